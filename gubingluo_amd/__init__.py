@@ -1,0 +1,241 @@
+"""gubingluo_amd -- MI355X-native 2-D acoustic prestack RTM (drop-in for gubingluo's rtm_gpu).
+
+The compute path is librtm_hip.so (hand-written gfx950 HIP kernels behind a
+C ABI, include/rtm_hip.h).  This module is a thin ctypes binding of that ABI
+for tests, the bench and Python callers; it holds no compute of its own and
+raises if the native library is not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "librtm_hip.so")
+EXE_PATH = os.path.join(PKG, "rtm_gpu")
+
+STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+
+# every symbol include/rtm_hip.h declares
+EXPORTED = [
+    "rtm_model", "rtm_real", "rtmh_last_error", "rtmh_version", "rtmh_device_count", "rtmh_create",
+    "rtmh_destroy", "rtmh_shot_geometry", "rtmh_run", "rtmh_fd_steps", "rtmh_coef_table", "rtmh_read_parfile",
+    "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
+]
+
+
+class Params(C.Structure):
+    """rtmh_params: the scalar arguments of rtm_model() (rtm_model.cu:12-17), same order."""
+    _fields_ = [
+        ("nshot", C.c_int), ("startshot", C.c_int), ("dshot", C.c_int), ("mode", C.c_int),
+        ("medium_flag", C.c_int), ("flag_layout", C.c_int), ("sgyflag", C.c_int),
+        ("endianflag", C.c_int), ("ismth", C.c_int),
+        ("nx", C.c_int), ("nz", C.c_int), ("nt", C.c_int),
+        ("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float),
+        ("npml", C.c_int),
+        ("direct", C.c_float), ("direct0", C.c_float), ("tsnap", C.c_float), ("fdom", C.c_float),
+        ("spx0", C.c_float), ("spz0", C.c_float), ("dspx", C.c_float), ("offset", C.c_float),
+        ("offsetmax", C.c_float), ("amp", C.c_float), ("alp", C.c_float),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [("batch", C.c_int), ("store_fmt", C.c_int), ("graph_chunk", C.c_int), ("tile_z", C.c_int),
+                ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [("image1", C.c_void_p), ("image2s", C.c_void_p), ("illum", C.c_void_p),
+                ("image_lap", C.c_void_p), ("image_norm", C.c_void_p),
+                ("shot_seis", C.c_void_p), ("shot_g1", C.c_void_p), ("shot_illum", C.c_void_p),
+                ("shot_seis_in", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("gpu_ms", C.c_double), ("fwd_ms", C.c_double), ("rev_ms", C.c_double),
+                ("cell_updates", C.c_double), ("nshots", C.c_int), ("batches", C.c_int), ("tile_z", C.c_int),
+                ("batch", C.c_int), ("hbm_bytes_alg", C.c_double), ("kern_ms", C.c_double * 4),
+                ("kern_calls", C.c_int * 4), ("step_bytes_fwd", C.c_double), ("step_bytes_rev", C.c_double)]
+
+
+class Geom(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ishot", "nx1", "nx2", "nxl", "nxp", "nzp", "nsx", "nsz", "nrec",
+                                       "rcol0", "noffset")]
+
+
+class Grid(C.Structure):
+    _fields_ = [("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float), ("nx", C.c_int), ("nz", C.c_int),
+                ("nt", C.c_int), ("lx", C.c_int), ("lz", C.c_int), ("vpmin", C.c_float), ("vpmax", C.c_float)]
+
+
+class Parfile(C.Structure):
+    _fields_ = [("dt0", C.c_float), ("nt0", C.c_int), ("tsnap", C.c_float),
+                ("xmax", C.c_float), ("zmax", C.c_float), ("dx0", C.c_float), ("dz0", C.c_float),
+                ("fdom", C.c_float), ("amp", C.c_float), ("alp", C.c_float),
+                ("spx0", C.c_float), ("spz0", C.c_float), ("dspx", C.c_float),
+                ("offset", C.c_float), ("offsetmax", C.c_float),
+                ("startshot", C.c_int), ("nshot", C.c_int), ("dshot", C.c_int), ("nshotall", C.c_int),
+                ("npml", C.c_int), ("medium_flag", C.c_int), ("ismth", C.c_int),
+                ("direct", C.c_float), ("direct0", C.c_float),
+                ("idevice", C.c_int), ("flag_layout", C.c_int), ("mode", C.c_int), ("dataflag", C.c_int),
+                ("sgyflag", C.c_int), ("endianflag", C.c_int), ("rhoflag", C.c_int),
+                ("vp_file", C.c_char * 256), ("shotgather", C.c_char * 256), ("rho_file", C.c_char * 256)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The native library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"librtm_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.rtmh_last_error.restype = C.c_char_p
+        L.rtmh_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().rtmh_last_error().decode()
+
+
+def device_count() -> int:
+    return int(lib().rtmh_device_count())
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(**kw) -> Params:
+    d = dict(nshot=1, startshot=1, dshot=1, mode=0, medium_flag=0, flag_layout=2, sgyflag=0, endianflag=0,
+             ismth=0, nx=100, nz=60, nt=200, dx=10.0, dz=10.0, dt=1e-3, npml=32, direct=1.5, direct0=0.05,
+             tsnap=0.1, fdom=20.0, spx0=500.0, spz0=20.0, dspx=50.0, offset=0.0, offsetmax=1000.0, amp=1.0,
+             alp=0.05)
+    d.update(kw)
+    P = Params()
+    for k, v in d.items():
+        setattr(P, k, v)
+    return P
+
+
+def coef_table() -> np.ndarray:
+    out = np.zeros(100, np.float32)
+    lib().rtmh_coef_table(_fp(out))
+    return out
+
+
+def geometry(P: Params, ishot: int) -> Geom:
+    g = Geom()
+    if lib().rtmh_shot_geometry(C.byref(P), C.c_int(ishot), C.byref(g)) != 0:
+        raise ValueError(f"shot {ishot} outside the model")
+    return g
+
+
+def shot_list(P: Params):
+    return list(range(P.startshot, P.nshot + 1, P.dshot))
+
+
+@dataclass
+class RunResult:
+    image1: np.ndarray
+    image2s: np.ndarray
+    illum: np.ndarray
+    image_lap: np.ndarray | None
+    image_norm: np.ndarray | None
+    shot_seis: list | None
+    shot_g1: list | None
+    shot_illum: list | None
+    stats: Stats
+
+
+class Context:
+    """One GPU's engine (rtmh_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        if lib().rtmh_create(C.c_int(device), C.byref(self._h)) != 0:
+            raise RuntimeError(f"rtmh_create failed: {last_error()}")
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().rtmh_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, P: Params, vp0, rho0, coef=None, *, batch=0, store_fmt=STORE_F32, graph_chunk=0, tile_z=0,
+            write_files=False, verbose=False, mem_budget=0.0, want_shots=False, want_post=True,
+            seis_in=None, profile=False) -> RunResult:
+        coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
+        vp0 = np.ascontiguousarray(vp0, np.float32)
+        rho0 = np.ascontiguousarray(rho0, np.float32)
+        nz, nx = P.nz, P.nx
+        assert vp0.shape == (nz, nx) and rho0.shape == (nz, nx)
+        img1 = np.zeros((nz, nx), np.float32)
+        img2 = np.zeros((nz, nx), np.float32)
+        ill = np.zeros((nz, nx), np.float32)
+        lap = np.zeros((nz, nx), np.float32) if want_post else None
+        nrm = np.zeros((nz, nx), np.float32) if want_post else None
+        out = Outputs(_fp(img1), _fp(img2), _fp(ill), _fp(lap), _fp(nrm), None, None, None, None)
+        shots = shot_list(P)
+        keep = []
+        seis = g1s = ils = None
+        if want_shots or seis_in is not None:
+            geoms = [geometry(P, s) for s in shots]
+        if want_shots:
+            seis = [np.zeros((P.nt, g.nrec), np.float32) for g in geoms]
+            g1s = [np.zeros((nz, g.nxl), np.float32) for g in geoms]
+            ils = [np.zeros((nz, g.nxl), np.float32) for g in geoms]
+            arrs = []
+            for lst in (seis, g1s, ils):
+                pa = (C.c_void_p * len(lst))(*[_fp(a) for a in lst])
+                arrs.append(pa)
+            keep.extend(arrs)
+            out.shot_seis = C.cast(arrs[0], C.c_void_p)
+            out.shot_g1 = C.cast(arrs[1], C.c_void_p)
+            out.shot_illum = C.cast(arrs[2], C.c_void_p)
+        if seis_in is not None:
+            si = [np.ascontiguousarray(a, np.float32) for a in seis_in]
+            for a, g in zip(si, geoms):
+                assert a.shape == (P.nt, g.nrec)
+            pa = (C.c_void_p * len(si))(*[_fp(a) for a in si])
+            keep.extend([pa, si])
+            out.shot_seis_in = C.cast(pa, C.c_void_p)
+        opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
+                      int(profile))
+        st = Stats()
+        rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),
+                            C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"rtmh_run failed ({rc}): {last_error()}")
+        return RunResult(img1, img2, ill, lap, nrm, seis, g1s, ils, st)
+
+    def fd_steps(self, P: Params, ishot, vp0, rho0, fields, nsteps, *, it0=0, with_source=False, coef=None):
+        """Advance fields (dict p, vx, vz, p1, p2 of shape (nzp, nxp), float32) in place."""
+        coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
+        for k in ("p", "vx", "vz", "p1", "p2"):
+            assert fields[k].dtype == np.float32 and fields[k].flags.c_contiguous
+        rc = lib().rtmh_fd_steps(self._h, C.byref(P), C.c_int(ishot), _fp(np.ascontiguousarray(vp0, np.float32)),
+                                 _fp(np.ascontiguousarray(rho0, np.float32)), _fp(coef), C.c_int(nsteps),
+                                 C.c_int(it0), C.c_int(int(with_source)), _fp(fields["p"]), _fp(fields["vx"]),
+                                 _fp(fields["vz"]), _fp(fields["p1"]), _fp(fields["p2"]))
+        if rc != 0:
+            raise RuntimeError(f"rtmh_fd_steps failed ({rc}): {last_error()}")

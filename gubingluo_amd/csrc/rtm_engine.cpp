@@ -1,0 +1,802 @@
+// rtm_engine.cpp -- host runtime of the RTM path: shot set-up, slot memory,
+// batched forward / reverse passes (plain launches or replayed hipGraph
+// chunks), FP64 stacking, outputs and the reference's file side effects.
+#include "rtm_engine.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "host_io.h"
+#include "rtm_launch.h"
+
+namespace rtm {
+
+static thread_local std::string g_err;
+void set_error(const std::string& s) { g_err = s; }
+const char* get_error() { return g_err.c_str(); }
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                           \
+      return -10;                                                                          \
+    }                                                                                      \
+  } while (0)
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------------ geometry ----
+bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s)
+{
+  const float dx = P.dx, dz = P.dz;
+  const int nx = P.nx, npml = P.npml;
+  int nx1, nx2, is = 0, noffset = 0;
+  float spx, spz;
+  if (P.flag_layout == 1) {                                  // rtm_model.cu:124-134
+    const int NX = (int)(P.offsetmax / dx + 1);
+    const int sp1 = (int)(P.spx0 / dx), np = (int)(P.dspx / dx), nr = (NX - 1) / 2;
+    is = sp1 + (ishot - 1) * np;
+    nx1 = imax(0, is - nr);
+    nx2 = imin(nx - 1, is + nr);
+    spx = (is - nx1) * dx;
+  } else if (P.flag_layout == 2) {                           // rtm_model.cu:627-630
+    nx1 = 0; nx2 = nx - 1;
+    spx = P.spx0 + (ishot - 1) * P.dspx;
+  } else if (P.flag_layout == 3 || P.flag_layout == 4) {     // rtm_model.cu:1126-1138, 1633-1645
+    const int NX = (int)(P.offsetmax / dx + 1);
+    const int sp1 = (int)(P.spx0 / dx), np = (int)(P.dspx / dx);
+    is = sp1 + (ishot - 1) * np;
+    noffset = (int)(P.offset / dx);
+    const int nr = NX - 1;
+    if (P.flag_layout == 3) { nx1 = imax(0, is); nx2 = imin(nx - 1, is + nr); }
+    else { nx1 = imax(0, is - nr); nx2 = imin(nx - 1, is); }
+    spx = (is - nx1) * dx;
+  } else {
+    return false;
+  }
+  const int nxl = nx2 - nx1 + 1;
+  if (nxl <= 0) return false;
+  spz = P.spz0;
+  rtmh_geom& g = s.g;
+  memset(&g, 0, sizeof(g));
+  g.ishot = ishot; g.nx1 = nx1; g.nx2 = nx2; g.nxl = nxl; g.noffset = noffset;
+  s.hk_nt = (short)(unsigned short)(P.nt);
+  s.hk_dt = (short)(unsigned short)(P.dt * 1000000);
+  s.hk_dx = dx;
+  if (P.flag_layout <= 2) {
+    s.hk_itraceall = (ishot - 1) * nxl + 1;
+    s.hk_offset = (int)(nx1 * dx - spx);
+    s.hk_sx = (int)spx;
+    s.hk_gx = (int)(nx1 * dx);
+    s.hk_ntrace = nxl;
+  } else {
+    s.hk_itraceall = (ishot - 1) * (nxl - noffset) + 1;
+    s.hk_offset = (int)P.offset;
+    s.hk_sx = (int)spx;
+    s.hk_gx = (int)(P.offset + nx1 * dx);
+    s.hk_ntrace = nxl - noffset;
+  }
+  spx = spx + npml * dx;
+  spz = spz + npml * dz;
+  g.nxp = nxl + 2 * npml;
+  g.nzp = P.nz + 2 * npml;
+  g.nsx = (int)(spx / dx);
+  g.nsz = (int)(spz / dz);
+  if (P.flag_layout <= 2) { g.nrec = nxl; g.rcol0 = npml; }
+  else if (P.flag_layout == 3) { g.nrec = nxl - noffset; g.rcol0 = npml + noffset; }
+  else { g.nrec = nxl - noffset; g.rcol0 = npml; }
+  if (g.nrec < 0) g.nrec = 0;
+  return true;
+}
+
+// padded-window value of a model at padded (iz, ix): the edge replication of
+// extendvel1 (Velocity_function.cpp:2011-2030) as a clamp into the window
+static inline float win_at(const float* m, int nx, int nz, int nx1, int nxl, int npml, int iz, int ix)
+{
+  const int r = imin(imax(iz - npml, 0), nz - 1);
+  const int c = nx1 + imin(imax(ix - npml, 0), nxl - 1);
+  return m[(size_t)r * nx + c];
+}
+
+void ricker(std::vector<float>& w, int nt, float fdom, float dt)
+{
+  const double pi = 3.1415926535897932;                       // common.h:11
+  w.resize(nt);
+  for (int it = 0; it < nt; it++) {
+    float t = (float)(pi * (double)fdom * fabs((double)(it * dt) - 1.0 / (double)fdom));
+    t *= t;
+    w[it] = (float)((1.0 - 2.0 * (double)t) * (double)expf(-t));
+  }
+}
+
+void taper_row(std::vector<float>& t, int nxl, int nwin)
+{
+  t.resize(nxl);
+  const float alpha = 0.06f;                                  // rtm_model.cu:71
+  for (int ix = 0; ix < nxl; ix++) {
+    float a = -alpha * alpha, v = 1.0f;
+    if (ix < nwin) { const float d = (float)(nwin - ix); a = a * d; a = a * d; v = expf(a); }
+    else if (ix >= nxl - nwin) { const float d = (float)(ix - nxl + nwin + 1); a = a * d; a = a * d; v = expf(a); }
+    t[ix] = v;
+  }
+}
+
+// PML profile (kernel.cu:174-269) and the damping factors of kernel.cu:592-593
+static void damping(std::vector<float>& c1, std::vector<float>& c2, int n, int npml, float vpmax, float h,
+                    bool half, float dt)
+{
+  const float R = 1.0e-10f;
+  const float d = npml * h;
+  const float d0 = (float)(1.5 * (double)vpmax * (double)logf(1 / R) / (double)d);
+  c1.resize(n); c2.resize(n);
+  for (int i = 0; i < n; i++) {
+    float dd = 0.0f;
+    if (i < npml) {
+      float t = half ? (float)((double)(float)(npml - i) - 0.5) : (float)(npml - i);
+      t = t / (float)npml; t = t * t; dd = d0 * t;
+    } else if (i >= n - npml) {
+      float t = half ? (float)((double)(float)(i - n + npml + 1) - 0.5) : (float)(i - n + npml + 1);
+      t = t / (float)npml; t = t * t; dd = d0 * t;
+    }
+    c1[i] = (float)(1.0 - 0.5 * (double)dt * (double)dd);
+    c2[i] = (float)(2.0 - (double)c1[i]);
+  }
+}
+
+void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s)
+{
+  const rtmh_geom& g = s.g;
+  const int npml = P.npml, nxp = g.nxp, nzp = g.nzp;
+  // per-shot vmax over the padded window (rtm_model.cu:207): edge replication
+  // adds no new values, so the window's column maxima of v^2 suffice
+  float m = colmax_vv[g.nx1];
+  for (int ix = g.nx1; ix <= g.nx2; ix++) m = colmax_vv[ix] > m ? colmax_vv[ix] : m;
+  s.vpmax = sqrtf(m);
+  damping(s.c1x, s.c2x, nxp, npml, s.vpmax, P.dx, false, P.dt);
+  damping(s.c1xv, s.c2xv, nxp - 1, npml, s.vpmax, P.dx, true, P.dt);
+  damping(s.c1z, s.c2z, nzp, npml, s.vpmax, P.dz, false, P.dt);
+  damping(s.c1zv, s.c2zv, nzp - 1, npml, s.vpmax, P.dz, true, P.dt);
+  s.c1xv.push_back(1.0f); s.c2xv.push_back(1.0f);            // pad to nxp / nzp
+  s.c1zv.push_back(1.0f); s.c2zv.push_back(1.0f);
+  // Gaussian source (kernel.cu:23-35): bounding box of its non-zero values
+  const float dx2 = P.dx * P.dx, dz2 = P.dz * P.dz;
+  int x0 = nxp, x1 = -1, z0 = nzp, z1 = -1;
+  std::vector<float> src((size_t)nxp * nzp);
+  for (int iz = 0; iz < nzp; iz++)
+    for (int ix = 0; ix < nxp; ix++) {
+      const float dist = (ix - g.nsx) * (ix - g.nsx) * dx2 + (iz - g.nsz) * (iz - g.nsz) * dz2;
+      const float v = P.amp * expf(-P.alp * P.alp * dist);
+      src[(size_t)iz * nxp + ix] = v;
+      if (v != 0.0f) { x0 = imin(x0, ix); x1 = imax(x1, ix); z0 = imin(z0, iz); z1 = imax(z1, iz); }
+    }
+  s.sa.clear();
+  if (x1 >= x0) {
+    s.sx0 = x0; s.sx1 = x1; s.sz0 = z0; s.sz1 = z1;
+    for (int iz = z0; iz <= z1; iz++)
+      for (int ix = x0; ix <= x1; ix++) s.sa.push_back(P.dt * src[(size_t)iz * nxp + ix]);
+  } else {
+    s.sx0 = 0; s.sx1 = -1; s.sz0 = 0; s.sz1 = -1;
+    s.sa.push_back(0.0f);
+  }
+  // direct-wave mute (kernel.cu:96-173) as the last muted sample per receiver
+  const float dt = P.dt;
+  const float _dt = (float)(1.0 / (double)dt);
+  const int nw = (int)(P.direct / (P.fdom * dt));
+  const int tlength = (int)(P.direct0 / dt);
+  const int coff = (P.flag_layout == 3) ? g.noffset : 0;
+  const float vvs = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, g.nsz, g.nsx);
+  const float vvsrc = vvs * vvs;
+  s.cut.assign(g.nrec > 0 ? g.nrec : 1, -1);
+  for (int ir = 0; ir < g.nrec; ir++) {
+    const int col = ir + npml + coff;
+    const float vr = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, npml, col);
+    const float vvr = vr * vr;
+    const float vtemp = (float)(0.482 * (1.0 / (double)sqrtf(vvr) + 1.0 / (double)sqrtf(vvsrc)));
+    float dist = (col - g.nsx) * (col - g.nsx) * dx2 + (npml - g.nsz) * (npml - g.nsz) * dz2;
+    dist = sqrtf(dist);
+    const int temp = tlength < P.nt ? tlength : P.nt;
+    const int c0 = (int)(dist * vtemp * _dt) + nw;
+    int c = c0 < P.nt ? c0 : P.nt;
+    c = c > temp ? c : temp;
+    c = c < P.nt ? c : P.nt;
+    s.cut[ir] = c;
+  }
+}
+
+// ------------------------------------------------------------------ pool ----
+DevPool::~DevPool() { trim(); }
+
+void* DevPool::get(size_t bytes)
+{
+  auto it = free_.lower_bound(bytes);
+  if (it != free_.end() && it->first <= bytes + bytes / 4) {
+    void* p = it->second;
+    free_.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    trim();
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  }
+  size_[p] = bytes;
+  held_ += bytes;
+  return p;
+}
+
+void DevPool::put(void* p)
+{
+  if (!p) return;
+  free_.insert({size_[p], p});
+}
+
+void DevPool::trim()
+{
+  for (auto& kv : free_) {
+    (void)hipFree(kv.second);
+    held_ -= size_[kv.second];
+    size_.erase(kv.second);
+  }
+  free_.clear();
+}
+
+// ---------------------------------------------------------------- engine ----
+bool GraphKey::operator<(const GraphKey& o) const
+{
+  if (ntiles != o.ntiles) return ntiles < o.ntiles;
+  if (tz != o.tz) return tz < o.tz;
+  if (phase != o.phase) return phase < o.phase;
+  if (save != o.save) return save < o.save;
+  if (quant != o.quant) return quant < o.quant;
+  if (k != o.k) return k < o.k;
+  if (shots != o.shots) return shots < o.shots;
+  return tiles < o.tiles;
+}
+
+Engine::Engine(int device) : device_(device)
+{
+  (void)hipSetDevice(device_);
+  (void)hipStreamCreateWithFlags(&st_, hipStreamNonBlocking);
+}
+
+Engine::~Engine()
+{
+  drop_graphs();
+  pool_.trim();
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Engine::drop_graphs()
+{
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+}
+
+static size_t store_elem(int fmt) { return fmt == ST_F32 ? 4 : 2; }
+
+// small per-slot arrays, in this order, each 256-byte aligned
+struct SmallLayout {
+  size_t c[8], sa, cut, taper, total;
+};
+static SmallLayout small_layout(const ShotSetup& s)
+{
+  SmallLayout L;
+  size_t o = 0;
+  const size_t nx = s.g.nxp, nz = s.g.nzp;
+  const size_t n8[8] = {nx, nx, nx, nx, nz, nz, nz, nz};
+  for (int i = 0; i < 8; i++) { L.c[i] = o; o += align256(n8[i] * 4); }
+  L.sa = o; o += align256(s.sa.size() * 4);
+  L.cut = o; o += align256(s.cut.size() * 4);
+  L.taper = o; o += align256(s.taper.size() * 4 + 4);
+  L.total = o;
+  return L;
+}
+
+size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
+{
+  const size_t pitch = pitch_of(s.g.nxp), nzp = s.g.nzp;
+  const size_t F = align256(nzp * pitch * 4);
+  const size_t nzi = nzp - 2 * (size_t)((s.g.nxp - s.g.nxl) / 2);
+  const size_t I = align256(nzi * s.g.nxl * 4);
+  const size_t strip = (size_t)40 * (nzi + s.g.nxl);
+  size_t b = small_layout(s).total + 10 * F + 2 * I;
+  b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
+  b += align256((size_t)nt * strip * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * strip * 4));
+  b += 2 * align256((size_t)nt * 4 * 4);
+  return b;
+}
+
+void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
+                         std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
+                         const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch)
+{
+  memset(&d, 0, sizeof(d));
+  const rtmh_geom& g = s.g;
+  d.nxp = g.nxp; d.nzp = g.nzp; d.npml = P.npml; d.pitch = pitch_of(g.nxp);
+  d.nxl = g.nxl; d.nzi = P.nz; d.nt = P.nt; d.phase = PH_IDLE;
+  d.it_off = 0; d.it_dir = 1;
+  d.record = 1; d.rcol0 = g.rcol0; d.nrec = g.nrec;
+  d.dt = P.dt; d.dtx = P.dt / P.dx; d.dtz = P.dt / P.dz;   // rtm_model.cu:73-74
+  d.rho = rho_dev; d.vv = vv_dev; d.mpitch = model_pitch;
+  d.mz0 = P.npml; d.mzn = P.nz; d.mx0 = g.nx1; d.mxs = P.npml; d.mxn = g.nxl;
+  // small arrays
+  const SmallLayout L = small_layout(s);
+  std::vector<char> img(L.total, 0);
+  const std::vector<float>* cv[8] = {&s.c1x, &s.c2x, &s.c1xv, &s.c2xv, &s.c1z, &s.c2z, &s.c1zv, &s.c2zv};
+  for (int i = 0; i < 8; i++) memcpy(img.data() + L.c[i], cv[i]->data(), cv[i]->size() * 4);
+  memcpy(img.data() + L.sa, s.sa.data(), s.sa.size() * 4);
+  memcpy(img.data() + L.cut, s.cut.data(), s.cut.size() * 4);
+  memcpy(img.data() + L.taper, s.taper.data(), s.taper.size() * 4);
+  uploads.push_back({base, std::move(img)});
+  const float** cp[8] = {&d.c1x, &d.c2x, &d.c1xv, &d.c2xv, &d.c1z, &d.c2z, &d.c1zv, &d.c2zv};
+  for (int i = 0; i < 8; i++) *cp[i] = (const float*)(base + L.c[i]);
+  d.sx0 = s.sx0; d.sx1 = s.sx1; d.sz0 = s.sz0; d.sz1 = s.sz1;
+  d.sa = (const float*)(base + L.sa);
+  d.cut = (const int*)(base + L.cut);
+  d.coef = coef_dev; d.wav = wav_dev;
+  size_t o = L.total;
+  const size_t F = align256((size_t)d.nzp * d.pitch * 4);
+  float** fp[10] = {&d.sp, &d.svx, &d.svz, &d.sp1, &d.sp2, &d.rp, &d.rvx, &d.rvz, &d.rp1, &d.rp2};
+  for (int i = 0; i < 10; i++) { *fp[i] = (float*)(base + o); o += F; }
+  const size_t I = align256((size_t)d.nzi * d.nxl * 4);
+  d.g1 = (float*)(base + o); o += I;
+  d.illum = (float*)(base + o); o += I;
+  d.rec = (float*)(base + o); o += align256((size_t)P.nt * (g.nrec > 0 ? g.nrec : 1) * 4);
+  d.strip_len = (long long)40 * (d.nzi + d.nxl);
+  d.store_fmt = store_fmt;
+  d.store = base + o;
+  o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * d.strip_len * 4));
+  d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4);
+  d.smax = (unsigned int*)(base + o); o += align256((size_t)P.nt * 4 * 4);
+}
+
+hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
+                                 int nslots, int store_fmt)
+{
+  auto it = graphs_.find(k);
+  if (it != graphs_.end()) return it->second;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  LaunchArgs a{dshots, dtiles, k.ntiles, 0};
+  for (int s = 0; s < k.k; s++) {
+    a.step = s;
+    (void)rtm_launch_step(a, k.phase, k.tz, k.save, st_);
+    if (k.quant) (void)rtm_launch_quant(dshots, dslots, nslots, s, 128, st_);
+  }
+  (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
+  if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) return nullptr;
+  hipGraphExec_t ex = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+  (void)hipGraphDestroy(g);
+  if (ex) graphs_[k] = ex;
+  (void)store_fmt;
+  return ex;
+}
+
+int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
+                     const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
+                     rtmh_stats* prof)
+{
+  const int save = phase == PH_FWD ? 1 : 0;
+  const int quant = (phase == PH_FWD && store_fmt != ST_F32) ? 1 : 0;
+  const int nslots = (int)slots.size();
+  int done = 0;
+  if (prof) {
+    // every kernel bracketed by its own event pair on the launch stream
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    LaunchArgs a{dshots, dtiles, ntiles, 0};
+    const int kb = phase == PH_FWD ? 0 : 2;
+    for (int s = 0; s < nsteps; s++) {
+      a.step = s;
+      HIPCHK(hipEventRecord(ev[0], st_));
+      HIPCHK(rtm_launch_phase(a, phase, tz, save, 0, st_));
+      HIPCHK(hipEventRecord(ev[1], st_));
+      HIPCHK(rtm_launch_phase(a, phase, tz, save, 1, st_));
+      HIPCHK(hipEventRecord(ev[2], st_));
+      if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, 128, st_));
+      HIPCHK(hipEventSynchronize(ev[2]));
+      float m0 = 0, m1 = 0;
+      HIPCHK(hipEventElapsedTime(&m0, ev[0], ev[1]));
+      HIPCHK(hipEventElapsedTime(&m1, ev[1], ev[2]));
+      prof->kern_ms[kb] += m0; prof->kern_ms[kb + 1] += m1;
+      prof->kern_calls[kb]++; prof->kern_calls[kb + 1]++;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return 0;
+  }
+  if (chunk > 0 && nsteps >= chunk) {
+    GraphKey key{ntiles, tz, phase, save, quant, chunk, dshots, dtiles};
+    hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
+    if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
+    const int nch = nsteps / chunk;
+    for (int c = 0; c < nch; c++) HIPCHK(hipGraphLaunch(ex, st_));
+    done = nch * chunk;
+  }
+  LaunchArgs a{dshots, dtiles, ntiles, 0};
+  for (int s = 0; s < nsteps - done; s++) {
+    a.step = s;
+    HIPCHK(rtm_launch_step(a, phase, tz, save, st_));
+    if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, 128, st_));
+  }
+  return 0;
+}
+
+static void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles)
+{
+  tiles.clear();
+  for (int k = 0; k < (int)ds.size(); k++)
+    for (int z0 = 0; z0 < ds[k].nzp; z0 += tz)
+      for (int x0 = 0; x0 < ds[k].nxp; x0 += RTM_TX) tiles.push_back(Tile{k, x0, z0, 0});
+}
+
+static bool params_ok(const rtmh_params& P, std::string& why)
+{
+  if (P.nx < 2 || P.nz < 2) { why = "nx, nz must be >= 2"; return false; }
+  if (P.nt < 2) { why = "nt must be >= 2"; return false; }
+  if (P.npml < 10) { why = "npml must be >= 10 (radius-10 strips)"; return false; }
+  if (!(P.dx > 0 && P.dz > 0 && P.dt > 0)) { why = "dx, dz, dt must be > 0"; return false; }
+  if (P.flag_layout < 1 || P.flag_layout > 4) { why = "flag_layout must be 1..4"; return false; }
+  if (P.dshot <= 0) { why = "dshot must be > 0"; return false; }
+  if (P.medium_flag != 0 || P.ismth != 0) {
+    why = "medium_flag / ismth model smoothing is not implemented";
+    return false;
+  }
+  return true;
+}
+
+int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
+                rtmh_outputs* out, rtmh_stats* stats)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  std::string why;
+  if (!params_ok(P, why)) { set_error(why); return -1; }
+  if (opt.store_fmt < 0 || opt.store_fmt > 2) { set_error("store_fmt must be 0, 1 or 2"); return -1; }
+  const int tz = opt.tile_z > 0 ? opt.tile_z : 16;
+  if (tz != 8 && tz != 16 && tz != 32) { set_error("tile_z must be 8, 16 or 32"); return -1; }
+  HIPCHK(hipSetDevice(device_));
+
+  const int nx = P.nx, nz = P.nz, nt = P.nt;
+  std::vector<int> shots;
+  for (int is = P.startshot; is <= P.nshot; is += P.dshot) shots.push_back(is);
+  const int nsh = (int)shots.size();
+
+  // resident model: rho and v^2 (extractvel1 squares, Velocity_function.cpp:2059)
+  const size_t nm = (size_t)nx * nz;
+  std::vector<float> vv(nm), colmax(nx, 0.0f);
+  for (size_t i = 0; i < nm; i++) vv[i] = vp0[i] * vp0[i];
+  for (int ix = 0; ix < nx; ix++) {
+    float m = vv[ix];
+    for (int iz = 0; iz < nz; iz++) m = vv[(size_t)iz * nx + ix] > m ? vv[(size_t)iz * nx + ix] : m;
+    colmax[ix] = m;
+  }
+  // per-shot set-up, taper window state carried in shot order (rtm_model.cu:70, 465-466)
+  std::vector<ShotSetup> setup(nsh);
+  int nwin = 50;
+  for (int k = 0; k < nsh; k++) {
+    if (!shot_geometry(P, shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
+    shot_setup(P, vp0, colmax.data(), setup[k]);
+    while (2 * nwin >= setup[k].g.nxl) nwin -= 3;
+    setup[k].nwin = nwin;
+    taper_row(setup[k].taper, setup[k].g.nxl, nwin);
+  }
+  std::vector<float> wav;
+  ricker(wav, nt, P.fdom, P.dt);
+
+  // device: model, coefficients, wavelet, FP64 stacks
+  const size_t common_bytes = 2 * align256(nm * 4) + align256(100 * 4) + align256((size_t)nt * 4) +
+                              3 * align256(nm * 8) + 3 * align256(nm * 4);
+  char* common = (char*)pool_.get(common_bytes);
+  if (!common) { set_error("out of device memory (model)"); return -3; }
+  float* d_rho = (float*)common;
+  float* d_vv = (float*)(common + align256(nm * 4));
+  float* d_coef = (float*)(common + 2 * align256(nm * 4));
+  float* d_wav = (float*)((char*)d_coef + align256(100 * 4));
+  double* d_i1 = (double*)((char*)d_wav + align256((size_t)nt * 4));
+  double* d_i2 = (double*)((char*)d_i1 + align256(nm * 8));
+  double* d_il = (double*)((char*)d_i2 + align256(nm * 8));
+  float* d_f1 = (float*)((char*)d_il + align256(nm * 8));
+  float* d_f2 = (float*)((char*)d_f1 + align256(nm * 4));
+  float* d_f3 = (float*)((char*)d_f2 + align256(nm * 4));
+  HIPCHK(hipMemcpyAsync(d_rho, rho0, nm * 4, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(d_coef, coef, 100 * 4, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(d_wav, wav.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st_));
+  HIPCHK(rtm_launch_zero_f64(d_i1, (long long)nm, st_));
+  HIPCHK(rtm_launch_zero_f64(d_i2, (long long)nm, st_));
+  HIPCHK(rtm_launch_zero_f64(d_il, (long long)nm, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+
+  // memory budget and batch size
+  size_t freeb = 0, totalb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  double budget = opt.mem_budget > 0 ? opt.mem_budget : 0.8 * (double)(freeb + pool_.held());
+  const int bmax = opt.batch > 0 ? opt.batch : 64;
+
+  hipEvent_t e0, e1, e2;
+  HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2));
+  double fwd_ms = 0, rev_ms = 0, cells = 0, algb = 0, step_fwd = 0, step_rev = 0;
+  if (stats) memset(stats, 0, sizeof(*stats));
+  int nbatches = 0, maxb = 0;
+  std::vector<double> img_tmp;  // for temp-image checkpoints
+  int rc = 0;
+
+  for (int k0 = 0; k0 < nsh && rc == 0;) {
+    // greedy batch of consecutive shots within the budget
+    int k1 = k0;
+    double used = 0;
+    while (k1 < nsh && k1 - k0 < bmax) {
+      const double b = (double)slot_bytes(setup[k1], nt, opt.store_fmt);
+      if (k1 > k0 && used + b > budget) break;
+      used += b;
+      k1++;
+    }
+    const int nb = k1 - k0;
+    maxb = nb > maxb ? nb : maxb;
+    nbatches++;
+    std::vector<void*> mem(nb, nullptr);
+    std::vector<ShotDev> ds(nb);
+    std::vector<std::pair<void*, std::vector<char>>> uploads;
+    for (int j = 0; j < nb; j++) {
+      mem[j] = pool_.get(slot_bytes(setup[k0 + j], nt, opt.store_fmt));
+      if (!mem[j]) { set_error("out of device memory (shot slots)"); rc = -3; break; }
+      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, d_rho, d_vv, nx);
+      ds[j].record = (P.mode == 0) ? 1 : 0;
+    }
+    if (rc != 0) { for (void* p : mem) pool_.put(p); break; }
+    std::vector<Tile> tiles;
+    build_tiles(ds, tz, tiles);
+    std::vector<int> slots(nb);
+    for (int j = 0; j < nb; j++) slots[j] = j;
+    // batch control block: descriptors, tiles, slot list, stack entries
+    const size_t cb_bytes = align256(nb * sizeof(ShotDev)) + align256(tiles.size() * sizeof(Tile)) +
+                            align256(nb * 4) + align256(nb * sizeof(StackShotHost));
+    char* cb = (char*)pool_.get(cb_bytes);
+    if (!cb) { set_error("out of device memory (batch)"); rc = -3; for (void* p : mem) pool_.put(p); break; }
+    ShotDev* d_ds = (ShotDev*)cb;
+    Tile* d_tiles = (Tile*)(cb + align256(nb * sizeof(ShotDev)));
+    int* d_slots = (int*)((char*)d_tiles + align256(tiles.size() * sizeof(Tile)));
+    StackShotHost* d_stk = (StackShotHost*)((char*)d_slots + align256(nb * 4));
+    for (auto& u : uploads) HIPCHK(hipMemcpyAsync(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice, st_));
+    for (int j = 0; j < nb; j++) { ds[j].phase = PH_FWD; ds[j].it_off = 0; ds[j].it_dir = 1; }
+    HIPCHK(hipMemcpyAsync(d_ds, ds.data(), nb * sizeof(ShotDev), hipMemcpyHostToDevice, st_));
+    HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
+    HIPCHK(hipMemcpyAsync(d_slots, slots.data(), nb * 4, hipMemcpyHostToDevice, st_));
+    for (int j = 0; j < nb; j++) HIPCHK(rtm_launch_phase_init(d_ds, j, PH_FWD, 512, st_));
+    if (opt.verbose)
+      for (int j = 0; j < nb; j++) printf("N0. %d shot source wavefield calculating......\n", shots[k0 + j]);
+
+    // ---- forward pass: propagation + record + boundary strips
+    HIPCHK(hipEventRecord(e0, st_));
+    rc = run_pass(PH_FWD, nt, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
+                  (opt.profile && stats) ? stats : nullptr);
+    if (rc) { for (void* p : mem) pool_.put(p); pool_.put(cb); break; }
+    HIPCHK(hipEventRecord(e1, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    fwd_ms += ms;
+
+    // seismograms: outputs, mode-0 files, mode-1 inputs
+    std::vector<float> rec;
+    for (int j = 0; j < nb && rc == 0; j++) {
+      const ShotSetup& s = setup[k0 + j];
+      const size_t nrv = (size_t)nt * s.g.nrec;
+      const int ishot = shots[k0 + j];
+      if (P.mode == 0) {
+        if ((out && out->shot_seis && out->shot_seis[k0 + j]) || opt.write_files) {
+          rec.resize(nrv);
+          HIPCHK(hipMemcpy(rec.data(), ds[j].rec, nrv * 4, hipMemcpyDeviceToHost));
+          if (out && out->shot_seis && out->shot_seis[k0 + j]) memcpy(out->shot_seis[k0 + j], rec.data(), nrv * 4);
+          if (opt.write_files) {
+            SuKey key;
+            key.iTraceAll = s.hk_itraceall; key.iTrace = 1; key.Offset = s.hk_offset; key.sx = s.hk_sx;
+            key.gx = s.hk_gx; key.Nt = s.hk_nt; key.Dt = s.hk_dt; key.Dx = s.hk_dx; key.Ntrace = s.hk_ntrace;
+            char path[64];
+            snprintf(path, sizeof(path), "./seisReal/%dseisreal.su", ishot);
+            rtmio_write_seis(rec.data(), key, 0, P.sgyflag, P.endianflag, path);
+          }
+        }
+      } else {
+        rec.assign(nrv, 0.0f);
+        if (out && out->shot_seis_in && out->shot_seis_in[k0 + j]) {
+          memcpy(rec.data(), out->shot_seis_in[k0 + j], nrv * 4);
+        } else {
+          char path[64];
+          snprintf(path, sizeof(path), "./seisReal/%dseisreal.su", ishot);
+          SuKey key;
+          if (rtmio_read_seis(rec.data(), &key, 0, P.sgyflag, path, (int)nrv) != 0) {
+            set_error(std::string("cannot read ") + path);
+            rc = -4;
+            break;
+          }
+        }
+        HIPCHK(hipMemcpy(ds[j].rec, rec.data(), nrv * 4, hipMemcpyHostToDevice));
+      }
+    }
+    if (rc) { for (void* p : mem) pool_.put(p); pool_.put(cb); break; }
+
+    // ---- reverse pass: reconstruction + receiver propagation + imaging
+    for (int j = 0; j < nb; j++) { ds[j].phase = PH_REV; ds[j].it_off = nt - 2; ds[j].it_dir = -1; }
+    HIPCHK(hipMemcpyAsync(d_ds, ds.data(), nb * sizeof(ShotDev), hipMemcpyHostToDevice, st_));
+    for (int j = 0; j < nb; j++) {
+      HIPCHK(rtm_launch_phase_init(d_ds, j, PH_REV, 512, st_));
+      HIPCHK(rtm_launch_seed(d_ds, j, ds[j].nrec, st_));
+    }
+    if (opt.verbose) printf("source wavefiled prepared over...\nBegin backward......\n");
+    HIPCHK(hipEventRecord(e1, st_));
+    rc = run_pass(PH_REV, nt - 1, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
+                  (opt.profile && stats) ? stats : nullptr);
+    if (rc) { for (void* p : mem) pool_.put(p); pool_.put(cb); break; }
+    HIPCHK(hipEventRecord(e2, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    HIPCHK(hipEventElapsedTime(&ms, e1, e2));
+    rev_ms += ms;
+
+    for (int j = 0; j < nb; j++) {
+      const ShotSetup& s = setup[k0 + j];
+      const size_t ni = (size_t)nz * s.g.nxl;
+      if (out && out->shot_g1 && out->shot_g1[k0 + j]) HIPCHK(hipMemcpy(out->shot_g1[k0 + j], ds[j].g1, ni * 4, hipMemcpyDeviceToHost));
+      if (out && out->shot_illum && out->shot_illum[k0 + j]) HIPCHK(hipMemcpy(out->shot_illum[k0 + j], ds[j].illum, ni * 4, hipMemcpyDeviceToHost));
+      const double nxz = (double)s.g.nxp * s.g.nzp, inn = (double)nz * s.g.nxl;
+      cells += 2.0 * nt * nxz + (double)(nt - 1) * inn;
+      // algorithmic bytes: FD step 32 B/cell + 16 B per PML cell (forward and
+      // receiver passes), reconstruction 32 B and imaging 16 B per interior
+      // cell, boundary strips written once and read once
+      const double pml = nxz - inn;
+      const double strip = 40.0 * (nz + s.g.nxl) * (double)store_elem(opt.store_fmt);
+      algb += 2.0 * nt * (32.0 * nxz + 16.0 * pml) + (double)(nt - 1) * inn * (32.0 + 16.0) + 2.0 * nt * strip;
+      if (k0 == 0) {
+        step_fwd += 32.0 * nxz + 16.0 * pml + strip;
+        step_rev += 32.0 * nxz + 16.0 * pml + inn * (32.0 + 16.0) + strip;
+      }
+    }
+    // ---- stack in shot order (FP64); temp images every 50 shots
+    std::vector<StackShotHost> stk(nb);
+    for (int j = 0; j < nb; j++) stk[j] = StackShotHost{ds[j].g1, ds[j].illum, (const float*)((const char*)mem[j] + small_layout(setup[k0 + j]).taper), setup[k0 + j].g.nx1, setup[k0 + j].g.nxl};
+    HIPCHK(hipMemcpyAsync(d_stk, stk.data(), nb * sizeof(StackShotHost), hipMemcpyHostToDevice, st_));
+    for (int j = 0; j < nb; j++) {
+      HIPCHK(rtm_launch_stack(d_stk + j, 1, nx, nz, d_i1, d_i2, d_il, st_));
+      const int ishot = shots[k0 + j];
+      if (opt.write_files && (ishot - 1) % 50 == 0) {
+        std::vector<double> a(nm);
+        std::vector<float> f(nm), zero(nm, 0.0f);
+        struct { double* d; const char* name; } outs[] = {{d_i2, "Image2s"}, {d_i1, "Image1"}, {d_il, "Illum"}, {d_i1, "Image2m"}};
+        for (auto& o : outs) {
+          HIPCHK(hipMemcpyAsync(a.data(), o.d, nm * 8, hipMemcpyDeviceToHost, st_));
+          HIPCHK(hipStreamSynchronize(st_));
+          for (size_t i = 0; i < nm; i++) f[i] = (float)a[i];
+          char path[64];
+          snprintf(path, sizeof(path), "./output/%stemp.su", o.name);
+          rtmio_write_image(f.data(), nz, nx, P.dx, path);
+        }
+        const char* zeros[] = {"Image3s", "Image4s", "Image5s", "Image3m", "Image4m", "Image5m", "Imagetrue"};
+        for (const char* zn : zeros) {
+          char path[64];
+          snprintf(path, sizeof(path), "./output/%stemp.su", zn);
+          rtmio_write_image(zero.data(), nz, nx, P.dx, path);
+        }
+      }
+    }
+    HIPCHK(hipStreamSynchronize(st_));
+    if (opt.verbose)
+      for (int j = 0; j < nb; j++) printf("%d shot finished: %g (s)\n", shots[k0 + j], 0.0);
+    for (void* p : mem) pool_.put(p);
+    pool_.put(cb);
+    k0 = k1;
+  }
+  if (rc != 0) { pool_.put(common); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); return rc; }
+
+  // ---- final images
+  HIPCHK(rtm_launch_round(d_i1, d_f1, (long long)nm, st_));
+  HIPCHK(rtm_launch_round(d_i2, d_f2, (long long)nm, st_));
+  HIPCHK(rtm_launch_round(d_il, d_f3, (long long)nm, st_));
+  if (out) {
+    if (out->image1) HIPCHK(hipMemcpyAsync(out->image1, d_f1, nm * 4, hipMemcpyDeviceToHost, st_));
+    if (out->image2s) HIPCHK(hipMemcpyAsync(out->image2s, d_f2, nm * 4, hipMemcpyDeviceToHost, st_));
+    if (out->illum) HIPCHK(hipMemcpyAsync(out->illum, d_f3, nm * 4, hipMemcpyDeviceToHost, st_));
+    if (out->image_lap || out->image_norm) {
+      // reuse the FP64 buffer d_i2 as float scratch after image2s is copied out
+      HIPCHK(hipStreamSynchronize(st_));
+      float* scratch = (float*)d_i1;
+      if (out->image_lap) {
+        HIPCHK(rtm_launch_laplace(d_f1, scratch, nx, nz, (int)P.dx, (int)P.dz, st_));
+        HIPCHK(hipMemcpyAsync(out->image_lap, scratch, nm * 4, hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+      }
+      if (out->image_norm) {
+        HIPCHK(rtm_launch_norm_multi(d_f1, d_f3, scratch, (long long)nm, st_));
+        HIPCHK(hipMemcpyAsync(out->image_norm, scratch, nm * 4, hipMemcpyDeviceToHost, st_));
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st_));
+  pool_.put(common);
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
+  if (stats) {
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->fwd_ms = fwd_ms; stats->rev_ms = rev_ms; stats->gpu_ms = fwd_ms + rev_ms;
+    stats->cell_updates = cells; stats->nshots = nsh; stats->batches = nbatches; stats->tile_z = tz;
+    stats->batch = maxb; stats->hbm_bytes_alg = algb;
+    stats->step_bytes_fwd = step_fwd; stats->step_bytes_rev = step_rev;
+  }
+  return 0;
+}
+
+int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const float* rho0, const float* coef,
+                     int nsteps, int it0, int with_source, float* pf, float* vxf, float* vzf, float* p1f, float* p2f)
+{
+  std::string why;
+  if (!params_ok(P, why)) { set_error(why); return -1; }
+  if (it0 < 0 || it0 + nsteps > P.nt) { set_error("it0 + nsteps exceeds nt"); return -1; }
+  HIPCHK(hipSetDevice(device_));
+  ShotSetup s;
+  if (!shot_geometry(P, ishot, s)) { set_error("shot window outside the model"); return -2; }
+  const int nx = P.nx, nz = P.nz;
+  const size_t nm = (size_t)nx * nz;
+  std::vector<float> vv(nm), colmax(nx);
+  for (size_t i = 0; i < nm; i++) vv[i] = vp0[i] * vp0[i];
+  for (int ix = 0; ix < nx; ix++) {
+    float m = vv[ix];
+    for (int iz = 0; iz < nz; iz++) m = vv[(size_t)iz * nx + ix] > m ? vv[(size_t)iz * nx + ix] : m;
+    colmax[ix] = m;
+  }
+  shot_setup(P, vp0, colmax.data(), s);
+  taper_row(s.taper, s.g.nxl, 1);
+  if (!with_source) { s.sx0 = 0; s.sx1 = -1; s.sz0 = 0; s.sz1 = -1; }
+  std::vector<float> wav;
+  ricker(wav, P.nt, P.fdom, P.dt);
+  const size_t cb = 2 * align256(nm * 4) + align256(400) + align256((size_t)P.nt * 4);
+  char* common = (char*)pool_.get(cb);
+  const size_t sb = slot_bytes(s, P.nt, ST_F32);
+  char* slot = (char*)pool_.get(sb);
+  std::vector<Tile> tiles;
+  const int tz = 16;
+  ShotDev d;
+  std::vector<std::pair<void*, std::vector<char>>> up;
+  float* d_rho = (float*)common;
+  float* d_vv = (float*)(common + align256(nm * 4));
+  float* d_coef = (float*)(common + 2 * align256(nm * 4));
+  float* d_wav = (float*)((char*)d_coef + align256(400));
+  layout_slot(P, s, ST_F32, slot, d, up, d_coef, d_wav, d_rho, d_vv, nx);
+  d.record = 0; d.phase = PH_FWD; d.it_off = it0; d.it_dir = 1;
+  std::vector<ShotDev> dv(1, d);
+  build_tiles(dv, tz, tiles);
+  char* ctl = (char*)pool_.get(align256(sizeof(ShotDev)) + align256(tiles.size() * sizeof(Tile)));
+  ShotDev* d_ds = (ShotDev*)ctl;
+  Tile* d_tiles = (Tile*)(ctl + align256(sizeof(ShotDev)));
+  HIPCHK(hipMemcpy(d_rho, rho0, nm * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_coef, coef, 400, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_wav, wav.data(), (size_t)P.nt * 4, hipMemcpyHostToDevice));
+  for (auto& u : up) HIPCHK(hipMemcpy(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_ds, &d, sizeof(ShotDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  // fields in: host [nzp*nxp] -> device pitch
+  const int nxp = s.g.nxp, nzp = s.g.nzp, pitch = d.pitch;
+  float* hf[5] = {pf, vxf, vzf, p1f, p2f};
+  float* dfp[5] = {d.sp, d.svx, d.svz, d.sp1, d.sp2};
+  for (int k = 0; k < 5; k++) {
+    HIPCHK(hipMemset(dfp[k], 0, (size_t)nzp * pitch * 4));
+    HIPCHK(hipMemcpy2D(dfp[k], (size_t)pitch * 4, hf[k], (size_t)nxp * 4, (size_t)nxp * 4, nzp, hipMemcpyHostToDevice));
+  }
+  LaunchArgs a{d_ds, d_tiles, (int)tiles.size(), 0};
+  for (int k = 0; k < nsteps; k++) {
+    a.step = k;
+    HIPCHK(rtm_launch_step(a, PH_FWD, tz, 0, st_));
+  }
+  HIPCHK(hipStreamSynchronize(st_));
+  for (int k = 0; k < 5; k++)
+    HIPCHK(hipMemcpy2D(hf[k], (size_t)nxp * 4, dfp[k], (size_t)pitch * 4, (size_t)nxp * 4, nzp, hipMemcpyDeviceToHost));
+  pool_.put(ctl); pool_.put(slot); pool_.put(common);
+  return 0;
+}
+
+}  // namespace rtm

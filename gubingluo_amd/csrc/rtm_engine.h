@@ -1,0 +1,105 @@
+// rtm_engine.h -- host runtime of the RTM path on one GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/rtm_hip.h"
+#include "rtm_dev.h"
+
+namespace rtm {
+
+// ---- per-shot host set-up -------------------------------------------------
+struct ShotSetup {
+  rtmh_geom g;
+  // SU key header of the shot (rtm_model.cu:137-145)
+  int hk_itraceall = 0, hk_offset = 0, hk_sx = 0, hk_gx = 0, hk_ntrace = 0;
+  short hk_nt = 0, hk_dt = 0;
+  float hk_dx = 0.f;
+  float vpmax = 0.f;
+  std::vector<float> c1x, c2x, c1xv, c2xv, c1z, c2z, c1zv, c2zv;  // damping factors
+  int sx0 = 0, sx1 = -1, sz0 = 0, sz1 = -1;                          // source box
+  std::vector<float> sa;                                              // dt*src in the box
+  std::vector<int> cut;                                               // mute
+  std::vector<float> taper;                                           // nxl
+  int nwin = 0;
+};
+
+// shot window and padded geometry (rtm_model.cu:122-153 and the layout
+// variants); returns false if the window is empty or the layout unknown
+bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s);
+// host set-up of one shot given the job's v^2 column maxima
+void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s);
+// Ricker of kernel.cu:13-22
+void ricker(std::vector<float>& w, int nt, float fdom, float dt);
+// taper row (kernel.cu:1591-1604)
+void taper_row(std::vector<float>& t, int nxl, int nwin);
+
+// ---- device memory pool ------------------------------------------------------
+class DevPool {
+ public:
+  ~DevPool();
+  void* get(size_t bytes);
+  void put(void* p);
+  void trim();
+  size_t held() const { return held_; }
+ private:
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> size_;
+  size_t held_ = 0;
+};
+
+// ---- engine -----------------------------------------------------------------
+struct SlotMem {
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
+struct GraphKey {
+  int ntiles, tz, phase, save, quant, k;
+  const void* shots;
+  const void* tiles;
+  bool operator<(const GraphKey& o) const;
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  int device() const { return device_; }
+  hipStream_t stream() const { return st_; }
+
+  int run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
+          rtmh_outputs* out, rtmh_stats* stats);
+  int fd_steps(const rtmh_params& P, int ishot, const float* vp0, const float* rho0, const float* coef, int nsteps,
+               int it0, int with_source, float* pf, float* vxf, float* vzf, float* p1f, float* p2f);
+
+  // bytes one slot of this geometry needs on the device
+  static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt);
+  static int pitch_of(int nxp) { return ((nxp + 63) / 64) * 64; }
+
+ private:
+  int device_;
+  hipStream_t st_ = nullptr;
+  DevPool pool_;
+  std::map<GraphKey, hipGraphExec_t> graphs_;
+
+  // build the device descriptor of one slot inside `mem`
+  void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
+                   std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
+                   const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch);
+  int run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
+               const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
+               rtmh_stats* prof);
+  hipGraphExec_t graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
+                           int nslots, int store_fmt);
+  void drop_graphs();
+};
+
+// thread-local error text for rtmh_last_error()
+void set_error(const std::string& s);
+const char* get_error();
+
+}  // namespace rtm

@@ -1,0 +1,749 @@
+// rtm_kernels.hip -- gfx950 kernels of the RTM hot path.
+//
+// Per time step two phases are launched over a tile table that may span many
+// shots (ragged windows):
+//   velocity phase  : vx, vz from p          (kernel.cu:430-500, 542-713 forward;
+//                                              kernel.cu:318-388 reverse)
+//   pressure phase  : p from vx, vz, + PML split pair, source, record, strips,
+//                     receiver insertion, imaging (kernel.cu:501-541, 714-1185,
+//                     36-95, 1186-1416)
+// A tile is 64 columns (one wave64 row, coalesced along x, the fast axis of
+// every array) by TZ rows; the fields it needs are staged once in LDS with
+// the radius-10 halo, and the x- and z-derivatives are read from LDS rows and
+// columns.  The per-cell arithmetic follows the reference operation order
+// with the fused multiply-adds written explicitly (see the oracle header), so
+// results are reproducible bit for bit against the FP32 CPU restatement.
+//
+// Compiled with -ffp-contract=off: no fusion other than the __builtin_fmaf
+// calls below; float division is IEEE (correctly rounded) as HIP compiles it.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "rtm_dev.h"
+
+#define LW 83              // 64 + 19: tile width incl. the x halo
+#define NTHR 256
+
+namespace {
+
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+
+// blockIdx -> tile: workgroups are dealt to the 8 XCDs round-robin, so give
+// each XCD a contiguous run of the tile table (neighbouring tiles of one shot
+// share their halos through that XCD's L2).
+__device__ __forceinline__ int xcd_tile(int ntiles)
+{
+  const int g = gridDim.x;                 // launched as a multiple of 8
+  const int per = g >> 3;
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per + (b >> 3);
+  return t < ntiles ? t : -1;
+}
+
+__device__ __forceinline__ float model_at(const float* __restrict__ m, const ShotDev& s, int iz, int ix)
+{
+  const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
+  const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
+  return m[(size_t)r * s.mpitch + c];
+}
+
+// stage f[zs .. zs+H) x [xs .. xs+W) into L (row-major, width W); cells
+// outside [0,nr) x [0,nc) read as 0.
+template <int W, int H>
+__device__ __forceinline__ void stage(float* __restrict__ L, const float* __restrict__ f, int pitch,
+                                      int nr, int nc, int zs, int xs)
+{
+  for (int e = threadIdx.x; e < W * H; e += NTHR) {
+    const int r = e / W, c = e - r * W;
+    const int gz = zs + r, gx = xs + c;
+    float v = 0.0f;
+    if (gz >= 0 && gz < nr && gx >= 0 && gx < nc) v = f[(size_t)gz * pitch + gx];
+    L[e] = v;
+  }
+}
+
+// ---------------------------------------------------------------- strips --
+// store offsets of the boundary strips (kernel.cu:1186-1293 packing); -1 if
+// the cell is not part of the block.
+__device__ __forceinline__ long long strip_vx(const ShotDev& s, int iz, int ix)
+{
+  if (iz < s.npml || iz >= s.nzp - s.npml) return -1;
+  const int r = iz - s.npml;
+  if (ix >= s.npml - RTM_R && ix < s.npml) return (long long)r * RTM_R + ix - (s.npml - RTM_R);
+  const int c0 = s.nxp - s.npml - 1;
+  if (ix >= c0 && ix < c0 + RTM_R) return (long long)RTM_R * s.nzi + (long long)r * RTM_R + ix - c0;
+  return -1;
+}
+__device__ __forceinline__ long long strip_vz(const ShotDev& s, int iz, int ix)
+{
+  if (ix < s.npml || ix >= s.nxp - s.npml) return -1;
+  const long long base = 2LL * RTM_R * s.nzi;
+  const int c = ix - s.npml;
+  if (iz >= s.npml - RTM_R && iz < s.npml) return base + (long long)(iz - (s.npml - RTM_R)) * s.nxl + c;
+  const int r0 = s.nzp - s.npml - 1;
+  if (iz >= r0 && iz < r0 + RTM_R) return base + (long long)RTM_R * s.nxl + (long long)(iz - r0) * s.nxl + c;
+  return -1;
+}
+__device__ __forceinline__ long long strip_p(const ShotDev& s, int iz, int ix)
+{
+  const long long base_lr = 2LL * RTM_R * s.nzi + 2LL * RTM_R * s.nxl;
+  const long long base_tb = base_lr + 2LL * RTM_R * s.nzi;
+  if (iz >= s.npml && iz < s.nzp - s.npml) {
+    const int r = iz - s.npml;
+    if (ix >= s.npml - RTM_R && ix < s.npml) return base_lr + (long long)r * RTM_R + ix - (s.npml - RTM_R);
+    const int c0 = s.nxp - s.npml;
+    if (ix >= c0 && ix < c0 + RTM_R) return base_lr + (long long)RTM_R * s.nzi + (long long)r * RTM_R + ix - c0;
+    return -1;
+  }
+  if (ix < s.npml || ix >= s.nxp - s.npml) return -1;
+  const int c = ix - s.npml;
+  if (iz >= s.npml - RTM_R && iz < s.npml) return base_tb + (long long)(iz - (s.npml - RTM_R)) * s.nxl + c;
+  const int r0 = s.nzp - s.npml;
+  if (iz >= r0 && iz < r0 + RTM_R) return base_tb + (long long)RTM_R * s.nxl + (long long)(iz - r0) * s.nxl + c;
+  return -1;
+}
+
+// 16-bit store: block index of a strip offset (0 vx, 1 vz, 2 p-lr, 3 p-tb)
+__device__ __forceinline__ int strip_block(const ShotDev& s, long long off)
+{
+  const long long a = 2LL * RTM_R * s.nzi, b = a + 2LL * RTM_R * s.nxl, c = b + 2LL * RTM_R * s.nzi;
+  return off < a ? 0 : (off < b ? 1 : (off < c ? 2 : 3));
+}
+
+__device__ __forceinline__ void store_put(const ShotDev& s, int it, long long off, float v)
+{
+  const long long k = (long long)it * s.strip_len + off;
+  if (s.store_fmt == ST_F32) {
+    ((float*)s.store)[k] = v;
+  } else {
+    // raw fp32 staging of step it (ping-pong by parity); quantised by k_quant
+    float* stage = (float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
+    (void)k;
+    stage[off] = v;
+    const unsigned int bits = __float_as_uint(fabsf(v));
+    atomicMax(&s.smax[(long long)it * 4 + strip_block(s, off)], bits);
+  }
+}
+
+__device__ __forceinline__ float store_get(const ShotDev& s, int it, long long off)
+{
+  const long long k = (long long)it * s.strip_len + off;
+  if (s.store_fmt == ST_F32) return ((const float*)s.store)[k];
+  const float sc = s.sscale[(long long)it * 4 + strip_block(s, off)];
+  if (s.store_fmt == ST_F16) return __half2float(((const __half*)s.store)[k]) * sc;
+  const unsigned short h = ((const unsigned short*)s.store)[k];
+  return __uint_as_float(((unsigned int)h) << 16) * sc;
+}
+
+// ------------------------------------------------------- velocity phase --
+// Forward velocity update of one wavefield over the whole padded grid:
+//   vx(iz,ix), ix <= nxp-2, radius min(10, ix+1, nxp-1-ix)   (norderx, kernel.cu:299-307)
+//   vz(iz,ix), iz <= nzp-2, radius min(10, iz+1, nzp-1-iz)   (norderz, kernel.cu:309-317)
+//   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho   (kernel.cu:588-594)
+// L holds p rows z0-9 .. z0+TZ+9, columns x0-9 .. x0+73.
+template <int TZ>
+__device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __restrict__ vx, float* __restrict__ vz,
+                                        const ShotDev& s, int x0, int z0, int save_it)
+{
+  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  const int ix = x0 + lx;
+  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch;
+  if (ix >= nxp) return;
+  const float* __restrict__ cf = s.coef;
+  float c10[RTM_R];
+#pragma unroll
+  for (int i = 0; i < RTM_R; i++) c10[i] = cf[90 + i];
+  const float dtx = s.dtx, dtz = s.dtz;
+  const bool hasvx = ix <= nxp - 2;
+  const int rx = imin(RTM_R, imin(ix + 1, nxp - 1 - ix));
+  float c1x = 1.0f, c2x = 1.0f;
+  if (hasvx) { c1x = s.c1xv[ix]; c2x = s.c2xv[ix]; }
+  const bool dampx = !(c1x == 1.0f && c2x == 1.0f);
+  for (int j = lz; j < TZ; j += 4) {
+    const int iz = z0 + j;
+    if (iz >= nzp) break;
+    const float rho = model_at(s.rho, s, iz, ix);
+    const float* __restrict__ Lr = L + (j + 9) * LW + lx + 9;   // -> p(iz, ix)
+    if (hasvx) {
+      float acc = 0.0f;
+      if (rx == RTM_R) {
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
+      } else {
+        const float* c = cf + (rx - 1) * RTM_R;
+        for (int i = 1; i <= rx; i++) acc = __builtin_fmaf(c[i - 1], Lr[i] - Lr[1 - i], acc);
+      }
+      acc = acc * dtx;
+      acc = acc / rho;
+      const size_t id = (size_t)iz * P + ix;
+      float v = vx[id];
+      v = dampx ? __builtin_fmaf(c1x, v, -acc) / c2x : v - acc;
+      vx[id] = v;
+      if (save_it >= 0) {
+        const long long o = strip_vx(s, iz, ix);
+        if (o >= 0) store_put(s, save_it, o, v);
+      }
+    }
+    if (iz <= nzp - 2) {
+      const int rz = imin(RTM_R, imin(iz + 1, nzp - 1 - iz));
+      const float c1z = s.c1zv[iz], c2z = s.c2zv[iz];
+      float acc = 0.0f;
+      const float* __restrict__ Lc = L + (j + 9) * LW + lx + 9;
+      if (rz == RTM_R) {
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lc[i * LW] - Lc[(1 - i) * LW], acc);
+      } else {
+        const float* c = cf + (rz - 1) * RTM_R;
+        for (int i = 1; i <= rz; i++) acc = __builtin_fmaf(c[i - 1], Lc[i * LW] - Lc[(1 - i) * LW], acc);
+      }
+      acc = acc * dtz;
+      acc = acc / rho;
+      const size_t id = (size_t)iz * P + ix;
+      float v = vz[id];
+      v = (c1z == 1.0f && c2z == 1.0f) ? v - acc : __builtin_fmaf(c1z, v, -acc) / c2z;
+      vz[id] = v;
+      if (save_it >= 0) {
+        const long long o = strip_vz(s, iz, ix);
+        if (o >= 0) store_put(s, save_it, o, v);
+      }
+    }
+  }
+}
+
+// Reverse velocity update of the source wavefield (kernel.cu:318-388 with the
+// strip restore of kernel.cu:1294-1347, rtm_model.cu:424-429): strips of step
+// it+1 are written back, interior cells get v += D.
+template <int TZ>
+__device__ __forceinline__ void vel_rev(const float* __restrict__ L, float* __restrict__ vx, float* __restrict__ vz,
+                                        const ShotDev& s, int x0, int z0, int it)
+{
+  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  const int ix = x0 + lx;
+  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
+  if (ix >= nxp) return;
+  float c10[RTM_R];
+#pragma unroll
+  for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
+  const float dtx = s.dtx, dtz = s.dtz;
+  const bool xin_vx = ix >= npml && ix <= nxp - npml - 2;
+  const bool xin_vz = ix >= npml && ix < nxp - npml;
+  for (int j = lz; j < TZ; j += 4) {
+    const int iz = z0 + j;
+    if (iz >= nzp) break;
+    const size_t id = (size_t)iz * P + ix;
+    const float* __restrict__ Lr = L + (j + 9) * LW + lx + 9;
+    // vx
+    {
+      const long long o = strip_vx(s, iz, ix);
+      if (o >= 0) {
+        vx[id] = store_get(s, it + 1, o);
+      } else if (xin_vx && iz >= npml && iz < nzp - npml) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
+        acc = acc * dtx;
+        acc = acc / model_at(s.rho, s, iz, ix);
+        vx[id] = vx[id] + acc;
+      }
+    }
+    // vz
+    {
+      const long long o = strip_vz(s, iz, ix);
+      if (o >= 0) {
+        vz[id] = store_get(s, it + 1, o);
+      } else if (xin_vz && iz >= npml && iz <= nzp - npml - 2) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i * LW] - Lr[(1 - i) * LW], acc);
+        acc = acc * dtz;
+        acc = acc / model_at(s.rho, s, iz, ix);
+        vz[id] = vz[id] + acc;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------- pressure phase --
+// Spatial derivatives of the velocities at p node (iz, ix) with radius r,
+// read from the staged tiles: VX rows z0..z0+TZ, columns x0-10..x0+73;
+// VZ rows z0-10..z0+TZ+9, columns x0..x0+64.
+struct DivV { float dxv, dzv; };
+
+__device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float* __restrict__ VZc, int r,
+                                      const float* c10, const float* __restrict__ cf,
+                                      float dtx, float dtz, float rho, float vv)
+{
+  float sx = 0.0f, sz = 0.0f;
+  if (r == RTM_R) {
+#pragma unroll
+    for (int i = 1; i <= RTM_R; i++) {
+      sx = __builtin_fmaf(c10[i - 1], VXr[i - 1] - VXr[-i], sx);
+      sz = __builtin_fmaf(c10[i - 1], VZc[(i - 1) * 64] - VZc[-i * 64], sz);
+    }
+  } else {
+    const float* c = cf + (r - 1) * RTM_R;
+    for (int i = 1; i <= r; i++) {
+      sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
+      sz = __builtin_fmaf(c[i - 1], VZc[(i - 1) * 64] - VZc[-i * 64], sz);
+    }
+  }
+  DivV d;
+  d.dxv = sx * dtx; d.dxv = d.dxv * rho; d.dxv = d.dxv * vv;     // kernel.cu:536
+  d.dzv = sz * dtz; d.dzv = d.dzv * rho; d.dzv = d.dzv * vv;     // kernel.cu:537
+  return d;
+}
+
+// forward pressure update of one cell; returns the new p.  Interior cells:
+// p -= dxv + dzv (kernel.cu:539).  PML cells: split pair (kernel.cu:780-783).
+__device__ __forceinline__ float pres_cell(float p, float* __restrict__ p1, float* __restrict__ p2, size_t id,
+                                           bool inner, const DivV& d, float c1x, float c2x, float c1z, float c2z)
+{
+  if (inner) return p - (d.dxv + d.dzv);
+  float q1 = p1[id], q2 = p2[id];
+  q1 = __builtin_fmaf(c1x, q1, -d.dxv) / c2x;
+  q2 = __builtin_fmaf(c1z, q2, -d.dzv) / c2z;
+  p1[id] = q1; p2[id] = q2;
+  return q1 + q2;
+}
+
+template <int TZ>
+__device__ __forceinline__ void stage_v(float* __restrict__ VX, float* __restrict__ VZ, const float* __restrict__ vx,
+                                        const float* __restrict__ vz, const ShotDev& s, int x0, int z0)
+{
+  stage<LW, TZ>(VX, vx, s.pitch, s.nzp, s.nxp - 1, z0, x0 - 10);
+  stage<64, TZ + 19>(VZ, vz, s.pitch, s.nzp - 1, s.nxp, z0 - 10, x0);
+}
+
+// forward pressure phase of the source wavefield: FD update, source
+// injection (kernel.cu:36-47), recording (kernel.cu:48-71), p strips.
+template <int TZ>
+__device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const float* __restrict__ VZ,
+                                             const ShotDev& s, int x0, int z0, int it, int save)
+{
+  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  const int ix = x0 + lx;
+  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
+  if (ix >= nxp) return;
+  float c10[RTM_R];
+#pragma unroll
+  for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
+  const float c1x = s.c1x[ix], c2x = s.c2x[ix];
+  const bool xin = ix >= npml && ix < nxp - npml;
+  const bool xring = ix == 0 || ix == nxp - 1;
+  const float wv = s.wav[it];
+  const int sw = s.sx1 - s.sx0 + 1;
+  for (int j = lz; j < TZ; j += 4) {
+    const int iz = z0 + j;
+    if (iz >= nzp) break;
+    const size_t id = (size_t)iz * P + ix;
+    float p = s.sp[id];
+    if (!xring && iz > 0 && iz < nzp - 1) {
+      const int r = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
+      const DivV d = div_v(VX + j * LW + lx + 10, VZ + (j + 10) * 64 + lx, r, c10, s.coef, s.dtx, s.dtz,
+                           model_at(s.rho, s, iz, ix), model_at(s.vv, s, iz, ix));
+      const bool inner = xin && iz >= npml && iz < nzp - npml;
+      p = pres_cell(p, s.sp1, s.sp2, id, inner, d, c1x, c2x, s.c1z[iz], s.c2z[iz]);
+    }
+    if (iz >= s.sz0 && iz <= s.sz1 && ix >= s.sx0 && ix <= s.sx1)
+      p = __builtin_fmaf(s.sa[(iz - s.sz0) * sw + ix - s.sx0], wv, p);
+    s.sp[id] = p;
+    if (s.record && iz == npml) {
+      const int ir = ix - s.rcol0;
+      if (ir >= 0 && ir < s.nrec) s.rec[(size_t)it * s.nrec + ir] = p;
+    }
+    if (save) {
+      const long long o = strip_p(s, iz, ix);
+      if (o >= 0) store_put(s, it, o, p);
+    }
+  }
+}
+
+// reverse pressure phase: source reconstruction (kernel.cu:389-429, strip
+// restore kernel.cu:1348-1400, source removal rtm_model.cu:437), receiver
+// forward step + Dirichlet insertion (kernel.cu:72-95) and imaging
+// (kernel.cu:1401-1416).
+template <int TZ>
+__device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const float* __restrict__ SZ,
+                                         const float* __restrict__ RX, const float* __restrict__ RZ,
+                                         const ShotDev& s, int x0, int z0, int it)
+{
+  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  const int ix = x0 + lx;
+  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
+  if (ix >= nxp) return;
+  float c10[RTM_R];
+#pragma unroll
+  for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
+  const float c1x = s.c1x[ix], c2x = s.c2x[ix];
+  const bool xin = ix >= npml && ix < nxp - npml;
+  const bool xring = ix == 0 || ix == nxp - 1;
+  const float wv = s.wav[it];
+  const int sw = s.sx1 - s.sx0 + 1;
+  const bool xsrc = ix >= s.sx0 && ix <= s.sx1;
+  for (int j = lz; j < TZ; j += 4) {
+    const int iz = z0 + j;
+    if (iz >= nzp) break;
+    const size_t id = (size_t)iz * P + ix;
+    const bool inner = xin && iz >= npml && iz < nzp - npml;
+    const float rho = model_at(s.rho, s, iz, ix);
+    const float vv = model_at(s.vv, s, iz, ix);
+    // source wavefield (interior + strips only; the rest is never read)
+    float ps = 0.0f;
+    {
+      const long long o = strip_p(s, iz, ix);
+      bool wr = false;
+      if (o >= 0) {
+        ps = store_get(s, it, o); wr = true;
+      } else if (inner) {
+        const DivV d = div_v(SX + j * LW + lx + 10, SZ + (j + 10) * 64 + lx, RTM_R, c10, s.coef, s.dtx, s.dtz, rho, vv);
+        ps = s.sp[id] + (d.dxv + d.dzv);
+        wr = true;
+      }
+      if (wr) {
+        if (xsrc && iz >= s.sz0 && iz <= s.sz1) ps = __builtin_fmaf(-s.sa[(iz - s.sz0) * sw + ix - s.sx0], wv, ps);
+        s.sp[id] = ps;
+      }
+    }
+    // receiver wavefield
+    float p = s.rp[id];
+    if (!xring && iz > 0 && iz < nzp - 1) {
+      const int r = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
+      const DivV d = div_v(RX + j * LW + lx + 10, RZ + (j + 10) * 64 + lx, r, c10, s.coef, s.dtx, s.dtz, rho, vv);
+      p = pres_cell(p, s.rp1, s.rp2, id, inner, d, c1x, c2x, s.c1z[iz], s.c2z[iz]);
+    }
+    if (iz == npml) {
+      const int ir = ix - s.rcol0;
+      if (ir >= 0 && ir < s.nrec) {
+        const float rv = (it <= s.cut[ir]) ? 0.0f : s.rec[(size_t)it * s.nrec + ir];
+        p = s.dt * rv;
+      }
+    }
+    s.rp[id] = p;
+    if (inner) {
+      const size_t ii = (size_t)(iz - npml) * s.nxl + ix - npml;
+      s.g1[ii] = __builtin_fmaf(ps, p, s.g1[ii]);
+      s.illum[ii] = __builtin_fmaf(ps, ps, s.illum[ii]);
+    }
+  }
+}
+
+}  // namespace
+
+// ================================================================ kernels ==
+template <int TZ>
+__global__ __launch_bounds__(NTHR) void k_vel_fwd(LaunchArgs a, int save)
+{
+  __shared__ float L[(TZ + 19) * LW];
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const Tile tl = a.tiles[t];
+  const ShotDev s = a.shots[tl.slot];
+  const int it = s.it_off + s.it_dir * a.step;
+  stage<LW, TZ + 19>(L, s.sp, s.pitch, s.nzp, s.nxp, tl.z0 - 9, tl.x0 - 9);
+  __syncthreads();
+  vel_fwd<TZ>(L, s.svx, s.svz, s, tl.x0, tl.z0, save ? it : -1);
+}
+
+template <int TZ>
+__global__ __launch_bounds__(NTHR) void k_pres_fwd(LaunchArgs a, int save)
+{
+  __shared__ float VX[TZ * LW];
+  __shared__ float VZ[(TZ + 19) * 64];
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const Tile tl = a.tiles[t];
+  const ShotDev s = a.shots[tl.slot];
+  const int it = s.it_off + s.it_dir * a.step;
+  stage_v<TZ>(VX, VZ, s.svx, s.svz, s, tl.x0, tl.z0);
+  __syncthreads();
+  pres_fwd_src<TZ>(VX, VZ, s, tl.x0, tl.z0, it, save);
+}
+
+template <int TZ>
+__global__ __launch_bounds__(NTHR) void k_vel_rev(LaunchArgs a)
+{
+  __shared__ float LS[(TZ + 19) * LW];
+  __shared__ float LR[(TZ + 19) * LW];
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const Tile tl = a.tiles[t];
+  const ShotDev s = a.shots[tl.slot];
+  const int it = s.it_off + s.it_dir * a.step;
+  stage<LW, TZ + 19>(LS, s.sp, s.pitch, s.nzp, s.nxp, tl.z0 - 9, tl.x0 - 9);
+  stage<LW, TZ + 19>(LR, s.rp, s.pitch, s.nzp, s.nxp, tl.z0 - 9, tl.x0 - 9);
+  __syncthreads();
+  vel_rev<TZ>(LS, s.svx, s.svz, s, tl.x0, tl.z0, it);
+  vel_fwd<TZ>(LR, s.rvx, s.rvz, s, tl.x0, tl.z0, -1);
+}
+
+template <int TZ>
+__global__ __launch_bounds__(NTHR) void k_pres_rev(LaunchArgs a)
+{
+  __shared__ float SX[TZ * LW];
+  __shared__ float SZ[(TZ + 19) * 64];
+  __shared__ float RX[TZ * LW];
+  __shared__ float RZ[(TZ + 19) * 64];
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const Tile tl = a.tiles[t];
+  const ShotDev s = a.shots[tl.slot];
+  const int it = s.it_off + s.it_dir * a.step;
+  stage_v<TZ>(SX, SZ, s.svx, s.svz, s, tl.x0, tl.z0);
+  stage_v<TZ>(RX, RZ, s.rvx, s.rvz, s, tl.x0, tl.z0);
+  __syncthreads();
+  pres_rev<TZ>(SX, SZ, RX, RZ, s, tl.x0, tl.z0, it);
+}
+
+// 16-bit boundary store: quantise the fp32 staging of step `it` of each slot
+// with the max-abs of its block: q = x / scale, scale = maxabs / 32768 (fp16)
+// or 1 (bf16 keeps the fp32 exponent range; scale kept for uniformity).
+__global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                 int nslots, int step)
+{
+  for (int k = 0; k < nslots; k++) {
+    const ShotDev s = shots[slots[k]];
+    const int it = s.it_off + s.it_dir * step;
+    const float* stage = (const float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
+    float sc[4];
+    for (int b = 0; b < 4; b++) {
+      const float m = __uint_as_float(s.smax[(long long)it * 4 + b]);
+      float v = 1.0f;
+      if (s.store_fmt == ST_F16) v = m > 0.0f ? m / 32768.0f : 1.0f;
+      sc[b] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) s.sscale[(long long)it * 4 + threadIdx.x] = sc[threadIdx.x];
+    const long long n = s.strip_len;
+    const long long a = 2LL * RTM_R * s.nzi, b = a + 2LL * RTM_R * s.nxl, c = b + 2LL * RTM_R * s.nzi;
+    for (long long e = (long long)blockIdx.x * NTHR + threadIdx.x; e < n; e += (long long)gridDim.x * NTHR) {
+      const int blk = e < a ? 0 : (e < b ? 1 : (e < c ? 2 : 3));
+      const float q = stage[e] / sc[blk];
+      const long long k2 = (long long)it * n + e;
+      if (s.store_fmt == ST_F16) {
+        ((__half*)s.store)[k2] = __float2half_rn(q);
+      } else {
+        // bf16, round to nearest even
+        unsigned int u = __float_as_uint(q);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        ((unsigned short*)s.store)[k2] = (unsigned short)(u >> 16);
+      }
+    }
+  }
+}
+
+// zero the wavefields of a slot for the phase that starts (FWD: source set;
+// REV: source velocities, receiver set and imaging accumulators), and seed
+// the receiver wavefield with the last trace sample (rtm_model.cu:366-410).
+__global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__ shots, int slot, int phase)
+{
+  const ShotDev s = shots[slot];
+  const long long n = (long long)s.nzp * s.pitch;
+  const long long ni = (long long)s.nzi * s.nxl;
+  const long long g0 = (long long)blockIdx.x * NTHR + threadIdx.x, gs = (long long)gridDim.x * NTHR;
+  if (phase == PH_FWD) {
+    for (long long e = g0; e < n; e += gs) { s.sp[e] = 0.f; s.svx[e] = 0.f; s.svz[e] = 0.f; s.sp1[e] = 0.f; s.sp2[e] = 0.f; }
+    if (s.store_fmt != ST_F32)
+      for (long long e = g0; e < 4LL * s.nt; e += gs) s.smax[e] = 0u;
+  } else {
+    for (long long e = g0; e < n; e += gs) {
+      s.svx[e] = 0.f; s.svz[e] = 0.f;
+      s.rp[e] = 0.f; s.rvx[e] = 0.f; s.rvz[e] = 0.f; s.rp1[e] = 0.f; s.rp2[e] = 0.f;
+    }
+    for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
+  }
+}
+
+__global__ __launch_bounds__(NTHR) void k_seed_receivers(const ShotDev* __restrict__ shots, int slot)
+{
+  const ShotDev s = shots[slot];
+  const int ir = blockIdx.x * NTHR + threadIdx.x;
+  if (ir >= s.nrec) return;
+  const int it = s.nt - 1;
+  const float rv = (it <= s.cut[ir]) ? 0.0f : s.rec[(size_t)it * s.nrec + ir];
+  s.rp[(size_t)s.npml * s.pitch + s.rcol0 + ir] = s.dt * rv;
+}
+
+// taper and stack the shots of a batch into the FP64 images, one image cell
+// per thread, shots in the given order (kernel.cu:1556-1615, rtm_model.cu:462-488)
+struct StackShot { const float* g1; const float* illum; const float* taper; int nx1, nxl; };
+
+__global__ __launch_bounds__(NTHR) void k_stack(const StackShot* __restrict__ sh, int nsh, int nx, int nz,
+                                                 double* __restrict__ img1, double* __restrict__ img2s,
+                                                 double* __restrict__ ill)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e >= (long long)nx * nz) return;
+  const int iz = (int)(e / nx), ix = (int)(e - (long long)iz * nx);
+  double a1 = img1[e], a2 = img2s[e], a3 = ill[e];
+  for (int k = 0; k < nsh; k++) {
+    const StackShot q = sh[k];
+    const int c = ix - q.nx1;
+    if (c < 0 || c >= q.nxl) continue;
+    const float t = q.taper[c];
+    const float g = q.g1[(size_t)iz * q.nxl + c] * t;
+    const float il = q.illum[(size_t)iz * q.nxl + c] * t;
+    a2 += (double)g / ((double)il + 1.0e-6);
+    a1 += (double)g;
+    a3 += (double)il;
+  }
+  img1[e] = a1; img2s[e] = a2; ill[e] = a3;
+}
+
+// final products from the FP64 stacks: fp32 images, the Laplacian-filtered
+// image (laplacefd1d, image_function.cpp:44-79, int spacings) and the
+// multi-shot normalised image (kernel.cu:1580-1590).
+__global__ __launch_bounds__(NTHR) void k_round(const double* __restrict__ a, float* __restrict__ o, long long n)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e < n) o[e] = (float)a[e];
+}
+
+__global__ __launch_bounds__(NTHR) void k_laplace(const float* __restrict__ im, float* __restrict__ out, int nx, int nz,
+                                                   int dxi, int dzi)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e >= (long long)nx * nz) return;
+  const int iz = (int)(e / nx), ix = (int)(e - (long long)iz * nx);
+  if (ix == 0 || iz == 0 || ix == nx - 1 || iz == nz - 1) { out[e] = 0.0f; return; }
+  const float c = im[e];
+  const float t1 = (float)(((double)(im[e + 1] + im[e - 1]) - 2.0 * (double)c) / (double)(dxi * dxi));
+  const float t2 = (float)(((double)(im[e + nx] + im[e - nx]) - 2.0 * (double)c) / (double)(dzi * dzi));
+  out[e] = t1 + t2;
+}
+
+__global__ __launch_bounds__(NTHR) void k_norm_multi(const float* __restrict__ sg, const float* __restrict__ ss,
+                                                      float* __restrict__ out, long long n)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e < n) out[e] = (float)((double)sg[e] / ((double)ss[e] + 1.0e-6));
+}
+
+__global__ __launch_bounds__(NTHR) void k_zero_f64(double* __restrict__ a, long long n)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e < n) a[e] = 0.0;
+}
+
+// advance the time origin of the slots of a captured chunk of `k` steps so
+// that replaying the same graph continues where the last replay stopped
+__global__ void k_bump(ShotDev* __restrict__ shots, const int* __restrict__ slots, int nslots, int k)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) {
+    ShotDev* s = &shots[slots[i]];
+    s->it_off += s->it_dir * k;
+  }
+}
+
+// =========================================================== launchers ==
+// (host side, same translation unit so the templates are instantiated here)
+#include "rtm_launch.h"
+
+static inline int grid8(int ntiles) { return ((ntiles + 7) / 8) * 8; }
+
+template <int TZ>
+static hipError_t launch_step_t(const LaunchArgs& a, int phase, int save, hipStream_t st)
+{
+  const dim3 g(grid8(a.ntiles)), b(NTHR);
+  if (phase == PH_FWD) {
+    hipLaunchKernelGGL(k_vel_fwd<TZ>, g, b, 0, st, a, save);
+    hipLaunchKernelGGL(k_pres_fwd<TZ>, g, b, 0, st, a, save);
+  } else {
+    hipLaunchKernelGGL(k_vel_rev<TZ>, g, b, 0, st, a);
+    hipLaunchKernelGGL(k_pres_rev<TZ>, g, b, 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+template <int TZ>
+static hipError_t launch_phase_t(const LaunchArgs& a, int phase, int save, int which, hipStream_t st)
+{
+  const dim3 g(grid8(a.ntiles)), b(NTHR);
+  if (phase == PH_FWD) {
+    if (which == 0) hipLaunchKernelGGL(k_vel_fwd<TZ>, g, b, 0, st, a, save);
+    else hipLaunchKernelGGL(k_pres_fwd<TZ>, g, b, 0, st, a, save);
+  } else {
+    if (which == 0) hipLaunchKernelGGL(k_vel_rev<TZ>, g, b, 0, st, a);
+    else hipLaunchKernelGGL(k_pres_rev<TZ>, g, b, 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_phase(const LaunchArgs& a, int phase, int tz, int save, int which, hipStream_t st)
+{
+  switch (tz) {
+    case 8: return launch_phase_t<8>(a, phase, save, which, st);
+    case 16: return launch_phase_t<16>(a, phase, save, which, st);
+    case 32: return launch_phase_t<32>(a, phase, save, which, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st)
+{
+  switch (tz) {
+    case 8: return launch_step_t<8>(a, phase, save, st);
+    case 16: return launch_step_t<16>(a, phase, save, st);
+    case 32: return launch_step_t<32>(a, phase, save, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_quant, dim3(nblocks), dim3(NTHR), 0, st, shots, slots, nslots, step);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_phase_init, dim3(nblocks), dim3(NTHR), 0, st, shots, slot, phase);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st)
+{
+  if (nrec <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seed_receivers, dim3((nrec + NTHR - 1) / NTHR), dim3(NTHR), 0, st, shots, slot);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_stack(const void* sh, int nsh, int nx, int nz, double* i1, double* i2, double* il, hipStream_t st)
+{
+  const long long n = (long long)nx * nz;
+  hipLaunchKernelGGL(k_stack, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st,
+                     (const StackShot*)sh, nsh, nx, nz, i1, i2, il);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_round(const double* a, float* o, long long n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_round, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, a, o, n);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_laplace(const float* im, float* out, int nx, int nz, int dxi, int dzi, hipStream_t st)
+{
+  const long long n = (long long)nx * nz;
+  hipLaunchKernelGGL(k_laplace, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, im, out, nx, nz, dxi, dzi);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, long long n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_norm_multi, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, sg, ss, out, n);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, a, n);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_bump(ShotDev* shots, const int* slots, int nslots, int k, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_bump, dim3((nslots + 63) / 64), dim3(64), 0, st, shots, slots, nslots, k);
+  return hipGetLastError();
+}

@@ -1,0 +1,24 @@
+// rtm_launch.h -- host-side launchers of the kernels in rtm_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rtm_dev.h"
+
+// one time step (velocity + pressure phase) of the tiles in `a`; phase is
+// PH_FWD or PH_REV, tz the tile height (8, 16 or 32), save=1 writes the
+// boundary strips of the forward pass.
+hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st);
+// the two phases of rtm_launch_step separately (which: 0 velocity, 1 pressure)
+hipError_t rtm_launch_phase(const LaunchArgs& a, int phase, int tz, int save, int which, hipStream_t st);
+hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, int nblocks, hipStream_t st);
+hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st);
+hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st);
+hipError_t rtm_launch_stack(const void* stack_shots, int nsh, int nx, int nz, double* i1, double* i2, double* il,
+                            hipStream_t st);
+hipError_t rtm_launch_round(const double* a, float* o, long long n, hipStream_t st);
+hipError_t rtm_launch_laplace(const float* im, float* out, int nx, int nz, int dxi, int dzi, hipStream_t st);
+hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, long long n, hipStream_t st);
+hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st);
+hipError_t rtm_launch_bump(ShotDev* shots, const int* slots, int nslots, int k, hipStream_t st);
+
+// per-shot entry of the stacking kernel (k_stack)
+struct StackShotHost { const float* g1; const float* illum; const float* taper; int nx1, nxl; };

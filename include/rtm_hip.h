@@ -1,0 +1,194 @@
+/*
+ * rtm_hip.h -- C ABI of librtm_hip.so, the MI355X (gfx950) implementation of
+ * the 2-D acoustic prestack RTM path of gubingluo.
+ *
+ * Two layers:
+ *   1. the drop-in entry points the reference's driver binds (same symbol,
+ *      same argument list, same meaning);
+ *   2. a native API (prefix rtmh_) with status-code returns, used by the
+ *      batched / reduced-precision / served forms, the tests and the bench.
+ * Plain C types only: host pointers and sizes; device memory never crosses
+ * this boundary.
+ */
+#ifndef RTM_HIP_H
+#define RTM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* 1. drop-in                                                          */
+/* ------------------------------------------------------------------ */
+
+/* Replaces rtm_model() of rtm_model.cu:12-17, declared by the caller at
+ * RTM_GPU_velstr.cpp:23-28 and called at RTM_GPU_velstr.cpp:229-232.
+ * Shots ishot = startshot, startshot+dshot, ... <= nshot (nshot is the last
+ * index); vp0/rho0 host [nz*nx] (m/s, kg/m^3); diffcoef host [100];
+ * wavelet ignored (the Ricker is regenerated, as the reference does on the
+ * GPU).  Outputs are host [nz*nx] arrays, overwritten: image1 (cross-
+ * correlation stack), image2s (per-shot illumination-normalised stack),
+ * image2m (= image1, the reference's normalisation calls are commented out),
+ * illum (illumination stack).  image3s/4s/5s/3m/4m/5m (decomposed, Poynting
+ * and energy-norm conditions) are outside this implementation and are
+ * returned as zeros.  File side effects as the reference: mode 0 writes
+ * ./seisReal/<ishot>seisreal.su, mode 1 reads it; ./output/Image*temp.su
+ * every 50 shots; ./output/<idevice>Imagetrue.su.  On failure prints the
+ * error and exits with status 1 (the reference exits 0 from check_gpu_error,
+ * kernel.cu:1616-1624). */
+void rtm_model(int idevice, int nshot, int startshot, int dshot, int mode, int medium_flag, int flag_layout,
+               int sgyflag, int endianflag, int ismth, int nx, int nz, int nt, float dx, float dz, float dt,
+               int npml, float direct, float direct0, float tsnap, float fdom, float spx0, float spz0, float dspx,
+               float offset, float offsetmax, float amp, float alp, float *vp0, float *rho0, float *diffcoef,
+               float *wavelet, float *image1, float *image2s, float *image3s, float *image4s, float *image5s,
+               float *image2m, float *image3m, float *image4m, float *image5m, float *illum);
+
+/* Link-compatible stand-in for rtm_real() (rtm_real.cu:17-22; the caller's
+ * prototype at RTM_GPU_velstr.cpp:30-35 differs in three positions).  The
+ * field-data path is not implemented: prints a message and exits with
+ * status 2. */
+void rtm_real(int idevice, int nshot, int startshot, int dshot, int medium_flag, int flag_layout, int sgyflag,
+              int endianflag, int ismth, int nx, int nz, int nt, float dx, float dz, float dt, int nx0, int nz0,
+              int nt0, float dx0, float dz0, float dt0, int npml, int lx, int lz, float tsnap, float fdom,
+              float spz0, float amp, float alp, int nw, int tlength, float *vp, float *rho, float *diffcoef,
+              float *wavelet, char filename_shotgather[40], int **shotprofileId, float *image1, float *image2s,
+              float *image3s, float *image4s, float *image5s, float *image2m, float *image3m, float *image4m,
+              float *image5m, float *illum);
+
+/* ------------------------------------------------------------------ */
+/* 2. native API                                                       */
+/* ------------------------------------------------------------------ */
+
+/* The scalar arguments of rtm_model(), same order and meaning. */
+typedef struct rtmh_params {
+  int nshot, startshot, dshot, mode, medium_flag, flag_layout, sgyflag, endianflag, ismth;
+  int nx, nz, nt;
+  float dx, dz, dt;
+  int npml;
+  float direct, direct0, tsnap, fdom, spx0, spz0, dspx, offset, offsetmax, amp, alp;
+} rtmh_params;
+
+enum { RTMH_STORE_F32 = 0, RTMH_STORE_F16 = 1, RTMH_STORE_BF16 = 2 };
+
+typedef struct rtmh_options {
+  int batch;          /* shots per launch; 0 = as many as fit the memory budget  */
+  int store_fmt;      /* boundary store: RTMH_STORE_F32 / _F16 / _BF16           */
+  int graph_chunk;    /* time steps per captured hipGraph; 0 = plain launches    */
+  int tile_z;         /* tile height 8/16/32; 0 = autotuned                      */
+  int write_files;    /* reproduce the reference's file side effects             */
+  int verbose;        /* 0 silent, 1 per-shot lines as the reference prints       */
+  double mem_budget;  /* bytes of HBM for shot slots; 0 = 80% of free memory      */
+  int profile;        /* 1: plain launches timed one by one with HIP events       */
+} rtmh_options;
+
+/* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */
+typedef struct rtmh_outputs {
+  float *image1, *image2s, *illum;
+  float *image_lap;         /* laplacefd1d(image1), image_function.cpp:44-79      */
+  float *image_norm;        /* image1/(illum+1e-6), kernel.cu:1580-1590           */
+  float **shot_seis;        /* per shot (job order): [nt*nrec] unmuted record     */
+  float **shot_g1;          /* per shot: [nz*nxl] cross-correlation, untapered    */
+  float **shot_illum;       /* per shot: [nz*nxl] illumination, untapered         */
+  const float **shot_seis_in; /* per shot: [nt*nrec] data to migrate (mode 1)     */
+} rtmh_outputs;
+
+typedef struct rtmh_stats {
+  double total_ms;          /* whole job incl. setup and copies                   */
+  double gpu_ms;            /* propagation passes (forward + reverse), HIP events */
+  double fwd_ms, rev_ms;
+  double cell_updates;      /* 2*nt*nxp*nzp + (nt-1)*nz*nxl summed over shots     */
+  int nshots, batches, tile_z, batch;
+  double hbm_bytes_alg;     /* algorithmic bytes of the propagation passes        */
+  /* profile=1 only: per kernel (0 vel_fwd, 1 pres_fwd, 2 vel_rev, 3 pres_rev)
+   * summed launch time and launch count */
+  double kern_ms[4];
+  int kern_calls[4];
+  double step_bytes_fwd;    /* algorithmic bytes of one forward step of the batch */
+  double step_bytes_rev;    /* ... of one reverse step                            */
+} rtmh_stats;
+
+/* last error message of this thread ("" if none) */
+const char *rtmh_last_error(void);
+/* library version string */
+const char *rtmh_version(void);
+/* number of HIP devices (0 without a GPU); does not initialise a context */
+int rtmh_device_count(void);
+
+typedef struct rtmh_ctx rtmh_ctx;
+int rtmh_create(int device, rtmh_ctx **out);
+int rtmh_destroy(rtmh_ctx *ctx);
+
+/* per-shot geometry of a job (window, padded dims, receivers); -1 if the
+ * shot lies outside the model */
+typedef struct rtmh_geom {
+  int ishot, nx1, nx2, nxl, nxp, nzp, nsx, nsz, nrec, rcol0, noffset;
+} rtmh_geom;
+int rtmh_shot_geometry(const rtmh_params *p, int ishot, rtmh_geom *g);
+
+/* Whole job as rtm_model() does it, with options; returns 0 or a negative
+ * status (message in rtmh_last_error()). */
+int rtmh_run(rtmh_ctx *ctx, const rtmh_params *p, const float *vp0, const float *rho0, const float *diffcoef,
+             const rtmh_options *opt, rtmh_outputs *out, rtmh_stats *stats);
+
+/* nsteps forward FD steps of one shot's source wavefield from given fields
+ * (unit-level parity of the velocity-stress + PML update).  All field arrays
+ * are host [nzp*nxp] (vx uses columns 0..nxp-2, vz rows 0..nzp-2; p1/p2 hold
+ * the split pressure pair at PML cells).  it0: wavelet index of the first
+ * step; with_source adds the source term as the reference does. */
+int rtmh_fd_steps(rtmh_ctx *ctx, const rtmh_params *p, int ishot, const float *vp0, const float *rho0,
+                  const float *diffcoef, int nsteps, int it0, int with_source, float *pf, float *vxf, float *vzf,
+                  float *p1f, float *p2f);
+
+
+/* ------------------------------------------------------------------ */
+/* 3. drop-in driver helpers (the host side of RTM_GPU_velstr.cpp)     */
+/* ------------------------------------------------------------------ */
+
+/* Liu-Yang staggered coefficient table, out[100], row r-1 = order 2r
+ * (FD_function.cpp:73-165 assembled as RTM_GPU_velstr.cpp:184-196). */
+void rtmh_coef_table(float *out);
+
+/* Parameter file of the driver (RTM_GPU_velstr.cpp:60-120). */
+typedef struct rtmh_parfile {
+  float dt0; int nt0; float tsnap;
+  float xmax, zmax, dx0, dz0;
+  float fdom, amp, alp;
+  float spx0, spz0, dspx;
+  float offset, offsetmax;
+  int startshot, nshot, dshot, nshotall;
+  int npml, medium_flag, ismth;
+  float direct, direct0;
+  int idevice, flag_layout, mode, dataflag, sgyflag, endianflag, rhoflag;
+  char vp_file[256], shotgather[256], rho_file[256];
+} rtmh_parfile;
+/* 0 ok, -1 missing file, -2 malformed */
+int rtmh_read_parfile(const char *path, rtmh_parfile *pf);
+
+/* Grid after the dispersion / stability guard get_constant()
+ * (RTM_GPU_velstr.cpp:319-362). */
+typedef struct rtmh_grid {
+  float dx, dz, dt; int nx, nz, nt, lx, lz; float vpmin, vpmax;
+} rtmh_grid;
+void rtmh_get_constant(const float *vp, float fdom, float dx0, float dz0, int nx0, int nz0, int nt0, float dt0,
+                       rtmh_grid *g);
+
+/* interpvel1 (Velocity_function.cpp:1946-1977). */
+void rtmh_interp_model(const float *vp0, float *vp, int nx0, int nz0, int nx, int nz, int lx, int lz);
+
+/* selfdoc banner (RTM_GPU_velstr.cpp:155-182) on stdout. */
+void rtmh_selfdoc(const rtmh_parfile *pf, int nx0, int nz0, const rtmh_grid *g);
+
+/* Laplacian-filtered image (laplacefd1d, image_function.cpp:44-79, integer
+ * spacings as its int arguments) and multi-shot normalised image
+ * (cuda_applyimagingconditionmulti, kernel.cu:1580-1590) on the GPU. */
+int rtmh_postprocess(int device, const float *image1, const float *illum, int nx, int nz, float dx, float dz,
+                     float *image_lap, float *image_norm);
+
+/* main() of RTM_GPU_velstr.cpp:37-317: reads ./input/cmd.in, runs, writes
+ * ./output/<dev>*.su (plus <dev>Image1lap.su and <dev>Imagenorm.su). */
+int rtmh_driver_main(int argc, char **argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTM_HIP_H */

@@ -1,0 +1,305 @@
+/*
+ * rtm_oracle.c -- CPU oracle for the 2-D acoustic prestack RTM path of the
+ * reference (gubingluo: RTM_GPU_velstr.cpp -> rtm_model.cu -> kernel.cu).
+ *
+ * *** TEST INFRASTRUCTURE ONLY ***
+ * This library is the checker.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load or call it, and only to compute the
+ * expected answer or the CPU baseline time.  The product library
+ * (gubingluo_amd) never links, loads or falls back to it.
+ *
+ * What is restated (file:line of the reference):
+ *   shot window + padded source position .... rtm_model.cu:122-153 (layout 1),
+ *                                             :626-652 (2), :1125-1155 (3), :1632-1660 (4)
+ *   window extraction, v^2, edge padding .... Velocity_function.cpp:2011-2073
+ *   per-shot vmax, PML profiles ............. rtm_model.cu:207, kernel.cu:174-269
+ *   Ricker wavelet, Gaussian source ......... kernel.cu:13-35
+ *   FD order maps ........................... kernel.cu:271-317
+ *   velocity / pressure updates + PML ....... kernel.cu:430-1185
+ *   record / insert / mute .................. kernel.cu:48-173
+ *   boundary strips save / restore .......... kernel.cu:1186-1400
+ *   reverse reconstruction .................. kernel.cu:318-429, rtm_model.cu:413-437
+ *   cross-correlation + illumination ........ kernel.cu:1401-1416
+ *   taper, stacking ......................... kernel.cu:1556-1615, rtm_model.cu:462-488
+ *   Laplacian filter ........................ image_function.cpp:44-79
+ *   multi-shot normalisation ................ kernel.cu:1580-1590, image_function.cpp:11-20
+ *
+ * Two pressure/PML forms:
+ *   literal      -- the kernels' own thread ranges (valid only for npml == 32,
+ *                   where the 32-wide PML blocks tile the absorbing layer);
+ *   generalised  -- every cell of the padded grid updated by the same
+ *                   per-cell formulas, for any npml >= 10.  Identical to the
+ *                   literal form at npml == 32 (checked by the tests).
+ *
+ * Pinning of each function is listed in DESIGN.md (oracle section); the
+ * checks live in tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+#define ORC_PI 3.1415926535897932   /* common.h:11 */
+#define R10 10                      /* N/2, common.h:57 */
+
+static inline int orc_imin(int a, int b) { return a < b ? a : b; }
+static inline int orc_imax(int a, int b) { return a > b ? a : b; }
+
+/* the scalar arguments of rtm_model() (rtm_model.cu:12-17) */
+typedef struct {
+    int nshot, startshot, dshot, mode, medium_flag, flag_layout, sgyflag, endianflag, ismth;
+    int nx, nz, nt;
+    float dx, dz, dt;
+    int npml;
+    float direct, direct0, tsnap, fdom, spx0, spz0, dspx, offset, offsetmax, amp, alp;
+} orc_params;
+
+/* per-shot geometry */
+typedef struct {
+    int ishot, nx1, nx2, nxl, nxp, nzp, nsx, nsz;
+    int nrec;      /* receivers recorded / inserted                     */
+    int rcol0;     /* padded column of receiver 0                       */
+    int noffset;   /* layouts 3/4                                       */
+    /* SU key header of the shot (rtm_model.cu:137-145) */
+    int hk_itraceall, hk_offset, hk_sx, hk_gx, hk_ntrace, hk_nt, hk_dt;
+    float hk_dx;
+} orc_geom;
+
+typedef struct {
+    int literal;          /* 1: literal kernel ranges (npml must be 32)        */
+    int forward_only;     /* stop after the forward propagation                */
+    int nt_limit;         /* >0: only this many forward steps (timing samples) */
+    int snap_it;          /* step whose wavefields are returned (-1: none)     */
+    const float *vvpad;   /* optional padded v^2 window (nzp*nxp)              */
+    const float *rhopad;  /* optional padded density window                    */
+} orc_shot_opts;
+
+/* ------------------------------------------------------------------ */
+ORC_EXPORT int orc_shot_geometry(const orc_params *P, int ishot, orc_geom *g)
+{
+    memset(g, 0, sizeof(*g));
+    const float dx = P->dx, dz = P->dz;
+    const int nx = P->nx, npml = P->npml;
+    int nx1, nx2, is = 0, noffset = 0;
+    float spx, spz;
+    g->ishot = ishot;
+    if (P->flag_layout == 1) {                                   /* rtm_model.cu:124-134 */
+        int NX = (int)(P->offsetmax / dx + 1);
+        int sp1 = (int)(P->spx0 / dx);
+        int np = (int)(P->dspx / dx);
+        int nr = (NX - 1) / 2;
+        is = sp1 + (ishot - 1) * np;
+        nx1 = orc_imax(0, is - nr);
+        nx2 = orc_imin(nx - 1, is + nr);
+        spx = (is - nx1) * dx;
+    } else if (P->flag_layout == 2) {                            /* rtm_model.cu:627-630 */
+        nx1 = 0; nx2 = nx - 1;
+        spx = P->spx0 + (ishot - 1) * P->dspx;
+    } else if (P->flag_layout == 3 || P->flag_layout == 4) {     /* :1126-1138, :1633-1645 */
+        int NX = (int)(P->offsetmax / dx + 1);
+        int sp1 = (int)(P->spx0 / dx);
+        int np = (int)(P->dspx / dx);
+        is = sp1 + (ishot - 1) * np;
+        noffset = (int)(P->offset / dx);
+        int nr = NX - 1;
+        if (P->flag_layout == 3) { nx1 = orc_imax(0, is); nx2 = orc_imin(nx - 1, is + nr); }
+        else { nx1 = orc_imax(0, is - nr); nx2 = orc_imin(nx - 1, is); }
+        spx = (is - nx1) * dx;
+    } else {
+        return -1;
+    }
+    int nxl = nx2 - nx1 + 1;
+    if (nxl <= 0) return -1;
+    spz = P->spz0;
+    g->nx1 = nx1; g->nx2 = nx2; g->nxl = nxl; g->noffset = noffset;
+    /* key header */
+    g->hk_nt = (int)(short)(unsigned short)(P->nt);
+    g->hk_dt = (int)(short)(unsigned short)(P->dt * 1000000);
+    g->hk_dx = dx;
+    if (P->flag_layout <= 2) {
+        g->hk_itraceall = (ishot - 1) * nxl + 1;
+        g->hk_offset = (int)(nx1 * dx - spx);
+        g->hk_sx = (int)(spx);
+        g->hk_gx = (int)(nx1 * dx);
+        g->hk_ntrace = nxl;
+    } else {
+        g->hk_itraceall = (ishot - 1) * (nxl - noffset) + 1;
+        g->hk_offset = (int)(P->offset);
+        g->hk_sx = (int)(spx);
+        g->hk_gx = (int)(P->offset + nx1 * dx);
+        g->hk_ntrace = nxl - noffset;
+    }
+    spx = spx + npml * dx;
+    spz = spz + npml * dz;
+    g->nxp = nxl + 2 * npml;
+    g->nzp = P->nz + 2 * npml;
+    g->nsx = (int)(spx / dx);
+    g->nsz = (int)(spz / dz);
+    /* receivers: record1/insert1 (layouts 1,2), record2 (3), record3 (4) */
+    if (P->flag_layout <= 2) { g->nrec = nxl; g->rcol0 = npml; }
+    else if (P->flag_layout == 3) { g->nrec = nxl - noffset; g->rcol0 = npml + noffset; }
+    else { g->nrec = nxl - noffset; g->rcol0 = npml; }
+    if (g->nrec < 0) g->nrec = 0;
+    return 0;
+}
+
+/* extractvel1/extractrho1 + extendvel1 (Velocity_function.cpp:2011-2073) */
+ORC_EXPORT void orc_window_pad(float *out, const float *in, int nx, int nz, int nx1, int nx2,
+                               int npml, int square)
+{
+    int nxl = nx2 - nx1 + 1, nxp = nxl + 2 * npml, nzp = nz + 2 * npml;
+    for (int iz = 0; iz < nz; iz++)
+        for (int ix = nx1; ix <= nx2; ix++) {
+            float v = in[(size_t)iz * nx + ix];
+            out[(size_t)(iz + npml) * nxp + ix - nx1 + npml] = square ? v * v : v;
+        }
+    for (int iz = npml; iz < nzp - npml; iz++)
+        for (int ix = 0; ix < npml; ix++) {
+            out[(size_t)iz * nxp + ix] = out[(size_t)iz * nxp + npml];
+            out[(size_t)iz * nxp + nxp - 1 - ix] = out[(size_t)iz * nxp + nxp - npml - 1];
+        }
+    for (int ix = 0; ix < nxp; ix++)
+        for (int iz = 0; iz < npml; iz++) {
+            out[(size_t)iz * nxp + ix] = out[(size_t)npml * nxp + ix];
+            out[(size_t)(nzp - 1 - iz) * nxp + ix] = out[(size_t)(nzp - 1 - npml) * nxp + ix];
+        }
+}
+
+/* cuda_mute1/2/3 (kernel.cu:96-173): last muted sample per receiver.
+ * vvpad: padded v^2 window of the shot (NULL: built from vp0). */
+ORC_EXPORT void orc_mute_cuts(const orc_params *P, const orc_geom *g, const float *unused,
+                              const float *vp0, const float *vvpad, int *cut)
+{
+    (void)unused;
+    const int nxp = g->nxp, npml = P->npml, nt = P->nt;
+    float *vv = NULL;
+    if (!vvpad) {
+        vv = (float *)malloc((size_t)nxp * g->nzp * sizeof(float));
+        orc_window_pad(vv, vp0, P->nx, P->nz, g->nx1, g->nx2, npml, 1);
+        vvpad = vv;
+    }
+    float dt = P->dt;
+    float _dt = (float)(1.0 / (double)dt);                      /* rtm_model.cu:75 */
+    float dx2 = P->dx * P->dx, dz2 = P->dz * P->dz;
+    int nw = (int)(P->direct / (P->fdom * dt));                 /* rtm_model.cu:22-23 */
+    int tlength = (int)(P->direct0 / dt);
+    int coff = (P->flag_layout == 3) ? g->noffset : 0;          /* mute2 shifts by noffset */
+    for (int ir = 0; ir < g->nrec; ir++) {
+        int col = ir + npml + coff;
+        float vtemp = (float)(0.482 * (1.0 / (double)sqrtf(vvpad[(size_t)npml * nxp + col]) +
+                                       1.0 / (double)sqrtf(vvpad[(size_t)g->nsz * nxp + g->nsx])));
+        float dist = (col - g->nsx) * (col - g->nsx) * dx2 + (npml - g->nsz) * (npml - g->nsz) * dz2;
+        dist = sqrtf(dist);
+        int temp = tlength < nt ? tlength : nt;
+        int c0 = (int)(dist * vtemp * _dt) + nw;
+        int c = c0 < nt ? c0 : nt;
+        c = c > temp ? c : temp;
+        c = c < nt ? c : nt;
+        cut[ir] = c;
+    }
+    free(vv);
+}
+
+/* cuda_ricker_wavelet (kernel.cu:13-22) in its float/double mixture */
+static float orc_ricker_f32(int it, float fdom, float dt)
+{
+    float temp = (float)(ORC_PI * (double)fdom * fabs((double)(it * dt) - 1.0 / (double)fdom));
+    temp *= temp;
+    return (float)((1.0 - 2.0 * (double)temp) * (double)expf(-temp));
+}
+
+ORC_EXPORT void orc_ricker(float *w, int nt, float fdom, float dt)
+{
+    for (int it = 0; it < nt; it++) w[it] = orc_ricker_f32(it, fdom, dt);
+}
+
+/* cuda_taper_calculate (kernel.cu:1591-1604), alpha = 0.06 (rtm_model.cu:71) */
+static float orc_taper(int ix, int nxl, int nwin)
+{
+    float alpha = 0.06f;
+    float a = -alpha * alpha;
+    if (ix < nwin) { float d = (float)(nwin - ix); a = a * d; a = a * d; return expf(a); }
+    if (ix >= nxl - nwin && ix < nxl) { float d = (float)(ix - nxl + nwin + 1); a = a * d; a = a * d; return expf(a); }
+    return 1.0f;
+}
+
+ORC_EXPORT void orc_taper_row(float *out, int nxl, int nwin)
+{
+    for (int ix = 0; ix < nxl; ix++) out[ix] = orc_taper(ix, nxl, nwin);
+}
+
+/* laplacefd1d (image_function.cpp:44-79); dx, dz are int as in the source */
+ORC_EXPORT void orc_laplacian(float *image, int nx, int nz, int dx, int dz)
+{
+    float *c = (float *)malloc((size_t)nx * nz * sizeof(float));
+    memcpy(c, image, (size_t)nx * nz * sizeof(float));
+    for (int ix = 1; ix < nx - 1; ix++)
+        for (int iz = 1; iz < nz - 1; iz++) {
+            float t1 = (float)(((double)(c[(size_t)iz * nx + ix + 1] + c[(size_t)iz * nx + ix - 1]) -
+                                2.0 * (double)c[(size_t)iz * nx + ix]) / (double)(dx * dx));
+            float t2 = (float)(((double)(c[(size_t)(iz + 1) * nx + ix] + c[(size_t)(iz - 1) * nx + ix]) -
+                                2.0 * (double)c[(size_t)iz * nx + ix]) / (double)(dz * dz));
+            image[(size_t)iz * nx + ix] = t1 + t2;
+        }
+    for (int ix = 0; ix < nx; ix++) { image[ix] = 0.0f; image[(size_t)(nz - 1) * nx + ix] = 0.0f; }
+    for (int iz = 0; iz < nz; iz++) { image[(size_t)iz * nx] = 0.0f; image[(size_t)iz * nx + nx - 1] = 0.0f; }
+    free(c);
+}
+
+/* cuda_applyimagingconditionmulti (kernel.cu:1580-1590) */
+ORC_EXPORT void orc_normalize_multi(float *out, const float *sg, const float *ss, size_t n)
+{
+    for (size_t i = 0; i < n; i++) out[i] = (float)((double)sg[i] / ((double)ss[i] + 1.0e-6));
+}
+
+/* scaleimage1d (image_function.cpp:11-20) */
+ORC_EXPORT void orc_scaleimage(const float *illum, float *grad, size_t n)
+{
+    float m = fabsf(illum[0]);
+    for (size_t i = 0; i < n; i++) m = fabsf(illum[i]) > m ? fabsf(illum[i]) : m;
+    float mi = (float)(0.0001 * (double)m);
+    for (size_t i = 0; i < n; i++) grad[i] /= illum[i] + mi;
+}
+
+/* Liu-Yang optimised staggered coefficients (FD_function.cpp:73-165)
+ * assembled as RTM_GPU_velstr.cpp:184-196: row r-1 holds order 2r. */
+ORC_EXPORT void orc_coef_table(float *out)
+{
+    static const float t[10][10] = {
+        {1.0f},
+        {0.1129042e+1f, -0.4301412e-1f},
+        {0.1185991e+1f, -0.7249965e-1f, 0.6301572e-2f},
+        {0.1217990e+1f, -0.9382142e-1f, 0.1507536e-1f, -0.1700324e-2f},
+        {0.1236607e+1f, -0.1082265e+0f, 0.2343440e-1f, -0.5033546e-2f, 0.6817483e-3f},
+        {0.1247662e+1f, -0.1175538e+0f, 0.2997970e-1f, -0.8719078e-2f, 0.2215897e-2f, -0.3462075e-3f},
+        {0.1254799e+1f, -0.1238928e+0f, 0.3494371e-1f, -0.1208897e-1f, 0.4132531e-2f, -0.1197110e-2f, 0.2122227e-3f},
+        {0.1259312e+1f, -0.1280347e+0f, 0.3841945e-1f, -0.1473229e-1f, 0.5924913e-2f, -0.2248618e-2f, 0.7179226e-3f, -0.1400855e-3f},
+        {0.1262502e+1f, -0.1310244e+0f, 0.4103928e-1f, -0.1686807e-1f, 0.7530520e-2f, -0.3345071e-2f, 0.1380367e-2f, -0.4808410e-3f, 0.1023759e-3f},
+        {0.1264748e+1f, -0.1331606e+0f, 0.4296909e-1f, -0.1851897e-1f, 0.8861071e-2f, -0.4347073e-2f, 0.2076101e-2f, -0.9164925e-3f, 0.3437446e-3f, -0.7874250e-4f},
+    };
+    for (int r = 0; r < 10; r++)
+        for (int i = 0; i < 10; i++) out[r * 10 + i] = t[r][i];
+}
+
+/* ------------------------------------------------------------------ */
+#define RT float
+#define SFX(n) n##_f32
+#define FMA fmaf
+#define ORACLE_IS_F64 0
+#include "rtm_oracle_impl.h"
+#undef RT
+#undef SFX
+#undef FMA
+#undef ORACLE_IS_F64
+
+#define RT double
+#define SFX(n) n##_f64
+#define FMA fma
+#define ORACLE_IS_F64 1
+#include "rtm_oracle_impl.h"
+#undef RT
+#undef SFX
+#undef FMA
+#undef ORACLE_IS_F64
