@@ -22,7 +22,8 @@ STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 # every symbol include/rtm_hip.h declares
 EXPORTED = [
     "rtm_model", "rtm_real", "rtmh_last_error", "rtmh_version", "rtmh_device_count", "rtmh_create",
-    "rtmh_destroy", "rtmh_shot_geometry", "rtmh_run", "rtmh_fd_steps", "rtmh_coef_table", "rtmh_read_parfile",
+    "rtmh_destroy", "rtmh_shot_geometry", "rtmh_run", "rtmh_model_create", "rtmh_model_destroy",
+    "rtmh_run_model", "rtmh_fd_steps", "rtmh_coef_table", "rtmh_read_parfile",
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
 ]
 
@@ -181,14 +182,30 @@ class Context:
         except Exception:
             pass
 
-    def run(self, P: Params, vp0, rho0, coef=None, *, batch=0, store_fmt=STORE_F32, graph_chunk=0, tile_z=0,
-            write_files=False, verbose=False, mem_budget=0.0, want_shots=False, want_post=True,
-            seis_in=None, profile=False) -> RunResult:
-        coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
+    def model_create(self, vp0, rho0) -> int:
+        """Upload a velocity/density model to HBM; returns its id."""
         vp0 = np.ascontiguousarray(vp0, np.float32)
         rho0 = np.ascontiguousarray(rho0, np.float32)
+        assert vp0.shape == rho0.shape and vp0.ndim == 2
+        mid = C.c_int(-1)
+        nz, nx = vp0.shape
+        if lib().rtmh_model_create(self._h, C.c_int(nx), C.c_int(nz), _fp(vp0), _fp(rho0), C.byref(mid)) != 0:
+            raise RuntimeError(f"rtmh_model_create failed: {last_error()}")
+        return int(mid.value)
+
+    def model_destroy(self, mid: int) -> None:
+        lib().rtmh_model_destroy(self._h, C.c_int(mid))
+
+    def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
+            graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
+            want_post=True, seis_in=None, profile=False) -> RunResult:
+        """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model."""
+        coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
         nz, nx = P.nz, P.nx
-        assert vp0.shape == (nz, nx) and rho0.shape == (nz, nx)
+        if model is None:
+            vp0 = np.ascontiguousarray(vp0, np.float32)
+            rho0 = np.ascontiguousarray(rho0, np.float32)
+            assert vp0.shape == (nz, nx) and rho0.shape == (nz, nx)
         img1 = np.zeros((nz, nx), np.float32)
         img2 = np.zeros((nz, nx), np.float32)
         ill = np.zeros((nz, nx), np.float32)
@@ -222,8 +239,12 @@ class Context:
         opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
                       int(profile))
         st = Stats()
-        rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),
-                            C.byref(st))
+        if model is None:
+            rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),
+                                C.byref(st))
+        else:
+            rc = lib().rtmh_run_model(self._h, C.c_int(model), C.byref(P), _fp(coef), C.byref(opt), C.byref(out),
+                                      C.byref(st))
         if rc != 0:
             raise RuntimeError(f"rtmh_run failed ({rc}): {last_error()}")
         return RunResult(img1, img2, ill, lap, nrm, seis, g1s, ils, st)

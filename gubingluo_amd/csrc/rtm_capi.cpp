@@ -65,6 +65,28 @@ extern "C" int rtmh_run(rtmh_ctx* ctx, const rtmh_params* p, const float* vp0, c
   return ctx->eng->run(*p, vp0, rho0, diffcoef, o, out, stats);
 }
 
+extern "C" int rtmh_model_create(rtmh_ctx* ctx, int nx, int nz, const float* vp0, const float* rho0, int* model_id)
+{
+  if (!ctx || !model_id) { rtm::set_error("null argument"); return -1; }
+  return ctx->eng->model_create(nx, nz, vp0, rho0, model_id);
+}
+
+extern "C" int rtmh_model_destroy(rtmh_ctx* ctx, int model_id)
+{
+  if (!ctx) { rtm::set_error("null argument"); return -1; }
+  return ctx->eng->model_destroy(model_id);
+}
+
+extern "C" int rtmh_run_model(rtmh_ctx* ctx, int model_id, const rtmh_params* p, const float* diffcoef,
+                              const rtmh_options* opt, rtmh_outputs* out, rtmh_stats* stats)
+{
+  if (!ctx || !p || !diffcoef) { rtm::set_error("null argument"); return -1; }
+  rtmh_options o;
+  memset(&o, 0, sizeof(o));
+  if (opt) o = *opt;
+  return ctx->eng->run_model(model_id, *p, diffcoef, o, out, stats);
+}
+
 extern "C" int rtmh_fd_steps(rtmh_ctx* ctx, const rtmh_params* p, int ishot, const float* vp0, const float* rho0,
                              const float* diffcoef, int nsteps, int it0, int with_source, float* pf, float* vxf,
                              float* vzf, float* p1f, float* p2f)

@@ -269,6 +269,9 @@ Engine::Engine(int device) : device_(device)
 Engine::~Engine()
 {
   drop_graphs();
+  for (auto& kv : models_) { (void)hipFree(kv.second.d_rho); (void)hipFree(kv.second.d_vv); }
+  models_.clear();
+  if (cb_) (void)hipFree(cb_);
   pool_.trim();
   if (st_) (void)hipStreamDestroy(st_);
 }
@@ -452,8 +455,87 @@ static bool params_ok(const rtmh_params& P, std::string& why)
   return true;
 }
 
+int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, int* id)
+{
+  if (nx < 2 || nz < 2 || !vp0 || !rho0) { set_error("bad model"); return -1; }
+  HIPCHK(hipSetDevice(device_));
+  Model m;
+  m.nx = nx; m.nz = nz;
+  const size_t nm = (size_t)nx * nz;
+  m.vp0.assign(vp0, vp0 + nm);
+  std::vector<float> vv(nm);
+  // v^2 as extractvel1 stores it (Velocity_function.cpp:2059)
+  for (size_t i = 0; i < nm; i++) vv[i] = vp0[i] * vp0[i];
+  m.colmax.assign(nx, 0.0f);
+  for (int ix = 0; ix < nx; ix++) {
+    float mx = vv[ix];
+    for (int iz = 0; iz < nz; iz++) mx = vv[(size_t)iz * nx + ix] > mx ? vv[(size_t)iz * nx + ix] : mx;
+    m.colmax[ix] = mx;
+  }
+  if (hipMalloc(&m.d_rho, nm * 4) != hipSuccess || hipMalloc(&m.d_vv, nm * 4) != hipSuccess) {
+    (void)hipFree(m.d_rho);
+    set_error("out of device memory (model)");
+    return -3;
+  }
+  HIPCHK(hipMemcpy(m.d_rho, rho0, nm * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice));
+  m.bytes = 2 * nm * 4;
+  const int k = next_model_++;
+  models_[k] = std::move(m);
+  *id = k;
+  return 0;
+}
+
+int Engine::model_destroy(int id)
+{
+  auto it = models_.find(id);
+  if (it == models_.end()) { set_error("no such model"); return -1; }
+  (void)hipFree(it->second.d_rho);
+  (void)hipFree(it->second.d_vv);
+  models_.erase(it);
+  return 0;
+}
+
+const Engine::Model* Engine::model(int id) const
+{
+  auto it = models_.find(id);
+  return it == models_.end() ? nullptr : &it->second;
+}
+
 int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
                 rtmh_outputs* out, rtmh_stats* stats)
+{
+  int id = -1;
+  int rc = model_create(P.nx, P.nz, vp0, rho0, &id);
+  if (rc) return rc;
+  rc = run_model(id, P, coef, opt, out, stats);
+  model_destroy(id);
+  return rc;
+}
+
+// make the persistent control block (descriptors, tiles, slot list, stack
+// entries) large enough; graphs captured against the old block are dropped
+int Engine::ensure_cb(int nslots, size_t ntiles)
+{
+  if (nslots <= cb_slots_ && ntiles <= cb_tiles_) return 0;
+  drop_graphs();
+  if (cb_) (void)hipFree(cb_);
+  cb_ = nullptr;
+  cb_slots_ = nslots > cb_slots_ ? nslots : cb_slots_;
+  if (cb_slots_ < 64) cb_slots_ = 64;
+  cb_tiles_ = ntiles > cb_tiles_ ? ntiles * 2 : cb_tiles_;
+  const size_t bytes = align256(cb_slots_ * sizeof(ShotDev)) + align256(cb_tiles_ * sizeof(Tile)) +
+                       align256(cb_slots_ * 4) + align256(cb_slots_ * sizeof(StackShotHost));
+  HIPCHK(hipMalloc(&cb_, bytes));
+  d_ds_ = (ShotDev*)cb_;
+  d_tiles_ = (Tile*)((char*)cb_ + align256(cb_slots_ * sizeof(ShotDev)));
+  d_slots_ = (int*)((char*)d_tiles_ + align256(cb_tiles_ * sizeof(Tile)));
+  d_stk_ = (StackShotHost*)((char*)d_slots_ + align256(cb_slots_ * 4));
+  return 0;
+}
+
+int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, const rtmh_options& opt,
+                      rtmh_outputs* out, rtmh_stats* stats)
 {
   auto t0 = std::chrono::steady_clock::now();
   std::string why;
@@ -461,28 +543,23 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
   if (opt.store_fmt < 0 || opt.store_fmt > 2) { set_error("store_fmt must be 0, 1 or 2"); return -1; }
   const int tz = opt.tile_z > 0 ? opt.tile_z : 16;
   if (tz != 8 && tz != 16 && tz != 32) { set_error("tile_z must be 8, 16 or 32"); return -1; }
+  const Model* M = model(model_id);
+  if (!M) { set_error("no such model"); return -1; }
+  if (M->nx != P.nx || M->nz != P.nz) { set_error("model size differs from nx, nz"); return -1; }
   HIPCHK(hipSetDevice(device_));
+  const float* vp0 = M->vp0.data();
 
   const int nx = P.nx, nz = P.nz, nt = P.nt;
   std::vector<int> shots;
   for (int is = P.startshot; is <= P.nshot; is += P.dshot) shots.push_back(is);
   const int nsh = (int)shots.size();
-
-  // resident model: rho and v^2 (extractvel1 squares, Velocity_function.cpp:2059)
   const size_t nm = (size_t)nx * nz;
-  std::vector<float> vv(nm), colmax(nx, 0.0f);
-  for (size_t i = 0; i < nm; i++) vv[i] = vp0[i] * vp0[i];
-  for (int ix = 0; ix < nx; ix++) {
-    float m = vv[ix];
-    for (int iz = 0; iz < nz; iz++) m = vv[(size_t)iz * nx + ix] > m ? vv[(size_t)iz * nx + ix] : m;
-    colmax[ix] = m;
-  }
   // per-shot set-up, taper window state carried in shot order (rtm_model.cu:70, 465-466)
   std::vector<ShotSetup> setup(nsh);
   int nwin = 50;
   for (int k = 0; k < nsh; k++) {
     if (!shot_geometry(P, shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
-    shot_setup(P, vp0, colmax.data(), setup[k]);
+    shot_setup(P, vp0, M->colmax.data(), setup[k]);
     while (2 * nwin >= setup[k].g.nxl) nwin -= 3;
     setup[k].nwin = nwin;
     taper_row(setup[k].taper, setup[k].g.nxl, nwin);
@@ -491,13 +568,13 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
   ricker(wav, nt, P.fdom, P.dt);
 
   // device: model, coefficients, wavelet, FP64 stacks
-  const size_t common_bytes = 2 * align256(nm * 4) + align256(100 * 4) + align256((size_t)nt * 4) +
+  const size_t common_bytes = align256(100 * 4) + align256((size_t)nt * 4) +
                               3 * align256(nm * 8) + 3 * align256(nm * 4);
   char* common = (char*)pool_.get(common_bytes);
-  if (!common) { set_error("out of device memory (model)"); return -3; }
-  float* d_rho = (float*)common;
-  float* d_vv = (float*)(common + align256(nm * 4));
-  float* d_coef = (float*)(common + 2 * align256(nm * 4));
+  if (!common) { set_error("out of device memory (job)"); return -3; }
+  const float* d_rho = M->d_rho;
+  const float* d_vv = M->d_vv;
+  float* d_coef = (float*)common;
   float* d_wav = (float*)((char*)d_coef + align256(100 * 4));
   double* d_i1 = (double*)((char*)d_wav + align256((size_t)nt * 4));
   double* d_i2 = (double*)((char*)d_i1 + align256(nm * 8));
@@ -505,8 +582,6 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
   float* d_f1 = (float*)((char*)d_il + align256(nm * 8));
   float* d_f2 = (float*)((char*)d_f1 + align256(nm * 4));
   float* d_f3 = (float*)((char*)d_f2 + align256(nm * 4));
-  HIPCHK(hipMemcpyAsync(d_rho, rho0, nm * 4, hipMemcpyHostToDevice, st_));
-  HIPCHK(hipMemcpyAsync(d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(d_coef, coef, 100 * 4, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(d_wav, wav.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st_));
   HIPCHK(rtm_launch_zero_f64(d_i1, (long long)nm, st_));
@@ -555,15 +630,11 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
     build_tiles(ds, tz, tiles);
     std::vector<int> slots(nb);
     for (int j = 0; j < nb; j++) slots[j] = j;
-    // batch control block: descriptors, tiles, slot list, stack entries
-    const size_t cb_bytes = align256(nb * sizeof(ShotDev)) + align256(tiles.size() * sizeof(Tile)) +
-                            align256(nb * 4) + align256(nb * sizeof(StackShotHost));
-    char* cb = (char*)pool_.get(cb_bytes);
-    if (!cb) { set_error("out of device memory (batch)"); rc = -3; for (void* p : mem) pool_.put(p); break; }
-    ShotDev* d_ds = (ShotDev*)cb;
-    Tile* d_tiles = (Tile*)(cb + align256(nb * sizeof(ShotDev)));
-    int* d_slots = (int*)((char*)d_tiles + align256(tiles.size() * sizeof(Tile)));
-    StackShotHost* d_stk = (StackShotHost*)((char*)d_slots + align256(nb * 4));
+    if (ensure_cb(nb, tiles.size()) != 0) { for (void* p : mem) pool_.put(p); rc = -3; break; }
+    ShotDev* d_ds = d_ds_;
+    Tile* d_tiles = d_tiles_;
+    int* d_slots = d_slots_;
+    StackShotHost* d_stk = d_stk_;
     for (auto& u : uploads) HIPCHK(hipMemcpyAsync(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice, st_));
     for (int j = 0; j < nb; j++) { ds[j].phase = PH_FWD; ds[j].it_off = 0; ds[j].it_dir = 1; }
     HIPCHK(hipMemcpyAsync(d_ds, ds.data(), nb * sizeof(ShotDev), hipMemcpyHostToDevice, st_));
@@ -577,7 +648,7 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
     HIPCHK(hipEventRecord(e0, st_));
     rc = run_pass(PH_FWD, nt, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
                   (opt.profile && stats) ? stats : nullptr);
-    if (rc) { for (void* p : mem) pool_.put(p); pool_.put(cb); break; }
+    if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e1, st_));
     HIPCHK(hipStreamSynchronize(st_));
     float ms = 0;
@@ -621,7 +692,7 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
         HIPCHK(hipMemcpy(ds[j].rec, rec.data(), nrv * 4, hipMemcpyHostToDevice));
       }
     }
-    if (rc) { for (void* p : mem) pool_.put(p); pool_.put(cb); break; }
+    if (rc) { for (void* p : mem) pool_.put(p); break; }
 
     // ---- reverse pass: reconstruction + receiver propagation + imaging
     for (int j = 0; j < nb; j++) { ds[j].phase = PH_REV; ds[j].it_off = nt - 2; ds[j].it_dir = -1; }
@@ -634,7 +705,7 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
     HIPCHK(hipEventRecord(e1, st_));
     rc = run_pass(PH_REV, nt - 1, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
                   (opt.profile && stats) ? stats : nullptr);
-    if (rc) { for (void* p : mem) pool_.put(p); pool_.put(cb); break; }
+    if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e2, st_));
     HIPCHK(hipStreamSynchronize(st_));
     HIPCHK(hipEventElapsedTime(&ms, e1, e2));
@@ -689,7 +760,6 @@ int Engine::run(const rtmh_params& P, const float* vp0, const float* rho0, const
     if (opt.verbose)
       for (int j = 0; j < nb; j++) printf("%d shot finished: %g (s)\n", shots[k0 + j], 0.0);
     for (void* p : mem) pool_.put(p);
-    pool_.put(cb);
     k0 = k1;
   }
   if (rc != 0) { pool_.put(common); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); return rc; }

@@ -8,6 +8,7 @@
 
 #include "../../include/rtm_hip.h"
 #include "rtm_dev.h"
+#include "rtm_launch.h"
 
 namespace rtm {
 
@@ -71,8 +72,23 @@ class Engine {
   int device() const { return device_; }
   hipStream_t stream() const { return st_; }
 
+  // device-resident velocity models (rho and v^2), shared by many jobs
+  struct Model {
+    int nx = 0, nz = 0;
+    float* d_rho = nullptr;
+    float* d_vv = nullptr;
+    std::vector<float> vp0;     // host copy: mute velocities
+    std::vector<float> colmax;  // host: column maxima of v^2 (per-shot vmax)
+    size_t bytes = 0;
+  };
+  int model_create(int nx, int nz, const float* vp0, const float* rho0, int* id);
+  int model_destroy(int id);
+  const Model* model(int id) const;
+
   int run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
           rtmh_outputs* out, rtmh_stats* stats);
+  int run_model(int model_id, const rtmh_params& P, const float* coef, const rtmh_options& opt, rtmh_outputs* out,
+                rtmh_stats* stats);
   int fd_steps(const rtmh_params& P, int ishot, const float* vp0, const float* rho0, const float* coef, int nsteps,
                int it0, int with_source, float* pf, float* vxf, float* vzf, float* p1f, float* p2f);
 
@@ -85,6 +101,17 @@ class Engine {
   hipStream_t st_ = nullptr;
   DevPool pool_;
   std::map<GraphKey, hipGraphExec_t> graphs_;
+  std::map<int, Model> models_;
+  int next_model_ = 1;
+  // persistent control block: captured graphs reference these addresses
+  void* cb_ = nullptr;
+  int cb_slots_ = 0;
+  size_t cb_tiles_ = 0;
+  ShotDev* d_ds_ = nullptr;
+  Tile* d_tiles_ = nullptr;
+  int* d_slots_ = nullptr;
+  struct StackShotHost* d_stk_ = nullptr;
+  int ensure_cb(int nslots, size_t ntiles);
 
   // build the device descriptor of one slot inside `mem`
   void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,

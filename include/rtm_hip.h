@@ -130,6 +130,15 @@ int rtmh_shot_geometry(const rtmh_params *p, int ishot, rtmh_geom *g);
 int rtmh_run(rtmh_ctx *ctx, const rtmh_params *p, const float *vp0, const float *rho0, const float *diffcoef,
              const rtmh_options *opt, rtmh_outputs *out, rtmh_stats *stats);
 
+/* Device-resident velocity model (density and v^2 kept in HBM across jobs;
+ * the per-shot windows of extractvel1/extendvel1 are read from it by
+ * clamped indexing instead of per-shot copies). */
+int rtmh_model_create(rtmh_ctx *ctx, int nx, int nz, const float *vp0, const float *rho0, int *model_id);
+int rtmh_model_destroy(rtmh_ctx *ctx, int model_id);
+/* rtmh_run on a resident model (p->nx, p->nz must match it). */
+int rtmh_run_model(rtmh_ctx *ctx, int model_id, const rtmh_params *p, const float *diffcoef,
+                   const rtmh_options *opt, rtmh_outputs *out, rtmh_stats *stats);
+
 /* nsteps forward FD steps of one shot's source wavefield from given fields
  * (unit-level parity of the velocity-stress + PML update).  All field arrays
  * are host [nzp*nxp] (vx uses columns 0..nxp-2, vz rows 0..nzp-2; p1/p2 hold
