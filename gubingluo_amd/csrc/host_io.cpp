@@ -86,16 +86,19 @@ extern "C" int rtmio_read_seis(float* seis, SuKey* key, int inflag, int sgyflag,
   SuKey k;
   if (read_key(fp, &k, 0) != 0) { fclose(fp); return -2; }
   int swap = 0;
-  if (k.Dt <= 0 || k.Nt <= 0) {                    // get_parameter endianness probe
-    fseek(fp, start, SEEK_SET);
-    SuKey k2;
-    read_key(fp, &k2, 1);
-    if (k2.Dt > 0 && k2.Nt > 0) { swap = 1; k = k2; }
-  }
+  fseek(fp, 0, SEEK_END);
+  const long sz = ftell(fp) - start;
+  fseek(fp, start, SEEK_SET);
+  SuKey k2;
+  read_key(fp, &k2, 1);
+  // get_parameter's probe (IO_function.cpp:253-259) swaps when dt or ns is
+  // not positive; a byte-swapped header can also look positive, so a header
+  // whose trace length does not tile the file loses to one that does
+  const bool fits = k.Nt > 0 && sz % (240L + 4L * k.Nt) == 0;
+  const bool fits2 = k2.Nt > 0 && k2.Dt > 0 && sz % (240L + 4L * k2.Nt) == 0;
+  if ((k.Dt <= 0 || k.Nt <= 0 || !fits) && fits2) { swap = 1; k = k2; }
   const int nt = k.Nt;
   if (nt <= 0) { fclose(fp); return -3; }
-  fseek(fp, 0, SEEK_END);
-  long sz = ftell(fp) - start;
   const int ntrace = (int)(sz / (240L + 4L * nt));
   k.Ntrace = ntrace;
   *key = k;

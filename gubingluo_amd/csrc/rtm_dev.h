@@ -6,7 +6,10 @@
 // many shots of different window widths (ragged batches) and, in the served
 // form, shots at different time steps and phases.
 //
-// Wavefield layout (per slot, all fp32, row pitch `pitch` floats, nzp rows):
+// Wavefield layout (per slot, all fp32, row pitch `pitch` floats; field
+// pointers address cell (0,0), with RTM_MZT zero rows above, RTM_MXL zero
+// columns to the left and at least 16 rows / 25 columns of zeros beyond the
+// tiled extent, so radius-10 tile staging never needs bounds checks):
 //   sp, svx, svz : source wavefield  (forward propagation, then reverse
 //                  reconstruction in place; rtm_model.cu:322-437)
 //   sp1, sp2     : split pressure pair of the source wavefield in PML cells
@@ -23,14 +26,21 @@
 
 #define RTM_R 10            // stencil radius (N/2, common.h:57)
 #define RTM_TX 64           // tile width (one wave64 row)
+#define RTM_MZT 16          // zero margin rows above row 0 of every field
+#define RTM_MXL 16          // zero margin columns left of column 0
+#define RTM_ZALIGN 32       // field rows are allocated for nzp rounded up to this + 16 below
+
+// tile flags (Tile::kind): which cell classes a tile contains
+enum RtmTileFlag : int { TF_STRIP = 1, TF_SRC = 2, TF_REC = 4 };
 
 enum RtmPhase : int { PH_IDLE = 0, PH_FWD = 1, PH_REV = 2 };
 enum RtmStoreFmt : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
 
 struct ShotDev {
   // geometry
-  int nxp, nzp, npml, pitch;
+  int nxp, nzp, npml, pitch;   // pitch: row stride of every field (floats)
   int nxl, nzi, nt, phase;
+  int frows, forg;             // rows allocated per field; offset of (0,0) in it
   int it_off, it_dir;          // time index = it_off + it_dir * step
   int record, rcol0, nrec, _pad0;
   float dt, dtx, dtz, _pad1;

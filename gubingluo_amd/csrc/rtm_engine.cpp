@@ -305,7 +305,7 @@ static SmallLayout small_layout(const ShotSetup& s)
 size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
 {
   const size_t pitch = pitch_of(s.g.nxp), nzp = s.g.nzp;
-  const size_t F = align256(nzp * pitch * 4);
+  const size_t F = align256((size_t)frows_of((int)nzp) * pitch * 4);
   const size_t nzi = nzp - 2 * (size_t)((s.g.nxp - s.g.nxl) / 2);
   const size_t I = align256(nzi * s.g.nxl * 4);
   const size_t strip = (size_t)40 * (nzi + s.g.nxl);
@@ -324,6 +324,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   const rtmh_geom& g = s.g;
   d.nxp = g.nxp; d.nzp = g.nzp; d.npml = P.npml; d.pitch = pitch_of(g.nxp);
   d.nxl = g.nxl; d.nzi = P.nz; d.nt = P.nt; d.phase = PH_IDLE;
+  d.frows = frows_of(g.nzp); d.forg = RTM_MZT * d.pitch + RTM_MXL;
   d.it_off = 0; d.it_dir = 1;
   d.record = 1; d.rcol0 = g.rcol0; d.nrec = g.nrec;
   d.dt = P.dt; d.dtx = P.dt / P.dx; d.dtz = P.dt / P.dz;   // rtm_model.cu:73-74
@@ -345,9 +346,9 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.cut = (const int*)(base + L.cut);
   d.coef = coef_dev; d.wav = wav_dev;
   size_t o = L.total;
-  const size_t F = align256((size_t)d.nzp * d.pitch * 4);
+  const size_t F = align256((size_t)d.frows * d.pitch * 4);
   float** fp[10] = {&d.sp, &d.svx, &d.svz, &d.sp1, &d.sp2, &d.rp, &d.rvx, &d.rvz, &d.rp1, &d.rp2};
-  for (int i = 0; i < 10; i++) { *fp[i] = (float*)(base + o); o += F; }
+  for (int i = 0; i < 10; i++) { *fp[i] = (float*)(base + o) + d.forg; o += F; }
   const size_t I = align256((size_t)d.nzi * d.nxl * 4);
   d.g1 = (float*)(base + o); o += I;
   d.illum = (float*)(base + o); o += I;
@@ -432,12 +433,28 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   return 0;
 }
 
+// tile table of a batch with the per-tile flags of the cell classes it holds
 static void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles)
 {
   tiles.clear();
-  for (int k = 0; k < (int)ds.size(); k++)
-    for (int z0 = 0; z0 < ds[k].nzp; z0 += tz)
-      for (int x0 = 0; x0 < ds[k].nxp; x0 += RTM_TX) tiles.push_back(Tile{k, x0, z0, 0});
+  for (int k = 0; k < (int)ds.size(); k++) {
+    const ShotDev& d = ds[k];
+    const int n = d.npml, R = RTM_R;
+    for (int z0 = 0; z0 < d.nzp; z0 += tz)
+      for (int x0 = 0; x0 < d.nxp; x0 += RTM_TX) {
+        const int z1 = z0 + tz - 1, x1 = x0 + RTM_TX - 1;
+        auto hit = [&](int za, int zb, int xa, int xb) { return z0 <= zb && z1 >= za && x0 <= xb && x1 >= xa; };
+        int f = 0;
+        // strips: columns n-R..n-1 and nxp-n-1..nxp-n+9 over interior rows,
+        // rows n-R..n-1 and nzp-n-1..nzp-n+9 over interior columns
+        if (hit(n, d.nzp - n - 1, n - R, n - 1) || hit(n, d.nzp - n - 1, d.nxp - n - 1, d.nxp - n + R - 1) ||
+            hit(n - R, n - 1, n, d.nxp - n - 1) || hit(d.nzp - n - 1, d.nzp - n + R - 1, n, d.nxp - n - 1))
+          f |= TF_STRIP;
+        if (d.sx1 >= d.sx0 && hit(d.sz0, d.sz1, d.sx0, d.sx1)) f |= TF_SRC;
+        if (d.nrec > 0 && hit(n, n, d.rcol0, d.rcol0 + d.nrec - 1)) f |= TF_REC;
+        tiles.push_back(Tile{k, x0, z0, f});
+      }
+  }
 }
 
 static bool params_ok(const rtmh_params& P, std::string& why)
@@ -854,7 +871,7 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   float* hf[5] = {pf, vxf, vzf, p1f, p2f};
   float* dfp[5] = {d.sp, d.svx, d.svz, d.sp1, d.sp2};
   for (int k = 0; k < 5; k++) {
-    HIPCHK(hipMemset(dfp[k], 0, (size_t)nzp * pitch * 4));
+    HIPCHK(hipMemset(dfp[k] - d.forg, 0, (size_t)d.frows * pitch * 4));
     HIPCHK(hipMemcpy2D(dfp[k], (size_t)pitch * 4, hf[k], (size_t)nxp * 4, (size_t)nxp * 4, nzp, hipMemcpyHostToDevice));
   }
   LaunchArgs a{d_ds, d_tiles, (int)tiles.size(), 0};

@@ -94,7 +94,8 @@ class Engine {
 
   // bytes one slot of this geometry needs on the device
   static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt);
-  static int pitch_of(int nxp) { return ((nxp + 63) / 64) * 64; }
+  static int pitch_of(int nxp) { return ((nxp + 63) / 64) * 64 + 64; }
+  static int frows_of(int nzp) { return RTM_MZT + ((nzp + RTM_ZALIGN - 1) / RTM_ZALIGN) * RTM_ZALIGN + 16; }
 
  private:
   int device_;

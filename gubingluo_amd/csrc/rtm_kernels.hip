@@ -2,27 +2,31 @@
 //
 // Per time step two phases are launched over a tile table that may span many
 // shots (ragged windows):
-//   velocity phase  : vx, vz from p          (kernel.cu:430-500, 542-713 forward;
-//                                              kernel.cu:318-388 reverse)
-//   pressure phase  : p from vx, vz, + PML split pair, source, record, strips,
-//                     receiver insertion, imaging (kernel.cu:501-541, 714-1185,
-//                     36-95, 1186-1416)
-// A tile is 64 columns (one wave64 row, coalesced along x, the fast axis of
-// every array) by TZ rows; the fields it needs are staged once in LDS with
-// the radius-10 halo, and the x- and z-derivatives are read from LDS rows and
-// columns.  The per-cell arithmetic follows the reference operation order
-// with the fused multiply-adds written explicitly (see the oracle header), so
-// results are reproducible bit for bit against the FP32 CPU restatement.
+//   velocity phase : vx, vz from p       (kernel.cu:430-500, 542-713 forward;
+//                                          kernel.cu:318-388 reverse)
+//   pressure phase : p from vx, vz, PML split pair, source, record, strips,
+//                    receiver insertion, imaging (kernel.cu:501-541, 714-1185,
+//                    36-95, 1186-1416)
+// A tile is 64 columns (one wave64 row: every array is x-fastest, so each
+// wave touches one contiguous 256-byte row segment) by TZ rows.  The operand
+// field is staged once in LDS with its radius-10 halo; x-derivatives read LDS
+// rows, z-derivatives a register window of the thread's own column (each
+// thread owns TZ/4 consecutive rows).  Fields carry zero margins (MZT rows
+// above, MXL columns left, more below/right), so staging needs no bounds
+// checks.  Per-tile flags computed on the host keep the strip / source /
+// receiver logic out of tiles that do not touch them.
 //
-// Compiled with -ffp-contract=off: no fusion other than the __builtin_fmaf
-// calls below; float division is IEEE (correctly rounded) as HIP compiles it.
+// The per-cell arithmetic follows the reference operation order with the
+// fused multiply-adds written explicitly (the oracle's canonical FP32 form),
+// so results match the CPU restatement bit for bit.  Compiled with
+// -ffp-contract=off; float division is IEEE as HIP compiles it.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
 #include "rtm_dev.h"
 
-#define LW 83              // 64 + 19: tile width incl. the x halo
+#define LW 83              // 64 + 19: staged width incl. the x halo
 #define NTHR 256
 
 namespace {
@@ -35,8 +39,7 @@ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 // share their halos through that XCD's L2).
 __device__ __forceinline__ int xcd_tile(int ntiles)
 {
-  const int g = gridDim.x;                 // launched as a multiple of 8
-  const int per = g >> 3;
+  const int per = gridDim.x >> 3;          // launched as a multiple of 8
   const int b = blockIdx.x;
   const int t = (b & 7) * per + (b >> 3);
   return t < ntiles ? t : -1;
@@ -49,18 +52,28 @@ __device__ __forceinline__ float model_at(const float* __restrict__ m, const Sho
   return m[(size_t)r * s.mpitch + c];
 }
 
-// stage f[zs .. zs+H) x [xs .. xs+W) into L (row-major, width W); cells
-// outside [0,nr) x [0,nc) read as 0.
+// stage f rows zs..zs+H-1, columns xs..xs+W-1 (W <= 128) into L[H][W]; all
+// global loads are issued before the LDS stores.
 template <int W, int H>
-__device__ __forceinline__ void stage(float* __restrict__ L, const float* __restrict__ f, int pitch,
-                                      int nr, int nc, int zs, int xs)
+__device__ __forceinline__ void stage(float* __restrict__ L, const float* __restrict__ f, int pitch, int zs, int xs)
 {
-  for (int e = threadIdx.x; e < W * H; e += NTHR) {
-    const int r = e / W, c = e - r * W;
-    const int gz = zs + r, gx = xs + c;
-    float v = 0.0f;
-    if (gz >= 0 && gz < nr && gx >= 0 && gx < nc) v = f[(size_t)gz * pitch + gx];
-    L[e] = v;
+  constexpr int K = (H + 3) / 4;
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  float a[K], b[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ly + 4 * k;
+    const float* src = f + (ptrdiff_t)(zs + imin(r, H - 1)) * pitch + xs;
+    a[k] = src[lx];
+    b[k] = (W > 64 && lx < W - 64) ? src[64 + lx] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int r = ly + 4 * k;
+    if (r < H) {
+      L[r * W + lx] = a[k];
+      if (W > 64 && lx < W - 64) L[r * W + 64 + lx] = b[k];
+    }
   }
 }
 
@@ -105,7 +118,7 @@ __device__ __forceinline__ long long strip_p(const ShotDev& s, int iz, int ix)
   return -1;
 }
 
-// 16-bit store: block index of a strip offset (0 vx, 1 vz, 2 p-lr, 3 p-tb)
+// 16-bit store: block of a strip offset (0 vx, 1 vz, 2 p-lr, 3 p-tb)
 __device__ __forceinline__ int strip_block(const ShotDev& s, long long off)
 {
   const long long a = 2LL * RTM_R * s.nzi, b = a + 2LL * RTM_R * s.nxl, c = b + 2LL * RTM_R * s.nzi;
@@ -114,16 +127,13 @@ __device__ __forceinline__ int strip_block(const ShotDev& s, long long off)
 
 __device__ __forceinline__ void store_put(const ShotDev& s, int it, long long off, float v)
 {
-  const long long k = (long long)it * s.strip_len + off;
   if (s.store_fmt == ST_F32) {
-    ((float*)s.store)[k] = v;
+    ((float*)s.store)[(long long)it * s.strip_len + off] = v;
   } else {
-    // raw fp32 staging of step it (ping-pong by parity); quantised by k_quant
-    float* stage = (float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
-    (void)k;
-    stage[off] = v;
-    const unsigned int bits = __float_as_uint(fabsf(v));
-    atomicMax(&s.smax[(long long)it * 4 + strip_block(s, off)], bits);
+    // fp32 staging of step it (by parity), quantised by k_quant after the step
+    float* stg = (float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
+    stg[off] = v;
+    atomicMax(&s.smax[(long long)it * 4 + strip_block(s, off)], __float_as_uint(fabsf(v)));
   }
 }
 
@@ -137,17 +147,33 @@ __device__ __forceinline__ float store_get(const ShotDev& s, int it, long long o
   return __uint_as_float(((unsigned int)h) << 16) * sc;
 }
 
+// radius-r staggered difference a[i] - b[i] with the coefficient row of order
+// 2r; the fma chain runs i = 1..r as the reference loops (kernel.cu:459-460)
+#define STENCIL_SUM(acc, r, c10, cf, A, B)                                                 \
+  do {                                                                                     \
+    if ((r) == RTM_R) {                                                                    \
+      _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], (A) - (B), acc); \
+    } else {                                                                               \
+      const float* cr_ = (cf) + ((r) - 1) * RTM_R;                                         \
+      for (int i = 1; i <= (r); i++) acc = __builtin_fmaf(cr_[i - 1], (A) - (B), acc);     \
+    }                                                                                      \
+  } while (0)
+
 // ------------------------------------------------------- velocity phase --
-// Forward velocity update of one wavefield over the whole padded grid:
+// Forward velocity update of one wavefield over a tile of the padded grid:
 //   vx(iz,ix), ix <= nxp-2, radius min(10, ix+1, nxp-1-ix)   (norderx, kernel.cu:299-307)
 //   vz(iz,ix), iz <= nzp-2, radius min(10, iz+1, nzp-1-iz)   (norderz, kernel.cu:309-317)
-//   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho   (kernel.cu:588-594)
+//   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho  (kernel.cu:588-594);
+//   c1 = c2 = 1 off the absorbing layer reduces it to v - D (kernel.cu:462).
 // L holds p rows z0-9 .. z0+TZ+9, columns x0-9 .. x0+73.
 template <int TZ>
 __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __restrict__ vx, float* __restrict__ vz,
-                                        const ShotDev& s, int x0, int z0, int save_it)
+                                        const ShotDev& s, int x0, int z0, int save_it, int flags,
+                                        const float (&vxo)[TZ / 4], const float (&vzo)[TZ / 4],
+                                        const float (&rho)[TZ / 4])
 {
-  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  constexpr int RPT = TZ / 4;
+  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
   const int ix = x0 + lx;
   const int nxp = s.nxp, nzp = s.nzp, P = s.pitch;
   if (ix >= nxp) return;
@@ -155,33 +181,27 @@ __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __re
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = cf[90 + i];
-  const float dtx = s.dtx, dtz = s.dtz;
   const bool hasvx = ix <= nxp - 2;
   const int rx = imin(RTM_R, imin(ix + 1, nxp - 1 - ix));
-  float c1x = 1.0f, c2x = 1.0f;
-  if (hasvx) { c1x = s.c1xv[ix]; c2x = s.c2xv[ix]; }
+  const float c1x = s.c1xv[ix], c2x = s.c2xv[ix];
   const bool dampx = !(c1x == 1.0f && c2x == 1.0f);
-  for (int j = lz; j < TZ; j += 4) {
-    const int iz = z0 + j;
-    if (iz >= nzp) break;
-    const float rho = model_at(s.rho, s, iz, ix);
-    const float* __restrict__ Lr = L + (j + 9) * LW + lx + 9;   // -> p(iz, ix)
-    if (hasvx) {
-      float acc = 0.0f;
-      if (rx == RTM_R) {
+  float col[RPT + 19];                                 // p(z0+j0-9+k, ix)
 #pragma unroll
-        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
-      } else {
-        const float* c = cf + (rx - 1) * RTM_R;
-        for (int i = 1; i <= rx; i++) acc = __builtin_fmaf(c[i - 1], Lr[i] - Lr[1 - i], acc);
-      }
-      acc = acc * dtx;
-      acc = acc / rho;
-      const size_t id = (size_t)iz * P + ix;
-      float v = vx[id];
-      v = dampx ? __builtin_fmaf(c1x, v, -acc) / c2x : v - acc;
+  for (int k = 0; k < RPT + 19; k++) col[k] = L[(j0 + k) * LW + lx + 9];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int iz = z0 + j0 + r;
+    if (iz >= nzp) break;
+    const size_t id = (size_t)iz * P + ix;
+    if (hasvx) {
+      const float* __restrict__ Lr = L + (j0 + r + 9) * LW + lx + 9;
+      float acc = 0.0f;
+      STENCIL_SUM(acc, rx, c10, cf, Lr[i], Lr[1 - i]);
+      acc = acc * s.dtx;
+      acc = acc / rho[r];
+      const float v = dampx ? __builtin_fmaf(c1x, vxo[r], -acc) / c2x : vxo[r] - acc;
       vx[id] = v;
-      if (save_it >= 0) {
+      if (save_it >= 0 && (flags & TF_STRIP)) {
         const long long o = strip_vx(s, iz, ix);
         if (o >= 0) store_put(s, save_it, o, v);
       }
@@ -190,21 +210,12 @@ __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __re
       const int rz = imin(RTM_R, imin(iz + 1, nzp - 1 - iz));
       const float c1z = s.c1zv[iz], c2z = s.c2zv[iz];
       float acc = 0.0f;
-      const float* __restrict__ Lc = L + (j + 9) * LW + lx + 9;
-      if (rz == RTM_R) {
-#pragma unroll
-        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lc[i * LW] - Lc[(1 - i) * LW], acc);
-      } else {
-        const float* c = cf + (rz - 1) * RTM_R;
-        for (int i = 1; i <= rz; i++) acc = __builtin_fmaf(c[i - 1], Lc[i * LW] - Lc[(1 - i) * LW], acc);
-      }
-      acc = acc * dtz;
-      acc = acc / rho;
-      const size_t id = (size_t)iz * P + ix;
-      float v = vz[id];
-      v = (c1z == 1.0f && c2z == 1.0f) ? v - acc : __builtin_fmaf(c1z, v, -acc) / c2z;
+      STENCIL_SUM(acc, rz, c10, cf, col[r + 9 + i], col[r + 10 - i]);
+      acc = acc * s.dtz;
+      acc = acc / rho[r];
+      const float v = (c1z == 1.0f && c2z == 1.0f) ? vzo[r] - acc : __builtin_fmaf(c1z, vzo[r], -acc) / c2z;
       vz[id] = v;
-      if (save_it >= 0) {
+      if (save_it >= 0 && (flags & TF_STRIP)) {
         const long long o = strip_vz(s, iz, ix);
         if (o >= 0) store_put(s, save_it, o, v);
       }
@@ -217,76 +228,75 @@ __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __re
 // it+1 are written back, interior cells get v += D.
 template <int TZ>
 __device__ __forceinline__ void vel_rev(const float* __restrict__ L, float* __restrict__ vx, float* __restrict__ vz,
-                                        const ShotDev& s, int x0, int z0, int it)
+                                        const ShotDev& s, int x0, int z0, int it, int flags,
+                                        const float (&vxo)[TZ / 4], const float (&vzo)[TZ / 4],
+                                        const float (&rho)[TZ / 4])
 {
-  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  constexpr int RPT = TZ / 4;
+  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
   const int ix = x0 + lx;
   const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
   if (ix >= nxp) return;
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
-  const float dtx = s.dtx, dtz = s.dtz;
   const bool xin_vx = ix >= npml && ix <= nxp - npml - 2;
   const bool xin_vz = ix >= npml && ix < nxp - npml;
-  for (int j = lz; j < TZ; j += 4) {
-    const int iz = z0 + j;
+  float col[RPT + 19];
+#pragma unroll
+  for (int k = 0; k < RPT + 19; k++) col[k] = L[(j0 + k) * LW + lx + 9];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int iz = z0 + j0 + r;
     if (iz >= nzp) break;
     const size_t id = (size_t)iz * P + ix;
-    const float* __restrict__ Lr = L + (j + 9) * LW + lx + 9;
-    // vx
-    {
-      const long long o = strip_vx(s, iz, ix);
-      if (o >= 0) {
-        vx[id] = store_get(s, it + 1, o);
-      } else if (xin_vx && iz >= npml && iz < nzp - npml) {
-        float acc = 0.0f;
+    const long long ovx = (flags & TF_STRIP) ? strip_vx(s, iz, ix) : -1;
+    if (ovx >= 0) {
+      vx[id] = store_get(s, it + 1, ovx);
+    } else if (xin_vx && iz >= npml && iz < nzp - npml) {
+      const float* __restrict__ Lr = L + (j0 + r + 9) * LW + lx + 9;
+      float acc = 0.0f;
 #pragma unroll
-        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
-        acc = acc * dtx;
-        acc = acc / model_at(s.rho, s, iz, ix);
-        vx[id] = vx[id] + acc;
-      }
+      for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
+      acc = acc * s.dtx;
+      acc = acc / rho[r];
+      vx[id] = vxo[r] + acc;
     }
-    // vz
-    {
-      const long long o = strip_vz(s, iz, ix);
-      if (o >= 0) {
-        vz[id] = store_get(s, it + 1, o);
-      } else if (xin_vz && iz >= npml && iz <= nzp - npml - 2) {
-        float acc = 0.0f;
+    const long long ovz = (flags & TF_STRIP) ? strip_vz(s, iz, ix) : -1;
+    if (ovz >= 0) {
+      vz[id] = store_get(s, it + 1, ovz);
+    } else if (xin_vz && iz >= npml && iz <= nzp - npml - 2) {
+      float acc = 0.0f;
 #pragma unroll
-        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i * LW] - Lr[(1 - i) * LW], acc);
-        acc = acc * dtz;
-        acc = acc / model_at(s.rho, s, iz, ix);
-        vz[id] = vz[id] + acc;
-      }
+      for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], col[r + 9 + i] - col[r + 10 - i], acc);
+      acc = acc * s.dtz;
+      acc = acc / rho[r];
+      vz[id] = vzo[r] + acc;
     }
   }
 }
 
 // ------------------------------------------------------- pressure phase --
-// Spatial derivatives of the velocities at p node (iz, ix) with radius r,
-// read from the staged tiles: VX rows z0..z0+TZ, columns x0-10..x0+73;
-// VZ rows z0-10..z0+TZ+9, columns x0..x0+64.
+// VX rows z0..z0+TZ-1, columns x0-10..x0+72; VZ rows z0-10..z0+TZ+8, columns x0..x0+63.
 struct DivV { float dxv, dzv; };
 
-__device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float* __restrict__ VZc, int r,
+template <int RPT>
+__device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float (&col)[RPT + 19], int r, int rad,
                                       const float* c10, const float* __restrict__ cf,
                                       float dtx, float dtz, float rho, float vv)
 {
   float sx = 0.0f, sz = 0.0f;
-  if (r == RTM_R) {
+  if (rad == RTM_R) {
 #pragma unroll
     for (int i = 1; i <= RTM_R; i++) {
       sx = __builtin_fmaf(c10[i - 1], VXr[i - 1] - VXr[-i], sx);
-      sz = __builtin_fmaf(c10[i - 1], VZc[(i - 1) * 64] - VZc[-i * 64], sz);
+      sz = __builtin_fmaf(c10[i - 1], col[r + 9 + i] - col[r + 10 - i], sz);
     }
   } else {
-    const float* c = cf + (r - 1) * RTM_R;
-    for (int i = 1; i <= r; i++) {
+    const float* c = cf + (rad - 1) * RTM_R;
+    for (int i = 1; i <= rad; i++) {
       sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
-      sz = __builtin_fmaf(c[i - 1], VZc[(i - 1) * 64] - VZc[-i * 64], sz);
+      sz = __builtin_fmaf(c[i - 1], col[r + 9 + i] - col[r + 10 - i], sz);
     }
   }
   DivV d;
@@ -295,8 +305,8 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
   return d;
 }
 
-// forward pressure update of one cell; returns the new p.  Interior cells:
-// p -= dxv + dzv (kernel.cu:539).  PML cells: split pair (kernel.cu:780-783).
+// forward pressure update of one cell.  Interior: p -= dxv + dzv
+// (kernel.cu:539).  PML: split pair, p = p1 + p2 (kernel.cu:780-783).
 __device__ __forceinline__ float pres_cell(float p, float* __restrict__ p1, float* __restrict__ p2, size_t id,
                                            bool inner, const DivV& d, float c1x, float c2x, float c1z, float c2z)
 {
@@ -310,19 +320,29 @@ __device__ __forceinline__ float pres_cell(float p, float* __restrict__ p1, floa
 
 template <int TZ>
 __device__ __forceinline__ void stage_v(float* __restrict__ VX, float* __restrict__ VZ, const float* __restrict__ vx,
-                                        const float* __restrict__ vz, const ShotDev& s, int x0, int z0)
+                                        const float* __restrict__ vz, int pitch, int x0, int z0)
 {
-  stage<LW, TZ>(VX, vx, s.pitch, s.nzp, s.nxp - 1, z0, x0 - 10);
-  stage<64, TZ + 19>(VZ, vz, s.pitch, s.nzp - 1, s.nxp, z0 - 10, x0);
+  stage<LW, TZ>(VX, vx, pitch, z0, x0 - 10);
+  stage<64, TZ + 19>(VZ, vz, pitch, z0 - 10, x0);
+}
+
+template <int RPT>
+__device__ __forceinline__ void load_col64(float (&col)[RPT + 19], const float* __restrict__ VZ, int j0, int lx)
+{
+#pragma unroll
+  for (int k = 0; k < RPT + 19; k++) col[k] = VZ[(j0 + k) * 64 + lx];
 }
 
 // forward pressure phase of the source wavefield: FD update, source
 // injection (kernel.cu:36-47), recording (kernel.cu:48-71), p strips.
 template <int TZ>
 __device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const float* __restrict__ VZ,
-                                             const ShotDev& s, int x0, int z0, int it, int save)
+                                             const ShotDev& s, int x0, int z0, int it, int save, int flags,
+                                             const float (&po)[TZ / 4], const float (&rho)[TZ / 4],
+                                             const float (&vv)[TZ / 4])
 {
-  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  constexpr int RPT = TZ / 4;
+  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
   const int ix = x0 + lx;
   const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
   if (ix >= nxp) return;
@@ -332,28 +352,28 @@ __device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const
   const float c1x = s.c1x[ix], c2x = s.c2x[ix];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
-  const float wv = s.wav[it];
-  const int sw = s.sx1 - s.sx0 + 1;
-  for (int j = lz; j < TZ; j += 4) {
-    const int iz = z0 + j;
+  float col[RPT + 19];
+  load_col64<RPT>(col, VZ, j0, lx);
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int iz = z0 + j0 + r;
     if (iz >= nzp) break;
     const size_t id = (size_t)iz * P + ix;
-    float p = s.sp[id];
+    float p = po[r];
     if (!xring && iz > 0 && iz < nzp - 1) {
-      const int r = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
-      const DivV d = div_v(VX + j * LW + lx + 10, VZ + (j + 10) * 64 + lx, r, c10, s.coef, s.dtx, s.dtz,
-                           model_at(s.rho, s, iz, ix), model_at(s.vv, s, iz, ix));
+      const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
+      const DivV d = div_v<RPT>(VX + (j0 + r) * LW + lx + 10, col, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
       const bool inner = xin && iz >= npml && iz < nzp - npml;
       p = pres_cell(p, s.sp1, s.sp2, id, inner, d, c1x, c2x, s.c1z[iz], s.c2z[iz]);
     }
-    if (iz >= s.sz0 && iz <= s.sz1 && ix >= s.sx0 && ix <= s.sx1)
-      p = __builtin_fmaf(s.sa[(iz - s.sz0) * sw + ix - s.sx0], wv, p);
+    if ((flags & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && ix >= s.sx0 && ix <= s.sx1)
+      p = __builtin_fmaf(s.sa[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], s.wav[it], p);
     s.sp[id] = p;
-    if (s.record && iz == npml) {
+    if ((flags & TF_REC) && s.record && iz == npml) {
       const int ir = ix - s.rcol0;
       if (ir >= 0 && ir < s.nrec) s.rec[(size_t)it * s.nrec + ir] = p;
     }
-    if (save) {
+    if (save && (flags & TF_STRIP)) {
       const long long o = strip_p(s, iz, ix);
       if (o >= 0) store_put(s, it, o, p);
     }
@@ -367,9 +387,12 @@ __device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const
 template <int TZ>
 __device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const float* __restrict__ SZ,
                                          const float* __restrict__ RX, const float* __restrict__ RZ,
-                                         const ShotDev& s, int x0, int z0, int it)
+                                         const ShotDev& s, int x0, int z0, int it, int flags,
+                                         const float (&pso)[TZ / 4], const float (&pro)[TZ / 4],
+                                         const float (&rho)[TZ / 4], const float (&vv)[TZ / 4])
 {
-  const int lx = threadIdx.x & 63, lz = threadIdx.x >> 6;
+  constexpr int RPT = TZ / 4;
+  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
   const int ix = x0 + lx;
   const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
   if (ix >= nxp) return;
@@ -379,41 +402,43 @@ __device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const flo
   const float c1x = s.c1x[ix], c2x = s.c2x[ix];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
-  const float wv = s.wav[it];
-  const int sw = s.sx1 - s.sx0 + 1;
-  const bool xsrc = ix >= s.sx0 && ix <= s.sx1;
-  for (int j = lz; j < TZ; j += 4) {
-    const int iz = z0 + j;
+  const bool xsrc = (flags & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
+  float cs[RPT + 19], cr[RPT + 19];
+  load_col64<RPT>(cs, SZ, j0, lx);
+  load_col64<RPT>(cr, RZ, j0, lx);
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int iz = z0 + j0 + r;
     if (iz >= nzp) break;
     const size_t id = (size_t)iz * P + ix;
     const bool inner = xin && iz >= npml && iz < nzp - npml;
-    const float rho = model_at(s.rho, s, iz, ix);
-    const float vv = model_at(s.vv, s, iz, ix);
     // source wavefield (interior + strips only; the rest is never read)
     float ps = 0.0f;
     {
-      const long long o = strip_p(s, iz, ix);
+      const long long o = (flags & TF_STRIP) ? strip_p(s, iz, ix) : -1;
       bool wr = false;
       if (o >= 0) {
-        ps = store_get(s, it, o); wr = true;
+        ps = store_get(s, it, o);
+        wr = true;
       } else if (inner) {
-        const DivV d = div_v(SX + j * LW + lx + 10, SZ + (j + 10) * 64 + lx, RTM_R, c10, s.coef, s.dtx, s.dtz, rho, vv);
-        ps = s.sp[id] + (d.dxv + d.dzv);
+        const DivV d = div_v<RPT>(SX + (j0 + r) * LW + lx + 10, cs, r, RTM_R, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+        ps = pso[r] + (d.dxv + d.dzv);
         wr = true;
       }
       if (wr) {
-        if (xsrc && iz >= s.sz0 && iz <= s.sz1) ps = __builtin_fmaf(-s.sa[(iz - s.sz0) * sw + ix - s.sx0], wv, ps);
+        if (xsrc && iz >= s.sz0 && iz <= s.sz1)
+          ps = __builtin_fmaf(-s.sa[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], s.wav[it], ps);
         s.sp[id] = ps;
       }
     }
     // receiver wavefield
-    float p = s.rp[id];
+    float p = pro[r];
     if (!xring && iz > 0 && iz < nzp - 1) {
-      const int r = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
-      const DivV d = div_v(RX + j * LW + lx + 10, RZ + (j + 10) * 64 + lx, r, c10, s.coef, s.dtx, s.dtz, rho, vv);
+      const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
+      const DivV d = div_v<RPT>(RX + (j0 + r) * LW + lx + 10, cr, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
       p = pres_cell(p, s.rp1, s.rp2, id, inner, d, c1x, c2x, s.c1z[iz], s.c2z[iz]);
     }
-    if (iz == npml) {
+    if ((flags & TF_REC) && iz == npml) {
       const int ir = ix - s.rcol0;
       if (ir >= 0 && ir < s.nrec) {
         const float rv = (it <= s.cut[ir]) ? 0.0f : s.rec[(size_t)it * s.nrec + ir];
@@ -429,26 +454,50 @@ __device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const flo
   }
 }
 
+// own-cell loads of a thread (rows j0..j0+RPT-1 of its column), issued before
+// the tile barrier so they overlap the staging; rows / columns past the grid
+// read the (zero) margin
+template <int RPT>
+__device__ __forceinline__ void own_load(float (&o)[RPT], const float* __restrict__ f, const ShotDev& s, int x0, int z0)
+{
+  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
+#pragma unroll
+  for (int r = 0; r < RPT; r++) o[r] = f[(ptrdiff_t)(z0 + j0 + r) * s.pitch + x0 + lx];
+}
+template <int RPT>
+__device__ __forceinline__ void own_model(float (&o)[RPT], const float* __restrict__ m, const ShotDev& s, int x0, int z0)
+{
+  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
+#pragma unroll
+  for (int r = 0; r < RPT; r++) o[r] = model_at(m, s, z0 + j0 + r, x0 + lx);
+}
+
 }  // namespace
 
 // ================================================================ kernels ==
 template <int TZ>
 __global__ __launch_bounds__(NTHR) void k_vel_fwd(LaunchArgs a, int save)
 {
+  constexpr int RPT = TZ / 4;
   __shared__ float L[(TZ + 19) * LW];
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
   const int it = s.it_off + s.it_dir * a.step;
-  stage<LW, TZ + 19>(L, s.sp, s.pitch, s.nzp, s.nxp, tl.z0 - 9, tl.x0 - 9);
+  float vxo[RPT], vzo[RPT], rho[RPT];
+  stage<LW, TZ + 19>(L, s.sp, s.pitch, tl.z0 - 9, tl.x0 - 9);
+  own_load<RPT>(vxo, s.svx, s, tl.x0, tl.z0);
+  own_load<RPT>(vzo, s.svz, s, tl.x0, tl.z0);
+  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
   __syncthreads();
-  vel_fwd<TZ>(L, s.svx, s.svz, s, tl.x0, tl.z0, save ? it : -1);
+  vel_fwd<TZ>(L, s.svx, s.svz, s, tl.x0, tl.z0, save ? it : -1, tl.kind, vxo, vzo, rho);
 }
 
 template <int TZ>
 __global__ __launch_bounds__(NTHR) void k_pres_fwd(LaunchArgs a, int save)
 {
+  constexpr int RPT = TZ / 4;
   __shared__ float VX[TZ * LW];
   __shared__ float VZ[(TZ + 19) * 64];
   const int t = xcd_tile(a.ntiles);
@@ -456,14 +505,19 @@ __global__ __launch_bounds__(NTHR) void k_pres_fwd(LaunchArgs a, int save)
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
   const int it = s.it_off + s.it_dir * a.step;
-  stage_v<TZ>(VX, VZ, s.svx, s.svz, s, tl.x0, tl.z0);
+  float po[RPT], rho[RPT], vv[RPT];
+  stage_v<TZ>(VX, VZ, s.svx, s.svz, s.pitch, tl.x0, tl.z0);
+  own_load<RPT>(po, s.sp, s, tl.x0, tl.z0);
+  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
+  own_model<RPT>(vv, s.vv, s, tl.x0, tl.z0);
   __syncthreads();
-  pres_fwd_src<TZ>(VX, VZ, s, tl.x0, tl.z0, it, save);
+  pres_fwd_src<TZ>(VX, VZ, s, tl.x0, tl.z0, it, save, tl.kind, po, rho, vv);
 }
 
 template <int TZ>
 __global__ __launch_bounds__(NTHR) void k_vel_rev(LaunchArgs a)
 {
+  constexpr int RPT = TZ / 4;
   __shared__ float LS[(TZ + 19) * LW];
   __shared__ float LR[(TZ + 19) * LW];
   const int t = xcd_tile(a.ntiles);
@@ -471,16 +525,23 @@ __global__ __launch_bounds__(NTHR) void k_vel_rev(LaunchArgs a)
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
   const int it = s.it_off + s.it_dir * a.step;
-  stage<LW, TZ + 19>(LS, s.sp, s.pitch, s.nzp, s.nxp, tl.z0 - 9, tl.x0 - 9);
-  stage<LW, TZ + 19>(LR, s.rp, s.pitch, s.nzp, s.nxp, tl.z0 - 9, tl.x0 - 9);
+  float svx[RPT], svz[RPT], rvx[RPT], rvz[RPT], rho[RPT];
+  stage<LW, TZ + 19>(LS, s.sp, s.pitch, tl.z0 - 9, tl.x0 - 9);
+  stage<LW, TZ + 19>(LR, s.rp, s.pitch, tl.z0 - 9, tl.x0 - 9);
+  own_load<RPT>(svx, s.svx, s, tl.x0, tl.z0);
+  own_load<RPT>(svz, s.svz, s, tl.x0, tl.z0);
+  own_load<RPT>(rvx, s.rvx, s, tl.x0, tl.z0);
+  own_load<RPT>(rvz, s.rvz, s, tl.x0, tl.z0);
+  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
   __syncthreads();
-  vel_rev<TZ>(LS, s.svx, s.svz, s, tl.x0, tl.z0, it);
-  vel_fwd<TZ>(LR, s.rvx, s.rvz, s, tl.x0, tl.z0, -1);
+  vel_rev<TZ>(LS, s.svx, s.svz, s, tl.x0, tl.z0, it, tl.kind, svx, svz, rho);
+  vel_fwd<TZ>(LR, s.rvx, s.rvz, s, tl.x0, tl.z0, -1, tl.kind, rvx, rvz, rho);
 }
 
 template <int TZ>
 __global__ __launch_bounds__(NTHR) void k_pres_rev(LaunchArgs a)
 {
+  constexpr int RPT = TZ / 4;
   __shared__ float SX[TZ * LW];
   __shared__ float SZ[(TZ + 19) * 64];
   __shared__ float RX[TZ * LW];
@@ -490,41 +551,44 @@ __global__ __launch_bounds__(NTHR) void k_pres_rev(LaunchArgs a)
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
   const int it = s.it_off + s.it_dir * a.step;
-  stage_v<TZ>(SX, SZ, s.svx, s.svz, s, tl.x0, tl.z0);
-  stage_v<TZ>(RX, RZ, s.rvx, s.rvz, s, tl.x0, tl.z0);
+  float pso[RPT], pro[RPT], rho[RPT], vv[RPT];
+  stage_v<TZ>(SX, SZ, s.svx, s.svz, s.pitch, tl.x0, tl.z0);
+  stage_v<TZ>(RX, RZ, s.rvx, s.rvz, s.pitch, tl.x0, tl.z0);
+  own_load<RPT>(pso, s.sp, s, tl.x0, tl.z0);
+  own_load<RPT>(pro, s.rp, s, tl.x0, tl.z0);
+  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
+  own_model<RPT>(vv, s.vv, s, tl.x0, tl.z0);
   __syncthreads();
-  pres_rev<TZ>(SX, SZ, RX, RZ, s, tl.x0, tl.z0, it);
+  pres_rev<TZ>(SX, SZ, RX, RZ, s, tl.x0, tl.z0, it, tl.kind, pso, pro, rho, vv);
 }
 
 // 16-bit boundary store: quantise the fp32 staging of step `it` of each slot
-// with the max-abs of its block: q = x / scale, scale = maxabs / 32768 (fp16)
-// or 1 (bf16 keeps the fp32 exponent range; scale kept for uniformity).
+// with the max-abs of its block: q = x / scale, scale = maxabs / 32768.
 __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
                                                  int nslots, int step)
 {
   for (int k = 0; k < nslots; k++) {
     const ShotDev s = shots[slots[k]];
     const int it = s.it_off + s.it_dir * step;
-    const float* stage = (const float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
+    const float* stg = (const float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
     float sc[4];
+#pragma unroll
     for (int b = 0; b < 4; b++) {
       const float m = __uint_as_float(s.smax[(long long)it * 4 + b]);
-      float v = 1.0f;
-      if (s.store_fmt == ST_F16) v = m > 0.0f ? m / 32768.0f : 1.0f;
-      sc[b] = v;
+      sc[b] = m > 0.0f ? m / 32768.0f : 1.0f;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 4) s.sscale[(long long)it * 4 + threadIdx.x] = sc[threadIdx.x];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int b = 0; b < 4; b++) s.sscale[(long long)it * 4 + b] = sc[b];
     const long long n = s.strip_len;
     const long long a = 2LL * RTM_R * s.nzi, b = a + 2LL * RTM_R * s.nxl, c = b + 2LL * RTM_R * s.nzi;
     for (long long e = (long long)blockIdx.x * NTHR + threadIdx.x; e < n; e += (long long)gridDim.x * NTHR) {
       const int blk = e < a ? 0 : (e < b ? 1 : (e < c ? 2 : 3));
-      const float q = stage[e] / sc[blk];
+      const float q = stg[e] / sc[blk];
       const long long k2 = (long long)it * n + e;
       if (s.store_fmt == ST_F16) {
         ((__half*)s.store)[k2] = __float2half_rn(q);
       } else {
-        // bf16, round to nearest even
-        unsigned int u = __float_as_uint(q);
+        unsigned int u = __float_as_uint(q);          // bf16, round to nearest even
         u += 0x7FFFu + ((u >> 16) & 1u);
         ((unsigned short*)s.store)[k2] = (unsigned short)(u >> 16);
       }
@@ -532,23 +596,28 @@ __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shot
   }
 }
 
-// zero the wavefields of a slot for the phase that starts (FWD: source set;
-// REV: source velocities, receiver set and imaging accumulators), and seed
-// the receiver wavefield with the last trace sample (rtm_model.cu:366-410).
+// zero the whole field allocations (margins included) that the starting
+// phase uses: FWD both sets; REV the source velocities (rtm_model.cu:366-372),
+// the receiver set and the imaging accumulators.
 __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__ shots, int slot, int phase)
 {
   const ShotDev s = shots[slot];
-  const long long n = (long long)s.nzp * s.pitch;
+  const long long n = (long long)s.frows * s.pitch;
   const long long ni = (long long)s.nzi * s.nxl;
   const long long g0 = (long long)blockIdx.x * NTHR + threadIdx.x, gs = (long long)gridDim.x * NTHR;
+  float *sp = s.sp - s.forg, *svx = s.svx - s.forg, *svz = s.svz - s.forg, *sp1 = s.sp1 - s.forg, *sp2 = s.sp2 - s.forg;
+  float *rp = s.rp - s.forg, *rvx = s.rvx - s.forg, *rvz = s.rvz - s.forg, *rp1 = s.rp1 - s.forg, *rp2 = s.rp2 - s.forg;
   if (phase == PH_FWD) {
-    for (long long e = g0; e < n; e += gs) { s.sp[e] = 0.f; s.svx[e] = 0.f; s.svz[e] = 0.f; s.sp1[e] = 0.f; s.sp2[e] = 0.f; }
+    for (long long e = g0; e < n; e += gs) {
+      sp[e] = 0.f; svx[e] = 0.f; svz[e] = 0.f; sp1[e] = 0.f; sp2[e] = 0.f;
+      rp[e] = 0.f; rvx[e] = 0.f; rvz[e] = 0.f; rp1[e] = 0.f; rp2[e] = 0.f;
+    }
     if (s.store_fmt != ST_F32)
       for (long long e = g0; e < 4LL * s.nt; e += gs) s.smax[e] = 0u;
   } else {
     for (long long e = g0; e < n; e += gs) {
-      s.svx[e] = 0.f; s.svz[e] = 0.f;
-      s.rp[e] = 0.f; s.rvx[e] = 0.f; s.rvz[e] = 0.f; s.rp1[e] = 0.f; s.rp2[e] = 0.f;
+      svx[e] = 0.f; svz[e] = 0.f;
+      rp[e] = 0.f; rvx[e] = 0.f; rvz[e] = 0.f; rp1[e] = 0.f; rp2[e] = 0.f;
     }
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
   }
@@ -564,8 +633,8 @@ __global__ __launch_bounds__(NTHR) void k_seed_receivers(const ShotDev* __restri
   s.rp[(size_t)s.npml * s.pitch + s.rcol0 + ir] = s.dt * rv;
 }
 
-// taper and stack the shots of a batch into the FP64 images, one image cell
-// per thread, shots in the given order (kernel.cu:1556-1615, rtm_model.cu:462-488)
+// taper and stack shots into the FP64 images, one image cell per thread,
+// shots in the given order (kernel.cu:1556-1615, rtm_model.cu:462-488)
 struct StackShot { const float* g1; const float* illum; const float* taper; int nx1, nxl; };
 
 __global__ __launch_bounds__(NTHR) void k_stack(const StackShot* __restrict__ sh, int nsh, int nx, int nz,
@@ -590,9 +659,9 @@ __global__ __launch_bounds__(NTHR) void k_stack(const StackShot* __restrict__ sh
   img1[e] = a1; img2s[e] = a2; ill[e] = a3;
 }
 
-// final products from the FP64 stacks: fp32 images, the Laplacian-filtered
-// image (laplacefd1d, image_function.cpp:44-79, int spacings) and the
-// multi-shot normalised image (kernel.cu:1580-1590).
+// final products: fp32 images from the FP64 stacks, the Laplacian-filtered
+// image (laplacefd1d, image_function.cpp:44-79, integer spacings) and the
+// multi-shot normalised image (kernel.cu:1580-1590)
 __global__ __launch_bounds__(NTHR) void k_round(const double* __restrict__ a, float* __restrict__ o, long long n)
 {
   const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
@@ -637,24 +706,9 @@ __global__ void k_bump(ShotDev* __restrict__ shots, const int* __restrict__ slot
 }
 
 // =========================================================== launchers ==
-// (host side, same translation unit so the templates are instantiated here)
 #include "rtm_launch.h"
 
 static inline int grid8(int ntiles) { return ((ntiles + 7) / 8) * 8; }
-
-template <int TZ>
-static hipError_t launch_step_t(const LaunchArgs& a, int phase, int save, hipStream_t st)
-{
-  const dim3 g(grid8(a.ntiles)), b(NTHR);
-  if (phase == PH_FWD) {
-    hipLaunchKernelGGL(k_vel_fwd<TZ>, g, b, 0, st, a, save);
-    hipLaunchKernelGGL(k_pres_fwd<TZ>, g, b, 0, st, a, save);
-  } else {
-    hipLaunchKernelGGL(k_vel_rev<TZ>, g, b, 0, st, a);
-    hipLaunchKernelGGL(k_pres_rev<TZ>, g, b, 0, st, a);
-  }
-  return hipGetLastError();
-}
 
 template <int TZ>
 static hipError_t launch_phase_t(const LaunchArgs& a, int phase, int save, int which, hipStream_t st)
@@ -682,12 +736,9 @@ hipError_t rtm_launch_phase(const LaunchArgs& a, int phase, int tz, int save, in
 
 hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st)
 {
-  switch (tz) {
-    case 8: return launch_step_t<8>(a, phase, save, st);
-    case 16: return launch_step_t<16>(a, phase, save, st);
-    case 32: return launch_step_t<32>(a, phase, save, st);
-    default: return hipErrorInvalidValue;
-  }
+  hipError_t e = rtm_launch_phase(a, phase, tz, save, 0, st);
+  if (e != hipSuccess) return e;
+  return rtm_launch_phase(a, phase, tz, save, 1, st);
 }
 
 hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, int nblocks, hipStream_t st)

@@ -39,7 +39,7 @@ class Geom(C.Structure):
 
 class ShotOpts(C.Structure):
     _fields_ = [("literal", C.c_int), ("forward_only", C.c_int), ("nt_limit", C.c_int),
-                ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p)]
+                ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p), ("exact_reverse", C.c_int)]
 
 
 _lib = None
@@ -150,7 +150,7 @@ class ShotResult:
 
 
 def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, literal=False, forward_only=False,
-             nt_limit=0, snap_it=-1, seis_in=None) -> ShotResult:
+             nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False) -> ShotResult:
     g = geometry(P, ishot)
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     rt = np.float64 if f64 else np.float32
@@ -160,7 +160,7 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, literal=False, forwar
     sf = np.zeros((g.nzp, g.nxp), np.float32) if snap_it >= 0 else None
     sr = np.zeros((g.nzp, g.nxp), np.float32) if snap_it >= 0 else None
     o = ShotOpts(literal=int(literal), forward_only=int(forward_only), nt_limit=int(nt_limit),
-                 snap_it=int(snap_it), vvpad=None, rhopad=None)
+                 snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse))
     vp0 = np.ascontiguousarray(vp0, np.float32)
     rho0 = np.ascontiguousarray(rho0, np.float32)
     si = None if seis_in is None else np.ascontiguousarray(seis_in, np.float32)

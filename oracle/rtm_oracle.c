@@ -74,6 +74,9 @@ typedef struct {
     int snap_it;          /* step whose wavefields are returned (-1: none)     */
     const float *vvpad;   /* optional padded v^2 window (nzp*nxp)              */
     const float *rhopad;  /* optional padded density window                    */
+    int exact_reverse;    /* checking aid, not reference semantics: remove the
+                             source at it+1 from interior cells only, so the
+                             reverse pass inverts the forward one */
 } orc_shot_opts;
 
 /* ------------------------------------------------------------------ */

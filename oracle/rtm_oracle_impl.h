@@ -623,7 +623,16 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
         SFX(rev_vel)(&S, &m);
         SFX(strips_io)(&S, store + (size_t)it * slen, 0, 2);
         SFX(rev_pres)(&S, &m);
-        SFX(add_source)(S.p, &m, it, -1);
+        if (o->exact_reverse) {
+            RT wv = m.wav[it + 1];
+            for (int iz = npml; iz < nzp - npml; iz++)
+                for (int ix = npml; ix < nxp - npml; ix++) {
+                    size_t id = (size_t)iz * nxp + ix;
+                    S.p[id] = FMA(-(m.dt * m.src[id]), wv, S.p[id]);
+                }
+        } else {
+            SFX(add_source)(S.p, &m, it, -1);
+        }
         /* receiver wavefield: forward step, Dirichlet insertion */
         SFX(fwd_step)(&Rw, &m, o->literal);
         for (int ir = 0; ir < g.nrec; ir++) Rw.p[rrow + g.rcol0 + ir] = m.dt * rec[(size_t)it * g.nrec + ir];
