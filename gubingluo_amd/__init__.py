@@ -25,6 +25,7 @@ EXPORTED = [
     "rtmh_destroy", "rtmh_shot_geometry", "rtmh_run", "rtmh_model_create", "rtmh_model_destroy",
     "rtmh_run_model", "rtmh_fd_steps", "rtmh_coef_table", "rtmh_read_parfile",
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
+    "rtmh_server_create", "rtmh_server_submit", "rtmh_server_wait", "rtmh_server_query", "rtmh_server_destroy",
 ]
 
 
@@ -45,7 +46,8 @@ class Params(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("batch", C.c_int), ("store_fmt", C.c_int), ("graph_chunk", C.c_int), ("tile_z", C.c_int),
-                ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int)]
+                ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int),
+                ("shot_offsetmax", C.c_void_p)]
 
 
 class Outputs(C.Structure):
@@ -198,7 +200,7 @@ class Context:
 
     def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
             graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
-            want_post=True, seis_in=None, profile=False) -> RunResult:
+            want_post=True, seis_in=None, profile=False, shot_offsetmax=None) -> RunResult:
         """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model."""
         coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
         nz, nx = P.nz, P.nx
@@ -216,7 +218,13 @@ class Context:
         keep = []
         seis = g1s = ils = None
         if want_shots or seis_in is not None:
-            geoms = [geometry(P, s) for s in shots]
+            geoms = []
+            for k, s in enumerate(shots):
+                Pk = P
+                if shot_offsetmax is not None:
+                    Pk = Params.from_buffer_copy(P)
+                    Pk.offsetmax = float(shot_offsetmax[k])
+                geoms.append(geometry(Pk, s))
         if want_shots:
             seis = [np.zeros((P.nt, g.nrec), np.float32) for g in geoms]
             g1s = [np.zeros((nz, g.nxl), np.float32) for g in geoms]
@@ -236,8 +244,10 @@ class Context:
             pa = (C.c_void_p * len(si))(*[_fp(a) for a in si])
             keep.extend([pa, si])
             out.shot_seis_in = C.cast(pa, C.c_void_p)
+        som = None if shot_offsetmax is None else np.ascontiguousarray(shot_offsetmax, np.float32)
+        keep.append(som)
         opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
-                      int(profile))
+                      int(profile), _fp(som))
         st = Stats()
         if model is None:
             rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),
@@ -260,3 +270,65 @@ class Context:
                                  _fp(fields["vz"]), _fp(fields["p1"]), _fp(fields["p2"]))
         if rc != 0:
             raise RuntimeError(f"rtmh_fd_steps failed ({rc}): {last_error()}")
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("mem_budget", C.c_double), ("max_slots", C.c_int), ("tile_z", C.c_int), ("store_fmt", C.c_int)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double),
+                ("cell_updates", C.c_double), ("nshots", C.c_int), ("status", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("ticks", C.c_longlong), ("launches", C.c_longlong), ("peak_slots", C.c_int),
+                ("active_slots", C.c_int), ("queued_requests", C.c_int), ("busy_s", C.c_double),
+                ("cell_updates", C.c_double)]
+
+
+class Server:
+    """Continuous-batching RTM server on one context (rtmh_server_*)."""
+
+    def __init__(self, ctx: Context, *, mem_budget=0.0, max_slots=0, tile_z=0, store_fmt=STORE_F32):
+        self._h = C.c_void_p()
+        o = ServerOptions(float(mem_budget), int(max_slots), int(tile_z), int(store_fmt))
+        if lib().rtmh_server_create(ctx._h, C.byref(o), C.byref(self._h)) != 0:
+            raise RuntimeError(f"rtmh_server_create failed: {last_error()}")
+        self.ctx = ctx
+
+    def submit(self, model: int, P: Params, coef=None) -> int:
+        coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
+        rid = C.c_int(-1)
+        if lib().rtmh_server_submit(self._h, C.c_int(model), C.byref(P), _fp(coef), C.byref(rid)) != 0:
+            raise RuntimeError(f"rtmh_server_submit failed: {last_error()}")
+        return int(rid.value)
+
+    def wait(self, rid: int, P: Params, timeout=0.0):
+        """Block until request rid is done; returns (image1, image2s, illum, image_lap, image_norm, stats)."""
+        nz, nx = P.nz, P.nx
+        ims = [np.zeros((nz, nx), np.float32) for _ in range(5)]
+        out = Outputs(*[_fp(a) for a in ims], None, None, None, None)
+        st = RequestStats()
+        rc = lib().rtmh_server_wait(self._h, C.c_int(rid), C.c_double(timeout), C.byref(out), C.byref(st))
+        if rc == 1:
+            raise TimeoutError(f"request {rid} not done")
+        if rc != 0:
+            raise RuntimeError(f"rtmh_server_wait failed ({rc}): {last_error()}")
+        return (*ims, st)
+
+    def query(self) -> ServerStats:
+        st = ServerStats()
+        lib().rtmh_server_query(self._h, C.byref(st))
+        return st
+
+    def close(self):
+        if self._h:
+            lib().rtmh_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

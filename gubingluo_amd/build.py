@@ -27,7 +27,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
 ]
-LIB_SOURCES = ["rtm_kernels.hip", "rtm_engine.cpp", "rtm_capi.cpp", "host_io.cpp", "host_driver.cpp"]
+LIB_SOURCES = ["rtm_kernels.hip", "rtm_engine.cpp", "rtm_server.cpp", "rtm_capi.cpp", "host_io.cpp",
+               "host_driver.cpp"]
 
 
 def _hipcc() -> str:

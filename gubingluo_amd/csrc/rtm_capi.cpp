@@ -16,6 +16,8 @@ struct rtmh_ctx {
   std::unique_ptr<Engine> eng;
 };
 
+rtm::Engine* rtmh_ctx_engine(rtmh_ctx* ctx) { return ctx->eng.get(); }
+
 extern "C" const char* rtmh_last_error(void) { return rtm::get_error(); }
 extern "C" const char* rtmh_version(void) { return "rtm-mi355x 0.1 (gfx950)"; }
 

@@ -434,10 +434,11 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
 }
 
 // tile table of a batch with the per-tile flags of the cell classes it holds
-static void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles)
+void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles, const std::vector<int>* slot_ids)
 {
   tiles.clear();
   for (int k = 0; k < (int)ds.size(); k++) {
+    const int sid = slot_ids ? (*slot_ids)[k] : k;
     const ShotDev& d = ds[k];
     const int n = d.npml, R = RTM_R;
     for (int z0 = 0; z0 < d.nzp; z0 += tz)
@@ -452,24 +453,27 @@ static void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile
           f |= TF_STRIP;
         if (d.sx1 >= d.sx0 && hit(d.sz0, d.sz1, d.sx0, d.sx1)) f |= TF_SRC;
         if (d.nrec > 0 && hit(n, n, d.rcol0, d.rcol0 + d.nrec - 1)) f |= TF_REC;
-        tiles.push_back(Tile{k, x0, z0, f});
+        tiles.push_back(Tile{sid, x0, z0, f});
       }
   }
 }
 
-static bool params_ok(const rtmh_params& P, std::string& why)
+int params_check(const rtmh_params& P, std::string& why);
+static bool params_ok(const rtmh_params& P, std::string& why) { return params_check(P, why) == 0; }
+
+int params_check(const rtmh_params& P, std::string& why)
 {
-  if (P.nx < 2 || P.nz < 2) { why = "nx, nz must be >= 2"; return false; }
-  if (P.nt < 2) { why = "nt must be >= 2"; return false; }
-  if (P.npml < 10) { why = "npml must be >= 10 (radius-10 strips)"; return false; }
-  if (!(P.dx > 0 && P.dz > 0 && P.dt > 0)) { why = "dx, dz, dt must be > 0"; return false; }
-  if (P.flag_layout < 1 || P.flag_layout > 4) { why = "flag_layout must be 1..4"; return false; }
-  if (P.dshot <= 0) { why = "dshot must be > 0"; return false; }
+  if (P.nx < 2 || P.nz < 2) { why = "nx, nz must be >= 2"; return -1; }
+  if (P.nt < 2) { why = "nt must be >= 2"; return -1; }
+  if (P.npml < 10) { why = "npml must be >= 10 (radius-10 strips)"; return -1; }
+  if (!(P.dx > 0 && P.dz > 0 && P.dt > 0)) { why = "dx, dz, dt must be > 0"; return -1; }
+  if (P.flag_layout < 1 || P.flag_layout > 4) { why = "flag_layout must be 1..4"; return -1; }
+  if (P.dshot <= 0) { why = "dshot must be > 0"; return -1; }
   if (P.medium_flag != 0 || P.ismth != 0) {
     why = "medium_flag / ismth model smoothing is not implemented";
-    return false;
+    return -1;
   }
-  return true;
+  return 0;
 }
 
 int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, int* id)
@@ -497,6 +501,7 @@ int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, in
   HIPCHK(hipMemcpy(m.d_rho, rho0, nm * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m.d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice));
   m.bytes = 2 * nm * 4;
+  std::lock_guard<std::mutex> lk(models_mu_);
   const int k = next_model_++;
   models_[k] = std::move(m);
   *id = k;
@@ -505,6 +510,7 @@ int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, in
 
 int Engine::model_destroy(int id)
 {
+  std::lock_guard<std::mutex> lk(models_mu_);
   auto it = models_.find(id);
   if (it == models_.end()) { set_error("no such model"); return -1; }
   (void)hipFree(it->second.d_rho);
@@ -513,8 +519,11 @@ int Engine::model_destroy(int id)
   return 0;
 }
 
+size_t Engine::taper_offset(const ShotSetup& s) { return small_layout(s).taper; }
+
 const Engine::Model* Engine::model(int id) const
 {
+  std::lock_guard<std::mutex> lk(models_mu_);
   auto it = models_.find(id);
   return it == models_.end() ? nullptr : &it->second;
 }
@@ -558,8 +567,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   std::string why;
   if (!params_ok(P, why)) { set_error(why); return -1; }
   if (opt.store_fmt < 0 || opt.store_fmt > 2) { set_error("store_fmt must be 0, 1 or 2"); return -1; }
-  const int tz = opt.tile_z > 0 ? opt.tile_z : 16;
-  if (tz != 8 && tz != 16 && tz != 32) { set_error("tile_z must be 8, 16 or 32"); return -1; }
+  if (opt.tile_z != 0 && opt.tile_z != 8 && opt.tile_z != 16 && opt.tile_z != 32) {
+    set_error("tile_z must be 0 (autotuned), 8, 16 or 32");
+    return -1;
+  }
+  int tz = opt.tile_z > 0 ? opt.tile_z : 16;
   const Model* M = model(model_id);
   if (!M) { set_error("no such model"); return -1; }
   if (M->nx != P.nx || M->nz != P.nz) { set_error("model size differs from nx, nz"); return -1; }
@@ -575,8 +587,10 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   std::vector<ShotSetup> setup(nsh);
   int nwin = 50;
   for (int k = 0; k < nsh; k++) {
-    if (!shot_geometry(P, shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
-    shot_setup(P, vp0, M->colmax.data(), setup[k]);
+    rtmh_params Pk = P;
+    if (opt.shot_offsetmax) Pk.offsetmax = opt.shot_offsetmax[k];
+    if (!shot_geometry(Pk, shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
+    shot_setup(Pk, vp0, M->colmax.data(), setup[k]);
     while (2 * nwin >= setup[k].g.nxl) nwin -= 3;
     setup[k].nwin = nwin;
     taper_row(setup[k].taper, setup[k].g.nxl, nwin);
@@ -615,6 +629,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   hipEvent_t e0, e1, e2;
   HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2));
   double fwd_ms = 0, rev_ms = 0, cells = 0, algb = 0, step_fwd = 0, step_rev = 0;
+  int last_tz = tz;
   if (stats) memset(stats, 0, sizeof(*stats));
   int nbatches = 0, maxb = 0;
   std::vector<double> img_tmp;  // for temp-image checkpoints
@@ -643,11 +658,28 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       ds[j].record = (P.mode == 0) ? 1 : 0;
     }
     if (rc != 0) { for (void* p : mem) pool_.put(p); break; }
-    std::vector<Tile> tiles;
+    // tile height: given, cached for this geometry class, or tuned on the
+    // first forward steps of this batch (every candidate computes the same
+    // values, so the tuning steps are part of the pass)
+    int maxp = 0, maxz = 0;
+    for (int j = 0; j < nb; j++) { maxp = std::max(maxp, ds[j].nxp); maxz = std::max(maxz, ds[j].nzp); }
+    const long long tkey = ((long long)(maxp / 64) << 40) | ((long long)(maxz / 32) << 20) | nb;
+    const int cands[3] = {8, 16, 32};
+    bool tune = false;
+    if (opt.tile_z == 0) {
+      auto ct = tuned_.find(tkey);
+      if (ct != tuned_.end()) tz = ct->second;
+      else tune = nt >= 3 * kTuneSteps + 2 && !opt.profile;
+    }
+    std::vector<Tile> tiles, tcand[3];
+    size_t tcap = 0;
+    if (tune)
+      for (int c = 0; c < 3; c++) { build_tiles(ds, cands[c], tcand[c]); tcap = std::max(tcap, tcand[c].size()); }
     build_tiles(ds, tz, tiles);
+    tcap = std::max(tcap, tiles.size());
     std::vector<int> slots(nb);
     for (int j = 0; j < nb; j++) slots[j] = j;
-    if (ensure_cb(nb, tiles.size()) != 0) { for (void* p : mem) pool_.put(p); rc = -3; break; }
+    if (ensure_cb(nb, tcap) != 0) { for (void* p : mem) pool_.put(p); rc = -3; break; }
     ShotDev* d_ds = d_ds_;
     Tile* d_tiles = d_tiles_;
     int* d_slots = d_slots_;
@@ -663,8 +695,37 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
 
     // ---- forward pass: propagation + record + boundary strips
     HIPCHK(hipEventRecord(e0, st_));
-    rc = run_pass(PH_FWD, nt, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
-                  (opt.profile && stats) ? stats : nullptr);
+    int fwd_done = 0;
+    if (tune) {
+      hipEvent_t te[4];
+      for (auto& e : te) HIPCHK(hipEventCreate(&e));
+      HIPCHK(hipEventRecord(te[0], st_));
+      for (int c = 0; c < 3; c++) {
+        HIPCHK(hipMemcpyAsync(d_tiles, tcand[c].data(), tcand[c].size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
+        LaunchArgs a{d_ds, d_tiles, (int)tcand[c].size(), 0};
+        for (int s2 = 0; s2 < kTuneSteps; s2++) {
+          a.step = c * kTuneSteps + s2;
+          HIPCHK(rtm_launch_step(a, PH_FWD, cands[c], 1, st_));
+          if (opt.store_fmt != ST_F32) HIPCHK(rtm_launch_quant(d_ds, d_slots, nb, a.step, 128, st_));
+        }
+        HIPCHK(hipEventRecord(te[c + 1], st_));
+      }
+      HIPCHK(rtm_launch_bump(d_ds, d_slots, nb, 3 * kTuneSteps, st_));
+      HIPCHK(hipEventSynchronize(te[3]));
+      float best = 1e30f;
+      for (int c = 0; c < 3; c++) {
+        float m = 0;
+        HIPCHK(hipEventElapsedTime(&m, te[c], te[c + 1]));
+        if (m < best) { best = m; tz = cands[c]; }
+      }
+      for (auto& e : te) (void)hipEventDestroy(e);
+      tuned_[tkey] = tz;
+      fwd_done = 3 * kTuneSteps;
+      build_tiles(ds, tz, tiles);
+      HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
+    }
+    rc = run_pass(PH_FWD, nt - fwd_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt,
+                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr);
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e1, st_));
     HIPCHK(hipStreamSynchronize(st_));
@@ -722,6 +783,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     HIPCHK(hipEventRecord(e1, st_));
     rc = run_pass(PH_REV, nt - 1, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
                   (opt.profile && stats) ? stats : nullptr);
+    last_tz = tz;
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e2, st_));
     HIPCHK(hipStreamSynchronize(st_));
@@ -810,7 +872,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   if (stats) {
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats->fwd_ms = fwd_ms; stats->rev_ms = rev_ms; stats->gpu_ms = fwd_ms + rev_ms;
-    stats->cell_updates = cells; stats->nshots = nsh; stats->batches = nbatches; stats->tile_z = tz;
+    stats->cell_updates = cells; stats->nshots = nsh; stats->batches = nbatches; stats->tile_z = last_tz;
     stats->batch = maxb; stats->hbm_bytes_alg = algb;
     stats->step_bytes_fwd = step_fwd; stats->step_bytes_rev = step_rev;
   }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -94,6 +95,14 @@ class Engine {
 
   // bytes one slot of this geometry needs on the device
   static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt);
+  // build the device descriptor of one slot inside `base` (uploads: host
+  // images of its small arrays to copy to the device)
+  static void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
+                          std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
+                          const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch);
+  // offset of the taper row inside a slot block
+  static size_t taper_offset(const ShotSetup& s);
+  std::mutex& models_mutex() { return models_mu_; }
   static int pitch_of(int nxp) { return ((nxp + 63) / 64) * 64 + 64; }
   static int frows_of(int nzp) { return RTM_MZT + ((nzp + RTM_ZALIGN - 1) / RTM_ZALIGN) * RTM_ZALIGN + 16; }
 
@@ -103,6 +112,9 @@ class Engine {
   DevPool pool_;
   std::map<GraphKey, hipGraphExec_t> graphs_;
   std::map<int, Model> models_;
+  std::map<long long, int> tuned_;      // geometry class -> tile height
+  static constexpr int kTuneSteps = 8;
+  mutable std::mutex models_mu_;
   int next_model_ = 1;
   // persistent control block: captured graphs reference these addresses
   void* cb_ = nullptr;
@@ -114,10 +126,6 @@ class Engine {
   struct StackShotHost* d_stk_ = nullptr;
   int ensure_cb(int nslots, size_t ntiles);
 
-  // build the device descriptor of one slot inside `mem`
-  void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
-                   std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                   const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch);
   int run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
                const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
                rtmh_stats* prof);
@@ -125,6 +133,11 @@ class Engine {
                            int nslots, int store_fmt);
   void drop_graphs();
 };
+
+// tile table of a list of slot descriptors with the per-tile flags
+// (tiles carry the index of the descriptor in `ds` plus `slot_base`)
+void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles, const std::vector<int>* slot_ids = nullptr);
+int params_check(const rtmh_params& P, std::string& why);
 
 // thread-local error text for rtmh_last_error()
 void set_error(const std::string& s);

@@ -29,6 +29,17 @@
 #define LW 83              // 64 + 19: staged width incl. the x halo
 #define NTHR 256
 
+// Pointers read from the descriptor are generic to the compiler, which would
+// then emit flat loads/stores (they count against the LDS wait counter too,
+// so every LDS wait would also wait for outstanding memory loads).  All
+// device-memory accesses go through these global-address-space views.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) int gint;
+typedef __attribute__((address_space(1))) unsigned int guint;
+typedef __attribute__((address_space(1))) unsigned short gushort;
+#define GF(p) ((gfloat*)(p))
+#define GCF(p) ((const gfloat*)(p))
+
 namespace {
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
@@ -49,7 +60,7 @@ __device__ __forceinline__ float model_at(const float* __restrict__ m, const Sho
 {
   const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
   const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
-  return m[(size_t)r * s.mpitch + c];
+  return GCF(m)[(size_t)r * s.mpitch + c];
 }
 
 // stage f rows zs..zs+H-1, columns xs..xs+W-1 (W <= 128) into L[H][W]; all
@@ -63,7 +74,7 @@ __device__ __forceinline__ void stage(float* __restrict__ L, const float* __rest
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int r = ly + 4 * k;
-    const float* src = f + (ptrdiff_t)(zs + imin(r, H - 1)) * pitch + xs;
+    const gfloat* src = GCF(f) + (ptrdiff_t)(zs + imin(r, H - 1)) * pitch + xs;
     a[k] = src[lx];
     b[k] = (W > 64 && lx < W - 64) ? src[64 + lx] : 0.0f;
   }
@@ -128,22 +139,23 @@ __device__ __forceinline__ int strip_block(const ShotDev& s, long long off)
 __device__ __forceinline__ void store_put(const ShotDev& s, int it, long long off, float v)
 {
   if (s.store_fmt == ST_F32) {
-    ((float*)s.store)[(long long)it * s.strip_len + off] = v;
+    GF(s.store)[(long long)it * s.strip_len + off] = v;
   } else {
     // fp32 staging of step it (by parity), quantised by k_quant after the step
-    float* stg = (float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
+    gfloat* stg = GF(s.store) + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
     stg[off] = v;
-    atomicMax(&s.smax[(long long)it * 4 + strip_block(s, off)], __float_as_uint(fabsf(v)));
+    __atomic_fetch_max((guint*)&s.smax[(long long)it * 4 + strip_block(s, off)], __float_as_uint(fabsf(v)),
+                       __ATOMIC_RELAXED);
   }
 }
 
 __device__ __forceinline__ float store_get(const ShotDev& s, int it, long long off)
 {
   const long long k = (long long)it * s.strip_len + off;
-  if (s.store_fmt == ST_F32) return ((const float*)s.store)[k];
-  const float sc = s.sscale[(long long)it * 4 + strip_block(s, off)];
-  if (s.store_fmt == ST_F16) return __half2float(((const __half*)s.store)[k]) * sc;
-  const unsigned short h = ((const unsigned short*)s.store)[k];
+  if (s.store_fmt == ST_F32) return GCF(s.store)[k];
+  const float sc = GCF(s.sscale)[(long long)it * 4 + strip_block(s, off)];
+  if (s.store_fmt == ST_F16) return __half2float(__ushort_as_half(((const gushort*)s.store)[k])) * sc;
+  const unsigned short h = ((const gushort*)s.store)[k];
   return __uint_as_float(((unsigned int)h) << 16) * sc;
 }
 
@@ -154,7 +166,7 @@ __device__ __forceinline__ float store_get(const ShotDev& s, int it, long long o
     if ((r) == RTM_R) {                                                                    \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], (A) - (B), acc); \
     } else {                                                                               \
-      const float* cr_ = (cf) + ((r) - 1) * RTM_R;                                         \
+      const gfloat* cr_ = GCF(cf) + ((r) - 1) * RTM_R;                                     \
       for (int i = 1; i <= (r); i++) acc = __builtin_fmaf(cr_[i - 1], (A) - (B), acc);     \
     }                                                                                      \
   } while (0)
@@ -180,10 +192,10 @@ __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __re
   const float* __restrict__ cf = s.coef;
   float c10[RTM_R];
 #pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = cf[90 + i];
+  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(cf)[90 + i];
   const bool hasvx = ix <= nxp - 2;
   const int rx = imin(RTM_R, imin(ix + 1, nxp - 1 - ix));
-  const float c1x = s.c1xv[ix], c2x = s.c2xv[ix];
+  const float c1x = GCF(s.c1xv)[ix], c2x = GCF(s.c2xv)[ix];
   const bool dampx = !(c1x == 1.0f && c2x == 1.0f);
   float col[RPT + 19];                                 // p(z0+j0-9+k, ix)
 #pragma unroll
@@ -200,7 +212,7 @@ __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __re
       acc = acc * s.dtx;
       acc = acc / rho[r];
       const float v = dampx ? __builtin_fmaf(c1x, vxo[r], -acc) / c2x : vxo[r] - acc;
-      vx[id] = v;
+      GF(vx)[id] = v;
       if (save_it >= 0 && (flags & TF_STRIP)) {
         const long long o = strip_vx(s, iz, ix);
         if (o >= 0) store_put(s, save_it, o, v);
@@ -208,13 +220,13 @@ __device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __re
     }
     if (iz <= nzp - 2) {
       const int rz = imin(RTM_R, imin(iz + 1, nzp - 1 - iz));
-      const float c1z = s.c1zv[iz], c2z = s.c2zv[iz];
+      const float c1z = GCF(s.c1zv)[iz], c2z = GCF(s.c2zv)[iz];
       float acc = 0.0f;
       STENCIL_SUM(acc, rz, c10, cf, col[r + 9 + i], col[r + 10 - i]);
       acc = acc * s.dtz;
       acc = acc / rho[r];
       const float v = (c1z == 1.0f && c2z == 1.0f) ? vzo[r] - acc : __builtin_fmaf(c1z, vzo[r], -acc) / c2z;
-      vz[id] = v;
+      GF(vz)[id] = v;
       if (save_it >= 0 && (flags & TF_STRIP)) {
         const long long o = strip_vz(s, iz, ix);
         if (o >= 0) store_put(s, save_it, o, v);
@@ -239,7 +251,7 @@ __device__ __forceinline__ void vel_rev(const float* __restrict__ L, float* __re
   if (ix >= nxp) return;
   float c10[RTM_R];
 #pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
+  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
   const bool xin_vx = ix >= npml && ix <= nxp - npml - 2;
   const bool xin_vz = ix >= npml && ix < nxp - npml;
   float col[RPT + 19];
@@ -252,7 +264,7 @@ __device__ __forceinline__ void vel_rev(const float* __restrict__ L, float* __re
     const size_t id = (size_t)iz * P + ix;
     const long long ovx = (flags & TF_STRIP) ? strip_vx(s, iz, ix) : -1;
     if (ovx >= 0) {
-      vx[id] = store_get(s, it + 1, ovx);
+      GF(vx)[id] = store_get(s, it + 1, ovx);
     } else if (xin_vx && iz >= npml && iz < nzp - npml) {
       const float* __restrict__ Lr = L + (j0 + r + 9) * LW + lx + 9;
       float acc = 0.0f;
@@ -260,18 +272,18 @@ __device__ __forceinline__ void vel_rev(const float* __restrict__ L, float* __re
       for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
       acc = acc * s.dtx;
       acc = acc / rho[r];
-      vx[id] = vxo[r] + acc;
+      GF(vx)[id] = vxo[r] + acc;
     }
     const long long ovz = (flags & TF_STRIP) ? strip_vz(s, iz, ix) : -1;
     if (ovz >= 0) {
-      vz[id] = store_get(s, it + 1, ovz);
+      GF(vz)[id] = store_get(s, it + 1, ovz);
     } else if (xin_vz && iz >= npml && iz <= nzp - npml - 2) {
       float acc = 0.0f;
 #pragma unroll
       for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], col[r + 9 + i] - col[r + 10 - i], acc);
       acc = acc * s.dtz;
       acc = acc / rho[r];
-      vz[id] = vzo[r] + acc;
+      GF(vz)[id] = vzo[r] + acc;
     }
   }
 }
@@ -293,7 +305,7 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
       sz = __builtin_fmaf(c10[i - 1], col[r + 9 + i] - col[r + 10 - i], sz);
     }
   } else {
-    const float* c = cf + (rad - 1) * RTM_R;
+    const gfloat* c = GCF(cf) + (rad - 1) * RTM_R;
     for (int i = 1; i <= rad; i++) {
       sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
       sz = __builtin_fmaf(c[i - 1], col[r + 9 + i] - col[r + 10 - i], sz);
@@ -311,10 +323,10 @@ __device__ __forceinline__ float pres_cell(float p, float* __restrict__ p1, floa
                                            bool inner, const DivV& d, float c1x, float c2x, float c1z, float c2z)
 {
   if (inner) return p - (d.dxv + d.dzv);
-  float q1 = p1[id], q2 = p2[id];
+  float q1 = GCF(p1)[id], q2 = GCF(p2)[id];
   q1 = __builtin_fmaf(c1x, q1, -d.dxv) / c2x;
   q2 = __builtin_fmaf(c1z, q2, -d.dzv) / c2z;
-  p1[id] = q1; p2[id] = q2;
+  GF(p1)[id] = q1; GF(p2)[id] = q2;
   return q1 + q2;
 }
 
@@ -348,8 +360,8 @@ __device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const
   if (ix >= nxp) return;
   float c10[RTM_R];
 #pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
-  const float c1x = s.c1x[ix], c2x = s.c2x[ix];
+  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
+  const float c1x = GCF(s.c1x)[ix], c2x = GCF(s.c2x)[ix];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
   float col[RPT + 19];
@@ -364,14 +376,14 @@ __device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const
       const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
       const DivV d = div_v<RPT>(VX + (j0 + r) * LW + lx + 10, col, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
       const bool inner = xin && iz >= npml && iz < nzp - npml;
-      p = pres_cell(p, s.sp1, s.sp2, id, inner, d, c1x, c2x, s.c1z[iz], s.c2z[iz]);
+      p = pres_cell(p, s.sp1, s.sp2, id, inner, d, c1x, c2x, GCF(s.c1z)[iz], GCF(s.c2z)[iz]);
     }
     if ((flags & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && ix >= s.sx0 && ix <= s.sx1)
-      p = __builtin_fmaf(s.sa[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], s.wav[it], p);
-    s.sp[id] = p;
+      p = __builtin_fmaf(GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
+    GF(s.sp)[id] = p;
     if ((flags & TF_REC) && s.record && iz == npml) {
       const int ir = ix - s.rcol0;
-      if (ir >= 0 && ir < s.nrec) s.rec[(size_t)it * s.nrec + ir] = p;
+      if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
     }
     if (save && (flags & TF_STRIP)) {
       const long long o = strip_p(s, iz, ix);
@@ -398,8 +410,8 @@ __device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const flo
   if (ix >= nxp) return;
   float c10[RTM_R];
 #pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = s.coef[90 + i];
-  const float c1x = s.c1x[ix], c2x = s.c2x[ix];
+  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
+  const float c1x = GCF(s.c1x)[ix], c2x = GCF(s.c2x)[ix];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
   const bool xsrc = (flags & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
@@ -427,8 +439,8 @@ __device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const flo
       }
       if (wr) {
         if (xsrc && iz >= s.sz0 && iz <= s.sz1)
-          ps = __builtin_fmaf(-s.sa[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], s.wav[it], ps);
-        s.sp[id] = ps;
+          ps = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], ps);
+        GF(s.sp)[id] = ps;
       }
     }
     // receiver wavefield
@@ -436,20 +448,20 @@ __device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const flo
     if (!xring && iz > 0 && iz < nzp - 1) {
       const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
       const DivV d = div_v<RPT>(RX + (j0 + r) * LW + lx + 10, cr, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
-      p = pres_cell(p, s.rp1, s.rp2, id, inner, d, c1x, c2x, s.c1z[iz], s.c2z[iz]);
+      p = pres_cell(p, s.rp1, s.rp2, id, inner, d, c1x, c2x, GCF(s.c1z)[iz], GCF(s.c2z)[iz]);
     }
     if ((flags & TF_REC) && iz == npml) {
       const int ir = ix - s.rcol0;
       if (ir >= 0 && ir < s.nrec) {
-        const float rv = (it <= s.cut[ir]) ? 0.0f : s.rec[(size_t)it * s.nrec + ir];
+        const float rv = (it <= ((const gint*)s.cut)[ir]) ? 0.0f : GCF(s.rec)[(size_t)it * s.nrec + ir];
         p = s.dt * rv;
       }
     }
-    s.rp[id] = p;
+    GF(s.rp)[id] = p;
     if (inner) {
       const size_t ii = (size_t)(iz - npml) * s.nxl + ix - npml;
-      s.g1[ii] = __builtin_fmaf(ps, p, s.g1[ii]);
-      s.illum[ii] = __builtin_fmaf(ps, ps, s.illum[ii]);
+      GF(s.g1)[ii] = __builtin_fmaf(ps, p, GCF(s.g1)[ii]);
+      GF(s.illum)[ii] = __builtin_fmaf(ps, ps, GCF(s.illum)[ii]);
     }
   }
 }
@@ -462,7 +474,7 @@ __device__ __forceinline__ void own_load(float (&o)[RPT], const float* __restric
 {
   const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
 #pragma unroll
-  for (int r = 0; r < RPT; r++) o[r] = f[(ptrdiff_t)(z0 + j0 + r) * s.pitch + x0 + lx];
+  for (int r = 0; r < RPT; r++) o[r] = GCF(f)[(ptrdiff_t)(z0 + j0 + r) * s.pitch + x0 + lx];
 }
 template <int RPT>
 __device__ __forceinline__ void own_model(float (&o)[RPT], const float* __restrict__ m, const ShotDev& s, int x0, int z0)
@@ -563,7 +575,7 @@ __global__ __launch_bounds__(NTHR) void k_pres_rev(LaunchArgs a)
 }
 
 // 16-bit boundary store: quantise the fp32 staging of step `it` of each slot
-// with the max-abs of its block: q = x / scale, scale = maxabs / 32768.
+// with the max-abs of its block: q = x / scale, scale = max(maxabs / 32768, FLT_MIN).
 __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
                                                  int nslots, int step)
 {
@@ -574,8 +586,10 @@ __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shot
     float sc[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) {
+      // a denormal max-abs (source tails before the wave arrives) must not
+      // underflow the scale to zero: |x| <= 32768 * FLT_MIN then anyway
       const float m = __uint_as_float(s.smax[(long long)it * 4 + b]);
-      sc[b] = m > 0.0f ? m / 32768.0f : 1.0f;
+      sc[b] = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
       for (int b = 0; b < 4; b++) s.sscale[(long long)it * 4 + b] = sc[b];

@@ -1,0 +1,607 @@
+// rtm_server.cpp -- the served form: continuous batching of RTM requests on
+// one GPU (rtmh_server_* in include/rtm_hip.h).
+//
+// A worker thread owns a stream and a slot table.  Each tick launches one
+// time step for every resident shot: the forward-phase shots in one
+// velocity + pressure launch pair, the reverse-phase shots in another, each
+// shot at its own time index (it = it_off + it_dir * tick).  Phase changes
+// and completions are known in advance from the tick count, so the host
+// never waits on the GPU to schedule; it only polls the events that close
+// requests.  Queued shots are admitted in arrival order whenever a slot and
+// budget are free (per-shot bytes are known before admission), so requests
+// that arrive while others run start at the next tick.  A finished shot's
+// image and illumination move to a small pending buffer, its slot is
+// released, and each request stacks its shots in job order into its own
+// FP64 images -- a request's result is the same alone or in any mix.
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/rtm_hip.h"
+#include "rtm_engine.h"
+#include "rtm_launch.h"
+
+namespace rtm {
+
+static inline size_t a256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+struct SrvRequest {
+  int id = 0, model_id = 0;
+  rtmh_params P;
+  std::vector<float> coef;
+  std::vector<ShotSetup> setup;
+  std::vector<int> shots;
+  int next_admit = 0, next_stack = 0;
+  std::vector<float*> pend_g1, pend_il;   // finished shots waiting to be stacked
+  std::vector<char> finished;
+  char* dev = nullptr;                     // coef, wavelet, tapers, stack entries, stacks, outputs
+  size_t dev_bytes = 0;
+  float *d_coef = nullptr, *d_wav = nullptr, *d_taper = nullptr;
+  StackShotHost* d_stk = nullptr;
+  double *d_i1 = nullptr, *d_i2 = nullptr, *d_il = nullptr;
+  float *d_f1 = nullptr, *d_f2 = nullptr, *d_f3 = nullptr, *d_lap = nullptr, *d_nrm = nullptr;
+  std::vector<size_t> taper_off;
+  std::vector<float> image1, image2s, illum, image_lap, image_norm;
+  std::vector<std::vector<float>> stage;   // host images uploaded asynchronously (kept alive)
+  double t_submit = 0, t_start = -1, t_done = -1, cells = 0;
+  hipEvent_t done_ev = nullptr;
+  bool finalizing = false, complete = false;
+  int status = 0;
+};
+
+struct SrvSlot {
+  int req = -1, k = -1, phase = PH_IDLE;
+  int steps_left = 0;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  std::vector<std::pair<void*, std::vector<char>>> uploads;
+};
+
+class Server {
+ public:
+  Server(Engine* eng, const rtmh_server_options& o);
+  ~Server();
+  int start();
+  int submit(int model_id, const rtmh_params& P, const float* coef, int* id);
+  int wait(int id, double timeout, rtmh_outputs* out, rtmh_request_stats* st);
+  void query(rtmh_server_stats* st);
+
+ private:
+  void loop();
+  double now() const
+  {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+  }
+  bool admit_one(SrvRequest& r);           // under mu_
+  int request_device(SrvRequest& r);       // under mu_
+  void transition(int sid);
+  void finish_shot(int sid);
+  void stack_ready(SrvRequest& r);
+  void finalize(SrvRequest& r);
+  void rebuild_tiles();
+  void fail(const char* what, hipError_t e);
+
+  Engine* eng_;
+  rtmh_server_options opt_;
+  hipStream_t st_ = nullptr;
+  DevPool pool_;
+  std::thread th_;
+  std::mutex mu_;
+  std::condition_variable cv_, cv_done_;
+  std::deque<int> queue_;
+  std::map<int, std::unique_ptr<SrvRequest>> reqs_;
+  std::vector<SrvSlot> slots_;
+  std::vector<ShotDev> hds_;
+  ShotDev* d_ds_ = nullptr;
+  Tile *d_tf_ = nullptr, *d_tr_ = nullptr;
+  int *d_qs_ = nullptr;
+  size_t tile_cap_ = 0;
+  int ntf_ = 0, ntr_ = 0, nq_ = 0;
+  std::vector<Tile> tf_, tr_;
+  std::vector<int> qslots_;
+  bool dirty_ = true, stop_ = false;
+  double budget_ = 0, used_ = 0;
+  long long tick_ = 0, launches_ = 0;
+  int next_id_ = 1, peak_ = 0, active_ = 0;
+  double busy_ = 0, cells_done_ = 0;
+  int err_ = 0;
+  std::string err_msg_;
+  std::chrono::steady_clock::time_point t0_;
+};
+
+Server::Server(Engine* eng, const rtmh_server_options& o) : eng_(eng), opt_(o)
+{
+  if (opt_.max_slots <= 0) opt_.max_slots = 256;
+  if (opt_.tile_z <= 0) opt_.tile_z = 16;
+  t0_ = std::chrono::steady_clock::now();
+}
+
+int Server::start()
+{
+  if (hipSetDevice(eng_->device()) != hipSuccess) { set_error("hipSetDevice failed"); return -10; }
+  if (hipStreamCreateWithFlags(&st_, hipStreamNonBlocking) != hipSuccess) { set_error("stream"); return -10; }
+  size_t fr = 0, tot = 0;
+  (void)hipMemGetInfo(&fr, &tot);
+  budget_ = opt_.mem_budget > 0 ? opt_.mem_budget : 0.8 * (double)fr;
+  slots_.assign(opt_.max_slots, SrvSlot());
+  hds_.assign(opt_.max_slots, ShotDev());
+  if (hipMalloc(&d_ds_, sizeof(ShotDev) * opt_.max_slots) != hipSuccess ||
+      hipMalloc(&d_qs_, sizeof(int) * opt_.max_slots) != hipSuccess) {
+    set_error("out of device memory (server control)");
+    return -3;
+  }
+  th_ = std::thread([this] { loop(); });
+  return 0;
+}
+
+Server::~Server()
+{
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  if (th_.joinable()) th_.join();
+  if (st_) (void)hipStreamSynchronize(st_);
+  for (auto& kv : reqs_) {
+    if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
+    if (kv.second->dev) pool_.put(kv.second->dev);
+  }
+  for (auto& s : slots_)
+    if (s.mem) pool_.put(s.mem);
+  (void)hipFree(d_ds_); (void)hipFree(d_qs_); (void)hipFree(d_tf_); (void)hipFree(d_tr_);
+  pool_.trim();
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Server::fail(const char* what, hipError_t e)
+{
+  err_ = -10;
+  err_msg_ = std::string(what) + ": " + hipGetErrorString(e);
+}
+
+int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* id)
+{
+  std::string why;
+  if (params_check(P, why) != 0) { set_error(why); return -1; }
+  if (P.mode != 0) { set_error("served requests model their data (mode 0)"); return -1; }
+  const Engine::Model* M = eng_->model(model_id);
+  if (!M || M->nx != P.nx || M->nz != P.nz) { set_error("no such model or size mismatch"); return -1; }
+  auto r = std::make_unique<SrvRequest>();
+  r->model_id = model_id;
+  r->P = P;
+  r->coef.assign(coef, coef + 100);
+  for (int is = P.startshot; is <= P.nshot; is += P.dshot) r->shots.push_back(is);
+  const int nsh = (int)r->shots.size();
+  if (nsh == 0) { set_error("no shots"); return -1; }
+  r->setup.resize(nsh);
+  int nwin = 50;
+  for (int k = 0; k < nsh; k++) {
+    if (!shot_geometry(P, r->shots[k], r->setup[k])) { set_error("shot window outside the model"); return -2; }
+    shot_setup(P, M->vp0.data(), M->colmax.data(), r->setup[k]);
+    while (2 * nwin >= r->setup[k].g.nxl) nwin -= 3;
+    r->setup[k].nwin = nwin;
+    taper_row(r->setup[k].taper, r->setup[k].g.nxl, nwin);
+    const double nxz = (double)r->setup[k].g.nxp * r->setup[k].g.nzp;
+    r->cells += 2.0 * P.nt * nxz + (double)(P.nt - 1) * P.nz * r->setup[k].g.nxl;
+  }
+  r->pend_g1.assign(nsh, nullptr);
+  r->pend_il.assign(nsh, nullptr);
+  r->finished.assign(nsh, 0);
+  std::lock_guard<std::mutex> lk(mu_);
+  if (err_) { set_error(err_msg_); return err_; }
+  r->id = next_id_++;
+  r->t_submit = now();
+  *id = r->id;
+  queue_.push_back(r->id);
+  reqs_[r->id] = std::move(r);
+  cv_.notify_all();
+  return 0;
+}
+
+// request-level device block: coefficients, wavelet, tapers, stack entries,
+// FP64 stacks and fp32 outputs
+int Server::request_device(SrvRequest& r)
+{
+  const rtmh_params& P = r.P;
+  const size_t nm = (size_t)P.nx * P.nz, nsh = r.shots.size();
+  size_t o = 0;
+  const size_t oc = o; o += a256(400);
+  const size_t ow = o; o += a256((size_t)P.nt * 4);
+  r.taper_off.resize(nsh);
+  const size_t ot = o;
+  for (size_t k = 0; k < nsh; k++) { r.taper_off[k] = o - ot; o += a256(r.setup[k].taper.size() * 4); }
+  const size_t os = o; o += a256(nsh * sizeof(StackShotHost));
+  const size_t oi = o; o += 3 * a256(nm * 8);
+  const size_t of = o; o += 5 * a256(nm * 4);
+  r.dev_bytes = o;
+  r.dev = (char*)pool_.get(o);
+  if (!r.dev) return -3;
+  r.d_coef = (float*)(r.dev + oc);
+  r.d_wav = (float*)(r.dev + ow);
+  r.d_taper = (float*)(r.dev + ot);
+  r.d_stk = (StackShotHost*)(r.dev + os);
+  r.d_i1 = (double*)(r.dev + oi);
+  r.d_i2 = (double*)(r.dev + oi + a256(nm * 8));
+  r.d_il = (double*)(r.dev + oi + 2 * a256(nm * 8));
+  float* f = (float*)(r.dev + of);
+  r.d_f1 = f; r.d_f2 = (float*)((char*)f + a256(nm * 4)); r.d_f3 = (float*)((char*)f + 2 * a256(nm * 4));
+  r.d_lap = (float*)((char*)f + 3 * a256(nm * 4)); r.d_nrm = (float*)((char*)f + 4 * a256(nm * 4));
+  std::vector<float> wav;
+  ricker(wav, P.nt, P.fdom, P.dt);
+  r.stage.push_back(r.coef);
+  r.stage.push_back(wav);
+  std::vector<float> tap((o - ot) / 4 + 1, 0.0f);
+  for (size_t k = 0; k < nsh; k++)
+    memcpy(tap.data() + r.taper_off[k] / 4, r.setup[k].taper.data(), r.setup[k].taper.size() * 4);
+  r.stage.push_back(tap);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(r.d_coef, r.stage[0].data(), 400, hipMemcpyHostToDevice, st_)) != hipSuccess ||
+      (e = hipMemcpyAsync(r.d_wav, r.stage[1].data(), (size_t)P.nt * 4, hipMemcpyHostToDevice, st_)) != hipSuccess ||
+      (e = hipMemcpyAsync(r.d_taper, r.stage[2].data(), (os - ot), hipMemcpyHostToDevice, st_)) != hipSuccess) {
+    fail("request upload", e);
+    return -10;
+  }
+  (void)rtm_launch_zero_f64(r.d_i1, (long long)nm, st_);
+  (void)rtm_launch_zero_f64(r.d_i2, (long long)nm, st_);
+  (void)rtm_launch_zero_f64(r.d_il, (long long)nm, st_);
+  return 0;
+}
+
+bool Server::admit_one(SrvRequest& r)
+{
+  int sid = -1;
+  for (int i = 0; i < (int)slots_.size(); i++)
+    if (slots_[i].phase == PH_IDLE && !slots_[i].mem) { sid = i; break; }
+  if (sid < 0) return false;
+  const int k = r.next_admit;
+  const ShotSetup& s = r.setup[k];
+  const size_t need = Engine::slot_bytes(s, r.P.nt, opt_.store_fmt);
+  size_t extra = 0;
+  if (!r.dev) {
+    const size_t nm = (size_t)r.P.nx * r.P.nz;
+    extra = 3 * nm * 8 + 5 * nm * 4 + (size_t)r.P.nt * 4;
+  }
+  // pending (finished, not yet stacked) buffers of a request are bounded by
+  // its resident shots; count them with the slot
+  const size_t pend = 2 * a256((size_t)r.P.nz * s.g.nxl * 4);
+  if (used_ + need + extra + pend > budget_ && active_ > 0) return false;
+  if (!r.dev) {
+    if (request_device(r) != 0) return false;
+    used_ += r.dev_bytes;
+    r.t_start = now();
+  }
+  SrvSlot& sl = slots_[sid];
+  sl.mem = pool_.get(need);
+  if (!sl.mem) return false;
+  sl.bytes = need + pend;
+  used_ += sl.bytes;
+  sl.req = r.id;
+  sl.k = k;
+  sl.phase = PH_FWD;
+  sl.steps_left = r.P.nt;
+  sl.uploads.clear();
+  const Engine::Model* M = eng_->model(r.model_id);
+  ShotDev& d = hds_[sid];
+  Engine::layout_slot(r.P, s, opt_.store_fmt, (char*)sl.mem, d, sl.uploads, r.d_coef, r.d_wav, M->d_rho, M->d_vv,
+                      r.P.nx);
+  d.record = 1;
+  d.phase = PH_FWD;
+  d.it_off = (int)(-tick_);
+  d.it_dir = 1;
+  hipError_t e;
+  for (auto& u : sl.uploads)
+    if ((e = hipMemcpyAsync(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice, st_)) != hipSuccess)
+      fail("slot upload", e);
+  if ((e = hipMemcpyAsync(d_ds_ + sid, &hds_[sid], sizeof(ShotDev), hipMemcpyHostToDevice, st_)) != hipSuccess)
+    fail("descriptor upload", e);
+  (void)rtm_launch_phase_init(d_ds_, sid, PH_FWD, 256, st_);
+  r.next_admit++;
+  active_++;
+  peak_ = active_ > peak_ ? active_ : peak_;
+  dirty_ = true;
+  return true;
+}
+
+void Server::rebuild_tiles()
+{
+  std::vector<ShotDev> fw, rv;
+  std::vector<int> fid, rid;
+  qslots_.clear();
+  for (int i = 0; i < (int)slots_.size(); i++) {
+    if (slots_[i].phase == PH_FWD) {
+      fw.push_back(hds_[i]); fid.push_back(i);
+      if (opt_.store_fmt != ST_F32) qslots_.push_back(i);
+    } else if (slots_[i].phase == PH_REV) {
+      rv.push_back(hds_[i]); rid.push_back(i);
+    }
+  }
+  build_tiles(fw, opt_.tile_z, tf_, &fid);
+  build_tiles(rv, opt_.tile_z, tr_, &rid);
+  const size_t need = tf_.size() > tr_.size() ? tf_.size() : tr_.size();
+  if (need > tile_cap_) {
+    (void)hipStreamSynchronize(st_);
+    (void)hipFree(d_tf_); (void)hipFree(d_tr_);
+    tile_cap_ = need * 2 + 1024;
+    if (hipMalloc(&d_tf_, tile_cap_ * sizeof(Tile)) != hipSuccess ||
+        hipMalloc(&d_tr_, tile_cap_ * sizeof(Tile)) != hipSuccess) {
+      err_ = -3; err_msg_ = "out of device memory (tiles)";
+      return;
+    }
+  }
+  // the previous tables may still be read by launches in flight: the copies
+  // are stream-ordered behind them
+  hipError_t e;
+  if (!tf_.empty() && (e = hipMemcpyAsync(d_tf_, tf_.data(), tf_.size() * sizeof(Tile), hipMemcpyHostToDevice, st_)) != hipSuccess)
+    fail("tile upload", e);
+  if (!tr_.empty() && (e = hipMemcpyAsync(d_tr_, tr_.data(), tr_.size() * sizeof(Tile), hipMemcpyHostToDevice, st_)) != hipSuccess)
+    fail("tile upload", e);
+  if (!qslots_.empty() && (e = hipMemcpyAsync(d_qs_, qslots_.data(), qslots_.size() * 4, hipMemcpyHostToDevice, st_)) != hipSuccess)
+    fail("slot list upload", e);
+  (void)hipStreamSynchronize(st_);   // host vectors are reused next rebuild
+  ntf_ = (int)tf_.size();
+  ntr_ = (int)tr_.size();
+  nq_ = (int)qslots_.size();
+  dirty_ = false;
+}
+
+void Server::transition(int sid)
+{
+  SrvSlot& sl = slots_[sid];
+  SrvRequest& r = *reqs_[sl.req];
+  if (sl.phase == PH_FWD) {
+    ShotDev& d = hds_[sid];
+    d.phase = PH_REV;
+    d.it_off = (int)(r.P.nt - 2 + tick_);      // it = nt-2 - (tick - tick_start)
+    d.it_dir = -1;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(d_ds_ + sid, &hds_[sid], sizeof(ShotDev), hipMemcpyHostToDevice, st_)) != hipSuccess)
+      fail("descriptor upload", e);
+    (void)rtm_launch_phase_init(d_ds_, sid, PH_REV, 256, st_);
+    (void)rtm_launch_seed(d_ds_, sid, d.nrec, st_);
+    sl.phase = PH_REV;
+    sl.steps_left = r.P.nt - 1;
+    dirty_ = true;
+  } else {
+    finish_shot(sid);
+  }
+}
+
+void Server::finish_shot(int sid)
+{
+  SrvSlot& sl = slots_[sid];
+  SrvRequest& r = *reqs_[sl.req];
+  const int k = sl.k;
+  const size_t n = (size_t)r.P.nz * r.setup[k].g.nxl * 4;
+  float* g = (float*)pool_.get(2 * a256(n));
+  if (!g) { err_ = -3; err_msg_ = "out of device memory (pending images)"; return; }
+  r.pend_g1[k] = g;
+  r.pend_il[k] = (float*)((char*)g + a256(n));
+  hipError_t e;
+  if ((e = hipMemcpyAsync(r.pend_g1[k], hds_[sid].g1, n, hipMemcpyDeviceToDevice, st_)) != hipSuccess ||
+      (e = hipMemcpyAsync(r.pend_il[k], hds_[sid].illum, n, hipMemcpyDeviceToDevice, st_)) != hipSuccess)
+    fail("pending copy", e);
+  r.finished[k] = 1;
+  pool_.put(sl.mem);
+  used_ -= sl.bytes;
+  used_ += 2 * a256(n);
+  sl = SrvSlot();
+  hds_[sid].phase = PH_IDLE;
+  active_--;
+  dirty_ = true;
+  stack_ready(r);
+}
+
+void Server::stack_ready(SrvRequest& r)
+{
+  const int nsh = (int)r.shots.size();
+  while (r.next_stack < nsh && r.finished[r.next_stack]) {
+    const int k = r.next_stack;
+    StackShotHost h{r.pend_g1[k], r.pend_il[k], r.d_taper + r.taper_off[k] / 4, r.setup[k].g.nx1, r.setup[k].g.nxl};
+    hipError_t e;
+    r.stage.push_back(std::vector<float>(sizeof(StackShotHost) / 4 + 1));
+    memcpy(r.stage.back().data(), &h, sizeof(h));
+    if ((e = hipMemcpyAsync(r.d_stk + k, r.stage.back().data(), sizeof(h), hipMemcpyHostToDevice, st_)) != hipSuccess)
+      fail("stack entry", e);
+    (void)rtm_launch_stack(r.d_stk + k, 1, r.P.nx, r.P.nz, r.d_i1, r.d_i2, r.d_il, st_);
+    const size_t n = (size_t)r.P.nz * r.setup[k].g.nxl * 4;
+    pool_.put(r.pend_g1[k]);          // stream-ordered behind the stacking kernel
+    used_ -= 2 * a256(n);
+    r.pend_g1[k] = r.pend_il[k] = nullptr;
+    r.next_stack++;
+  }
+  if (r.next_stack == nsh && !r.finalizing) finalize(r);
+}
+
+void Server::finalize(SrvRequest& r)
+{
+  const rtmh_params& P = r.P;
+  const size_t nm = (size_t)P.nx * P.nz;
+  (void)rtm_launch_round(r.d_i1, r.d_f1, (long long)nm, st_);
+  (void)rtm_launch_round(r.d_i2, r.d_f2, (long long)nm, st_);
+  (void)rtm_launch_round(r.d_il, r.d_f3, (long long)nm, st_);
+  (void)rtm_launch_laplace(r.d_f1, r.d_lap, P.nx, P.nz, (int)P.dx, (int)P.dz, st_);
+  (void)rtm_launch_norm_multi(r.d_f1, r.d_f3, r.d_nrm, (long long)nm, st_);
+  r.image1.resize(nm); r.image2s.resize(nm); r.illum.resize(nm); r.image_lap.resize(nm); r.image_norm.resize(nm);
+  float* dst[5] = {r.image1.data(), r.image2s.data(), r.illum.data(), r.image_lap.data(), r.image_norm.data()};
+  float* src[5] = {r.d_f1, r.d_f2, r.d_f3, r.d_lap, r.d_nrm};
+  hipError_t e;
+  for (int i = 0; i < 5; i++)
+    if ((e = hipMemcpyAsync(dst[i], src[i], nm * 4, hipMemcpyDeviceToHost, st_)) != hipSuccess) fail("result copy", e);
+  (void)hipEventCreateWithFlags(&r.done_ev, hipEventDisableTiming);
+  (void)hipEventRecord(r.done_ev, st_);
+  r.finalizing = true;
+}
+
+void Server::loop()
+{
+  (void)hipSetDevice(eng_->device());
+  std::deque<std::pair<long long, hipEvent_t>> inflight;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      // admission: first come first served over shots
+      while (!queue_.empty() && !err_) {
+        SrvRequest& r = *reqs_[queue_.front()];
+        if (r.next_admit >= (int)r.shots.size()) { queue_.pop_front(); continue; }
+        if (!admit_one(r)) break;
+      }
+      // close requests whose results have landed
+      for (auto& kv : reqs_) {
+        SrvRequest& r = *kv.second;
+        if (r.finalizing && !r.complete && hipEventQuery(r.done_ev) == hipSuccess) {
+          r.complete = true;
+          r.t_done = now();
+          r.stage.clear();
+          pool_.put(r.dev);
+          r.dev = nullptr;
+          used_ -= r.dev_bytes;
+          cells_done_ += r.cells;
+          cv_done_.notify_all();
+        }
+      }
+      bool pending_final = false;
+      for (auto& kv : reqs_) pending_final |= (kv.second->finalizing && !kv.second->complete);
+      if (stop_) break;
+      if (active_ == 0) {
+        if (pending_final) {
+          lk.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(200));
+          continue;
+        }
+        cv_.wait_for(lk, std::chrono::milliseconds(20));
+        continue;
+      }
+    }
+    const double tb = now();
+    if (dirty_) rebuild_tiles();
+    if (err_) { cv_done_.notify_all(); std::this_thread::sleep_for(std::chrono::milliseconds(5)); continue; }
+    if (ntf_ > 0) {
+      LaunchArgs a{d_ds_, d_tf_, ntf_, (int)tick_};
+      if (rtm_launch_step(a, PH_FWD, opt_.tile_z, 1, st_) != hipSuccess) fail("forward launch", hipGetLastError());
+      if (nq_ > 0) (void)rtm_launch_quant(d_ds_, d_qs_, nq_, (int)tick_, 128, st_);
+      launches_ += 2;
+    }
+    if (ntr_ > 0) {
+      LaunchArgs a{d_ds_, d_tr_, ntr_, (int)tick_};
+      if (rtm_launch_step(a, PH_REV, opt_.tile_z, 0, st_) != hipSuccess) fail("reverse launch", hipGetLastError());
+      launches_ += 2;
+    }
+    tick_++;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (int i = 0; i < (int)slots_.size(); i++) {
+        if (slots_[i].phase == PH_IDLE) continue;
+        if (--slots_[i].steps_left == 0) transition(i);
+      }
+    }
+    // bound how far the host runs ahead of the GPU
+    if ((tick_ & 7) == 0) {
+      hipEvent_t ev;
+      (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      (void)hipEventRecord(ev, st_);
+      inflight.push_back({tick_, ev});
+      while (inflight.size() > 4) {
+        (void)hipEventSynchronize(inflight.front().second);
+        (void)hipEventDestroy(inflight.front().second);
+        inflight.pop_front();
+      }
+    }
+    busy_ += now() - tb;
+  }
+  for (auto& p : inflight) { (void)hipEventSynchronize(p.second); (void)hipEventDestroy(p.second); }
+}
+
+int Server::wait(int id, double timeout, rtmh_outputs* out, rtmh_request_stats* st)
+{
+  std::unique_lock<std::mutex> lk(mu_);
+  auto it = reqs_.find(id);
+  if (it == reqs_.end()) { set_error("no such request"); return -1; }
+  SrvRequest& r = *it->second;
+  const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout > 0 ? timeout : 1e9);
+  while (!r.complete && !err_) {
+    if (cv_done_.wait_until(lk, until) == std::cv_status::timeout && !r.complete) return 1;
+  }
+  if (err_ && !r.complete) { set_error(err_msg_); return err_; }
+  if (out) {
+    const size_t nm = (size_t)r.P.nx * r.P.nz * 4;
+    if (out->image1) memcpy(out->image1, r.image1.data(), nm);
+    if (out->image2s) memcpy(out->image2s, r.image2s.data(), nm);
+    if (out->illum) memcpy(out->illum, r.illum.data(), nm);
+    if (out->image_lap) memcpy(out->image_lap, r.image_lap.data(), nm);
+    if (out->image_norm) memcpy(out->image_norm, r.image_norm.data(), nm);
+  }
+  if (st) {
+    st->submit_s = r.t_submit; st->start_s = r.t_start; st->done_s = r.t_done;
+    st->cell_updates = r.cells; st->nshots = (int)r.shots.size(); st->status = r.status;
+  }
+  return 0;
+}
+
+void Server::query(rtmh_server_stats* st)
+{
+  std::lock_guard<std::mutex> lk(mu_);
+  st->ticks = tick_; st->launches = launches_; st->peak_slots = peak_; st->active_slots = active_;
+  st->queued_requests = (int)queue_.size(); st->busy_s = busy_; st->cell_updates = cells_done_;
+}
+
+}  // namespace rtm
+
+// ------------------------------------------------------------------ C ABI --
+struct rtmh_server {
+  std::unique_ptr<rtm::Server> s;
+};
+
+// the engine of a context (rtm_capi.cpp)
+rtm::Engine* rtmh_ctx_engine(rtmh_ctx* ctx);
+
+extern "C" int rtmh_server_create(rtmh_ctx* ctx, const rtmh_server_options* opt, rtmh_server** out)
+{
+  if (!ctx || !out) { rtm::set_error("null argument"); return -1; }
+  rtmh_server_options o;
+  memset(&o, 0, sizeof(o));
+  if (opt) o = *opt;
+  if (o.store_fmt < 0 || o.store_fmt > 2) { rtm::set_error("store_fmt must be 0, 1 or 2"); return -1; }
+  if (o.tile_z != 0 && o.tile_z != 8 && o.tile_z != 16 && o.tile_z != 32) { rtm::set_error("tile_z"); return -1; }
+  auto* h = new rtmh_server;
+  h->s.reset(new rtm::Server(rtmh_ctx_engine(ctx), o));
+  const int rc = h->s->start();
+  if (rc) { delete h; return rc; }
+  *out = h;
+  return 0;
+}
+
+extern "C" int rtmh_server_submit(rtmh_server* srv, int model_id, const rtmh_params* p, const float* diffcoef,
+                                  int* request_id)
+{
+  if (!srv || !p || !diffcoef || !request_id) { rtm::set_error("null argument"); return -1; }
+  return srv->s->submit(model_id, *p, diffcoef, request_id);
+}
+
+extern "C" int rtmh_server_wait(rtmh_server* srv, int request_id, double timeout_s, rtmh_outputs* out,
+                                rtmh_request_stats* stats)
+{
+  if (!srv) { rtm::set_error("null argument"); return -1; }
+  return srv->s->wait(request_id, timeout_s, out, stats);
+}
+
+extern "C" int rtmh_server_query(rtmh_server* srv, rtmh_server_stats* st)
+{
+  if (!srv || !st) { rtm::set_error("null argument"); return -1; }
+  srv->s->query(st);
+  return 0;
+}
+
+extern "C" int rtmh_server_destroy(rtmh_server* srv)
+{
+  delete srv;
+  return 0;
+}

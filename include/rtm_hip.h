@@ -79,6 +79,8 @@ typedef struct rtmh_options {
   int verbose;        /* 0 silent, 1 per-shot lines as the reference prints       */
   double mem_budget;  /* bytes of HBM for shot slots; 0 = 80% of free memory      */
   int profile;        /* 1: plain launches timed one by one with HIP events       */
+  const float *shot_offsetmax; /* per shot (job order) aperture overriding offsetmax
+                                  (ragged windows); NULL: offsetmax for every shot */
 } rtmh_options;
 
 /* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */
@@ -196,6 +198,46 @@ int rtmh_postprocess(int device, const float *image1, const float *illum, int nx
 /* main() of RTM_GPU_velstr.cpp:37-317: reads ./input/cmd.in, runs, writes
  * ./output/<dev>*.su (plus <dev>Image1lap.su and <dev>Imagenorm.su). */
 int rtmh_driver_main(int argc, char **argv);
+
+/* ------------------------------------------------------------------ */
+/* 4. served form: continuous batching of RTM requests on one GPU      */
+/* ------------------------------------------------------------------ */
+/* A worker thread advances every resident shot by one time step per tick:
+ * shots of different requests, grid sizes, time steps and phases (forward /
+ * reverse) share the same launches.  Queued shots are admitted first come
+ * first served as slots and HBM budget free up; each request's shots are
+ * stacked in job order, so a request's images do not depend on what else
+ * runs beside it.  Requests are mode-0 jobs (data modelled, then migrated)
+ * on models created with rtmh_model_create on the same context, which must
+ * outlive them.  A context serves either rtmh_run* calls or one server at a
+ * time. */
+typedef struct rtmh_server rtmh_server;
+typedef struct rtmh_server_options {
+  double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of free */
+  int max_slots;      /* shots resident at once; 0 = 256                          */
+  int tile_z;         /* 8 / 16 / 32; 0 = 16                                      */
+  int store_fmt;      /* boundary store precision of every request                */
+} rtmh_server_options;
+typedef struct rtmh_request_stats {
+  double submit_s, start_s, done_s;   /* seconds since the server started          */
+  double cell_updates;
+  int nshots, status;
+} rtmh_request_stats;
+int rtmh_server_create(rtmh_ctx *ctx, const rtmh_server_options *opt, rtmh_server **out);
+/* queue one job; returns its request id */
+int rtmh_server_submit(rtmh_server *srv, int model_id, const rtmh_params *p, const float *diffcoef,
+                       int *request_id);
+/* 0 when done (outputs copied, stats filled), 1 on timeout, < 0 on error */
+int rtmh_server_wait(rtmh_server *srv, int request_id, double timeout_s, rtmh_outputs *out,
+                     rtmh_request_stats *stats);
+/* server-wide counters: ticks, launches, peak resident shots, busy seconds */
+typedef struct rtmh_server_stats {
+  long long ticks, launches;
+  int peak_slots, active_slots, queued_requests;
+  double busy_s, cell_updates;
+} rtmh_server_stats;
+int rtmh_server_query(rtmh_server *srv, rtmh_server_stats *st);
+int rtmh_server_destroy(rtmh_server *srv);
 
 #ifdef __cplusplus
 }
