@@ -1,18 +1,30 @@
 #!/usr/bin/env python3
 """bench.py -- 2-D acoustic prestack RTM on one MI355X.
 
-Workload (the headline configuration): plain RTM of a 200-shot survey over a
-seeded smooth-random layered model 1500-4500 m/s, nx=1024 nz=512, dx=10 m,
-PML 50, Ricker 15 Hz, dt=0.8 ms nt=4000, shots every 50 m, split-spread 301
-receivers (layout 1), fp32 stencil, all three imaging conditions
-(cross-correlation, source-illumination normalised, Laplacian filtered).
-Data: modelled by the same code from the synthetic model (no field data).
+Default workload (the headline configuration, --workload survey): plain RTM
+of a 200-shot survey over a seeded smooth-random layered model 1500-4500 m/s,
+nx=1024 nz=512, dx=10 m, PML 50, Ricker 15 Hz, dt=0.8 ms nt=4000, shots every
+50 m, split-spread 301 receivers (layout 1), fp32 stencil, all three imaging
+conditions (cross-correlation, source-illumination normalised, Laplacian
+filtered).  Data: modelled by the same code from the synthetic model.
 
 A "step" is the migration of one batch of --shots-per-step shots (forward
 modelling with boundary saving, reverse reconstruction + receiver
 propagation + imaging, FP64 stacking).  The velocity model is resident in
 HBM before the timed region (rtmh_model_create); every step runs through the
 C ABI (rtmh_run_model), which returns after its stream has drained.
+
+Other workloads (one JSON line each, same keys):
+  --workload batched    2048x768, 256 shots with per-shot aperture windows
+                        U[400,1600] columns, 64 shots per launch, hipGraph
+                        time loop, autotuned tile height
+  --workload precision  2048x1024 dx=7.5 m nt=6000 25 Hz: throughput with
+                        FP32 / FP16 / BF16 boundary stores, and the image
+                        error of each against the FP64 CPU oracle
+  --workload serve      Poisson stream (4 req/s) over 8 resident models,
+                        grids 256^2..2048x1024, nt 1000-6000, 1-32 shots,
+                        FP16 boundaries, continuous batching; against the
+                        plain one-shot-per-launch GPU form on the same stream
 
 Output: ONE JSON line on stdout (progress goes to stderr).
 """
@@ -41,7 +53,7 @@ def cells_of(P, geoms):
     return sum(2.0 * nt * g.nxp * g.nzp + (nt - 1.0) * P.nz * g.nxl for g in geoms)
 
 
-def cpu_baseline(P_full, vp, rho, min_seconds=10.0, max_shots=6):
+def cpu_baseline(P_full, vp, rho, min_seconds=10.0, max_shots=6, first=100):
     """The oracle (FP32 restatement, OpenMP over rows) on a bounded sample of
     the same survey: whole shots (forward + reconstruction + receiver +
     imaging at the full nt) from the middle of the survey, until at least
@@ -54,7 +66,7 @@ def cpu_baseline(P_full, vp, rho, min_seconds=10.0, max_shots=6):
     shots = []
     t0 = time.perf_counter()
     for k in range(max_shots):
-        ishot = 100 + k
+        ishot = first + k
         d.update(nshot=ishot, startshot=ishot)
         Po = O.make_params(**d)
         g = O.geometry(Po, ishot)
@@ -69,29 +81,40 @@ def cpu_baseline(P_full, vp, rho, min_seconds=10.0, max_shots=6):
                       f"{cells / 1e9:.2f} Gcell in {dt:.1f} s; {len(shots) / dt * 3600.0:.0f} shots/hour"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--shots-per-step", type=int, default=20)
-    ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: all shots of the step)")
-    ap.add_argument("--store", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--chunk", type=int, default=50, help="time steps per replayed hipGraph (0: plain launches)")
-    ap.add_argument("--tz", type=int, default=0, help="tile height (0: default)")
-    ap.add_argument("--nt", type=int, default=0, help="override nt (shorter runs; the headline uses 4000)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    args = ap.parse_args()
+def roofline_of(recs, P):
+    """Dominant kernel: the reverse time step k_rev (one launch per step:
+    reconstruction, receiver wavefield, imaging), timed by HIP events on the
+    engine's stream around every reverse pass of the timed region; the
+    algorithmic bytes per launch come from the per-cell figures (engine
+    stats step_bytes_rev)."""
+    rev_steps = sum(s.batches * (P.nt - 1) for s, _, _ in recs)
+    rev_ms = sum(s.rev_ms for s, _, _ in recs)
+    step_bytes = float(np.mean([s.step_bytes_rev for s, _, _ in recs]))
+    avg_step_s = rev_ms * 1e-3 / rev_steps
+    achieved = step_bytes / avg_step_s / 1e9
+    fwd_steps = sum(s.batches * P.nt for s, _, _ in recs)
+    fwd_ms = sum(s.fwd_ms for s, _, _ in recs)
+    fwd_bytes = float(np.mean([s.step_bytes_fwd for s, _, _ in recs]))
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_rev (reverse time step: reconstruction + receiver wavefield + imaging)",
+            "avg_launch_us": round(avg_step_s * 1e6, 2), "alg_bytes_per_launch": step_bytes,
+            "forward_step": {"kernel": "k_fwd", "achieved": round(fwd_bytes / (fwd_ms * 1e-3 / fwd_steps) / 1e9, 1),
+                             "avg_launch_us": round(fwd_ms * 1e3 / fwd_steps, 2),
+                             "alg_bytes_per_launch": fwd_bytes}}
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: every shot is independent and this build measures one GPU's "
-              "throughput (the multi-GPU design shards shots and sums the FP64 stacks once; it is not run here).",
-              file=sys.stderr)
-        sys.exit(2)
 
-    import gubingluo_amd as G
-    from gubingluo_amd import synth
+def base_line(args, value, unit, elapsed, dtype, data, config, extra):
+    out = {"metric": "grid-cell updates/s", "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(elapsed / max(args.steps, 1) * 1e3, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data,
+           "config": config}
+    out.update(extra)
+    return out
 
+
+# ------------------------------------------------------------------ survey --
+def run_survey(args, G, synth):
     P, vp, rho = synth.cfg2(nshot=200)
     if args.nt:
         P.nt = args.nt
@@ -99,7 +122,6 @@ def main():
     store = {"f32": G.STORE_F32, "f16": G.STORE_F16, "bf16": G.STORE_BF16}[args.store]
     log(f"[bench] cfg2 survey: {P.nx}x{P.nz} nt={P.nt} npml={P.npml}, {S} shots/step, "
         f"steps={args.steps} warmup={args.warmup} store={args.store} chunk={args.chunk}")
-
     ctx = G.Context(0)
     mid = ctx.model_create(vp, rho)
 
@@ -116,7 +138,6 @@ def main():
     for k in range(args.warmup):
         _, dt, _ = step(k)
         log(f"[bench] warmup step {k}: {dt:.2f} s")
-
     recs = []
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -124,60 +145,232 @@ def main():
         recs.append((r.stats, dt, shots))
         s = r.stats
         log(f"[bench] step {k}: {len(shots)} shots {dt:.2f} s, {s.cell_updates / dt / 1e9:.1f} Gcell/s "
-            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f})")
+            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f}, tz {s.tile_z})")
     elapsed = time.perf_counter() - t0
-
     cells = sum(s.cell_updates for s, _, _ in recs)
     nshots = sum(len(sh) for _, _, sh in recs)
     lat = np.concatenate([np.full(len(sh), dt) for _, dt, sh in recs])
-    # dominant kernel: the reverse time step (k_vel_rev + k_pres_rev launched
-    # back to back), timed by HIP events on the engine's stream over the
-    # timed region; algorithmic bytes per step from the per-unit figures
-    rev_steps = sum(s.batches * (P.nt - 1) for s, _, _ in recs)
-    rev_ms = sum(s.rev_ms for s, _, _ in recs)
-    step_bytes = float(np.mean([s.step_bytes_rev for s, _, _ in recs]))
-    avg_step_s = rev_ms * 1e-3 / rev_steps
-    achieved = step_bytes / avg_step_s / 1e9
-    fwd_steps = sum(s.batches * P.nt for s, _, _ in recs)
-    fwd_ms = sum(s.fwd_ms for s, _, _ in recs)
-    fwd_bytes = float(np.mean([s.step_bytes_fwd for s, _, _ in recs]))
-
-    out = {
-        "metric": "grid-cell updates/s",
-        "value": round(cells / elapsed / 1e9, 3),
-        "unit": "Gcell/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if store == G.STORE_F32 else f"f32 stencil, {args.store} boundaries",
-        "data": "synthetic: seeded smooth-random layered model, data modelled in the same run",
-        "config": {"workload": "plain RTM, 200-shot survey, 1024x512 dx=10 m, PML 50, Ricker 15 Hz, dt=0.8 ms "
-                               f"nt={P.nt}, shots every 50 m, split-spread 301 receivers, fp32, 3 imaging conditions",
-                   "shots_per_step": S, "batch": recs[0][0].batch, "graph_chunk": args.chunk,
-                   "tile_z": recs[0][0].tile_z, "store": args.store},
-        "shots_per_hour": round(nshots / elapsed * 3600.0, 1),
-        "p50_shot_latency_s": round(float(np.percentile(lat, 50)), 3),
-        "p99_shot_latency_s": round(float(np.percentile(lat, 99)), 3),
-        "hbm_pct_roofline": round(100.0 * achieved / HBM_PEAK_GBS, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "reverse time step (k_vel_rev + k_pres_rev)",
-                     "avg_launch_us": round(avg_step_s * 1e6, 2), "alg_bytes_per_launch": step_bytes,
-                     "forward_step": {"achieved": round(fwd_bytes / (fwd_ms * 1e-3 / fwd_steps) / 1e9, 1),
-                                      "avg_launch_us": round(fwd_ms * 1e3 / fwd_steps, 2),
-                                      "alg_bytes_per_launch": fwd_bytes}},
-        "cpu_baseline": None,
-    }
+    roof = roofline_of(recs, P)
+    out = base_line(
+        args, round(cells / elapsed / 1e9, 3), "Gcell/s", elapsed,
+        "f32" if store == G.STORE_F32 else f"f32 stencil, {args.store} boundaries",
+        "synthetic: seeded smooth-random layered model, data modelled in the same run",
+        {"workload": "plain RTM, 200-shot survey, 1024x512 dx=10 m, PML 50, Ricker 15 Hz, dt=0.8 ms "
+                     f"nt={P.nt}, shots every 50 m, split-spread 301 receivers, fp32, 3 imaging conditions",
+         "shots_per_step": S, "batch": recs[0][0].batch, "graph_chunk": args.chunk,
+         "tile_z": recs[0][0].tile_z, "store": args.store},
+        {"shots_per_hour": round(nshots / elapsed * 3600.0, 1),
+         "p50_shot_latency_s": round(float(np.percentile(lat, 50)), 3),
+         "p99_shot_latency_s": round(float(np.percentile(lat, 99)), 3),
+         "hbm_pct_roofline": round(100.0 * roof["achieved"] / HBM_PEAK_GBS, 1),
+         "roofline": roof, "cpu_baseline": None})
     ctx.model_destroy(mid)
     ctx.close()
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(P, vp, rho)
         log(f"[bench] cpu baseline {out['cpu_baseline']}")
+    return out
+
+
+# ----------------------------------------------------------------- batched --
+def run_batched(args, G, synth):
+    P, vp, rho, om = synth.cfg3()
+    if args.nt:
+        P.nt = args.nt
+    S = args.shots_per_step or 64
+    log(f"[bench] cfg3 batched: {P.nx}x{P.nz} nt={P.nt}, {S} shots/step (ragged windows), steps={args.steps}")
+    ctx = G.Context(0)
+    mid = ctx.model_create(vp, rho)
+
+    def step(k):
+        first = 1 + (k * S) % 256
+        last = min(first + S - 1, 256)
+        Q = G.Params.from_buffer_copy(P)
+        Q.startshot, Q.nshot, Q.dshot = first, last, 1
+        t0 = time.perf_counter()
+        r = ctx.run(Q, model=mid, batch=S, graph_chunk=args.chunk, tile_z=args.tz, want_post=True,
+                    shot_offsetmax=om[first - 1:last])
+        return r, time.perf_counter() - t0, list(range(first, last + 1))
+
+    for k in range(args.warmup):
+        _, dt, _ = step(k)
+        log(f"[bench] warmup step {k}: {dt:.2f} s")
+    recs = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r, dt, shots = step(args.warmup + k)
+        recs.append((r.stats, dt, shots))
+        s = r.stats
+        log(f"[bench] step {k}: {len(shots)} shots {dt:.2f} s, {s.cell_updates / dt / 1e9:.1f} Gcell/s "
+            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f}, tz {s.tile_z})")
+    elapsed = time.perf_counter() - t0
+    cells = sum(s.cell_updates for s, _, _ in recs)
+    nshots = sum(len(sh) for _, _, sh in recs)
+    roof = roofline_of(recs, P)
+    widths = [int(2 * np.floor(om[i] / P.dx) + 1) for i in range(len(om))]
+    out = base_line(
+        args, round(cells / elapsed / 1e9, 3), "Gcell/s", elapsed, "f32",
+        "synthetic: seeded smooth-random layered model, data modelled in the same run",
+        {"workload": f"batched RTM, 2048x768 dx=10 m, 256 shots every 80 m, per-shot windows U[400,1600] columns "
+                     f"(drawn {min(widths)}..{max(widths)}, clipped at the model edges), nt={P.nt}",
+         "shots_per_step": S, "batch": S, "graph_chunk": args.chunk, "tile_z": recs[0][0].tile_z},
+        {"shots_per_hour": round(nshots / elapsed * 3600.0, 1), "hbm_pct_roofline":
+         round(100.0 * roof["achieved"] / HBM_PEAK_GBS, 1), "roofline": roof, "cpu_baseline": None})
+    ctx.model_destroy(mid)
+    ctx.close()
+    return out
+
+
+# --------------------------------------------------------------- precision --
+def run_precision(args, G, synth):
+    P, vp, rho = synth.cfg4()
+    if args.nt:
+        P.nt = args.nt
+    S = args.shots_per_step or 8
+    ctx = G.Context(0)
+    mid = ctx.model_create(vp, rho)
+    res = {}
+    for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
+        Q = G.Params.from_buffer_copy(P)
+        Q.startshot, Q.nshot, Q.dshot = 61, 60 + S, 1
+        ctx.run(Q, model=mid, batch=S, store_fmt=fmt, graph_chunk=args.chunk, tile_z=args.tz, want_post=False) \
+            if args.warmup else None
+        t0 = time.perf_counter()
+        r = ctx.run(Q, model=mid, batch=S, store_fmt=fmt, graph_chunk=args.chunk, tile_z=args.tz, want_post=True)
+        dt = time.perf_counter() - t0
+        res[name] = {"gcell_s": round(r.stats.cell_updates / dt / 1e9, 2), "shots_per_hour": round(S / dt * 3600, 1),
+                     "seconds": round(dt, 2)}
+        log(f"[bench] {name}: {S} shots {dt:.2f} s, {r.stats.cell_updates / dt / 1e9:.1f} Gcell/s")
+    # image error of one shot against the FP64 CPU oracle (bounded: err_nt steps)
+    err = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        Q = G.Params.from_buffer_copy(P)
+        ne = args.err_nt or P.nt
+        Q.nt = ne
+        Q.startshot = Q.nshot = 64
+        Qo = O.make_params(**{n: getattr(Q, n) for n, _ in Q._fields_})
+        log(f"[bench] FP64 oracle, shot 64, nt={ne} ...")
+        t0 = time.perf_counter()
+        i1, i2, il = O.rtm(Qo, vp, rho, f64=True)
+        log(f"[bench] oracle {time.perf_counter() - t0:.1f} s")
+        err = {"sample": f"shot 64 of the survey, nt={ne}", "rel_l2": {}}
+        for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
+            r = ctx.run(Q, model=mid, batch=1, store_fmt=fmt, want_post=False)
+            e = {k: float(np.linalg.norm((a - b).astype(np.float64)) / max(np.linalg.norm(b.astype(np.float64)), 1e-300))
+                 for k, a, b in (("image1", r.image1, i1), ("image2s", r.image2s, i2), ("illum", r.illum, il))}
+            err["rel_l2"][name] = {k: float(f"{v:.3e}") for k, v in e.items()}
+            log(f"[bench] {name} vs FP64 oracle: {err['rel_l2'][name]}")
+    ctx.model_destroy(mid)
+    ctx.close()
+    best = res["f16"]
+    return base_line(
+        args, best["gcell_s"], "Gcell/s", best["seconds"], "f32 stencil, f16 boundaries",
+        "synthetic: seeded smooth-random layered model, data modelled in the same run",
+        {"workload": f"precision variant, 2048x1024 dx=7.5 m, PML 50, Ricker 25 Hz, dt=0.6 ms nt={P.nt}, "
+                     f"full-aperture receivers, {S} shots per launch", "shots_per_step": S, "batch": S},
+        {"per_store": res, "error_vs_fp64_oracle": err, "cpu_baseline": None})
+
+
+# ------------------------------------------------------------------- serve --
+def run_serve(args, G, synth):
+    models = synth.cfg5_models()
+    reqs = synth.cfg5_stream(args.requests, models)
+    ctx = G.Context(0)
+    mids = [ctx.model_create(v, r) for v, r in models]
+    total_shots = sum(P.nshot for _, _, P in reqs)
+    total_cells = 0.0
+    for _, m, P in reqs:
+        total_cells += cells_of(P, [G.geometry(P, s) for s in range(1, P.nshot + 1)])
+    log(f"[bench] serve: {len(reqs)} requests, {total_shots} shots, {total_cells / 1e12:.2f} Tcell, "
+        f"arrivals over {reqs[-1][0]:.1f} s")
+    # served form: continuous batching, FP16 boundaries
+    lat = []
+    with G.Server(ctx, store_fmt=G.STORE_F16, mem_budget=args.mem_budget) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for (ta, m, P) in reqs:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            ids.append((srv.submit(mids[m], P), P))
+        for rid, P in ids:
+            *_, st = srv.wait(rid, P, timeout=3600)
+            lat.append(st.done_s - st.submit_s)
+        served_s = time.perf_counter() - t0
+        q = srv.query()
+    log(f"[bench] served: {served_s:.1f} s, peak slots {q.peak_slots}, ticks {q.ticks}")
+    # plain one-shot-per-launch GPU form: requests one after another, one shot
+    # per launch, per-step launches (no graph), FP32 boundaries
+    plain_s = 0.0
+    nplain = min(args.plain_requests or len(reqs), len(reqs))
+    plain_shots = 0
+    plain_lat = []
+    for (ta, m, P) in reqs[:nplain]:
+        t1 = time.perf_counter()
+        ctx.run(P, model=mids[m], batch=1, graph_chunk=0, tile_z=32, store_fmt=G.STORE_F32, want_post=True)
+        d = time.perf_counter() - t1
+        plain_s += d
+        plain_shots += P.nshot
+        plain_lat.append(d)
+    log(f"[bench] plain: {nplain} requests in {plain_s:.1f} s")
+    for m in mids:
+        ctx.model_destroy(m)
+    ctx.close()
+    sph = total_shots / served_s * 3600.0
+    psph = plain_shots / plain_s * 3600.0
+    return base_line(
+        args, round(total_cells / served_s / 1e9, 3), "Gcell/s", served_s, "f32 stencil, f16 boundaries",
+        "synthetic: 8 seeded layered models, Poisson arrivals, data modelled in the same run",
+        {"workload": f"serving: Poisson stream 4 req/s, {len(reqs)} requests, grids 256^2..2048x1024, "
+                     "nt U[1000,6000], 1-32 shots, 8 resident models, fp16 boundaries, continuous batching"},
+        {"shots_per_hour": round(sph, 1), "p50_request_latency_s": round(float(np.percentile(lat, 50)), 2),
+         "p99_request_latency_s": round(float(np.percentile(lat, 99)), 2), "peak_slots": q.peak_slots,
+         "plain_form": {"shots_per_hour": round(psph, 1), "requests": nplain,
+                        "p50_request_s": round(float(np.percentile(plain_lat, 50)), 2),
+                        "note": "one shot per launch, per-step launches, fp32 boundaries, requests serially"},
+         "served_vs_plain_shots_per_hour": round(sph / psph, 2), "cpu_baseline": None})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="survey", choices=["survey", "batched", "precision", "serve"])
+    ap.add_argument("--shots-per-step", type=int, default=0, help="shots per step (survey default 20)")
+    ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: all shots of the step)")
+    ap.add_argument("--store", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--chunk", type=int, default=50, help="time steps per replayed hipGraph (0: plain launches)")
+    ap.add_argument("--tz", type=int, default=0, help="tile height 16/32/64 (0: autotuned)")
+    ap.add_argument("--nt", type=int, default=0, help="override nt (shorter runs)")
+    ap.add_argument("--err-nt", type=int, default=0, help="precision: nt of the oracle error sample (0: full)")
+    ap.add_argument("--requests", type=int, default=16, help="serve: requests in the stream")
+    ap.add_argument("--plain-requests", type=int, default=0, help="serve: requests run in the plain form (0: all)")
+    ap.add_argument("--mem-budget", type=float, default=0.0, help="serve: HBM budget in bytes (0: 85%% free)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle error")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: every shot is independent and this build measures one GPU's "
+              "throughput (the multi-GPU design shards shots and sums the FP64 stacks once; it is not run here).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import gubingluo_amd as G
+    from gubingluo_amd import synth
+
+    if args.workload == "survey":
+        if not args.shots_per_step:
+            args.shots_per_step = 20
+        out = run_survey(args, G, synth)
+    elif args.workload == "batched":
+        out = run_batched(args, G, synth)
+    elif args.workload == "precision":
+        out = run_precision(args, G, synth)
+    else:
+        out = run_serve(args, G, synth)
     print(json.dumps(out), flush=True)
 
 

@@ -6,16 +6,23 @@
 // many shots of different window widths (ragged batches) and, in the served
 // form, shots at different time steps and phases.
 //
-// Wavefield layout (per slot, all fp32, row pitch `pitch` floats; field
-// pointers address cell (0,0), with RTM_MZT zero rows above, RTM_MXL zero
-// columns to the left and at least 16 rows / 25 columns of zeros beyond the
-// tiled extent, so radius-10 tile staging never needs bounds checks):
-//   sp, svx, svz : source wavefield  (forward propagation, then reverse
-//                  reconstruction in place; rtm_model.cu:322-437)
-//   sp1, sp2     : split pressure pair of the source wavefield in PML cells
-//   rp, rvx, rvz, rp1, rp2 : receiver wavefield (rtm_model.cu:439-450)
+// Field layout (per slot, fp32, row pitch `pitch` floats).  Field pointers
+// address cell (0,0); every allocation has RTM_MZT rows above row 0, RTM_MXL
+// columns left of column 0 and at least 32 rows / 32 columns beyond the tiled
+// extent, so the radius-20 reads of a tile (velocities of the halo need the
+// pressure 10 cells further out) never need bounds checks.
+//   sp[2], svx[2], svz[2] : source wavefield, two time levels (step `it`
+//                           reads set it&1 and writes set (it+1)&1)
+//   rp[2], rvx[2], rvz[2] : receiver wavefield, same scheme
+//   q1, q2                : split pressure pair in PML cells (in place; the
+//                           source field's in the forward phase, the
+//                           receiver field's in the reverse phase)
+//   mrho, mvv             : the shot's model window (rho, v^2), edge
+//                           replicated into the PML and the margins
 //   vx(iz,ix) sits at x=(ix+1/2)dx and uses columns 0..nxp-2; vz(iz,ix) at
 //   z=(iz+1/2)dz uses rows 0..nzp-2 (kernel.cu:215-216 array shapes).
+// Damping factors are stored with the same margins (index -RTM_MXL.. of the
+// column arrays, -RTM_MZT.. of the row arrays; 1.0 outside the grid).
 // Imaging accumulators g1, illum: nzi x nxl, pitch nxl.
 // Boundary store: nt steps x strip_len values, packed per step as
 //   vx lr [2][nzi][10] | vz tb [2][10][nxl] | p lr [2][nzi][10] | p tb [2][10][nxl]
@@ -26,12 +33,19 @@
 
 #define RTM_R 10            // stencil radius (N/2, common.h:57)
 #define RTM_TX 64           // tile width (one wave64 row)
-#define RTM_MZT 16          // zero margin rows above row 0 of every field
-#define RTM_MXL 16          // zero margin columns left of column 0
-#define RTM_ZALIGN 32       // field rows are allocated for nzp rounded up to this + 16 below
+#define RTM_MZT 32          // margin rows above row 0 of every field
+#define RTM_MXL 32          // margin columns left of column 0
+#define RTM_ZALIGN 64       // field rows are allocated for nzp rounded up to this
+#define RTM_MZB 32          // margin rows below the rounded extent
 
-// tile flags (Tile::kind): which cell classes a tile contains
-enum RtmTileFlag : int { TF_STRIP = 1, TF_SRC = 2, TF_REC = 4 };
+// tile flags (Tile::kind)
+enum RtmTileFlag : int {
+  TF_STRIP = 1,   // the tile or its halo holds boundary-strip cells
+  TF_SRC = 2,     // the tile holds source cells
+  TF_REC = 4,     // the tile holds receiver cells
+  TF_GEN = 8,     // the tile or its halo leaves the undamped interior (per-cell rules)
+  TF_INT = 16,    // reverse phase: the tile holds reconstructed cells (interior or strips)
+};
 
 enum RtmPhase : int { PH_IDLE = 0, PH_FWD = 1, PH_REV = 2 };
 enum RtmStoreFmt : int { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
@@ -44,13 +58,16 @@ struct ShotDev {
   int it_off, it_dir;          // time index = it_off + it_dir * step
   int record, rcol0, nrec, _pad0;
   float dt, dtx, dtz, _pad1;
-  // model window: value(iz,ix) = m[clamp(iz-mz0,0,mzn-1)*mpitch + mx0 + clamp(ix-mxs,0,mxn-1)]
+  // model window source (resident model): value(iz,ix) =
+  //   m[clamp(iz-mz0,0,mzn-1)*mpitch + mx0 + clamp(ix-mxs,0,mxn-1)]
   const float* rho;
   const float* vv;             // velocity squared (extractvel1, Velocity_function.cpp:2059)
   int mpitch, mz0, mzn, mx0, mxs, mxn, _pad2, _pad3;
   // wavefields
-  float *sp, *svx, *svz, *sp1, *sp2;
-  float *rp, *rvx, *rvz, *rp1, *rp2;
+  float *sp[2], *svx[2], *svz[2];
+  float *rp[2], *rvx[2], *rvz[2];
+  float *q1, *q2;
+  float *mrho, *mvv;
   float *g1, *illum;
   // per-column / per-row damping factors c1 = 1-dt*d/2, c2 = 2-c1
   const float *c1x, *c2x, *c1xv, *c2xv, *c1z, *c2z, *c1zv, *c2zv;

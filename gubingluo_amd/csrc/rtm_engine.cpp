@@ -284,6 +284,10 @@ void Engine::drop_graphs()
 
 static size_t store_elem(int fmt) { return fmt == ST_F32 ? 4 : 2; }
 
+// fields of a slot: two levels of p, vx, vz for the source and the receiver
+// wavefield, the PML pair, the model window (rho, v^2)
+static constexpr int kSlotFields = 16;
+
 // small per-slot arrays, in this order, each 256-byte aligned
 struct SmallLayout {
   size_t c[8], sa, cut, taper, total;
@@ -292,7 +296,9 @@ static SmallLayout small_layout(const ShotSetup& s)
 {
   SmallLayout L;
   size_t o = 0;
-  const size_t nx = s.g.nxp, nz = s.g.nzp;
+  // damping arrays carry the field margins: columns -RTM_MXL .. pitch-RTM_MXL-1,
+  // rows -RTM_MZT .. frows-RTM_MZT-1 (value 1 outside the grid)
+  const size_t nx = Engine::pitch_of(s.g.nxp), nz = Engine::frows_of(s.g.nzp);
   const size_t n8[8] = {nx, nx, nx, nx, nz, nz, nz, nz};
   for (int i = 0; i < 8; i++) { L.c[i] = o; o += align256(n8[i] * 4); }
   L.sa = o; o += align256(s.sa.size() * 4);
@@ -309,7 +315,7 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
   const size_t nzi = nzp - 2 * (size_t)((s.g.nxp - s.g.nxl) / 2);
   const size_t I = align256(nzi * s.g.nxl * 4);
   const size_t strip = (size_t)40 * (nzi + s.g.nxl);
-  size_t b = small_layout(s).total + 10 * F + 2 * I;
+  size_t b = small_layout(s).total + kSlotFields * F + 2 * I;
   b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
   b += align256((size_t)nt * strip * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * strip * 4));
   b += 2 * align256((size_t)nt * 4 * 4);
@@ -334,21 +340,29 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   const SmallLayout L = small_layout(s);
   std::vector<char> img(L.total, 0);
   const std::vector<float>* cv[8] = {&s.c1x, &s.c2x, &s.c1xv, &s.c2xv, &s.c1z, &s.c2z, &s.c1zv, &s.c2zv};
-  for (int i = 0; i < 8; i++) memcpy(img.data() + L.c[i], cv[i]->data(), cv[i]->size() * 4);
+  const int pad[8] = {RTM_MXL, RTM_MXL, RTM_MXL, RTM_MXL, RTM_MZT, RTM_MZT, RTM_MZT, RTM_MZT};
+  const int len[8] = {d.pitch, d.pitch, d.pitch, d.pitch, d.frows, d.frows, d.frows, d.frows};
+  for (int i = 0; i < 8; i++) {
+    float* dst = (float*)(img.data() + L.c[i]);
+    for (int k = 0; k < len[i]; k++) dst[k] = 1.0f;
+    memcpy(dst + pad[i], cv[i]->data(), cv[i]->size() * 4);
+  }
   memcpy(img.data() + L.sa, s.sa.data(), s.sa.size() * 4);
   memcpy(img.data() + L.cut, s.cut.data(), s.cut.size() * 4);
   memcpy(img.data() + L.taper, s.taper.data(), s.taper.size() * 4);
   uploads.push_back({base, std::move(img)});
   const float** cp[8] = {&d.c1x, &d.c2x, &d.c1xv, &d.c2xv, &d.c1z, &d.c2z, &d.c1zv, &d.c2zv};
-  for (int i = 0; i < 8; i++) *cp[i] = (const float*)(base + L.c[i]);
+  for (int i = 0; i < 8; i++) *cp[i] = (const float*)(base + L.c[i]) + pad[i];
   d.sx0 = s.sx0; d.sx1 = s.sx1; d.sz0 = s.sz0; d.sz1 = s.sz1;
   d.sa = (const float*)(base + L.sa);
   d.cut = (const int*)(base + L.cut);
   d.coef = coef_dev; d.wav = wav_dev;
   size_t o = L.total;
   const size_t F = align256((size_t)d.frows * d.pitch * 4);
-  float** fp[10] = {&d.sp, &d.svx, &d.svz, &d.sp1, &d.sp2, &d.rp, &d.rvx, &d.rvz, &d.rp1, &d.rp2};
-  for (int i = 0; i < 10; i++) { *fp[i] = (float*)(base + o) + d.forg; o += F; }
+  float** fp[kSlotFields] = {&d.sp[0], &d.sp[1], &d.svx[0], &d.svx[1], &d.svz[0], &d.svz[1],
+                             &d.rp[0], &d.rp[1], &d.rvx[0], &d.rvx[1], &d.rvz[0], &d.rvz[1],
+                             &d.q1, &d.q2, &d.mrho, &d.mvv};
+  for (int i = 0; i < kSlotFields; i++) { *fp[i] = (float*)(base + o) + d.forg; o += F; }
   const size_t I = align256((size_t)d.nzi * d.nxl * 4);
   d.g1 = (float*)(base + o); o += I;
   d.illum = (float*)(base + o); o += I;
@@ -393,7 +407,8 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   const int nslots = (int)slots.size();
   int done = 0;
   if (prof) {
-    // every kernel bracketed by its own event pair on the launch stream
+    // every step kernel (and the quantisation after it) bracketed by its own
+    // event pair on the launch stream
     hipEvent_t ev[3];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     LaunchArgs a{dshots, dtiles, ntiles, 0};
@@ -401,17 +416,16 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     for (int s = 0; s < nsteps; s++) {
       a.step = s;
       HIPCHK(hipEventRecord(ev[0], st_));
-      HIPCHK(rtm_launch_phase(a, phase, tz, save, 0, st_));
+      HIPCHK(rtm_launch_step(a, phase, tz, save, st_));
       HIPCHK(hipEventRecord(ev[1], st_));
-      HIPCHK(rtm_launch_phase(a, phase, tz, save, 1, st_));
-      HIPCHK(hipEventRecord(ev[2], st_));
       if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, 128, st_));
+      HIPCHK(hipEventRecord(ev[2], st_));
       HIPCHK(hipEventSynchronize(ev[2]));
       float m0 = 0, m1 = 0;
       HIPCHK(hipEventElapsedTime(&m0, ev[0], ev[1]));
       HIPCHK(hipEventElapsedTime(&m1, ev[1], ev[2]));
-      prof->kern_ms[kb] += m0; prof->kern_ms[kb + 1] += m1;
-      prof->kern_calls[kb]++; prof->kern_calls[kb + 1]++;
+      prof->kern_ms[kb] += m0; prof->kern_calls[kb]++;
+      if (quant) { prof->kern_ms[kb + 1] += m1; prof->kern_calls[kb + 1]++; }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return 0;
@@ -433,26 +447,37 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   return 0;
 }
 
-// tile table of a batch with the per-tile flags of the cell classes it holds
+// tile table of a batch with the per-tile flags (TF_*) of the cell classes
+// the tile and the halo its step reads hold
 void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles, const std::vector<int>* slot_ids)
 {
   tiles.clear();
   for (int k = 0; k < (int)ds.size(); k++) {
     const int sid = slot_ids ? (*slot_ids)[k] : k;
     const ShotDev& d = ds[k];
-    const int n = d.npml, R = RTM_R;
-    for (int z0 = 0; z0 < d.nzp; z0 += tz)
-      for (int x0 = 0; x0 < d.nxp; x0 += RTM_TX) {
+    const int n = d.npml, R = RTM_R, nxp = d.nxp, nzp = d.nzp;
+    for (int z0 = 0; z0 < nzp; z0 += tz)
+      for (int x0 = 0; x0 < nxp; x0 += RTM_TX) {
         const int z1 = z0 + tz - 1, x1 = x0 + RTM_TX - 1;
-        auto hit = [&](int za, int zb, int xa, int xb) { return z0 <= zb && z1 >= za && x0 <= xb && x1 >= xa; };
+        // rectangle [za,zb] x [xa,xb] against the tile grown by (hz, hx)
+        auto hit = [&](int hz, int hx, int za, int zb, int xa, int xb) {
+          return z0 - hz <= zb && z1 + hz >= za && x0 - hx <= xb && x1 + hx >= xa;
+        };
         int f = 0;
-        // strips: columns n-R..n-1 and nxp-n-1..nxp-n+9 over interior rows,
-        // rows n-R..n-1 and nzp-n-1..nzp-n+9 over interior columns
-        if (hit(n, d.nzp - n - 1, n - R, n - 1) || hit(n, d.nzp - n - 1, d.nxp - n - 1, d.nxp - n + R - 1) ||
-            hit(n - R, n - 1, n, d.nxp - n - 1) || hit(d.nzp - n - 1, d.nzp - n + R - 1, n, d.nxp - n - 1))
+        // strips (vx, p: columns n-R..n-1 and nxp-n-1..nxp-n+9 over interior
+        // rows; vz, p: rows n-R..n-1 and nzp-n-1..nzp-n+9 over interior
+        // columns), against the velocity halo of the tile
+        if (hit(10, 10, n, nzp - n - 1, n - R, n - 1) || hit(10, 10, n, nzp - n - 1, nxp - n - 1, nxp - n + R - 1) ||
+            hit(10, 10, n - R, n - 1, n, nxp - n - 1) || hit(10, 10, nzp - n - 1, nzp - n + R - 1, n, nxp - n - 1))
           f |= TF_STRIP;
-        if (d.sx1 >= d.sx0 && hit(d.sz0, d.sz1, d.sx0, d.sx1)) f |= TF_SRC;
-        if (d.nrec > 0 && hit(n, n, d.rcol0, d.rcol0 + d.nrec - 1)) f |= TF_REC;
+        if (d.sx1 >= d.sx0 && hit(0, 0, d.sz0, d.sz1, d.sx0, d.sx1)) f |= TF_SRC;
+        if (d.nrec > 0 && hit(0, 0, n, n, d.rcol0, d.rcol0 + d.nrec - 1)) f |= TF_REC;
+        // undamped full-radius interior for every velocity the step computes
+        // (vx columns x0-10..x0+73, vz rows z0-10..z0+tz+8)
+        const bool inside = x0 - 10 >= n && x1 + 10 <= nxp - n - 2 && z0 - 10 >= n && z1 + 9 <= nzp - n - 2;
+        if (!inside) f |= TF_GEN;
+        // reverse phase: reconstructed cells (interior and strips)
+        if (hit(0, 0, n - R, nzp - n + R - 1, n - R, nxp - n + R - 1)) f |= TF_INT;
         tiles.push_back(Tile{sid, x0, z0, f});
       }
   }
@@ -567,11 +592,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   std::string why;
   if (!params_ok(P, why)) { set_error(why); return -1; }
   if (opt.store_fmt < 0 || opt.store_fmt > 2) { set_error("store_fmt must be 0, 1 or 2"); return -1; }
-  if (opt.tile_z != 0 && opt.tile_z != 8 && opt.tile_z != 16 && opt.tile_z != 32) {
-    set_error("tile_z must be 0 (autotuned), 8, 16 or 32");
+  if (opt.tile_z != 0 && opt.tile_z != 16 && opt.tile_z != 32 && opt.tile_z != 64) {
+    set_error("tile_z must be 0 (autotuned), 16, 32 or 64");
     return -1;
   }
-  int tz = opt.tile_z > 0 ? opt.tile_z : 16;
+  int tz = opt.tile_z > 0 ? opt.tile_z : 32;
   const Model* M = model(model_id);
   if (!M) { set_error("no such model"); return -1; }
   if (M->nx != P.nx || M->nz != P.nz) { set_error("model size differs from nx, nz"); return -1; }
@@ -664,7 +689,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     int maxp = 0, maxz = 0;
     for (int j = 0; j < nb; j++) { maxp = std::max(maxp, ds[j].nxp); maxz = std::max(maxz, ds[j].nzp); }
     const long long tkey = ((long long)(maxp / 64) << 40) | ((long long)(maxz / 32) << 20) | nb;
-    const int cands[3] = {8, 16, 32};
+    const int cands[3] = {16, 32, 64};
     bool tune = false;
     if (opt.tile_z == 0) {
       auto ct = tuned_.find(tkey);
@@ -907,7 +932,7 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   const size_t sb = slot_bytes(s, P.nt, ST_F32);
   char* slot = (char*)pool_.get(sb);
   std::vector<Tile> tiles;
-  const int tz = 16;
+  const int tz = 32;
   ShotDev d;
   std::vector<std::pair<void*, std::vector<char>>> up;
   float* d_rho = (float*)common;
@@ -930,12 +955,16 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   HIPCHK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   // fields in: host [nzp*nxp] -> device pitch
   const int nxp = s.g.nxp, nzp = s.g.nzp, pitch = d.pitch;
+  // fields in at level it0&1, out of level (it0+nsteps)&1; the model window
+  // and the zeroed levels come from the phase-start kernel
+  HIPCHK(rtm_launch_phase_init(d_ds, 0, PH_FWD, 512, st_));
+  HIPCHK(hipStreamSynchronize(st_));
   float* hf[5] = {pf, vxf, vzf, p1f, p2f};
-  float* dfp[5] = {d.sp, d.svx, d.svz, d.sp1, d.sp2};
-  for (int k = 0; k < 5; k++) {
-    HIPCHK(hipMemset(dfp[k] - d.forg, 0, (size_t)d.frows * pitch * 4));
-    HIPCHK(hipMemcpy2D(dfp[k], (size_t)pitch * 4, hf[k], (size_t)nxp * 4, (size_t)nxp * 4, nzp, hipMemcpyHostToDevice));
-  }
+  const int l0 = it0 & 1, l1 = (it0 + nsteps) & 1;
+  float* din[5] = {d.sp[l0], d.svx[l0], d.svz[l0], d.q1, d.q2};
+  float* dout[5] = {d.sp[l1], d.svx[l1], d.svz[l1], d.q1, d.q2};
+  for (int k = 0; k < 5; k++)
+    HIPCHK(hipMemcpy2D(din[k], (size_t)pitch * 4, hf[k], (size_t)nxp * 4, (size_t)nxp * 4, nzp, hipMemcpyHostToDevice));
   LaunchArgs a{d_ds, d_tiles, (int)tiles.size(), 0};
   for (int k = 0; k < nsteps; k++) {
     a.step = k;
@@ -943,7 +972,7 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   }
   HIPCHK(hipStreamSynchronize(st_));
   for (int k = 0; k < 5; k++)
-    HIPCHK(hipMemcpy2D(hf[k], (size_t)nxp * 4, dfp[k], (size_t)pitch * 4, (size_t)nxp * 4, nzp, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy2D(hf[k], (size_t)nxp * 4, dout[k], (size_t)pitch * 4, (size_t)nxp * 4, nzp, hipMemcpyDeviceToHost));
   pool_.put(ctl); pool_.put(slot); pool_.put(common);
   return 0;
 }

@@ -103,8 +103,11 @@ class Engine {
   // offset of the taper row inside a slot block
   static size_t taper_offset(const ShotSetup& s);
   std::mutex& models_mutex() { return models_mu_; }
-  static int pitch_of(int nxp) { return ((nxp + 63) / 64) * 64 + 64; }
-  static int frows_of(int nzp) { return RTM_MZT + ((nzp + RTM_ZALIGN - 1) / RTM_ZALIGN) * RTM_ZALIGN + 16; }
+  // row pitch: RTM_MXL margin columns, the grid rounded up to the tile width,
+  // 32 more; rows: RTM_MZT margin rows, the grid rounded up to the tallest
+  // tile, RTM_MZB more
+  static int pitch_of(int nxp) { return RTM_MXL + ((nxp + 63) / 64) * 64 + 32; }
+  static int frows_of(int nzp) { return RTM_MZT + ((nzp + RTM_ZALIGN - 1) / RTM_ZALIGN) * RTM_ZALIGN + RTM_MZB; }
 
  private:
   int device_;

@@ -1,33 +1,39 @@
 // rtm_kernels.hip -- gfx950 kernels of the RTM hot path.
 //
-// Per time step two phases are launched over a tile table that may span many
-// shots (ragged windows):
-//   velocity phase : vx, vz from p       (kernel.cu:430-500, 542-713 forward;
-//                                          kernel.cu:318-388 reverse)
-//   pressure phase : p from vx, vz, PML split pair, source, record, strips,
-//                    receiver insertion, imaging (kernel.cu:501-541, 714-1185,
-//                    36-95, 1186-1416)
-// A tile is 64 columns (one wave64 row: every array is x-fastest, so each
-// wave touches one contiguous 256-byte row segment) by TZ rows.  The operand
-// field is staged once in LDS with its radius-10 halo; x-derivatives read LDS
-// rows, z-derivatives a register window of the thread's own column (each
-// thread owns TZ/4 consecutive rows).  Fields carry zero margins (MZT rows
-// above, MXL columns left, more below/right), so staging needs no bounds
-// checks.  Per-tile flags computed on the host keep the strip / source /
-// receiver logic out of tiles that do not touch them.
+// One launch per time step.  A workgroup owns a tile of 64 columns (one
+// wave64 row: every array is x-fastest, so a wave touches one contiguous
+// 256-byte row segment) by TZ rows of one shot, and does the whole step for
+// it: the velocity update (kernel.cu:430-500, 542-713 forward; 318-388
+// reverse) of the tile and of the radius-10 halo that the pressure stencil
+// reads, then the pressure update (kernel.cu:501-541, 714-1185 forward;
+// 389-429 reverse) with source (kernel.cu:36-47), recording (48-71),
+// receiver insertion (72-95), boundary strips (1186-1400) and imaging
+// (1401-1416).  The halo velocities are computed again by the neighbouring
+// tiles instead of being read back, so a step reads p, vx, vz and the model
+// once and writes p, vx, vz once per cell.  Fields have two time levels
+// (step it reads level it&1 and writes level (it+1)&1), so a tile's reads of
+// its neighbours' cells never race with their writes.
 //
-// The per-cell arithmetic follows the reference operation order with the
-// fused multiply-adds written explicitly (the oracle's canonical FP32 form),
-// so results match the CPU restatement bit for bit.  Compiled with
-// -ffp-contract=off; float division is IEEE as HIP compiles it.
+// LDS: the pressure of the tile and its halo (columns x0-20..x0+83 of the
+// tile rows, rows z0-19..z0+TZ+18 of the tile columns), then -- reusing the
+// same bytes -- the new velocities (vx: columns x0-10..x0+73; vz: rows
+// z0-10..z0+TZ+8).  x-derivatives read LDS rows; z-derivatives read a
+// register window of the thread's own column.  The reverse step runs the
+// source reconstruction and then the receiver wavefield through the same
+// LDS; the imaging products use the reconstructed pressure from registers.
+//
+// Arithmetic follows the reference operation order with explicit fused
+// multiply-adds (the oracle's canonical FP32 form), so every cell matches the
+// CPU restatement bit for bit; the redundant halo velocities are the same
+// operations on the same inputs.  Compiled with -ffp-contract=off; float
+// division is IEEE as HIP compiles it.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
 #include "rtm_dev.h"
 
-#define LW 83              // 64 + 19: staged width incl. the x halo
-#define NTHR 256
+#define NTHR 256   // threads of the auxiliary kernels
 
 // Pointers read from the descriptor are generic to the compiler, which would
 // then emit flat loads/stores (they count against the LDS wait counter too,
@@ -56,87 +62,55 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
   return t < ntiles ? t : -1;
 }
 
-__device__ __forceinline__ float model_at(const float* __restrict__ m, const ShotDev& s, int iz, int ix)
-{
-  const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
-  const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
-  return GCF(m)[(size_t)r * s.mpitch + c];
-}
-
-// stage f rows zs..zs+H-1, columns xs..xs+W-1 (W <= 128) into L[H][W]; all
-// global loads are issued before the LDS stores.
-template <int W, int H>
-__device__ __forceinline__ void stage(float* __restrict__ L, const float* __restrict__ f, int pitch, int zs, int xs)
-{
-  constexpr int K = (H + 3) / 4;
-  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
-  float a[K], b[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    const int r = ly + 4 * k;
-    const gfloat* src = GCF(f) + (ptrdiff_t)(zs + imin(r, H - 1)) * pitch + xs;
-    a[k] = src[lx];
-    b[k] = (W > 64 && lx < W - 64) ? src[64 + lx] : 0.0f;
-  }
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    const int r = ly + 4 * k;
-    if (r < H) {
-      L[r * W + lx] = a[k];
-      if (W > 64 && lx < W - 64) L[r * W + 64 + lx] = b[k];
-    }
-  }
-}
-
 // ---------------------------------------------------------------- strips --
 // store offsets of the boundary strips (kernel.cu:1186-1293 packing); -1 if
 // the cell is not part of the block.
-__device__ __forceinline__ long long strip_vx(const ShotDev& s, int iz, int ix)
+__device__ __forceinline__ int strip_vx(const ShotDev& s, int iz, int ix)
 {
   if (iz < s.npml || iz >= s.nzp - s.npml) return -1;
   const int r = iz - s.npml;
-  if (ix >= s.npml - RTM_R && ix < s.npml) return (long long)r * RTM_R + ix - (s.npml - RTM_R);
+  if (ix >= s.npml - RTM_R && ix < s.npml) return r * RTM_R + ix - (s.npml - RTM_R);
   const int c0 = s.nxp - s.npml - 1;
-  if (ix >= c0 && ix < c0 + RTM_R) return (long long)RTM_R * s.nzi + (long long)r * RTM_R + ix - c0;
+  if (ix >= c0 && ix < c0 + RTM_R) return RTM_R * s.nzi + r * RTM_R + ix - c0;
   return -1;
 }
-__device__ __forceinline__ long long strip_vz(const ShotDev& s, int iz, int ix)
+__device__ __forceinline__ int strip_vz(const ShotDev& s, int iz, int ix)
 {
   if (ix < s.npml || ix >= s.nxp - s.npml) return -1;
-  const long long base = 2LL * RTM_R * s.nzi;
+  const int base = 2 * RTM_R * s.nzi;
   const int c = ix - s.npml;
-  if (iz >= s.npml - RTM_R && iz < s.npml) return base + (long long)(iz - (s.npml - RTM_R)) * s.nxl + c;
+  if (iz >= s.npml - RTM_R && iz < s.npml) return base + (iz - (s.npml - RTM_R)) * s.nxl + c;
   const int r0 = s.nzp - s.npml - 1;
-  if (iz >= r0 && iz < r0 + RTM_R) return base + (long long)RTM_R * s.nxl + (long long)(iz - r0) * s.nxl + c;
+  if (iz >= r0 && iz < r0 + RTM_R) return base + RTM_R * s.nxl + (iz - r0) * s.nxl + c;
   return -1;
 }
-__device__ __forceinline__ long long strip_p(const ShotDev& s, int iz, int ix)
+__device__ __forceinline__ int strip_p(const ShotDev& s, int iz, int ix)
 {
-  const long long base_lr = 2LL * RTM_R * s.nzi + 2LL * RTM_R * s.nxl;
-  const long long base_tb = base_lr + 2LL * RTM_R * s.nzi;
+  const int base_lr = 2 * RTM_R * s.nzi + 2 * RTM_R * s.nxl;
+  const int base_tb = base_lr + 2 * RTM_R * s.nzi;
   if (iz >= s.npml && iz < s.nzp - s.npml) {
     const int r = iz - s.npml;
-    if (ix >= s.npml - RTM_R && ix < s.npml) return base_lr + (long long)r * RTM_R + ix - (s.npml - RTM_R);
+    if (ix >= s.npml - RTM_R && ix < s.npml) return base_lr + r * RTM_R + ix - (s.npml - RTM_R);
     const int c0 = s.nxp - s.npml;
-    if (ix >= c0 && ix < c0 + RTM_R) return base_lr + (long long)RTM_R * s.nzi + (long long)r * RTM_R + ix - c0;
+    if (ix >= c0 && ix < c0 + RTM_R) return base_lr + RTM_R * s.nzi + r * RTM_R + ix - c0;
     return -1;
   }
   if (ix < s.npml || ix >= s.nxp - s.npml) return -1;
   const int c = ix - s.npml;
-  if (iz >= s.npml - RTM_R && iz < s.npml) return base_tb + (long long)(iz - (s.npml - RTM_R)) * s.nxl + c;
+  if (iz >= s.npml - RTM_R && iz < s.npml) return base_tb + (iz - (s.npml - RTM_R)) * s.nxl + c;
   const int r0 = s.nzp - s.npml;
-  if (iz >= r0 && iz < r0 + RTM_R) return base_tb + (long long)RTM_R * s.nxl + (long long)(iz - r0) * s.nxl + c;
+  if (iz >= r0 && iz < r0 + RTM_R) return base_tb + RTM_R * s.nxl + (iz - r0) * s.nxl + c;
   return -1;
 }
 
 // 16-bit store: block of a strip offset (0 vx, 1 vz, 2 p-lr, 3 p-tb)
-__device__ __forceinline__ int strip_block(const ShotDev& s, long long off)
+__device__ __forceinline__ int strip_block(const ShotDev& s, int off)
 {
-  const long long a = 2LL * RTM_R * s.nzi, b = a + 2LL * RTM_R * s.nxl, c = b + 2LL * RTM_R * s.nzi;
+  const int a = 2 * RTM_R * s.nzi, b = a + 2 * RTM_R * s.nxl, c = b + 2 * RTM_R * s.nzi;
   return off < a ? 0 : (off < b ? 1 : (off < c ? 2 : 3));
 }
 
-__device__ __forceinline__ void store_put(const ShotDev& s, int it, long long off, float v)
+__device__ __forceinline__ void store_put(const ShotDev& s, int it, int off, float v)
 {
   if (s.store_fmt == ST_F32) {
     GF(s.store)[(long long)it * s.strip_len + off] = v;
@@ -149,7 +123,7 @@ __device__ __forceinline__ void store_put(const ShotDev& s, int it, long long of
   }
 }
 
-__device__ __forceinline__ float store_get(const ShotDev& s, int it, long long off)
+__device__ __forceinline__ float store_get(const ShotDev& s, int it, int off)
 {
   const long long k = (long long)it * s.strip_len + off;
   if (s.store_fmt == ST_F32) return GCF(s.store)[k];
@@ -159,156 +133,264 @@ __device__ __forceinline__ float store_get(const ShotDev& s, int it, long long o
   return __uint_as_float(((unsigned int)h) << 16) * sc;
 }
 
-// radius-r staggered difference a[i] - b[i] with the coefficient row of order
-// 2r; the fma chain runs i = 1..r as the reference loops (kernel.cu:459-460)
-#define STENCIL_SUM(acc, r, c10, cf, A, B)                                                 \
-  do {                                                                                     \
-    if ((r) == RTM_R) {                                                                    \
+// ------------------------------------------------------------ tile shape --
+// G waves of 64 lanes; each thread owns RPT consecutive rows of one column.
+template <int G_, int RPT_>
+struct Geo {
+  static constexpr int G = G_, RPT = RPT_;
+  static constexpr int TZ = G * RPT;                   // tile rows
+  static constexpr int NT = 64 * G;                    // threads
+  static constexpr int PCW = 104;                      // p row: columns x0-20 .. x0+83
+  static constexpr int VXW = 84;                       // vx row: columns x0-10 .. x0+73
+  static constexpr int ZR = TZ + 19;                   // vz rows z0-10 .. z0+TZ+8
+  static constexpr int KZ = (ZR + G - 1) / G;          // vz rows per thread (consecutive)
+  static constexpr int PZR = G * KZ + 19;              // p column rows held (>= TZ + 38)
+  static constexpr int KP = (TZ + 38 + G - 1) / G;     // p column rows staged per thread
+  static constexpr int KX = (TZ * VXW + NT - 1) / NT;  // vx cells per thread
+  static constexpr int LDS_A = PZR * 64 + TZ * PCW;    // PZ | PC
+  static constexpr int LDS_B = TZ * VXW + ZR * 64;     // VXL | VZL
+  static constexpr int LDS = LDS_A > LDS_B ? LDS_A : LDS_B;
+};
+
+// radius-r staggered difference a[i] - b[i] with the coefficient row of
+// order 2r; the fma chain runs i = 1..r as the reference loops
+// (kernel.cu:459-460).  The loop is unrolled with a predicate so that the
+// operand windows stay in registers (a runtime index would put them in
+// scratch memory).
+#define STENCIL_SUM(acc, r, c10, cf, A, B)                                                        \
+  do {                                                                                            \
+    if ((r) == RTM_R) {                                                                           \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], (A) - (B), acc); \
-    } else {                                                                               \
-      const gfloat* cr_ = GCF(cf) + ((r) - 1) * RTM_R;                                     \
-      for (int i = 1; i <= (r); i++) acc = __builtin_fmaf(cr_[i - 1], (A) - (B), acc);     \
-    }                                                                                      \
+    } else {                                                                                      \
+      const gfloat* cr_ = GCF(cf) + ((r) - 1) * RTM_R;                                            \
+      _Pragma("unroll") for (int i = 1; i <= RTM_R; i++)                                          \
+        if (i <= (r)) acc = __builtin_fmaf(cr_[i - 1], (A) - (B), acc);                           \
+    }                                                                                             \
   } while (0)
 
-// ------------------------------------------------------- velocity phase --
-// Forward velocity update of one wavefield over a tile of the padded grid:
-//   vx(iz,ix), ix <= nxp-2, radius min(10, ix+1, nxp-1-ix)   (norderx, kernel.cu:299-307)
-//   vz(iz,ix), iz <= nzp-2, radius min(10, iz+1, nzp-1-iz)   (norderz, kernel.cu:309-317)
-//   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho  (kernel.cu:588-594);
-//   c1 = c2 = 1 off the absorbing layer reduces it to v - D (kernel.cu:462).
-// L holds p rows z0-9 .. z0+TZ+9, columns x0-9 .. x0+73.
-template <int TZ>
-__device__ __forceinline__ void vel_fwd(const float* __restrict__ L, float* __restrict__ vx, float* __restrict__ vz,
-                                        const ShotDev& s, int x0, int z0, int save_it, int flags,
-                                        const float (&vxo)[TZ / 4], const float (&vzo)[TZ / 4],
-                                        const float (&rho)[TZ / 4])
+// field kinds of one pass
+enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
+
+// Stage p rows z0-19 .. z0+TZ+18 of the tile columns into PZ (pitch 64) and
+// the tile rows with their x halo into PC (pitch 104, column c = x-(x0-20)).
+template <class Q>
+__device__ __forceinline__ void stage_p(float* __restrict__ PZ, float* __restrict__ PC, const float* __restrict__ p,
+                                        int pitch, int x0, int z0, int lx, int g)
 {
-  constexpr int RPT = TZ / 4;
-  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
-  const int ix = x0 + lx;
-  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch;
-  if (ix >= nxp) return;
-  const float* __restrict__ cf = s.coef;
-  float c10[RTM_R];
+  float a[Q::KP], w[Q::RPT];
+  const gfloat* base = GCF(p) + (ptrdiff_t)(z0 - 19) * pitch + x0;
+  const int wc = lx < 20 ? lx - 20 : lx + 44;          // wing columns (lanes < 40)
 #pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(cf)[90 + i];
-  const bool hasvx = ix <= nxp - 2;
-  const int rx = imin(RTM_R, imin(ix + 1, nxp - 1 - ix));
-  const float c1x = GCF(s.c1xv)[ix], c2x = GCF(s.c2xv)[ix];
-  const bool dampx = !(c1x == 1.0f && c2x == 1.0f);
-  float col[RPT + 19];                                 // p(z0+j0-9+k, ix)
+  for (int m = 0; m < Q::KP; m++) {
+    const int k = g + Q::G * m;
+    a[m] = 0.0f;
+    if (k < Q::TZ + 38) a[m] = base[k * pitch + lx];
+  }
 #pragma unroll
-  for (int k = 0; k < RPT + 19; k++) col[k] = L[(j0 + k) * LW + lx + 9];
+  for (int m = 0; m < Q::RPT; m++) {
+    const int j = g + Q::G * m;                        // tile row
+    w[m] = 0.0f;
+    if (lx < 40) w[m] = base[(j + 19) * pitch + wc];
+  }
 #pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int iz = z0 + j0 + r;
-    if (iz >= nzp) break;
-    const size_t id = (size_t)iz * P + ix;
-    if (hasvx) {
-      const float* __restrict__ Lr = L + (j0 + r + 9) * LW + lx + 9;
-      float acc = 0.0f;
-      STENCIL_SUM(acc, rx, c10, cf, Lr[i], Lr[1 - i]);
-      acc = acc * s.dtx;
-      acc = acc / rho[r];
-      const float v = dampx ? __builtin_fmaf(c1x, vxo[r], -acc) / c2x : vxo[r] - acc;
-      GF(vx)[id] = v;
-      if (save_it >= 0 && (flags & TF_STRIP)) {
-        const long long o = strip_vx(s, iz, ix);
-        if (o >= 0) store_put(s, save_it, o, v);
-      }
+  for (int m = 0; m < Q::KP; m++) {
+    const int k = g + Q::G * m;
+    if (k < Q::TZ + 38) {
+      PZ[k * 64 + lx] = a[m];
+      if (k >= 19 && k < 19 + Q::TZ) PC[(k - 19) * Q::PCW + 20 + lx] = a[m];
     }
-    if (iz <= nzp - 2) {
-      const int rz = imin(RTM_R, imin(iz + 1, nzp - 1 - iz));
-      const float c1z = GCF(s.c1zv)[iz], c2z = GCF(s.c2zv)[iz];
-      float acc = 0.0f;
-      STENCIL_SUM(acc, rz, c10, cf, col[r + 9 + i], col[r + 10 - i]);
-      acc = acc * s.dtz;
-      acc = acc / rho[r];
-      const float v = (c1z == 1.0f && c2z == 1.0f) ? vzo[r] - acc : __builtin_fmaf(c1z, vzo[r], -acc) / c2z;
-      GF(vz)[id] = v;
-      if (save_it >= 0 && (flags & TF_STRIP)) {
-        const long long o = strip_vz(s, iz, ix);
-        if (o >= 0) store_put(s, save_it, o, v);
-      }
-    }
+  }
+#pragma unroll
+  for (int m = 0; m < Q::RPT; m++) {
+    const int j = g + Q::G * m;
+    if (lx < 40) PC[j * Q::PCW + (lx < 20 ? lx : lx + 64)] = w[m];
   }
 }
 
-// Reverse velocity update of the source wavefield (kernel.cu:318-388 with the
-// strip restore of kernel.cu:1294-1347, rtm_model.cu:424-429): strips of step
-// it+1 are written back, interior cells get v += D.
-template <int TZ>
-__device__ __forceinline__ void vel_rev(const float* __restrict__ L, float* __restrict__ vx, float* __restrict__ vz,
-                                        const ShotDev& s, int x0, int z0, int it, int flags,
-                                        const float (&vxo)[TZ / 4], const float (&vzo)[TZ / 4],
-                                        const float (&rho)[TZ / 4])
+// ---------------------------------------------------------- velocities --
+// vz of rows z0-10+kz (kz = g*KZ + k) of the thread's column, vx of the
+// cells c = k*NT + tid of the TZ x 84 block.  Forward-type fields (source in
+// the forward phase, receiver in the reverse phase):
+//   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho
+//   (kernel.cu:588-594); c1 = c2 = 1 off the absorbing layer reduces it to
+//   v - D (kernel.cu:462); radius min(10, distance to the grid edge)
+//   (cuda_norderx/z, kernel.cu:299-317).
+// Reconstruction: strip cells restored from the store of step it+1
+// (kernel.cu:1294-1347), interior cells v + D (kernel.cu:318-388).
+template <class Q>
+struct VelOut {
+  float vz[Q::KZ];
+  float vx[Q::KX];
+};
+
+template <class Q, int FK>
+__device__ __forceinline__ void velocities(VelOut<Q>& o, const float* __restrict__ PZ, const float* __restrict__ PC,
+                                           const ShotDev& s, const Tile& tl, const float* __restrict__ vxc,
+                                           const float* __restrict__ vzc, float* __restrict__ vxn,
+                                           float* __restrict__ vzn, int it, int save, int lx, int g, int tid)
 {
-  constexpr int RPT = TZ / 4;
-  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
-  const int ix = x0 + lx;
-  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
-  if (ix >= nxp) return;
+  const int x0 = tl.x0, z0 = tl.z0, P = s.pitch, nxp = s.nxp, nzp = s.nzp, npml = s.npml;
+  const bool gen = (tl.kind & TF_GEN) != 0;
+  const bool strips = (tl.kind & TF_STRIP) != 0;
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
-  const bool xin_vx = ix >= npml && ix <= nxp - npml - 2;
-  const bool xin_vz = ix >= npml && ix < nxp - npml;
-  float col[RPT + 19];
+  const float* __restrict__ cf = s.coef;
+  const float dtx = s.dtx, dtz = s.dtz;
+
+  // ---- loads of the old velocities and density (issued with the staging)
+  float vzo[Q::KZ], rz[Q::KZ], vxo[Q::KX], rx[Q::KX];
+  const int ix = x0 + lx;
 #pragma unroll
-  for (int k = 0; k < RPT + 19; k++) col[k] = L[(j0 + k) * LW + lx + 9];
-#pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int iz = z0 + j0 + r;
-    if (iz >= nzp) break;
-    const size_t id = (size_t)iz * P + ix;
-    const long long ovx = (flags & TF_STRIP) ? strip_vx(s, iz, ix) : -1;
-    if (ovx >= 0) {
-      GF(vx)[id] = store_get(s, it + 1, ovx);
-    } else if (xin_vx && iz >= npml && iz < nzp - npml) {
-      const float* __restrict__ Lr = L + (j0 + r + 9) * LW + lx + 9;
-      float acc = 0.0f;
-#pragma unroll
-      for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Lr[i] - Lr[1 - i], acc);
-      acc = acc * s.dtx;
-      acc = acc / rho[r];
-      GF(vx)[id] = vxo[r] + acc;
+  for (int k = 0; k < Q::KZ; k++) {
+    const int kz = g * Q::KZ + k;
+    vzo[k] = 0.0f;
+    rz[k] = 1.0f;
+    if (kz < Q::ZR) {
+      const int id = (z0 - 10 + kz) * P + ix;
+      vzo[k] = GCF(vzc)[id];
+      rz[k] = GCF(s.mrho)[id];
     }
-    const long long ovz = (flags & TF_STRIP) ? strip_vz(s, iz, ix) : -1;
-    if (ovz >= 0) {
-      GF(vz)[id] = store_get(s, it + 1, ovz);
-    } else if (xin_vz && iz >= npml && iz <= nzp - npml - 2) {
-      float acc = 0.0f;
+  }
 #pragma unroll
-      for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], col[r + 9 + i] - col[r + 10 - i], acc);
-      acc = acc * s.dtz;
-      acc = acc / rho[r];
-      GF(vz)[id] = vzo[r] + acc;
+  for (int k = 0; k < Q::KX; k++) {
+    const int c = k * Q::NT + tid;
+    vxo[k] = 0.0f;
+    rx[k] = 1.0f;
+    if (c < Q::TZ * Q::VXW) {
+      const int row = c / Q::VXW, col = c - row * Q::VXW;
+      const int id = (z0 + row) * P + x0 - 10 + col;
+      vxo[k] = GCF(vxc)[id];
+      rx[k] = GCF(s.mrho)[id];
     }
+  }
+  __syncthreads();                                     // p staged
+
+  // ---- vz
+  {
+    float cw[Q::KZ + 19];                              // p rows z0-19+g*KZ+m of the column
+#pragma unroll
+    for (int m = 0; m < Q::KZ + 19; m++) cw[m] = PZ[(g * Q::KZ + m) * 64 + lx];
+#pragma unroll
+    for (int k = 0; k < Q::KZ; k++) {
+      const int kz = g * Q::KZ + k;
+      const int iz = z0 - 10 + kz;
+      float v = 0.0f;
+      if (kz < Q::ZR) {
+        if (FK == FK_RCN) {
+          const int so = strips ? strip_vz(s, iz, ix) : -1;
+          if (so >= 0) {
+            v = store_get(s, it + 1, so);
+          } else if (!gen || (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2)) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], cw[k + 9 + i] - cw[k + 10 - i], acc);
+            acc = acc * dtz;
+            acc = acc / rz[k];
+            v = vzo[k] + acc;
+          }
+        } else if (!gen) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], cw[k + 9 + i] - cw[k + 10 - i], acc);
+          acc = acc * dtz;
+          acc = acc / rz[k];
+          v = vzo[k] - acc;
+        } else if (iz >= 0 && iz <= nzp - 2 && ix < nxp) {
+          const int rr = imin(RTM_R, imin(iz + 1, nzp - 1 - iz));
+          float acc = 0.0f;
+          STENCIL_SUM(acc, rr, c10, cf, cw[k + 9 + i], cw[k + 10 - i]);
+          acc = acc * dtz;
+          acc = acc / rz[k];
+          const float c1 = GCF(s.c1zv)[iz], c2 = GCF(s.c2zv)[iz];
+          v = (c1 == 1.0f && c2 == 1.0f) ? vzo[k] - acc : __builtin_fmaf(c1, vzo[k], -acc) / c2;
+        }
+        // own rows: write the new level (cells that exist for vz)
+        if (kz >= 10 && kz < 10 + Q::TZ && ix < nxp && iz <= nzp - 2) {
+          GF(vzn)[iz * P + ix] = v;
+          if (FK == FK_SRC && save && strips) {
+            const int so = strip_vz(s, iz, ix);
+            if (so >= 0) store_put(s, it, so, v);
+          }
+        }
+      }
+      o.vz[k] = v;
+    }
+  }
+  // ---- vx
+#pragma unroll
+  for (int k = 0; k < Q::KX; k++) {
+    const int c = k * Q::NT + tid;
+    float v = 0.0f;
+    if (c < Q::TZ * Q::VXW) {
+      const int row = c / Q::VXW, col = c - row * Q::VXW;
+      const int iz = z0 + row, jx = x0 - 10 + col;
+      const float* __restrict__ Pr = PC + row * Q::PCW + col + 10;   // p at column jx
+      if (FK == FK_RCN) {
+        const int so = strips ? strip_vx(s, iz, jx) : -1;
+        if (so >= 0) {
+          v = store_get(s, it + 1, so);
+        } else if (!gen || (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml)) {
+          float acc = 0.0f;
+#pragma unroll
+          for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Pr[i] - Pr[1 - i], acc);
+          acc = acc * dtx;
+          acc = acc / rx[k];
+          v = vxo[k] + acc;
+        }
+      } else if (!gen) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Pr[i] - Pr[1 - i], acc);
+        acc = acc * dtx;
+        acc = acc / rx[k];
+        v = vxo[k] - acc;
+      } else if (jx >= 0 && jx <= nxp - 2 && iz < nzp) {
+        const int rr = imin(RTM_R, imin(jx + 1, nxp - 1 - jx));
+        float acc = 0.0f;
+        STENCIL_SUM(acc, rr, c10, cf, Pr[i], Pr[1 - i]);
+        acc = acc * dtx;
+        acc = acc / rx[k];
+        const float c1 = GCF(s.c1xv)[jx], c2 = GCF(s.c2xv)[jx];
+        v = (c1 == 1.0f && c2 == 1.0f) ? vxo[k] - acc : __builtin_fmaf(c1, vxo[k], -acc) / c2;
+      }
+      if (col >= 10 && col < 74 && iz < nzp && jx <= nxp - 2) {
+        GF(vxn)[iz * P + jx] = v;
+        if (FK == FK_SRC && save && strips) {
+          const int so = strip_vx(s, iz, jx);
+          if (so >= 0) store_put(s, it, so, v);
+        }
+      }
+    }
+    o.vx[k] = v;
   }
 }
 
-// ------------------------------------------------------- pressure phase --
-// VX rows z0..z0+TZ-1, columns x0-10..x0+72; VZ rows z0-10..z0+TZ+8, columns x0..x0+63.
+// ------------------------------------------------------------ pressure --
 struct DivV { float dxv, dzv; };
 
-template <int RPT>
-__device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float (&col)[RPT + 19], int r, int rad,
-                                      const float* c10, const float* __restrict__ cf,
-                                      float dtx, float dtz, float rho, float vv)
+// VXr points at vx of the cell's column in its LDS row; cz is the column
+// window of vz rows; r the thread-local row
+template <int NW>
+__device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float (&cz)[NW], int r, int rad,
+                                      const float* c10, const float* __restrict__ cf, float dtx, float dtz,
+                                      float rho, float vv)
 {
   float sx = 0.0f, sz = 0.0f;
   if (rad == RTM_R) {
 #pragma unroll
     for (int i = 1; i <= RTM_R; i++) {
       sx = __builtin_fmaf(c10[i - 1], VXr[i - 1] - VXr[-i], sx);
-      sz = __builtin_fmaf(c10[i - 1], col[r + 9 + i] - col[r + 10 - i], sz);
+      sz = __builtin_fmaf(c10[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
     }
   } else {
     const gfloat* c = GCF(cf) + (rad - 1) * RTM_R;
-    for (int i = 1; i <= rad; i++) {
-      sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
-      sz = __builtin_fmaf(c[i - 1], col[r + 9 + i] - col[r + 10 - i], sz);
+#pragma unroll
+    for (int i = 1; i <= RTM_R; i++) {
+      if (i <= rad) {
+        sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
+        sz = __builtin_fmaf(c[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
+      }
     }
   }
   DivV d;
@@ -317,261 +399,191 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
   return d;
 }
 
-// forward pressure update of one cell.  Interior: p -= dxv + dzv
-// (kernel.cu:539).  PML: split pair, p = p1 + p2 (kernel.cu:780-783).
-__device__ __forceinline__ float pres_cell(float p, float* __restrict__ p1, float* __restrict__ p2, size_t id,
-                                           bool inner, const DivV& d, float c1x, float c2x, float c1z, float c2z)
+// One field through one time step on a tile: p of the current level pc (with
+// velocities vxc, vzc) to the next level (pn, vxn, vzn).  On return pnew
+// holds the new pressure of the thread's own cells (rows g*RPT+r); ps is the
+// reconstructed source pressure of the same cells (receiver pass imaging).
+template <class Q, int FK>
+__device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDev& s, const Tile& tl, int it, int save,
+                                           const float* __restrict__ pc, const float* __restrict__ vxc,
+                                           const float* __restrict__ vzc, float* __restrict__ pn,
+                                           float* __restrict__ vxn, float* __restrict__ vzn,
+                                           const float (&ps)[Q::RPT], float (&pnew)[Q::RPT])
 {
-  if (inner) return p - (d.dxv + d.dzv);
-  float q1 = GCF(p1)[id], q2 = GCF(p2)[id];
-  q1 = __builtin_fmaf(c1x, q1, -d.dxv) / c2x;
-  q2 = __builtin_fmaf(c1z, q2, -d.dzv) / c2z;
-  GF(p1)[id] = q1; GF(p2)[id] = q2;
-  return q1 + q2;
-}
-
-template <int TZ>
-__device__ __forceinline__ void stage_v(float* __restrict__ VX, float* __restrict__ VZ, const float* __restrict__ vx,
-                                        const float* __restrict__ vz, int pitch, int x0, int z0)
-{
-  stage<LW, TZ>(VX, vx, pitch, z0, x0 - 10);
-  stage<64, TZ + 19>(VZ, vz, pitch, z0 - 10, x0);
-}
-
-template <int RPT>
-__device__ __forceinline__ void load_col64(float (&col)[RPT + 19], const float* __restrict__ VZ, int j0, int lx)
-{
-#pragma unroll
-  for (int k = 0; k < RPT + 19; k++) col[k] = VZ[(j0 + k) * 64 + lx];
-}
-
-// forward pressure phase of the source wavefield: FD update, source
-// injection (kernel.cu:36-47), recording (kernel.cu:48-71), p strips.
-template <int TZ>
-__device__ __forceinline__ void pres_fwd_src(const float* __restrict__ VX, const float* __restrict__ VZ,
-                                             const ShotDev& s, int x0, int z0, int it, int save, int flags,
-                                             const float (&po)[TZ / 4], const float (&rho)[TZ / 4],
-                                             const float (&vv)[TZ / 4])
-{
-  constexpr int RPT = TZ / 4;
-  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
+  float* PZ = lds;
+  float* PC = lds + Q::PZR * 64;
+  float* VXL = lds;
+  float* VZL = lds + Q::TZ * Q::VXW;
+  const int tid = threadIdx.x, lx = tid & 63;
+  const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int x0 = tl.x0, z0 = tl.z0, P = s.pitch, nxp = s.nxp, nzp = s.nzp, npml = s.npml;
   const int ix = x0 + lx;
-  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
-  if (ix >= nxp) return;
+  const bool gen = (tl.kind & TF_GEN) != 0;
+
+  stage_p<Q>(PZ, PC, pc, P, x0, z0, lx, g);
+  VelOut<Q> vo;
+  velocities<Q, FK>(vo, PZ, PC, s, tl, vxc, vzc, vxn, vzn, it, save, lx, g, tid);
+
+  // own old pressure out of LDS, model of the own cells
+  float po[Q::RPT], rho[Q::RPT], vv[Q::RPT];
+#pragma unroll
+  for (int r = 0; r < Q::RPT; r++) {
+    const int j = g * Q::RPT + r;
+    po[r] = PC[j * Q::PCW + 20 + lx];
+    const int id = (z0 + j) * P + ix;
+    rho[r] = GCF(s.mrho)[id];
+    vv[r] = GCF(s.mvv)[id];
+  }
+  __syncthreads();                                     // PZ/PC consumed
+#pragma unroll
+  for (int k = 0; k < Q::KZ; k++) {
+    const int kz = g * Q::KZ + k;
+    if (kz < Q::ZR) VZL[kz * 64 + lx] = vo.vz[k];
+  }
+#pragma unroll
+  for (int k = 0; k < Q::KX; k++) {
+    const int c = k * Q::NT + tid;
+    if (c < Q::TZ * Q::VXW) VXL[c] = vo.vx[k];
+  }
+  __syncthreads();                                     // new velocities staged
+
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
-  const float c1x = GCF(s.c1x)[ix], c2x = GCF(s.c2x)[ix];
+  float cz[Q::RPT + 19];
+#pragma unroll
+  for (int m = 0; m < Q::RPT + 19; m++) cz[m] = VZL[(g * Q::RPT + m) * 64 + lx];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
-  float col[RPT + 19];
-  load_col64<RPT>(col, VZ, j0, lx);
+  float c1x = 1.0f, c2x = 1.0f;
+  if (FK != FK_RCN && gen && ix < nxp) { c1x = GCF(s.c1x)[ix]; c2x = GCF(s.c2x)[ix]; }
+  const bool xsrc = (tl.kind & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
+  const bool strips = (tl.kind & TF_STRIP) != 0;
+
 #pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int iz = z0 + j0 + r;
-    if (iz >= nzp) break;
-    const size_t id = (size_t)iz * P + ix;
+  for (int r = 0; r < Q::RPT; r++) {
+    const int j = g * Q::RPT + r;
+    const int iz = z0 + j;
     float p = po[r];
-    if (!xring && iz > 0 && iz < nzp - 1) {
-      const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
-      const DivV d = div_v<RPT>(VX + (j0 + r) * LW + lx + 10, col, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+    const float* __restrict__ VXr = VXL + j * Q::VXW + lx + 10;
+    const bool incell = ix < nxp && iz < nzp;
+    if (FK == FK_RCN) {
       const bool inner = xin && iz >= npml && iz < nzp - npml;
-      p = pres_cell(p, s.sp1, s.sp2, id, inner, d, c1x, c2x, GCF(s.c1z)[iz], GCF(s.c2z)[iz]);
-    }
-    if ((flags & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && ix >= s.sx0 && ix <= s.sx1)
-      p = __builtin_fmaf(GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
-    GF(s.sp)[id] = p;
-    if ((flags & TF_REC) && s.record && iz == npml) {
-      const int ir = ix - s.rcol0;
-      if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
-    }
-    if (save && (flags & TF_STRIP)) {
-      const long long o = strip_p(s, iz, ix);
-      if (o >= 0) store_put(s, it, o, p);
-    }
-  }
-}
-
-// reverse pressure phase: source reconstruction (kernel.cu:389-429, strip
-// restore kernel.cu:1348-1400, source removal rtm_model.cu:437), receiver
-// forward step + Dirichlet insertion (kernel.cu:72-95) and imaging
-// (kernel.cu:1401-1416).
-template <int TZ>
-__device__ __forceinline__ void pres_rev(const float* __restrict__ SX, const float* __restrict__ SZ,
-                                         const float* __restrict__ RX, const float* __restrict__ RZ,
-                                         const ShotDev& s, int x0, int z0, int it, int flags,
-                                         const float (&pso)[TZ / 4], const float (&pro)[TZ / 4],
-                                         const float (&rho)[TZ / 4], const float (&vv)[TZ / 4])
-{
-  constexpr int RPT = TZ / 4;
-  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
-  const int ix = x0 + lx;
-  const int nxp = s.nxp, nzp = s.nzp, P = s.pitch, npml = s.npml;
-  if (ix >= nxp) return;
-  float c10[RTM_R];
-#pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
-  const float c1x = GCF(s.c1x)[ix], c2x = GCF(s.c2x)[ix];
-  const bool xin = ix >= npml && ix < nxp - npml;
-  const bool xring = ix == 0 || ix == nxp - 1;
-  const bool xsrc = (flags & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
-  float cs[RPT + 19], cr[RPT + 19];
-  load_col64<RPT>(cs, SZ, j0, lx);
-  load_col64<RPT>(cr, RZ, j0, lx);
-#pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int iz = z0 + j0 + r;
-    if (iz >= nzp) break;
-    const size_t id = (size_t)iz * P + ix;
-    const bool inner = xin && iz >= npml && iz < nzp - npml;
-    // source wavefield (interior + strips only; the rest is never read)
-    float ps = 0.0f;
-    {
-      const long long o = (flags & TF_STRIP) ? strip_p(s, iz, ix) : -1;
+      const int so = strips ? strip_p(s, iz, ix) : -1;
       bool wr = false;
-      if (o >= 0) {
-        ps = store_get(s, it, o);
+      if (so >= 0) {
+        p = store_get(s, it, so);                      // kernel.cu:1348-1400
         wr = true;
-      } else if (inner) {
-        const DivV d = div_v<RPT>(SX + (j0 + r) * LW + lx + 10, cs, r, RTM_R, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
-        ps = pso[r] + (d.dxv + d.dzv);
+      } else if (!gen || inner) {
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+        p = p + (d.dxv + d.dzv);                       // kernel.cu:389-429
         wr = true;
       }
-      if (wr) {
-        if (xsrc && iz >= s.sz0 && iz <= s.sz1)
-          ps = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], ps);
-        GF(s.sp)[id] = ps;
+      if (wr && xsrc && iz >= s.sz0 && iz <= s.sz1)    // source removal (rtm_model.cu:437)
+        p = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
+      if (wr) GF(pn)[iz * P + ix] = p;
+    } else {
+      if (!gen) {
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+        p = p - (d.dxv + d.dzv);                       // kernel.cu:539
+      } else if (incell && !xring && iz > 0 && iz < nzp - 1) {
+        const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
+        const DivV d = div_v(VXr, cz, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+        if (xin && iz >= npml && iz < nzp - npml) {
+          p = p - (d.dxv + d.dzv);
+        } else {                                       // split pair, p = p1 + p2 (kernel.cu:780-783)
+          const int id = iz * P + ix;
+          float q1 = GCF(s.q1)[id], q2 = GCF(s.q2)[id];
+          q1 = __builtin_fmaf(c1x, q1, -d.dxv) / c2x;
+          q2 = __builtin_fmaf(GCF(s.c1z)[iz], q2, -d.dzv) / GCF(s.c2z)[iz];
+          GF(s.q1)[id] = q1; GF(s.q2)[id] = q2;
+          p = q1 + q2;
+        }
+      }
+      if (FK == FK_SRC) {
+        if (xsrc && iz >= s.sz0 && iz <= s.sz1)        // kernel.cu:36-47
+          p = __builtin_fmaf(GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
+        if (incell) {
+          GF(pn)[iz * P + ix] = p;
+          if ((tl.kind & TF_REC) && s.record && iz == npml) {   // kernel.cu:48-71
+            const int ir = ix - s.rcol0;
+            if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
+          }
+          if (save && strips) {
+            const int so = strip_p(s, iz, ix);
+            if (so >= 0) store_put(s, it, so, p);
+          }
+        }
+      } else {                                         // receiver wavefield
+        if ((tl.kind & TF_REC) && iz == npml) {        // Dirichlet insertion (kernel.cu:72-95)
+          const int ir = ix - s.rcol0;
+          if (ir >= 0 && ir < s.nrec) {
+            const float rv = (it <= ((const gint*)s.cut)[ir]) ? 0.0f : GCF(s.rec)[(size_t)it * s.nrec + ir];
+            p = s.dt * rv;
+          }
+        }
+        if (incell) {
+          GF(pn)[iz * P + ix] = p;
+          if (xin && iz >= npml && iz < nzp - npml) {  // imaging (kernel.cu:1401-1416)
+            const int ii = (iz - npml) * s.nxl + ix - npml;
+            GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, GCF(s.g1)[ii]);
+            GF(s.illum)[ii] = __builtin_fmaf(ps[r], ps[r], GCF(s.illum)[ii]);
+          }
+        }
       }
     }
-    // receiver wavefield
-    float p = pro[r];
-    if (!xring && iz > 0 && iz < nzp - 1) {
-      const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
-      const DivV d = div_v<RPT>(RX + (j0 + r) * LW + lx + 10, cr, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
-      p = pres_cell(p, s.rp1, s.rp2, id, inner, d, c1x, c2x, GCF(s.c1z)[iz], GCF(s.c2z)[iz]);
-    }
-    if ((flags & TF_REC) && iz == npml) {
-      const int ir = ix - s.rcol0;
-      if (ir >= 0 && ir < s.nrec) {
-        const float rv = (it <= ((const gint*)s.cut)[ir]) ? 0.0f : GCF(s.rec)[(size_t)it * s.nrec + ir];
-        p = s.dt * rv;
-      }
-    }
-    GF(s.rp)[id] = p;
-    if (inner) {
-      const size_t ii = (size_t)(iz - npml) * s.nxl + ix - npml;
-      GF(s.g1)[ii] = __builtin_fmaf(ps, p, GCF(s.g1)[ii]);
-      GF(s.illum)[ii] = __builtin_fmaf(ps, ps, GCF(s.illum)[ii]);
-    }
+    pnew[r] = p;
   }
-}
-
-// own-cell loads of a thread (rows j0..j0+RPT-1 of its column), issued before
-// the tile barrier so they overlap the staging; rows / columns past the grid
-// read the (zero) margin
-template <int RPT>
-__device__ __forceinline__ void own_load(float (&o)[RPT], const float* __restrict__ f, const ShotDev& s, int x0, int z0)
-{
-  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
-#pragma unroll
-  for (int r = 0; r < RPT; r++) o[r] = GCF(f)[(ptrdiff_t)(z0 + j0 + r) * s.pitch + x0 + lx];
-}
-template <int RPT>
-__device__ __forceinline__ void own_model(float (&o)[RPT], const float* __restrict__ m, const ShotDev& s, int x0, int z0)
-{
-  const int lx = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * RPT;
-#pragma unroll
-  for (int r = 0; r < RPT; r++) o[r] = model_at(m, s, z0 + j0 + r, x0 + lx);
 }
 
 }  // namespace
 
 // ================================================================ kernels ==
-template <int TZ>
-__global__ __launch_bounds__(NTHR) void k_vel_fwd(LaunchArgs a, int save)
+// forward time step of the source wavefield (modelling, recording, strips)
+template <int G, int RPT>
+__global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
 {
-  constexpr int RPT = TZ / 4;
-  __shared__ float L[(TZ + 19) * LW];
+  using Q = Geo<G, RPT>;
+  __shared__ float lds[Q::LDS];
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
   const int it = s.it_off + s.it_dir * a.step;
-  float vxo[RPT], vzo[RPT], rho[RPT];
-  stage<LW, TZ + 19>(L, s.sp, s.pitch, tl.z0 - 9, tl.x0 - 9);
-  own_load<RPT>(vxo, s.svx, s, tl.x0, tl.z0);
-  own_load<RPT>(vzo, s.svz, s, tl.x0, tl.z0);
-  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
-  __syncthreads();
-  vel_fwd<TZ>(L, s.svx, s.svz, s, tl.x0, tl.z0, save ? it : -1, tl.kind, vxo, vzo, rho);
+  const bool odd = it & 1;                             // (a runtime index into s would spill it to scratch)
+  float none[Q::RPT], pnew[Q::RPT];
+#pragma unroll
+  for (int r = 0; r < Q::RPT; r++) none[r] = 0.0f;
+  field_step<Q, FK_SRC>(lds, s, tl, it, save, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
+                        odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
+                        odd ? s.svz[0] : s.svz[1], none, pnew);
 }
 
-template <int TZ>
-__global__ __launch_bounds__(NTHR) void k_pres_fwd(LaunchArgs a, int save)
+// reverse time step: source reconstruction, then the receiver wavefield and
+// the imaging products of the tile
+template <int G, int RPT>
+__global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
 {
-  constexpr int RPT = TZ / 4;
-  __shared__ float VX[TZ * LW];
-  __shared__ float VZ[(TZ + 19) * 64];
+  using Q = Geo<G, RPT>;
+  __shared__ float lds[Q::LDS];
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
   const int it = s.it_off + s.it_dir * a.step;
-  float po[RPT], rho[RPT], vv[RPT];
-  stage_v<TZ>(VX, VZ, s.svx, s.svz, s.pitch, tl.x0, tl.z0);
-  own_load<RPT>(po, s.sp, s, tl.x0, tl.z0);
-  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
-  own_model<RPT>(vv, s.vv, s, tl.x0, tl.z0);
-  __syncthreads();
-  pres_fwd_src<TZ>(VX, VZ, s, tl.x0, tl.z0, it, save, tl.kind, po, rho, vv);
-}
-
-template <int TZ>
-__global__ __launch_bounds__(NTHR) void k_vel_rev(LaunchArgs a)
-{
-  constexpr int RPT = TZ / 4;
-  __shared__ float LS[(TZ + 19) * LW];
-  __shared__ float LR[(TZ + 19) * LW];
-  const int t = xcd_tile(a.ntiles);
-  if (t < 0) return;
-  const Tile tl = a.tiles[t];
-  const ShotDev s = a.shots[tl.slot];
-  const int it = s.it_off + s.it_dir * a.step;
-  float svx[RPT], svz[RPT], rvx[RPT], rvz[RPT], rho[RPT];
-  stage<LW, TZ + 19>(LS, s.sp, s.pitch, tl.z0 - 9, tl.x0 - 9);
-  stage<LW, TZ + 19>(LR, s.rp, s.pitch, tl.z0 - 9, tl.x0 - 9);
-  own_load<RPT>(svx, s.svx, s, tl.x0, tl.z0);
-  own_load<RPT>(svz, s.svz, s, tl.x0, tl.z0);
-  own_load<RPT>(rvx, s.rvx, s, tl.x0, tl.z0);
-  own_load<RPT>(rvz, s.rvz, s, tl.x0, tl.z0);
-  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
-  __syncthreads();
-  vel_rev<TZ>(LS, s.svx, s.svz, s, tl.x0, tl.z0, it, tl.kind, svx, svz, rho);
-  vel_fwd<TZ>(LR, s.rvx, s.rvz, s, tl.x0, tl.z0, -1, tl.kind, rvx, rvz, rho);
-}
-
-template <int TZ>
-__global__ __launch_bounds__(NTHR) void k_pres_rev(LaunchArgs a)
-{
-  constexpr int RPT = TZ / 4;
-  __shared__ float SX[TZ * LW];
-  __shared__ float SZ[(TZ + 19) * 64];
-  __shared__ float RX[TZ * LW];
-  __shared__ float RZ[(TZ + 19) * 64];
-  const int t = xcd_tile(a.ntiles);
-  if (t < 0) return;
-  const Tile tl = a.tiles[t];
-  const ShotDev s = a.shots[tl.slot];
-  const int it = s.it_off + s.it_dir * a.step;
-  float pso[RPT], pro[RPT], rho[RPT], vv[RPT];
-  stage_v<TZ>(SX, SZ, s.svx, s.svz, s.pitch, tl.x0, tl.z0);
-  stage_v<TZ>(RX, RZ, s.rvx, s.rvz, s.pitch, tl.x0, tl.z0);
-  own_load<RPT>(pso, s.sp, s, tl.x0, tl.z0);
-  own_load<RPT>(pro, s.rp, s, tl.x0, tl.z0);
-  own_model<RPT>(rho, s.rho, s, tl.x0, tl.z0);
-  own_model<RPT>(vv, s.vv, s, tl.x0, tl.z0);
-  __syncthreads();
-  pres_rev<TZ>(SX, SZ, RX, RZ, s, tl.x0, tl.z0, it, tl.kind, pso, pro, rho, vv);
+  const bool odd = it & 1;
+  float ps[Q::RPT], pr[Q::RPT], none[Q::RPT];
+#pragma unroll
+  for (int r = 0; r < Q::RPT; r++) ps[r] = none[r] = 0.0f;
+  if (tl.kind & TF_INT) {
+    field_step<Q, FK_RCN>(lds, s, tl, it, 0, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
+                          odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
+                          odd ? s.svz[0] : s.svz[1], none, ps);
+    __syncthreads();                                   // LDS reused by the receiver pass
+  }
+  field_step<Q, FK_RCV>(lds, s, tl, it, 0, odd ? s.rp[1] : s.rp[0], odd ? s.rvx[1] : s.rvx[0],
+                        odd ? s.rvz[1] : s.rvz[0], odd ? s.rp[0] : s.rp[1], odd ? s.rvx[0] : s.rvx[1],
+                        odd ? s.rvz[0] : s.rvz[1], ps, pr);
 }
 
 // 16-bit boundary store: quantise the fp32 staging of step `it` of each slot
@@ -610,33 +622,43 @@ __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shot
   }
 }
 
-// zero the whole field allocations (margins included) that the starting
-// phase uses: FWD both sets; REV the source velocities (rtm_model.cu:366-372),
-// the receiver set and the imaging accumulators.
+// Phase start of one slot.  FWD: zero both levels of both wavefields and the
+// PML pair, and copy the shot's model window into the slot (edge replication
+// into the PML, extendvel1 Velocity_function.cpp:2011-2030, and on into the
+// margins).  REV: zero both levels of the source velocities (rtm_model.cu:
+// 366-372), the receiver set, the PML pair and the imaging accumulators.
 __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__ shots, int slot, int phase)
 {
   const ShotDev s = shots[slot];
   const long long n = (long long)s.frows * s.pitch;
   const long long ni = (long long)s.nzi * s.nxl;
   const long long g0 = (long long)blockIdx.x * NTHR + threadIdx.x, gs = (long long)gridDim.x * NTHR;
-  float *sp = s.sp - s.forg, *svx = s.svx - s.forg, *svz = s.svz - s.forg, *sp1 = s.sp1 - s.forg, *sp2 = s.sp2 - s.forg;
-  float *rp = s.rp - s.forg, *rvx = s.rvx - s.forg, *rvz = s.rvz - s.forg, *rp1 = s.rp1 - s.forg, *rp2 = s.rp2 - s.forg;
+  float* f[14] = {s.sp[0], s.sp[1], s.svx[0], s.svx[1], s.svz[0], s.svz[1], s.rp[0], s.rp[1],
+                  s.rvx[0], s.rvx[1], s.rvz[0], s.rvz[1], s.q1, s.q2};
+  const int first = phase == PH_FWD ? 0 : 2;
+  for (long long e = g0; e < n; e += gs)
+#pragma unroll
+    for (int k = 0; k < 14; k++)
+      if (k >= first) (f[k] - s.forg)[e] = 0.0f;
   if (phase == PH_FWD) {
     for (long long e = g0; e < n; e += gs) {
-      sp[e] = 0.f; svx[e] = 0.f; svz[e] = 0.f; sp1[e] = 0.f; sp2[e] = 0.f;
-      rp[e] = 0.f; rvx[e] = 0.f; rvz[e] = 0.f; rp1[e] = 0.f; rp2[e] = 0.f;
+      const int row = (int)(e / s.pitch), col = (int)(e - (long long)row * s.pitch);
+      const int iz = row - RTM_MZT, ix = col - RTM_MXL;
+      const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
+      const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
+      const size_t m = (size_t)r * s.mpitch + c;
+      (s.mrho - s.forg)[e] = s.rho[m];
+      (s.mvv - s.forg)[e] = s.vv[m];
     }
     if (s.store_fmt != ST_F32)
       for (long long e = g0; e < 4LL * s.nt; e += gs) s.smax[e] = 0u;
   } else {
-    for (long long e = g0; e < n; e += gs) {
-      svx[e] = 0.f; svz[e] = 0.f;
-      rp[e] = 0.f; rvx[e] = 0.f; rvz[e] = 0.f; rp1[e] = 0.f; rp2[e] = 0.f;
-    }
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
   }
 }
 
+// receiver wavefield at it = nt-1 (rtm_model.cu:366-372 with the insertion
+// of kernel.cu:72-95), written to the level the first reverse step reads
 __global__ __launch_bounds__(NTHR) void k_seed_receivers(const ShotDev* __restrict__ shots, int slot)
 {
   const ShotDev s = shots[slot];
@@ -644,7 +666,8 @@ __global__ __launch_bounds__(NTHR) void k_seed_receivers(const ShotDev* __restri
   if (ir >= s.nrec) return;
   const int it = s.nt - 1;
   const float rv = (it <= s.cut[ir]) ? 0.0f : s.rec[(size_t)it * s.nrec + ir];
-  s.rp[(size_t)s.npml * s.pitch + s.rcol0 + ir] = s.dt * rv;
+  float* rp = ((s.nt - 2) & 1) ? s.rp[1] : s.rp[0];
+  rp[(size_t)s.npml * s.pitch + s.rcol0 + ir] = s.dt * rv;
 }
 
 // taper and stack shots into the FP64 images, one image cell per thread,
@@ -724,35 +747,24 @@ __global__ void k_bump(ShotDev* __restrict__ shots, const int* __restrict__ slot
 
 static inline int grid8(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
-template <int TZ>
-static hipError_t launch_phase_t(const LaunchArgs& a, int phase, int save, int which, hipStream_t st)
+template <int G, int RPT>
+static hipError_t launch_step_t(const LaunchArgs& a, int phase, int save, hipStream_t st)
 {
-  const dim3 g(grid8(a.ntiles)), b(NTHR);
-  if (phase == PH_FWD) {
-    if (which == 0) hipLaunchKernelGGL(k_vel_fwd<TZ>, g, b, 0, st, a, save);
-    else hipLaunchKernelGGL(k_pres_fwd<TZ>, g, b, 0, st, a, save);
-  } else {
-    if (which == 0) hipLaunchKernelGGL(k_vel_rev<TZ>, g, b, 0, st, a);
-    else hipLaunchKernelGGL(k_pres_rev<TZ>, g, b, 0, st, a);
-  }
+  const dim3 g(grid8(a.ntiles)), b(64 * G);
+  if (phase == PH_FWD) hipLaunchKernelGGL((k_fwd<G, RPT>), g, b, 0, st, a, save);
+  else hipLaunchKernelGGL((k_rev<G, RPT>), g, b, 0, st, a);
   return hipGetLastError();
-}
-
-hipError_t rtm_launch_phase(const LaunchArgs& a, int phase, int tz, int save, int which, hipStream_t st)
-{
-  switch (tz) {
-    case 8: return launch_phase_t<8>(a, phase, save, which, st);
-    case 16: return launch_phase_t<16>(a, phase, save, which, st);
-    case 32: return launch_phase_t<32>(a, phase, save, which, st);
-    default: return hipErrorInvalidValue;
-  }
 }
 
 hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st)
 {
-  hipError_t e = rtm_launch_phase(a, phase, tz, save, 0, st);
-  if (e != hipSuccess) return e;
-  return rtm_launch_phase(a, phase, tz, save, 1, st);
+  if (a.ntiles <= 0) return hipSuccess;
+  switch (tz) {
+    case 16: return launch_step_t<4, 4>(a, phase, save, st);
+    case 32: return launch_step_t<4, 8>(a, phase, save, st);
+    case 64: return launch_step_t<8, 8>(a, phase, save, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, int nblocks, hipStream_t st)

@@ -3,12 +3,10 @@
 #include <hip/hip_runtime.h>
 #include "rtm_dev.h"
 
-// one time step (velocity + pressure phase) of the tiles in `a`; phase is
-// PH_FWD or PH_REV, tz the tile height (8, 16 or 32), save=1 writes the
-// boundary strips of the forward pass.
+// one time step of the tiles in `a` (one launch): phase PH_FWD (source
+// modelling; save=1 writes the boundary strips) or PH_REV (reconstruction,
+// receiver wavefield, imaging); tz the tile height (16, 32 or 64)
 hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st);
-// the two phases of rtm_launch_step separately (which: 0 velocity, 1 pressure)
-hipError_t rtm_launch_phase(const LaunchArgs& a, int phase, int tz, int save, int which, hipStream_t st);
 hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, int nblocks, hipStream_t st);
 hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st);
 hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st);

@@ -120,7 +120,7 @@ class Server {
 Server::Server(Engine* eng, const rtmh_server_options& o) : eng_(eng), opt_(o)
 {
   if (opt_.max_slots <= 0) opt_.max_slots = 256;
-  if (opt_.tile_z <= 0) opt_.tile_z = 16;
+  if (opt_.tile_z <= 0) opt_.tile_z = 32;
   t0_ = std::chrono::steady_clock::now();
 }
 
@@ -570,7 +570,10 @@ extern "C" int rtmh_server_create(rtmh_ctx* ctx, const rtmh_server_options* opt,
   memset(&o, 0, sizeof(o));
   if (opt) o = *opt;
   if (o.store_fmt < 0 || o.store_fmt > 2) { rtm::set_error("store_fmt must be 0, 1 or 2"); return -1; }
-  if (o.tile_z != 0 && o.tile_z != 8 && o.tile_z != 16 && o.tile_z != 32) { rtm::set_error("tile_z"); return -1; }
+  if (o.tile_z != 0 && o.tile_z != 16 && o.tile_z != 32 && o.tile_z != 64) {
+    rtm::set_error("tile_z must be 0 (32), 16, 32 or 64");
+    return -1;
+  }
   auto* h = new rtmh_server;
   h->s.reset(new rtm::Server(rtmh_ctx_engine(ctx), o));
   const int rc = h->s->start();

@@ -69,3 +69,69 @@ def cfg2(nshot=200, seed=2):
                     dx=dx, dz=dx, dt=0.8e-3, npml=50, fdom=15.0, spx0=100.0, spz0=20.0, dspx=50.0,
                     offset=0.0, offsetmax=3000.0, amp=1.0, alp=0.05, direct=1.5, direct0=0.05, tsnap=0.1)
     return P, vp, rho
+
+
+def cfg3(nshot=256, seed=3, window_seed=33):
+    """Batched RTM: 2048x768, dx=10, npml 50, Ricker 15 Hz, dt=0.8 ms nt=4000, 256 shots every
+    80 m, split-spread windows (layout 1) whose width is drawn per shot from U[400, 1600] columns.
+    Returns (P, vp, rho, offsetmax per shot)."""
+    nx, nz, dx = 2048, 768, 10.0
+    vp = layered_random(nx, nz, dx, dx, seed=seed)
+    rho = np.full_like(vp, 2000.0)
+    P = make_params(nshot=nshot, startshot=1, dshot=1, mode=0, flag_layout=1, nx=nx, nz=nz, nt=4000,
+                    dx=dx, dz=dx, dt=0.8e-3, npml=50, fdom=15.0, spx0=50.0, spz0=20.0, dspx=80.0,
+                    offset=0.0, offsetmax=3000.0, amp=1.0, alp=0.05, direct=1.5, direct0=0.05, tsnap=0.1)
+    rng = np.random.default_rng(window_seed)
+    width = rng.uniform(400.0, 1600.0, nshot)
+    # layout 1 window: 2*nr+1 columns, nr = offsetmax/dx
+    offsetmax = (np.floor((width - 1.0) / 2.0) * dx).astype(np.float32)
+    return P, vp, rho, offsetmax
+
+
+def cfg4(nshot=128, seed=4):
+    """Precision variant: 2048x1024, dx=dz=7.5 m, npml 50, Ricker 25 Hz, dt=0.6 ms nt=6000,
+    128 shots every 120 m, full-aperture receivers (layout 2)."""
+    nx, nz, dx = 2048, 1024, 7.5
+    vp = layered_random(nx, nz, dx, dx, seed=seed, sigma_m=30.0)
+    rho = np.full_like(vp, 2000.0)
+    P = make_params(nshot=nshot, startshot=1, dshot=1, mode=0, flag_layout=2, nx=nx, nz=nz, nt=6000,
+                    dx=dx, dz=dx, dt=0.6e-3, npml=50, fdom=25.0, spx0=60.0, spz0=15.0, dspx=120.0,
+                    offset=0.0, offsetmax=20000.0, amp=1.0, alp=0.05, direct=1.5, direct0=0.05, tsnap=0.1)
+    return P, vp, rho
+
+
+CFG5_NX = (256, 512, 768, 1024, 1280, 1536, 1792, 2048)
+CFG5_NZ = (256, 384, 512, 640, 768, 896, 1024)
+
+
+def cfg5_models(nmodels=8, seed=5):
+    """8 resident models with grids drawn from 256x256 .. 2048x1024 (dx=dz=10 m)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nmodels):
+        nx = int(rng.choice(CFG5_NX))
+        nz = int(rng.choice(CFG5_NZ))
+        vp = layered_random(nx, nz, 10.0, 10.0, seed=100 + k)
+        out.append((vp, np.full_like(vp, 2000.0)))
+    return out
+
+
+def cfg5_stream(nreq, models, rate=4.0, seed=55):
+    """Poisson arrivals (rate per second) of RTM requests: model index, nt ~ U[1000, 6000],
+    1..32 shots spread evenly over the model, split-spread 3000 m (layout 1), 15 Hz, dt=0.8 ms."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for _ in range(nreq):
+        t += rng.exponential(1.0 / rate)
+        m = int(rng.integers(0, len(models)))
+        nz, nx = models[m][0].shape
+        nt = int(rng.integers(1000, 6001))
+        ns = int(rng.integers(1, 33))
+        step = max(1, nx // ns)
+        P = make_params(nshot=ns, startshot=1, dshot=1, mode=0, flag_layout=1, nx=nx, nz=nz, nt=nt,
+                        dx=10.0, dz=10.0, dt=0.8e-3, npml=50, fdom=15.0, spx0=10.0 * (step // 2),
+                        spz0=20.0, dspx=10.0 * step, offset=0.0, offsetmax=3000.0, amp=1.0, alp=0.05,
+                        direct=1.5, direct0=0.05, tsnap=0.1)
+        reqs.append((t, m, P))
+    return reqs

@@ -74,7 +74,7 @@ typedef struct rtmh_options {
   int batch;          /* shots per launch; 0 = as many as fit the memory budget  */
   int store_fmt;      /* boundary store: RTMH_STORE_F32 / _F16 / _BF16           */
   int graph_chunk;    /* time steps per captured hipGraph; 0 = plain launches    */
-  int tile_z;         /* tile height 8/16/32; 0 = autotuned                      */
+  int tile_z;         /* tile height 16/32/64; 0 = autotuned                     */
   int write_files;    /* reproduce the reference's file side effects             */
   int verbose;        /* 0 silent, 1 per-shot lines as the reference prints       */
   double mem_budget;  /* bytes of HBM for shot slots; 0 = 80% of free memory      */
@@ -215,7 +215,7 @@ typedef struct rtmh_server rtmh_server;
 typedef struct rtmh_server_options {
   double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of free */
   int max_slots;      /* shots resident at once; 0 = 256                          */
-  int tile_z;         /* 8 / 16 / 32; 0 = 16                                      */
+  int tile_z;         /* 16 / 32 / 64; 0 = 32                                     */
   int store_fmt;      /* boundary store precision of every request                */
 } rtmh_server_options;
 typedef struct rtmh_request_stats {

@@ -42,11 +42,12 @@ def to_ref_fields(f, g, npml):
     return out
 
 
-@pytest.mark.parametrize("npml", [32, 40])
-def test_fd_steps_bit_exact(gpu_ctx, npml):
-    """forward velocity-stress + PML steps from random fields, incl. source"""
-    vp, rho = small_model()
-    kw = job(npml=npml)
+@pytest.mark.parametrize("npml,nz,nx", [(32, 60, 120), (40, 60, 120), (32, 170, 300)])
+def test_fd_steps_bit_exact(gpu_ctx, npml, nz, nx):
+    """forward velocity-stress + PML steps from random fields, incl. source;
+    the 300x170 grid also has tiles whose halo is all undamped interior"""
+    vp, rho = small_model(nz=nz, nx=nx)
+    kw = job(npml=npml, nz=nz, nx=nx, spx0=10.0 * nx / 2)
     P, Po = G.make_params(**kw), O.make_params(**kw)
     g = G.geometry(P, 1)
     rng = np.random.default_rng(1)
@@ -119,3 +120,48 @@ def test_mode1_migrates_given_data(gpu_ctx):
     P1 = G.make_params(**dict(kw, mode=1))
     r1 = gpu_ctx.run(P1, vp, rho, want_shots=True, seis_in=r0.shot_seis)
     assert np.array_equal(r0.image1, r1.image1)
+
+
+def test_large_grid_single_shot_vs_oracle(gpu_ctx):
+    """grid large enough for interior tiles (undamped halo: the fast path)"""
+    vp, rho = small_model(nz=170, nx=300, seed=7)
+    kw = job(nx=300, nz=170, nt=260, flag_layout=1, offsetmax=1200.0, spx0=1500.0, npml=32)
+    P, Po = G.make_params(**kw), O.make_params(**kw)
+    print("[large_grid] 300x170 layout 1", flush=True)
+    r = gpu_ctx.run(P, vp, rho, want_shots=True, tile_z=32)
+    o = O.run_shot(Po, 1, vp, rho)
+    assert np.array_equal(r.shot_seis[0], o.seis)
+    assert np.array_equal(r.shot_g1[0], o.g1)
+    assert np.array_equal(r.shot_illum[0], o.illum)
+
+
+def test_tile_heights_and_autotuner_invariant(gpu_ctx):
+    """every tile height (and the autotuned choice) computes the same cells"""
+    vp, rho = small_model(nz=150, nx=260, seed=8)
+    kw = job(nx=260, nz=150, nshot=3, flag_layout=1, offsetmax=900.0, spx0=700.0, dspx=600.0, nt=240)
+    P = G.make_params(**kw)
+    print("[tile_heights] 16/32/64/auto", flush=True)
+    runs = [gpu_ctx.run(P, vp, rho, batch=3, tile_z=tz, want_shots=True) for tz in (16, 32, 64, 0)]
+    assert runs[3].stats.tile_z in (16, 32, 64)
+    for r in runs[1:]:
+        for x, y in zip(runs[0].shot_g1, r.shot_g1):
+            assert np.array_equal(x, y)
+        for x, y in zip(runs[0].shot_seis, r.shot_seis):
+            assert np.array_equal(x, y)
+        assert np.array_equal(runs[0].image1, r.image1) and np.array_equal(runs[0].illum, r.illum)
+
+
+def test_ragged_windows_vs_oracle(gpu_ctx):
+    """one batch of shots with different aperture windows (per-shot offsetmax)"""
+    vp, rho = small_model(nz=60, nx=260, seed=9)
+    kw = job(nx=260, nz=60, nshot=4, flag_layout=1, spx0=400.0, dspx=500.0, nt=220)
+    om = np.array([300.0, 900.0, 550.0, 1200.0], np.float32)
+    P = G.make_params(**kw)
+    print("[ragged] per-shot offsetmax", om.tolist(), flush=True)
+    r = gpu_ctx.run(P, vp, rho, batch=4, want_shots=True, shot_offsetmax=om)
+    for k in range(4):
+        Po = O.make_params(**dict(kw, offsetmax=float(om[k]), startshot=k + 1, nshot=k + 1))
+        o = O.run_shot(Po, k + 1, vp, rho)
+        assert r.shot_g1[k].shape == o.g1.shape
+        assert np.array_equal(r.shot_seis[k], o.seis)
+        assert np.array_equal(r.shot_g1[k], o.g1)
