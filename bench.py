@@ -208,7 +208,7 @@ def run_batched(args, G, synth):
     cells = sum(s.cell_updates for s, _, _ in recs)
     nshots = sum(len(sh) for _, _, sh in recs)
     roof = roofline_of(recs, P)
-    widths = [int(2 * np.floor(om[i] / P.dx) + 1) for i in range(len(om))]
+    widths = [int(np.floor(om[i] / P.dx + 1)) for i in range(len(om))]
     out = base_line(
         args, round(cells / elapsed / 1e9, 3), "Gcell/s", elapsed, "f32",
         "synthetic: seeded smooth-random layered model, data modelled in the same run",

@@ -146,20 +146,45 @@ struct Geo {
   static constexpr int KZ = (ZR + G - 1) / G;          // vz rows per thread (consecutive)
   static constexpr int PZR = G * KZ + 19;              // p column rows held (>= TZ + 38)
   static constexpr int KP = (TZ + 38 + G - 1) / G;     // p column rows staged per thread
-  static constexpr int KX = (TZ * VXW + NT - 1) / NT;  // vx cells per thread
+  static constexpr int KH = (20 * TZ + NT - 1) / NT;   // halo vx cells per thread
   static constexpr int LDS_A = PZR * 64 + TZ * PCW;    // PZ | PC
   static constexpr int LDS_B = TZ * VXW + ZR * 64;     // VXL | VZL
   static constexpr int LDS = LDS_A > LDS_B ? LDS_A : LDS_B;
 };
 
+// ------------------------------------------------------- buffer access --
+// Every field of a slot is one allocation of frows x pitch floats.  Tile
+// accesses are raw buffer loads / stores with the row offset in a scalar
+// register and the lane's column offset in a vector register, so the
+// addressing costs no vector instructions per access.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t field_rsrc(const float* f00, const ShotDev& s)
+{
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(f00 - s.forg), (short)0, s.frows * s.pitch * 4, 0x00020000);
+}
+__device__ __forceinline__ float bld(rsrc_t r, int voff, int soff)
+{
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(rsrc_t r, float v, int voff, int soff)
+{
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+// true for every lane of the wave if any active lane has c: branches on it
+// are uniform, so the rare cases (reduced radius near the grid edge, damped
+// cells) cost their instructions only in the waves that hold them
+__device__ __forceinline__ bool any_lane(bool c) { return __builtin_amdgcn_ballot_w64(c) != 0; }
+
 // radius-r staggered difference a[i] - b[i] with the coefficient row of
 // order 2r; the fma chain runs i = 1..r as the reference loops
-// (kernel.cu:459-460).  The loop is unrolled with a predicate so that the
-// operand windows stay in registers (a runtime index would put them in
-// scratch memory).
+// (kernel.cu:459-460).  The reduced-radius loop is unrolled with a
+// predicate so that the operand windows stay in registers (a runtime index
+// would put them in scratch memory); row r-1 = 9 of the table is c10, so
+// lanes at full radius give the same sums in either branch.
 #define STENCIL_SUM(acc, r, c10, cf, A, B)                                                        \
   do {                                                                                            \
-    if ((r) == RTM_R) {                                                                           \
+    if (!any_lane((r) != RTM_R)) {                                                                \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], (A) - (B), acc); \
     } else {                                                                                      \
       const gfloat* cr_ = GCF(cf) + ((r) - 1) * RTM_R;                                            \
@@ -171,200 +196,50 @@ struct Geo {
 // field kinds of one pass
 enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
 
-// Stage p rows z0-19 .. z0+TZ+18 of the tile columns into PZ (pitch 64) and
-// the tile rows with their x halo into PC (pitch 104, column c = x-(x0-20)).
-template <class Q>
-__device__ __forceinline__ void stage_p(float* __restrict__ PZ, float* __restrict__ PC, const float* __restrict__ p,
-                                        int pitch, int x0, int z0, int lx, int g)
-{
-  float a[Q::KP], w[Q::RPT];
-  const gfloat* base = GCF(p) + (ptrdiff_t)(z0 - 19) * pitch + x0;
-  const int wc = lx < 20 ? lx - 20 : lx + 44;          // wing columns (lanes < 40)
-#pragma unroll
-  for (int m = 0; m < Q::KP; m++) {
-    const int k = g + Q::G * m;
-    a[m] = 0.0f;
-    if (k < Q::TZ + 38) a[m] = base[k * pitch + lx];
-  }
-#pragma unroll
-  for (int m = 0; m < Q::RPT; m++) {
-    const int j = g + Q::G * m;                        // tile row
-    w[m] = 0.0f;
-    if (lx < 40) w[m] = base[(j + 19) * pitch + wc];
-  }
-#pragma unroll
-  for (int m = 0; m < Q::KP; m++) {
-    const int k = g + Q::G * m;
-    if (k < Q::TZ + 38) {
-      PZ[k * 64 + lx] = a[m];
-      if (k >= 19 && k < 19 + Q::TZ) PC[(k - 19) * Q::PCW + 20 + lx] = a[m];
-    }
-  }
-#pragma unroll
-  for (int m = 0; m < Q::RPT; m++) {
-    const int j = g + Q::G * m;
-    if (lx < 40) PC[j * Q::PCW + (lx < 20 ? lx : lx + 64)] = w[m];
-  }
-}
-
-// ---------------------------------------------------------- velocities --
-// vz of rows z0-10+kz (kz = g*KZ + k) of the thread's column, vx of the
-// cells c = k*NT + tid of the TZ x 84 block.  Forward-type fields (source in
+// New velocity of one cell from the p window Pr (Pr[i] = p at +i along the
+// derivative), old value vo and density rh.  Forward-type fields (source in
 // the forward phase, receiver in the reverse phase):
 //   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho
 //   (kernel.cu:588-594); c1 = c2 = 1 off the absorbing layer reduces it to
 //   v - D (kernel.cu:462); radius min(10, distance to the grid edge)
-//   (cuda_norderx/z, kernel.cu:299-317).
+//   (cuda_norderx/z, kernel.cu:299-317).  VALID false: the velocity does
+//   not exist there (its value is never read); the damping arrays carry 1
+//   outside the grid.
 // Reconstruction: strip cells restored from the store of step it+1
 // (kernel.cu:1294-1347), interior cells v + D (kernel.cu:318-388).
-template <class Q>
-struct VelOut {
-  float vz[Q::KZ];
-  float vx[Q::KX];
-};
-
-template <class Q, int FK>
-__device__ __forceinline__ void velocities(VelOut<Q>& o, const float* __restrict__ PZ, const float* __restrict__ PC,
-                                           const ShotDev& s, const Tile& tl, const float* __restrict__ vxc,
-                                           const float* __restrict__ vzc, float* __restrict__ vxn,
-                                           float* __restrict__ vzn, int it, int save, int lx, int g, int tid)
-{
-  const int x0 = tl.x0, z0 = tl.z0, P = s.pitch, nxp = s.nxp, nzp = s.nzp, npml = s.npml;
-  const bool gen = (tl.kind & TF_GEN) != 0;
-  const bool strips = (tl.kind & TF_STRIP) != 0;
-  float c10[RTM_R];
-#pragma unroll
-  for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
-  const float* __restrict__ cf = s.coef;
-  const float dtx = s.dtx, dtz = s.dtz;
-
-  // ---- loads of the old velocities and density (issued with the staging)
-  float vzo[Q::KZ], rz[Q::KZ], vxo[Q::KX], rx[Q::KX];
-  const int ix = x0 + lx;
-#pragma unroll
-  for (int k = 0; k < Q::KZ; k++) {
-    const int kz = g * Q::KZ + k;
-    vzo[k] = 0.0f;
-    rz[k] = 1.0f;
-    if (kz < Q::ZR) {
-      const int id = (z0 - 10 + kz) * P + ix;
-      vzo[k] = GCF(vzc)[id];
-      rz[k] = GCF(s.mrho)[id];
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < Q::KX; k++) {
-    const int c = k * Q::NT + tid;
-    vxo[k] = 0.0f;
-    rx[k] = 1.0f;
-    if (c < Q::TZ * Q::VXW) {
-      const int row = c / Q::VXW, col = c - row * Q::VXW;
-      const int id = (z0 + row) * P + x0 - 10 + col;
-      vxo[k] = GCF(vxc)[id];
-      rx[k] = GCF(s.mrho)[id];
-    }
-  }
-  __syncthreads();                                     // p staged
-
-  // ---- vz
-  {
-    float cw[Q::KZ + 19];                              // p rows z0-19+g*KZ+m of the column
-#pragma unroll
-    for (int m = 0; m < Q::KZ + 19; m++) cw[m] = PZ[(g * Q::KZ + m) * 64 + lx];
-#pragma unroll
-    for (int k = 0; k < Q::KZ; k++) {
-      const int kz = g * Q::KZ + k;
-      const int iz = z0 - 10 + kz;
-      float v = 0.0f;
-      if (kz < Q::ZR) {
-        if (FK == FK_RCN) {
-          const int so = strips ? strip_vz(s, iz, ix) : -1;
-          if (so >= 0) {
-            v = store_get(s, it + 1, so);
-          } else if (!gen || (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2)) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], cw[k + 9 + i] - cw[k + 10 - i], acc);
-            acc = acc * dtz;
-            acc = acc / rz[k];
-            v = vzo[k] + acc;
-          }
-        } else if (!gen) {
-          float acc = 0.0f;
-#pragma unroll
-          for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], cw[k + 9 + i] - cw[k + 10 - i], acc);
-          acc = acc * dtz;
-          acc = acc / rz[k];
-          v = vzo[k] - acc;
-        } else if (iz >= 0 && iz <= nzp - 2 && ix < nxp) {
-          const int rr = imin(RTM_R, imin(iz + 1, nzp - 1 - iz));
-          float acc = 0.0f;
-          STENCIL_SUM(acc, rr, c10, cf, cw[k + 9 + i], cw[k + 10 - i]);
-          acc = acc * dtz;
-          acc = acc / rz[k];
-          const float c1 = GCF(s.c1zv)[iz], c2 = GCF(s.c2zv)[iz];
-          v = (c1 == 1.0f && c2 == 1.0f) ? vzo[k] - acc : __builtin_fmaf(c1, vzo[k], -acc) / c2;
-        }
-        // own rows: write the new level (cells that exist for vz)
-        if (kz >= 10 && kz < 10 + Q::TZ && ix < nxp && iz <= nzp - 2) {
-          GF(vzn)[iz * P + ix] = v;
-          if (FK == FK_SRC && save && strips) {
-            const int so = strip_vz(s, iz, ix);
-            if (so >= 0) store_put(s, it, so, v);
-          }
-        }
-      }
-      o.vz[k] = v;
-    }
-  }
-  // ---- vx
-#pragma unroll
-  for (int k = 0; k < Q::KX; k++) {
-    const int c = k * Q::NT + tid;
-    float v = 0.0f;
-    if (c < Q::TZ * Q::VXW) {
-      const int row = c / Q::VXW, col = c - row * Q::VXW;
-      const int iz = z0 + row, jx = x0 - 10 + col;
-      const float* __restrict__ Pr = PC + row * Q::PCW + col + 10;   // p at column jx
-      if (FK == FK_RCN) {
-        const int so = strips ? strip_vx(s, iz, jx) : -1;
-        if (so >= 0) {
-          v = store_get(s, it + 1, so);
-        } else if (!gen || (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml)) {
-          float acc = 0.0f;
-#pragma unroll
-          for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Pr[i] - Pr[1 - i], acc);
-          acc = acc * dtx;
-          acc = acc / rx[k];
-          v = vxo[k] + acc;
-        }
-      } else if (!gen) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], Pr[i] - Pr[1 - i], acc);
-        acc = acc * dtx;
-        acc = acc / rx[k];
-        v = vxo[k] - acc;
-      } else if (jx >= 0 && jx <= nxp - 2 && iz < nzp) {
-        const int rr = imin(RTM_R, imin(jx + 1, nxp - 1 - jx));
-        float acc = 0.0f;
-        STENCIL_SUM(acc, rr, c10, cf, Pr[i], Pr[1 - i]);
-        acc = acc * dtx;
-        acc = acc / rx[k];
-        const float c1 = GCF(s.c1xv)[jx], c2 = GCF(s.c2xv)[jx];
-        v = (c1 == 1.0f && c2 == 1.0f) ? vxo[k] - acc : __builtin_fmaf(c1, vxo[k], -acc) / c2;
-      }
-      if (col >= 10 && col < 74 && iz < nzp && jx <= nxp - 2) {
-        GF(vxn)[iz * P + jx] = v;
-        if (FK == FK_SRC && save && strips) {
-          const int so = strip_vx(s, iz, jx);
-          if (so >= 0) store_put(s, it, so, v);
-        }
-      }
-    }
-    o.vx[k] = v;
-  }
-}
+#define VEL_CELL(FK, gen, strips, STRIPF, IZ, IX, VALID, RAD, INTERIOR, P_AT, VO, RH, DTH, C1V, C2V, OUT)   \
+  do {                                                                                                     \
+    float v_ = 0.0f;                                                                                       \
+    if (FK == FK_RCN) {                                                                                    \
+      const int so_ = (strips) ? STRIPF(s, IZ, IX) : -1;                                                   \
+      if (so_ >= 0) {                                                                                      \
+        v_ = store_get(s, it + 1, so_);                                                                    \
+      } else if (!(gen) || (INTERIOR)) {                                                                   \
+        float acc = 0.0f;                                                                                  \
+        _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
+        acc = acc * (DTH);                                                                                 \
+        acc = acc / (RH);                                                                                  \
+        v_ = (VO) + acc;                                                                                   \
+      }                                                                                                    \
+    } else if (!(gen)) {                                                                                   \
+      float acc = 0.0f;                                                                                    \
+      _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
+      acc = acc * (DTH);                                                                                   \
+      acc = acc / (RH);                                                                                    \
+      v_ = (VO) - acc;                                                                                     \
+    } else {                                                                                               \
+      const int rr_ = (VALID) ? (RAD) : RTM_R;                                                             \
+      float acc = 0.0f;                                                                                    \
+      STENCIL_SUM(acc, rr_, c10, s.coef, P_AT(i), P_AT(1 - i));                                            \
+      acc = acc * (DTH);                                                                                   \
+      acc = acc / (RH);                                                                                    \
+      const float c1_ = (C1V), c2_ = (C2V);                                                                \
+      /* c1 = c2 = 1 gives fma(1, v, -D) / 1 == v - D exactly */                                           \
+      if (any_lane(c1_ != 1.0f || c2_ != 1.0f)) v_ = __builtin_fmaf(c1_, (VO), -acc) / c2_;                 \
+      else v_ = (VO) - acc;                                                                                \
+    }                                                                                                      \
+    OUT = v_;                                                                                              \
+  } while (0)
 
 // ------------------------------------------------------------ pressure --
 struct DivV { float dxv, dzv; };
@@ -403,115 +278,258 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
 // velocities vxc, vzc) to the next level (pn, vxn, vzn).  On return pnew
 // holds the new pressure of the thread's own cells (rows g*RPT+r); ps is the
 // reconstructed source pressure of the same cells (receiver pass imaging).
+//
+// Thread roles (lane lx, wave g):
+//   p staging  rows g+G*m of the tile columns (PZ) and their x wings (PC)
+//   vz         rows z0-10 + g*KZ + k of column x0+lx (z window from PZ)
+//   vx         own rows z0 + g*RPT + r of column x0+lx, and the halo
+//              columns x0-10..x0-1, x0+64..x0+73 dealt over all threads
+//   p          own rows z0 + g*RPT + r of column x0+lx
 template <class Q, int FK>
 __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDev& s, const Tile& tl, int it, int save,
-                                           const float* __restrict__ pc, const float* __restrict__ vxc,
-                                           const float* __restrict__ vzc, float* __restrict__ pn,
-                                           float* __restrict__ vxn, float* __restrict__ vzn,
-                                           const float (&ps)[Q::RPT], float (&pnew)[Q::RPT])
+                                           const float* pc, const float* vxc, const float* vzc, float* pn, float* vxn,
+                                           float* vzn, const float (&ps)[Q::RPT], float (&pnew)[Q::RPT])
 {
+  constexpr int G = Q::G, RPT = Q::RPT, TZ = Q::TZ, NT = Q::NT, KZ = Q::KZ, KP = Q::KP, KH = Q::KH;
+  constexpr int PCW = Q::PCW, VXW = Q::VXW, ZR = Q::ZR;
   float* PZ = lds;
   float* PC = lds + Q::PZR * 64;
   float* VXL = lds;
-  float* VZL = lds + Q::TZ * Q::VXW;
+  float* VZL = lds + TZ * VXW;
   const int tid = threadIdx.x, lx = tid & 63;
   const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int x0 = tl.x0, z0 = tl.z0, P = s.pitch, nxp = s.nxp, nzp = s.nzp, npml = s.npml;
+  const int x0 = tl.x0, z0 = tl.z0, nxp = s.nxp, nzp = s.nzp, npml = s.npml;
   const int ix = x0 + lx;
   const bool gen = (tl.kind & TF_GEN) != 0;
-
-  stage_p<Q>(PZ, PC, pc, P, x0, z0, lx, g);
-  VelOut<Q> vo;
-  velocities<Q, FK>(vo, PZ, PC, s, tl, vxc, vzc, vxn, vzn, it, save, lx, g, tid);
-
-  // own old pressure out of LDS, model of the own cells
-  float po[Q::RPT], rho[Q::RPT], vv[Q::RPT];
-#pragma unroll
-  for (int r = 0; r < Q::RPT; r++) {
-    const int j = g * Q::RPT + r;
-    po[r] = PC[j * Q::PCW + 20 + lx];
-    const int id = (z0 + j) * P + ix;
-    rho[r] = GCF(s.mrho)[id];
-    vv[r] = GCF(s.mvv)[id];
-  }
-  __syncthreads();                                     // PZ/PC consumed
-#pragma unroll
-  for (int k = 0; k < Q::KZ; k++) {
-    const int kz = g * Q::KZ + k;
-    if (kz < Q::ZR) VZL[kz * 64 + lx] = vo.vz[k];
-  }
-#pragma unroll
-  for (int k = 0; k < Q::KX; k++) {
-    const int c = k * Q::NT + tid;
-    if (c < Q::TZ * Q::VXW) VXL[c] = vo.vx[k];
-  }
-  __syncthreads();                                     // new velocities staged
-
+  const bool strips = (tl.kind & TF_STRIP) != 0;
+  const int pr4 = s.pitch * 4;
+  const int srow0 = (z0 + RTM_MZT) * pr4;             // byte offset of tile row 0
+  const int vcol = (ix + RTM_MXL) * 4;                // byte offset of the lane's column
+  const rsrc_t Rp = field_rsrc(pc, s), Rvx = field_rsrc(vxc, s), Rvz = field_rsrc(vzc, s);
+  const rsrc_t Rrho = field_rsrc(s.mrho, s);
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
-  float cz[Q::RPT + 19];
+  const float dtx = s.dtx, dtz = s.dtz;
+
+  // ---- 1. loads: p with its halo, old velocities and density of every
+  //         velocity cell this tile computes
+  float a[KP], w[RPT];
+  {
+    const int wcol = (x0 + (lx < 20 ? lx - 20 : lx + 44) + RTM_MXL) * 4;
 #pragma unroll
-  for (int m = 0; m < Q::RPT + 19; m++) cz[m] = VZL[(g * Q::RPT + m) * 64 + lx];
+    for (int m = 0; m < KP; m++) {
+      const int k = g + G * m;
+      a[m] = 0.0f;
+      if (k < TZ + 38) a[m] = bld(Rp, vcol, srow0 + (k - 19) * pr4);
+    }
+#pragma unroll
+    for (int m = 0; m < RPT; m++) {
+      w[m] = 0.0f;
+      if (lx < 40) w[m] = bld(Rp, wcol, srow0 + (g + G * m) * pr4);
+    }
+  }
+  float vzo[KZ], rz[KZ];
+#pragma unroll
+  for (int k = 0; k < KZ; k++) {
+    const int kz = g * KZ + k;
+    vzo[k] = 0.0f;
+    rz[k] = 1.0f;
+    if (kz < ZR) {
+      const int so = srow0 + (kz - 10) * pr4;
+      vzo[k] = bld(Rvz, vcol, so);
+      rz[k] = bld(Rrho, vcol, so);
+    }
+  }
+  float vxo[RPT], rho[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int so = srow0 + (g * RPT + r) * pr4;
+    vxo[r] = bld(Rvx, vcol, so);
+    rho[r] = bld(Rrho, vcol, so);
+  }
+  float hvo[KH], hrh[KH];
+  int hrow[KH], hcol[KH];
+#pragma unroll
+  for (int k = 0; k < KH; k++) {
+    const int h = k * NT + tid;
+    hvo[k] = 0.0f;
+    hrh[k] = 1.0f;
+    hrow[k] = h / 20;
+    const int hc = h - hrow[k] * 20;
+    hcol[k] = hc < 10 ? hc : hc + 64;                  // VXL column: x = x0 - 10 + hcol
+    if (h < 20 * TZ) {
+      const int off = srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4;
+      hvo[k] = bld(Rvx, off, 0);
+      hrh[k] = bld(Rrho, off, 0);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < KP; m++) {
+    const int k = g + G * m;
+    if (k < TZ + 38) {
+      PZ[k * 64 + lx] = a[m];
+      if (k >= 19 && k < 19 + TZ) PC[(k - 19) * PCW + 20 + lx] = a[m];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < RPT; m++)
+    if (lx < 40) PC[(g + G * m) * PCW + (lx < 20 ? lx : lx + 64)] = w[m];
+  __syncthreads();                                     // p staged
+
+  // ---- 2. new velocities
+  float vzn_[KZ], vxn_[RPT], hvn[KH];
+  {
+    const rsrc_t Rvzn = field_rsrc(vzn, s);
+    float cw[KZ + 19];                                 // p rows z0-19+g*KZ+m of the column
+#pragma unroll
+    for (int m = 0; m < KZ + 19; m++) cw[m] = PZ[(g * KZ + m) * 64 + lx];
+#pragma unroll
+    for (int k = 0; k < KZ; k++) {
+      const int kz = g * KZ + k;
+      const int iz = z0 - 10 + kz;
+#define PZA(i) cw[k + 9 + (i)]
+      VEL_CELL(FK, gen, strips, strip_vz, iz, ix, (iz >= 0 && iz <= nzp - 2 && ix < nxp),
+               imin(RTM_R, imin(iz + 1, nzp - 1 - iz)), (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], rz[k], dtz,
+               GCF(s.c1zv)[iz], GCF(s.c2zv)[iz], vzn_[k]);
+#undef PZA
+      if (kz < ZR && kz >= 10 && kz < 10 + TZ && ix < nxp && iz <= nzp - 2) {
+        bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
+        if (FK == FK_SRC && save && strips) {
+          const int so = strip_vz(s, iz, ix);
+          if (so >= 0) store_put(s, it, so, vzn_[k]);
+        }
+      }
+    }
+  }
+  {
+    const rsrc_t Rvxn = field_rsrc(vxn, s);
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int j = g * RPT + r, iz = z0 + j;
+      const float* __restrict__ Pr = PC + j * PCW + lx + 20;   // p at column ix
+#define PCA(i) Pr[(i)]
+      VEL_CELL(FK, gen, strips, strip_vx, iz, ix, (ix <= nxp - 2 && iz < nzp),
+               imin(RTM_R, imin(ix + 1, nxp - 1 - ix)), (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], rho[r], dtx,
+               GCF(s.c1xv)[ix], GCF(s.c2xv)[ix], vxn_[r]);
+#undef PCA
+      if (iz < nzp && ix <= nxp - 2) {
+        bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
+        if (FK == FK_SRC && save && strips) {
+          const int so = strip_vx(s, iz, ix);
+          if (so >= 0) store_put(s, it, so, vxn_[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KH; k++) {
+      const int h = k * NT + tid;
+      const int iz = z0 + hrow[k], jx = x0 - 10 + hcol[k];
+      const float* __restrict__ Pr = PC + hrow[k] * PCW + hcol[k] + 10;   // p at column jx
+      hvn[k] = 0.0f;
+      if (h < 20 * TZ) {
+#define PCA(i) Pr[(i)]
+        VEL_CELL(FK, gen, strips, strip_vx, iz, jx, (jx >= 0 && jx <= nxp - 2 && iz < nzp),
+                 imin(RTM_R, imin(jx + 1, nxp - 1 - jx)), (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hrh[k], dtx,
+                 GCF(s.c1xv)[jx], GCF(s.c2xv)[jx], hvn[k]);
+#undef PCA
+      }
+    }
+  }
+
+  // own old pressure out of LDS; model of the own cells
+  float po[RPT], vv[RPT];
+  const rsrc_t Rvv = field_rsrc(s.mvv, s);
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int j = g * RPT + r;
+    po[r] = PC[j * PCW + 20 + lx];
+    vv[r] = bld(Rvv, vcol, srow0 + j * pr4);
+  }
+  __syncthreads();                                     // PZ/PC consumed
+#pragma unroll
+  for (int k = 0; k < KZ; k++) {
+    const int kz = g * KZ + k;
+    if (kz < ZR) VZL[kz * 64 + lx] = vzn_[k];
+  }
+#pragma unroll
+  for (int r = 0; r < RPT; r++) VXL[(g * RPT + r) * VXW + lx + 10] = vxn_[r];
+#pragma unroll
+  for (int k = 0; k < KH; k++)
+    if (k * NT + tid < 20 * TZ) VXL[hrow[k] * VXW + hcol[k]] = hvn[k];
+  __syncthreads();                                     // new velocities staged
+
+  // ---- 3. new pressure of the own cells
+  float cz[RPT + 19];
+#pragma unroll
+  for (int m = 0; m < RPT + 19; m++) cz[m] = VZL[(g * RPT + m) * 64 + lx];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
   float c1x = 1.0f, c2x = 1.0f;
   if (FK != FK_RCN && gen && ix < nxp) { c1x = GCF(s.c1x)[ix]; c2x = GCF(s.c2x)[ix]; }
   const bool xsrc = (tl.kind & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
-  const bool strips = (tl.kind & TF_STRIP) != 0;
+  const rsrc_t Rpn = field_rsrc(pn, s);
 
 #pragma unroll
-  for (int r = 0; r < Q::RPT; r++) {
-    const int j = g * Q::RPT + r;
+  for (int r = 0; r < RPT; r++) {
+    const int j = g * RPT + r;
     const int iz = z0 + j;
+    const int so = srow0 + j * pr4;
     float p = po[r];
-    const float* __restrict__ VXr = VXL + j * Q::VXW + lx + 10;
+    const float* __restrict__ VXr = VXL + j * VXW + lx + 10;
     const bool incell = ix < nxp && iz < nzp;
     if (FK == FK_RCN) {
       const bool inner = xin && iz >= npml && iz < nzp - npml;
-      const int so = strips ? strip_p(s, iz, ix) : -1;
+      const int sp_ = strips ? strip_p(s, iz, ix) : -1;
       bool wr = false;
-      if (so >= 0) {
-        p = store_get(s, it, so);                      // kernel.cu:1348-1400
+      if (sp_ >= 0) {
+        p = store_get(s, it, sp_);                     // kernel.cu:1348-1400
         wr = true;
       } else if (!gen || inner) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
         p = p + (d.dxv + d.dzv);                       // kernel.cu:389-429
         wr = true;
       }
       if (wr && xsrc && iz >= s.sz0 && iz <= s.sz1)    // source removal (rtm_model.cu:437)
         p = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
-      if (wr) GF(pn)[iz * P + ix] = p;
+      if (wr) bst(Rpn, p, vcol, so);
     } else {
       if (!gen) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
         p = p - (d.dxv + d.dzv);                       // kernel.cu:539
-      } else if (incell && !xring && iz > 0 && iz < nzp - 1) {
-        const int rad = imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz);
-        const DivV d = div_v(VXr, cz, r, rad, c10, s.coef, s.dtx, s.dtz, rho[r], vv[r]);
-        if (xin && iz >= npml && iz < nzp - npml) {
-          p = p - (d.dxv + d.dzv);
-        } else {                                       // split pair, p = p1 + p2 (kernel.cu:780-783)
-          const int id = iz * P + ix;
-          float q1 = GCF(s.q1)[id], q2 = GCF(s.q2)[id];
+      } else {
+        // the outer ring keeps its value (only the source is added there);
+        // cells off the grid are computed with radius 10 and dropped
+        const bool upd = incell && !xring && iz > 0 && iz < nzp - 1;
+        const int rad = upd ? imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz) : RTM_R;
+        DivV d;
+        if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
+        else d = div_v(VXr, cz, r, rad, c10, s.coef, dtx, dtz, rho[r], vv[r]);
+        const bool split = upd && !(xin && iz >= npml && iz < nzp - npml);
+        float pu = p - (d.dxv + d.dzv);
+        if (any_lane(split) && split) {                // split pair, p = p1 + p2 (kernel.cu:780-783)
+          const rsrc_t R1 = field_rsrc(s.q1, s), R2 = field_rsrc(s.q2, s);
+          float q1 = bld(R1, vcol, so), q2 = bld(R2, vcol, so);
           q1 = __builtin_fmaf(c1x, q1, -d.dxv) / c2x;
           q2 = __builtin_fmaf(GCF(s.c1z)[iz], q2, -d.dzv) / GCF(s.c2z)[iz];
-          GF(s.q1)[id] = q1; GF(s.q2)[id] = q2;
-          p = q1 + q2;
+          bst(R1, q1, vcol, so);
+          bst(R2, q2, vcol, so);
+          pu = q1 + q2;
         }
+        if (upd) p = pu;
       }
       if (FK == FK_SRC) {
         if (xsrc && iz >= s.sz0 && iz <= s.sz1)        // kernel.cu:36-47
           p = __builtin_fmaf(GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
         if (incell) {
-          GF(pn)[iz * P + ix] = p;
+          bst(Rpn, p, vcol, so);
           if ((tl.kind & TF_REC) && s.record && iz == npml) {   // kernel.cu:48-71
             const int ir = ix - s.rcol0;
             if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
           }
           if (save && strips) {
-            const int so = strip_p(s, iz, ix);
-            if (so >= 0) store_put(s, it, so, p);
+            const int sp_ = strip_p(s, iz, ix);
+            if (sp_ >= 0) store_put(s, it, sp_, p);
           }
         }
       } else {                                         // receiver wavefield
@@ -523,7 +541,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           }
         }
         if (incell) {
-          GF(pn)[iz * P + ix] = p;
+          bst(Rpn, p, vcol, so);
           if (xin && iz >= npml && iz < nzp - npml) {  // imaging (kernel.cu:1401-1416)
             const int ii = (iz - npml) * s.nxl + ix - npml;
             GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, GCF(s.g1)[ii]);

@@ -82,9 +82,9 @@ def cfg3(nshot=256, seed=3, window_seed=33):
                     dx=dx, dz=dx, dt=0.8e-3, npml=50, fdom=15.0, spx0=50.0, spz0=20.0, dspx=80.0,
                     offset=0.0, offsetmax=3000.0, amp=1.0, alp=0.05, direct=1.5, direct0=0.05, tsnap=0.1)
     rng = np.random.default_rng(window_seed)
-    width = rng.uniform(400.0, 1600.0, nshot)
-    # layout 1 window: 2*nr+1 columns, nr = offsetmax/dx
-    offsetmax = (np.floor((width - 1.0) / 2.0) * dx).astype(np.float32)
+    width = np.floor(rng.uniform(400.0, 1600.0, nshot))
+    # layout 1 (rtm_model.cu:124-134): NX = offsetmax/dx + 1 columns, nr = (NX-1)/2 on each side
+    offsetmax = ((width - 1.0) * dx).astype(np.float32)
     return P, vp, rho, offsetmax
 
 
