@@ -82,7 +82,6 @@ struct ShotDev {
   // boundary store
   void* store;
   float* sscale;               // nt x 4 scales (16-bit formats)
-  unsigned int* smax;          // nt x 4 running max-abs bit patterns (16-bit formats)
   long long strip_len;         // values per step
   int store_fmt, _pad4;
 };

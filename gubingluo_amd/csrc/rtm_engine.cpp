@@ -318,7 +318,7 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
   size_t b = small_layout(s).total + kSlotFields * F + 2 * I;
   b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
   b += align256((size_t)nt * strip * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * strip * 4));
-  b += 2 * align256((size_t)nt * 4 * 4);
+  b += align256((size_t)nt * 4 * 4);
   return b;
 }
 
@@ -372,7 +372,6 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.store = base + o;
   o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * d.strip_len * 4));
   d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4);
-  d.smax = (unsigned int*)(base + o); o += align256((size_t)P.nt * 4 * 4);
 }
 
 hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
@@ -386,7 +385,7 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
   for (int s = 0; s < k.k; s++) {
     a.step = s;
     (void)rtm_launch_step(a, k.phase, k.tz, k.save, st_);
-    if (k.quant) (void)rtm_launch_quant(dshots, dslots, nslots, s, 128, st_);
+    if (k.quant) (void)rtm_launch_quant(dshots, dslots, nslots, s, st_);
   }
   (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
   if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) return nullptr;
@@ -418,7 +417,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
       HIPCHK(hipEventRecord(ev[0], st_));
       HIPCHK(rtm_launch_step(a, phase, tz, save, st_));
       HIPCHK(hipEventRecord(ev[1], st_));
-      if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, 128, st_));
+      if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, st_));
       HIPCHK(hipEventRecord(ev[2], st_));
       HIPCHK(hipEventSynchronize(ev[2]));
       float m0 = 0, m1 = 0;
@@ -442,7 +441,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   for (int s = 0; s < nsteps - done; s++) {
     a.step = s;
     HIPCHK(rtm_launch_step(a, phase, tz, save, st_));
-    if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, 128, st_));
+    if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, st_));
   }
   return 0;
 }
@@ -596,7 +595,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     set_error("tile_z must be 0 (autotuned), 16, 32 or 64");
     return -1;
   }
-  int tz = opt.tile_z > 0 ? opt.tile_z : 32;
+  int tz = opt.tile_z > 0 ? opt.tile_z : 16;
   const Model* M = model(model_id);
   if (!M) { set_error("no such model"); return -1; }
   if (M->nx != P.nx || M->nz != P.nz) { set_error("model size differs from nx, nz"); return -1; }
@@ -731,7 +730,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         for (int s2 = 0; s2 < kTuneSteps; s2++) {
           a.step = c * kTuneSteps + s2;
           HIPCHK(rtm_launch_step(a, PH_FWD, cands[c], 1, st_));
-          if (opt.store_fmt != ST_F32) HIPCHK(rtm_launch_quant(d_ds, d_slots, nb, a.step, 128, st_));
+          if (opt.store_fmt != ST_F32) HIPCHK(rtm_launch_quant(d_ds, d_slots, nb, a.step, st_));
         }
         HIPCHK(hipEventRecord(te[c + 1], st_));
       }

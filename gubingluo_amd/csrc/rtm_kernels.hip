@@ -103,6 +103,56 @@ __device__ __forceinline__ int strip_p(const ShotDev& s, int iz, int ix)
   return -1;
 }
 
+// The same offsets split into a row part (wave-uniform: the lanes of a wave
+// share their rows) and a column part (per lane, computed once per tile):
+// own-column cells then cost one add and a compare.
+struct StripCols {
+  int vx;    // vx lr column part, -1 off the vx strip columns
+  int plr;   // p lr column part, -1 off the p strip columns
+  int tb;    // ix - npml on the interior columns (vz / p tb blocks), -1 elsewhere
+};
+__device__ __forceinline__ StripCols strip_cols(const ShotDev& s, int ix)
+{
+  const int n = s.npml;
+  const int blr = 2 * RTM_R * s.nzi + 2 * RTM_R * s.nxl;
+  StripCols c;
+  c.vx = (ix >= n - RTM_R && ix < n) ? ix - (n - RTM_R)
+       : ((ix >= s.nxp - n - 1 && ix < s.nxp - n - 1 + RTM_R) ? RTM_R * s.nzi + ix - (s.nxp - n - 1) : -1);
+  c.plr = (ix >= n - RTM_R && ix < n) ? blr + ix - (n - RTM_R)
+        : ((ix >= s.nxp - n && ix < s.nxp - n + RTM_R) ? blr + RTM_R * s.nzi + ix - (s.nxp - n) : -1);
+  c.tb = (ix >= n && ix < s.nxp - n) ? ix - n : -1;
+  return c;
+}
+// row part of the vz block offset (rows n-10..n-1 and nzp-n-1..nzp-n+8), -1
+__device__ __forceinline__ int strip_row_vz(const ShotDev& s, int iz)
+{
+  const int n = s.npml, b = 2 * RTM_R * s.nzi;
+  if (iz >= n - RTM_R && iz < n) return b + (iz - (n - RTM_R)) * s.nxl;
+  const int r0 = s.nzp - n - 1;
+  if (iz >= r0 && iz < r0 + RTM_R) return b + RTM_R * s.nxl + (iz - r0) * s.nxl;
+  return -1;
+}
+// row part of the lr blocks (rows n..nzp-n-1), -1
+__device__ __forceinline__ int strip_row_lr(const ShotDev& s, int iz)
+{
+  return (iz >= s.npml && iz < s.nzp - s.npml) ? (iz - s.npml) * RTM_R : -1;
+}
+// row part of the p tb block (rows n-10..n-1 and nzp-n..nzp-n+9), -1
+__device__ __forceinline__ int strip_row_ptb(const ShotDev& s, int iz)
+{
+  const int n = s.npml, b = 4 * RTM_R * s.nzi + 2 * RTM_R * s.nxl;
+  if (iz >= n - RTM_R && iz < n) return b + (iz - (n - RTM_R)) * s.nxl;
+  const int r0 = s.nzp - n;
+  if (iz >= r0 && iz < r0 + RTM_R) return b + RTM_R * s.nxl + (iz - r0) * s.nxl;
+  return -1;
+}
+__device__ __forceinline__ int soff_vz(int row, const StripCols& c) { return (row >= 0 && c.tb >= 0) ? row + c.tb : -1; }
+__device__ __forceinline__ int soff_vx(int lr, const StripCols& c) { return (lr >= 0 && c.vx >= 0) ? lr + c.vx : -1; }
+__device__ __forceinline__ int soff_p(int lr, int tb, const StripCols& c)
+{
+  return (lr >= 0 && c.plr >= 0) ? lr + c.plr : ((tb >= 0 && c.tb >= 0) ? tb + c.tb : -1);
+}
+
 // 16-bit store: block of a strip offset (0 vx, 1 vz, 2 p-lr, 3 p-tb)
 __device__ __forceinline__ int strip_block(const ShotDev& s, int off)
 {
@@ -115,11 +165,10 @@ __device__ __forceinline__ void store_put(const ShotDev& s, int it, int off, flo
   if (s.store_fmt == ST_F32) {
     GF(s.store)[(long long)it * s.strip_len + off] = v;
   } else {
-    // fp32 staging of step it (by parity), quantised by k_quant after the step
+    // fp32 staging of step it (by parity), scaled and quantised by k_quant
+    // after the step
     gfloat* stg = GF(s.store) + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
     stg[off] = v;
-    __atomic_fetch_max((guint*)&s.smax[(long long)it * 4 + strip_block(s, off)], __float_as_uint(fabsf(v)),
-                       __ATOMIC_RELAXED);
   }
 }
 
@@ -140,14 +189,15 @@ struct Geo {
   static constexpr int G = G_, RPT = RPT_;
   static constexpr int TZ = G * RPT;                   // tile rows
   static constexpr int NT = 64 * G;                    // threads
-  static constexpr int PCW = 104;                      // p row: columns x0-20 .. x0+83
+  static constexpr int PW = 104;                       // p row: columns x0-20 .. x0+83
   static constexpr int VXW = 84;                       // vx row: columns x0-10 .. x0+73
   static constexpr int ZR = TZ + 19;                   // vz rows z0-10 .. z0+TZ+8
   static constexpr int KZ = (ZR + G - 1) / G;          // vz rows per thread (consecutive)
-  static constexpr int PZR = G * KZ + 19;              // p column rows held (>= TZ + 38)
-  static constexpr int KP = (TZ + 38 + G - 1) / G;     // p column rows staged per thread
+  static constexpr int PZR = G * KZ + 19;              // p rows held (>= TZ + 38; the last
+                                                       // thread's window may run past TZ+38)
+  static constexpr int KP = (TZ + 38 + G - 1) / G;     // p rows staged per thread
   static constexpr int KH = (20 * TZ + NT - 1) / NT;   // halo vx cells per thread
-  static constexpr int LDS_A = PZR * 64 + TZ * PCW;    // PZ | PC
+  static constexpr int LDS_A = PZR * PW;               // p rows z0-19.. (x halo on the tile rows)
   static constexpr int LDS_B = TZ * VXW + ZR * 64;     // VXL | VZL
   static constexpr int LDS = LDS_A > LDS_B ? LDS_A : LDS_B;
 };
@@ -169,6 +219,13 @@ __device__ __forceinline__ float bld(rsrc_t r, int voff, int soff)
 __device__ __forceinline__ void bst(rsrc_t r, float v, int voff, int soff)
 {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+// load one dword per lane into LDS: lane i writes lds_row[i] (lds_row is
+// wave-uniform)
+typedef __attribute__((address_space(3))) void lds_void_t;
+__device__ __forceinline__ void bld_lds(rsrc_t r, float* lds_row, int voff, int soff)
+{
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void_t*)lds_row, 4, voff, soff, 0, 0);
 }
 
 // true for every lane of the wave if any active lane has c: branches on it
@@ -207,11 +264,11 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
 //   outside the grid.
 // Reconstruction: strip cells restored from the store of step it+1
 // (kernel.cu:1294-1347), interior cells v + D (kernel.cu:318-388).
-#define VEL_CELL(FK, gen, strips, STRIPF, IZ, IX, VALID, RAD, INTERIOR, P_AT, VO, RH, DTH, C1V, C2V, OUT)   \
+#define VEL_CELL(FK, gen, strips, SOFF, VALID, RAD, INTERIOR, P_AT, VO, RH, DTH, C1V, C2V, OUT)            \
   do {                                                                                                     \
     float v_ = 0.0f;                                                                                       \
     if (FK == FK_RCN) {                                                                                    \
-      const int so_ = (strips) ? STRIPF(s, IZ, IX) : -1;                                                   \
+      const int so_ = (strips) ? (SOFF) : -1;                                                              \
       if (so_ >= 0) {                                                                                      \
         v_ = store_get(s, it + 1, so_);                                                                    \
       } else if (!(gen) || (INTERIOR)) {                                                                   \
@@ -291,10 +348,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
                                            float* vzn, const float (&ps)[Q::RPT], float (&pnew)[Q::RPT])
 {
   constexpr int G = Q::G, RPT = Q::RPT, TZ = Q::TZ, NT = Q::NT, KZ = Q::KZ, KP = Q::KP, KH = Q::KH;
-  constexpr int PCW = Q::PCW, VXW = Q::VXW, ZR = Q::ZR;
-  float* PZ = lds;
-  float* PC = lds + Q::PZR * 64;
-  float* VXL = lds;
+  constexpr int PW = Q::PW, VXW = Q::VXW, ZR = Q::ZR;
+  float* PS = lds;                                     // p rows z0-19.., columns x0-20..x0+83
+  float* VXL = lds;                                    // after the second barrier
   float* VZL = lds + TZ * VXW;
   const int tid = threadIdx.x, lx = tid & 63;
   const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -312,21 +368,25 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
   const float dtx = s.dtx, dtz = s.dtz;
 
-  // ---- 1. loads: p with its halo, old velocities and density of every
-  //         velocity cell this tile computes
-  float a[KP], w[RPT];
+  // ---- 1. loads: p with its halo straight into LDS (buffer loads with the
+  //         lds modifier: lane i of a wave writes its dword to base + 4 i),
+  //         old velocities and density of every velocity cell into registers
   {
-    const int wcol = (x0 + (lx < 20 ? lx - 20 : lx + 44) + RTM_MXL) * 4;
+    // wings: lanes 0..19 load columns x0-20.. and x0+64.. with two
+    // instructions of uniform LDS base (the base of an lds load is one value
+    // per wave, so the two wings must not share an instruction)
+    const int wl = (x0 - 20 + lx + RTM_MXL) * 4, wr = (x0 + 64 + lx + RTM_MXL) * 4;
 #pragma unroll
     for (int m = 0; m < KP; m++) {
       const int k = g + G * m;
-      a[m] = 0.0f;
-      if (k < TZ + 38) a[m] = bld(Rp, vcol, srow0 + (k - 19) * pr4);
-    }
-#pragma unroll
-    for (int m = 0; m < RPT; m++) {
-      w[m] = 0.0f;
-      if (lx < 40) w[m] = bld(Rp, wcol, srow0 + (g + G * m) * pr4);
+      if (k < TZ + 38) {
+        const int so = srow0 + (k - 19) * pr4;
+        bld_lds(Rp, PS + k * PW + 20, vcol, so);
+        if (k >= 19 && k < 19 + TZ && lx < 20) {
+          bld_lds(Rp, PS + k * PW, wl, so);
+          bld_lds(Rp, PS + k * PW + 84, wr, so);
+        }
+      }
     }
   }
   float vzo[KZ], rz[KZ];
@@ -364,17 +424,20 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       hrh[k] = bld(Rrho, off, 0);
     }
   }
-#pragma unroll
-  for (int m = 0; m < KP; m++) {
-    const int k = g + G * m;
-    if (k < TZ + 38) {
-      PZ[k * 64 + lx] = a[m];
-      if (k >= 19 && k < 19 + TZ) PC[(k - 19) * PCW + 20 + lx] = a[m];
-    }
+  // per-lane column data of the generic path: existence, radius, damping
+  bool vxv = true, dampx = false;
+  int rrx = RTM_R;
+  float c1xv = 1.0f, c2xv = 1.0f;
+  StripCols sc{-1, -1, -1};
+  if (gen) {
+    vxv = ix <= nxp - 2;
+    rrx = vxv ? imin(RTM_R, imin(ix + 1, nxp - 1 - ix)) : RTM_R;
+    c1xv = GCF(s.c1xv)[ix];
+    c2xv = GCF(s.c2xv)[ix];
+    dampx = !(c1xv == 1.0f && c2xv == 1.0f);
   }
-#pragma unroll
-  for (int m = 0; m < RPT; m++)
-    if (lx < 40) PC[(g + G * m) * PCW + (lx < 20 ? lx : lx + 64)] = w[m];
+  if (strips) sc = strip_cols(s, ix);
+  __builtin_amdgcn_s_waitcnt(0);                       // the LDS writes of the staging loads
   __syncthreads();                                     // p staged
 
   // ---- 2. new velocities
@@ -383,22 +446,20 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     const rsrc_t Rvzn = field_rsrc(vzn, s);
     float cw[KZ + 19];                                 // p rows z0-19+g*KZ+m of the column
 #pragma unroll
-    for (int m = 0; m < KZ + 19; m++) cw[m] = PZ[(g * KZ + m) * 64 + lx];
+    for (int m = 0; m < KZ + 19; m++) cw[m] = PS[(g * KZ + m) * PW + 20 + lx];
 #pragma unroll
     for (int k = 0; k < KZ; k++) {
       const int kz = g * KZ + k;
-      const int iz = z0 - 10 + kz;
+      const int iz = z0 - 10 + kz;                     // wave-uniform
 #define PZA(i) cw[k + 9 + (i)]
-      VEL_CELL(FK, gen, strips, strip_vz, iz, ix, (iz >= 0 && iz <= nzp - 2 && ix < nxp),
-               imin(RTM_R, imin(iz + 1, nzp - 1 - iz)), (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], rz[k], dtz,
+      const int svz = strips ? soff_vz(strip_row_vz(s, iz), sc) : -1;
+      VEL_CELL(FK, gen, strips, svz, (iz >= 0 && iz <= nzp - 2), imin(RTM_R, imin(iz + 1, nzp - 1 - iz)),
+               (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], rz[k], dtz,
                GCF(s.c1zv)[iz], GCF(s.c2zv)[iz], vzn_[k]);
 #undef PZA
       if (kz < ZR && kz >= 10 && kz < 10 + TZ && ix < nxp && iz <= nzp - 2) {
         bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
-        if (FK == FK_SRC && save && strips) {
-          const int so = strip_vz(s, iz, ix);
-          if (so >= 0) store_put(s, it, so, vzn_[k]);
-        }
+        if (FK == FK_SRC && save && strips && svz >= 0) store_put(s, it, svz, vzn_[k]);
       }
     }
   }
@@ -407,30 +468,29 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       const int j = g * RPT + r, iz = z0 + j;
-      const float* __restrict__ Pr = PC + j * PCW + lx + 20;   // p at column ix
+      const float* __restrict__ Pr = PS + (j + 19) * PW + lx + 20;   // p at column ix
 #define PCA(i) Pr[(i)]
-      VEL_CELL(FK, gen, strips, strip_vx, iz, ix, (ix <= nxp - 2 && iz < nzp),
-               imin(RTM_R, imin(ix + 1, nxp - 1 - ix)), (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], rho[r], dtx,
-               GCF(s.c1xv)[ix], GCF(s.c2xv)[ix], vxn_[r]);
+      const int svx = strips ? soff_vx(strip_row_lr(s, iz), sc) : -1;
+      VEL_CELL(FK, gen, strips, svx, vxv, rrx,
+               (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], rho[r], dtx,
+               c1xv, c2xv, vxn_[r]);
 #undef PCA
       if (iz < nzp && ix <= nxp - 2) {
         bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
-        if (FK == FK_SRC && save && strips) {
-          const int so = strip_vx(s, iz, ix);
-          if (so >= 0) store_put(s, it, so, vxn_[r]);
-        }
+        if (FK == FK_SRC && save && strips && svx >= 0) store_put(s, it, svx, vxn_[r]);
       }
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
       const int h = k * NT + tid;
       const int iz = z0 + hrow[k], jx = x0 - 10 + hcol[k];
-      const float* __restrict__ Pr = PC + hrow[k] * PCW + hcol[k] + 10;   // p at column jx
+      const float* __restrict__ Pr = PS + (hrow[k] + 19) * PW + hcol[k] + 10;   // p at column jx
       hvn[k] = 0.0f;
       if (h < 20 * TZ) {
+        const bool hv = jx >= 0 && jx <= nxp - 2;
 #define PCA(i) Pr[(i)]
-        VEL_CELL(FK, gen, strips, strip_vx, iz, jx, (jx >= 0 && jx <= nxp - 2 && iz < nzp),
-                 imin(RTM_R, imin(jx + 1, nxp - 1 - jx)), (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hrh[k], dtx,
+        VEL_CELL(FK, gen, strips, strip_vx(s, iz, jx), hv, (hv ? imin(RTM_R, imin(jx + 1, nxp - 1 - jx)) : RTM_R),
+                 (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hrh[k], dtx,
                  GCF(s.c1xv)[jx], GCF(s.c2xv)[jx], hvn[k]);
 #undef PCA
       }
@@ -443,10 +503,10 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
     const int j = g * RPT + r;
-    po[r] = PC[j * PCW + 20 + lx];
+    po[r] = PS[(j + 19) * PW + 20 + lx];
     vv[r] = bld(Rvv, vcol, srow0 + j * pr4);
   }
-  __syncthreads();                                     // PZ/PC consumed
+  __syncthreads();                                     // PS consumed
 #pragma unroll
   for (int k = 0; k < KZ; k++) {
     const int kz = g * KZ + k;
@@ -465,6 +525,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   for (int m = 0; m < RPT + 19; m++) cz[m] = VZL[(g * RPT + m) * 64 + lx];
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
+  const int radx = imin(RTM_R, imin(ix, nxp - 1 - ix));
   float c1x = 1.0f, c2x = 1.0f;
   if (FK != FK_RCN && gen && ix < nxp) { c1x = GCF(s.c1x)[ix]; c2x = GCF(s.c2x)[ix]; }
   const bool xsrc = (tl.kind & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
@@ -473,14 +534,14 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
     const int j = g * RPT + r;
-    const int iz = z0 + j;
+    const int iz = z0 + j;                             // wave-uniform
     const int so = srow0 + j * pr4;
     float p = po[r];
     const float* __restrict__ VXr = VXL + j * VXW + lx + 10;
     const bool incell = ix < nxp && iz < nzp;
     if (FK == FK_RCN) {
       const bool inner = xin && iz >= npml && iz < nzp - npml;
-      const int sp_ = strips ? strip_p(s, iz, ix) : -1;
+      const int sp_ = strips ? soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc) : -1;
       bool wr = false;
       if (sp_ >= 0) {
         p = store_get(s, it, sp_);                     // kernel.cu:1348-1400
@@ -500,8 +561,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       } else {
         // the outer ring keeps its value (only the source is added there);
         // cells off the grid are computed with radius 10 and dropped
-        const bool upd = incell && !xring && iz > 0 && iz < nzp - 1;
-        const int rad = upd ? imin(imin(imin(RTM_R, ix), imin(iz, nxp - 1 - ix)), nzp - 1 - iz) : RTM_R;
+        const bool zupd = iz > 0 && iz < nzp - 1;
+        const bool upd = zupd && incell && !xring;
+        const int rad = upd ? imin(radx, imin(iz, nzp - 1 - iz)) : RTM_R;
         DivV d;
         if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
         else d = div_v(VXr, cz, r, rad, c10, s.coef, dtx, dtz, rho[r], vv[r]);
@@ -528,7 +590,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
             if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
           }
           if (save && strips) {
-            const int sp_ = strip_p(s, iz, ix);
+            const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
             if (sp_ >= 0) store_put(s, it, sp_, p);
           }
         }
@@ -604,38 +666,43 @@ __global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
                         odd ? s.rvz[0] : s.rvz[1], ps, pr);
 }
 
-// 16-bit boundary store: quantise the fp32 staging of step `it` of each slot
-// with the max-abs of its block: q = x / scale, scale = max(maxabs / 32768, FLT_MIN).
+// 16-bit boundary store: one workgroup per (slot, block) of step `it`
+// reduces the max-abs of the block's fp32 staging, keeps the scale
+// max(maxabs / 32768, FLT_MIN) (1 for an all-zero block) and stores q = x / scale
+// (a denormal max-abs, e.g. source tails before the wave arrives, must not
+// underflow the scale to zero: |x| <= 32768 * FLT_MIN then anyway)
 __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
                                                  int nslots, int step)
 {
-  for (int k = 0; k < nslots; k++) {
-    const ShotDev s = shots[slots[k]];
-    const int it = s.it_off + s.it_dir * step;
-    const float* stg = (const float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
-    float sc[4];
+  __shared__ float red[NTHR / 64];
+  const int k = blockIdx.x >> 2, b = blockIdx.x & 3;
+  if (k >= nslots) return;
+  const ShotDev s = shots[slots[k]];
+  const int it = s.it_off + s.it_dir * step;
+  const float* stg = (const float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
+  const int e1 = 2 * RTM_R * s.nzi, e2 = e1 + 2 * RTM_R * s.nxl, e3 = e2 + 2 * RTM_R * s.nzi;
+  const int lo = b == 0 ? 0 : (b == 1 ? e1 : (b == 2 ? e2 : e3));
+  const int hi = b == 0 ? e1 : (b == 1 ? e2 : (b == 2 ? e3 : (int)s.strip_len));
+  float m = 0.0f;
+  for (int e = lo + (int)threadIdx.x; e < hi; e += NTHR) m = fmaxf(m, fabsf(stg[e]));
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      // a denormal max-abs (source tails before the wave arrives) must not
-      // underflow the scale to zero: |x| <= 32768 * FLT_MIN then anyway
-      const float m = __uint_as_float(s.smax[(long long)it * 4 + b]);
-      sc[b] = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-      for (int b = 0; b < 4; b++) s.sscale[(long long)it * 4 + b] = sc[b];
-    const long long n = s.strip_len;
-    const long long a = 2LL * RTM_R * s.nzi, b = a + 2LL * RTM_R * s.nxl, c = b + 2LL * RTM_R * s.nzi;
-    for (long long e = (long long)blockIdx.x * NTHR + threadIdx.x; e < n; e += (long long)gridDim.x * NTHR) {
-      const int blk = e < a ? 0 : (e < b ? 1 : (e < c ? 2 : 3));
-      const float q = stg[e] / sc[blk];
-      const long long k2 = (long long)it * n + e;
-      if (s.store_fmt == ST_F16) {
-        ((__half*)s.store)[k2] = __float2half_rn(q);
-      } else {
-        unsigned int u = __float_as_uint(q);          // bf16, round to nearest even
-        u += 0x7FFFu + ((u >> 16) & 1u);
-        ((unsigned short*)s.store)[k2] = (unsigned short)(u >> 16);
-      }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = red[0];
+#pragma unroll
+  for (int w = 1; w < NTHR / 64; w++) m = fmaxf(m, red[w]);
+  const float sc = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
+  if (threadIdx.x == 0) s.sscale[(long long)it * 4 + b] = sc;
+  const long long base = (long long)it * s.strip_len;
+  for (int e = lo + (int)threadIdx.x; e < hi; e += NTHR) {
+    const float q = stg[e] / sc;
+    if (s.store_fmt == ST_F16) {
+      ((__half*)s.store)[base + e] = __float2half_rn(q);
+    } else {
+      unsigned int u = __float_as_uint(q);            // bf16, round to nearest even
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      ((unsigned short*)s.store)[base + e] = (unsigned short)(u >> 16);
     }
   }
 }
@@ -668,8 +735,6 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
       (s.mrho - s.forg)[e] = s.rho[m];
       (s.mvv - s.forg)[e] = s.vv[m];
     }
-    if (s.store_fmt != ST_F32)
-      for (long long e = g0; e < 4LL * s.nt; e += gs) s.smax[e] = 0u;
   } else {
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
   }
@@ -785,9 +850,10 @@ hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hip
   }
 }
 
-hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, int nblocks, hipStream_t st)
+hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_quant, dim3(nblocks), dim3(NTHR), 0, st, shots, slots, nslots, step);
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quant, dim3(4 * nslots), dim3(NTHR), 0, st, shots, slots, nslots, step);
   return hipGetLastError();
 }
 

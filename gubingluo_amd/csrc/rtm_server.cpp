@@ -120,7 +120,7 @@ class Server {
 Server::Server(Engine* eng, const rtmh_server_options& o) : eng_(eng), opt_(o)
 {
   if (opt_.max_slots <= 0) opt_.max_slots = 256;
-  if (opt_.tile_z <= 0) opt_.tile_z = 32;
+  if (opt_.tile_z <= 0) opt_.tile_z = 16;
   t0_ = std::chrono::steady_clock::now();
 }
 
@@ -487,7 +487,7 @@ void Server::loop()
     if (ntf_ > 0) {
       LaunchArgs a{d_ds_, d_tf_, ntf_, (int)tick_};
       if (rtm_launch_step(a, PH_FWD, opt_.tile_z, 1, st_) != hipSuccess) fail("forward launch", hipGetLastError());
-      if (nq_ > 0) (void)rtm_launch_quant(d_ds_, d_qs_, nq_, (int)tick_, 128, st_);
+      if (nq_ > 0) (void)rtm_launch_quant(d_ds_, d_qs_, nq_, (int)tick_, st_);
       launches_ += 2;
     }
     if (ntr_ > 0) {
@@ -571,7 +571,7 @@ extern "C" int rtmh_server_create(rtmh_ctx* ctx, const rtmh_server_options* opt,
   if (opt) o = *opt;
   if (o.store_fmt < 0 || o.store_fmt > 2) { rtm::set_error("store_fmt must be 0, 1 or 2"); return -1; }
   if (o.tile_z != 0 && o.tile_z != 16 && o.tile_z != 32 && o.tile_z != 64) {
-    rtm::set_error("tile_z must be 0 (32), 16, 32 or 64");
+    rtm::set_error("tile_z must be 0 (16), 16, 32 or 64");
     return -1;
   }
   auto* h = new rtmh_server;

@@ -215,7 +215,7 @@ typedef struct rtmh_server rtmh_server;
 typedef struct rtmh_server_options {
   double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of free */
   int max_slots;      /* shots resident at once; 0 = 256                          */
-  int tile_z;         /* 16 / 32 / 64; 0 = 32                                     */
+  int tile_z;         /* 16 / 32 / 64; 0 = 16                                     */
   int store_fmt;      /* boundary store precision of every request                */
 } rtmh_server_options;
 typedef struct rtmh_request_stats {
