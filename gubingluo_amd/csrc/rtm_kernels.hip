@@ -237,14 +237,15 @@ __device__ __forceinline__ bool any_lane(bool c) { return __builtin_amdgcn_ballo
 // order 2r; the fma chain runs i = 1..r as the reference loops
 // (kernel.cu:459-460).  The reduced-radius loop is unrolled with a
 // predicate so that the operand windows stay in registers (a runtime index
-// would put them in scratch memory); row r-1 = 9 of the table is c10, so
-// lanes at full radius give the same sums in either branch.
+// would put them in scratch memory); cf is the coefficient table in LDS
+// (per-lane rows); row r-1 = 9 of the table is c10, so lanes at full radius
+// give the same sums in either branch.
 #define STENCIL_SUM(acc, r, c10, cf, A, B)                                                        \
   do {                                                                                            \
     if (!any_lane((r) != RTM_R)) {                                                                \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], (A) - (B), acc); \
     } else {                                                                                      \
-      const gfloat* cr_ = GCF(cf) + ((r) - 1) * RTM_R;                                            \
+      const float* cr_ = (cf) + ((r) - 1) * RTM_R;                                                \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++)                                          \
         if (i <= (r)) acc = __builtin_fmaf(cr_[i - 1], (A) - (B), acc);                           \
     }                                                                                             \
@@ -287,7 +288,7 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
     } else {                                                                                               \
       const int rr_ = (VALID) ? (RAD) : RTM_R;                                                             \
       float acc = 0.0f;                                                                                    \
-      STENCIL_SUM(acc, rr_, c10, s.coef, P_AT(i), P_AT(1 - i));                                            \
+      STENCIL_SUM(acc, rr_, c10, ctab, P_AT(i), P_AT(1 - i));                                              \
       acc = acc * (DTH);                                                                                   \
       acc = acc / (RH);                                                                                    \
       const float c1_ = (C1V), c2_ = (C2V);                                                                \
@@ -316,7 +317,7 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
       sz = __builtin_fmaf(c10[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
     }
   } else {
-    const gfloat* c = GCF(cf) + (rad - 1) * RTM_R;
+    const float* c = cf + (rad - 1) * RTM_R;           // coefficient table in LDS
 #pragma unroll
     for (int i = 1; i <= RTM_R; i++) {
       if (i <= rad) {
@@ -345,7 +346,8 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
 template <class Q, int FK>
 __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDev& s, const Tile& tl, int it, int save,
                                            const float* pc, const float* vxc, const float* vzc, float* pn, float* vxn,
-                                           float* vzn, const float (&ps)[Q::RPT], float (&pnew)[Q::RPT])
+                                           float* vzn, const float* __restrict__ ctab, const float (&ps)[Q::RPT],
+                                           float (&pnew)[Q::RPT])
 {
   constexpr int G = Q::G, RPT = Q::RPT, TZ = Q::TZ, NT = Q::NT, KZ = Q::KZ, KP = Q::KP, KH = Q::KH;
   constexpr int PW = Q::PW, VXW = Q::VXW, ZR = Q::ZR;
@@ -547,7 +549,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         p = store_get(s, it, sp_);                     // kernel.cu:1348-1400
         wr = true;
       } else if (!gen || inner) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab, dtx, dtz, rho[r], vv[r]);
         p = p + (d.dxv + d.dzv);                       // kernel.cu:389-429
         wr = true;
       }
@@ -556,7 +558,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       if (wr) bst(Rpn, p, vcol, so);
     } else {
       if (!gen) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab, dtx, dtz, rho[r], vv[r]);
         p = p - (d.dxv + d.dzv);                       // kernel.cu:539
       } else {
         // the outer ring keeps its value (only the source is added there);
@@ -565,8 +567,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         const bool upd = zupd && incell && !xring;
         const int rad = upd ? imin(radx, imin(iz, nzp - 1 - iz)) : RTM_R;
         DivV d;
-        if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, s.coef, dtx, dtz, rho[r], vv[r]);
-        else d = div_v(VXr, cz, r, rad, c10, s.coef, dtx, dtz, rho[r], vv[r]);
+        if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, ctab, dtx, dtz, rho[r], vv[r]);
+        else d = div_v(VXr, cz, r, rad, c10, ctab, dtx, dtz, rho[r], vv[r]);
         const bool split = upd && !(xin && iz >= npml && iz < nzp - npml);
         float pu = p - (d.dxv + d.dzv);
         if (any_lane(split) && split) {                // split pair, p = p1 + p2 (kernel.cu:780-783)
@@ -625,10 +627,12 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
 {
   using Q = Geo<G, RPT>;
   __shared__ float lds[Q::LDS];
+  __shared__ float ctab[100];                          // coefficient table (reduced radii near the edges)
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
+  if (threadIdx.x < 100) ctab[threadIdx.x] = GCF(s.coef)[threadIdx.x];
   const int it = s.it_off + s.it_dir * a.step;
   const bool odd = it & 1;                             // (a runtime index into s would spill it to scratch)
   float none[Q::RPT], pnew[Q::RPT];
@@ -636,7 +640,7 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
   for (int r = 0; r < Q::RPT; r++) none[r] = 0.0f;
   field_step<Q, FK_SRC>(lds, s, tl, it, save, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
                         odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
-                        odd ? s.svz[0] : s.svz[1], none, pnew);
+                        odd ? s.svz[0] : s.svz[1], ctab, none, pnew);
 }
 
 // reverse time step: source reconstruction, then the receiver wavefield and
@@ -646,10 +650,12 @@ __global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
 {
   using Q = Geo<G, RPT>;
   __shared__ float lds[Q::LDS];
+  __shared__ float ctab[100];
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
+  if (threadIdx.x < 100) ctab[threadIdx.x] = GCF(s.coef)[threadIdx.x];
   const int it = s.it_off + s.it_dir * a.step;
   const bool odd = it & 1;
   float ps[Q::RPT], pr[Q::RPT], none[Q::RPT];
@@ -658,12 +664,12 @@ __global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
   if (tl.kind & TF_INT) {
     field_step<Q, FK_RCN>(lds, s, tl, it, 0, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
                           odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
-                          odd ? s.svz[0] : s.svz[1], none, ps);
+                          odd ? s.svz[0] : s.svz[1], ctab, none, ps);
     __syncthreads();                                   // LDS reused by the receiver pass
   }
   field_step<Q, FK_RCV>(lds, s, tl, it, 0, odd ? s.rp[1] : s.rp[0], odd ? s.rvx[1] : s.rvx[0],
                         odd ? s.rvz[1] : s.rvz[0], odd ? s.rp[0] : s.rp[1], odd ? s.rvx[0] : s.rvx[1],
-                        odd ? s.rvz[0] : s.rvz[1], ps, pr);
+                        odd ? s.rvz[0] : s.rvz[1], ctab, ps, pr);
 }
 
 // 16-bit boundary store: one workgroup per (slot, block) of step `it`
