@@ -81,7 +81,17 @@ def cpu_baseline(P_full, vp, rho, min_seconds=10.0, max_shots=6, first=100):
                       f"{cells / 1e9:.2f} Gcell in {dt:.1f} s; {len(shots) / dt * 3600.0:.0f} shots/hour"}
 
 
-def roofline_of(recs, P):
+def traffic_ratio(kernel):
+    """HBM bytes / algorithmic bytes of a step kernel, measured offline with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/step_kernel_traffic.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "step_kernel_traffic.json")) as f:
+            return float(json.load(f)[kernel]["traffic_over_algorithmic"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline_of(recs, P, copy_gbs=None):
     """Dominant kernel: the reverse time step k_rev (one launch per step:
     reconstruction, receiver wavefield, imaging), timed by HIP events on the
     engine's stream around every reverse pass of the timed region; the
@@ -95,8 +105,14 @@ def roofline_of(recs, P):
     fwd_steps = sum(s.batches * P.nt for s, _, _ in recs)
     fwd_ms = sum(s.fwd_ms for s, _, _ in recs)
     fwd_bytes = float(np.mean([s.step_bytes_fwd for s, _, _ in recs]))
+    ratio = traffic_ratio("k_rev")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if ratio is None else round(ratio * step_bytes),
+            "traffic_source": None if ratio is None else
+            f"HBM/algorithmic bytes {ratio} of k_rev from rocprofv3 FETCH_SIZE+WRITE_SIZE "
+            "(profiles/k_fwd_k_rev_cfg2_20shots_traffic.md) times this run's algorithmic bytes",
+            "copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
             "kernel": "k_rev (reverse time step: reconstruction + receiver wavefield + imaging)",
             "avg_launch_us": round(avg_step_s * 1e6, 2), "alg_bytes_per_launch": step_bytes,
             "forward_step": {"kernel": "k_fwd", "achieved": round(fwd_bytes / (fwd_ms * 1e-3 / fwd_steps) / 1e9, 1),
@@ -123,6 +139,9 @@ def run_survey(args, G, synth):
     log(f"[bench] cfg2 survey: {P.nx}x{P.nz} nt={P.nt} npml={P.npml}, {S} shots/step, "
         f"steps={args.steps} warmup={args.warmup} store={args.store} chunk={args.chunk}")
     ctx = G.Context(0)
+    copy_gbs = ctx.copy_probe(args.copy_mib << 20, 3) if args.copy_mib else None
+    if copy_gbs:
+        log(f"[bench] device-to-device copy of {args.copy_mib} MiB: {copy_gbs:.0f} GB/s (read + write)")
     mid = ctx.model_create(vp, rho)
 
     def step(k):
@@ -150,7 +169,7 @@ def run_survey(args, G, synth):
     cells = sum(s.cell_updates for s, _, _ in recs)
     nshots = sum(len(sh) for _, _, sh in recs)
     lat = np.concatenate([np.full(len(sh), dt) for _, dt, sh in recs])
-    roof = roofline_of(recs, P)
+    roof = roofline_of(recs, P, copy_gbs)
     out = base_line(
         args, round(cells / elapsed / 1e9, 3), "Gcell/s", elapsed,
         "f32" if store == G.STORE_F32 else f"f32 stencil, {args.store} boundaries",
@@ -349,6 +368,8 @@ def main():
     ap.add_argument("--requests", type=int, default=16, help="serve: requests in the stream")
     ap.add_argument("--plain-requests", type=int, default=0, help="serve: requests run in the plain form (0: all)")
     ap.add_argument("--mem-budget", type=float, default=0.0, help="serve: HBM budget in bytes (0: 85%% free)")
+    ap.add_argument("--copy-mib", type=int, default=256,
+                    help="survey: size of the device-to-device copy probe (achievable bandwidth; 0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle error")
     args = ap.parse_args()
 

@@ -23,7 +23,7 @@ STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 EXPORTED = [
     "rtm_model", "rtm_real", "rtmh_last_error", "rtmh_version", "rtmh_device_count", "rtmh_create",
     "rtmh_destroy", "rtmh_shot_geometry", "rtmh_run", "rtmh_model_create", "rtmh_model_destroy",
-    "rtmh_run_model", "rtmh_fd_steps", "rtmh_coef_table", "rtmh_read_parfile",
+    "rtmh_run_model", "rtmh_fd_steps", "rtmh_copy_probe", "rtmh_coef_table", "rtmh_read_parfile",
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
     "rtmh_server_create", "rtmh_server_submit", "rtmh_server_wait", "rtmh_server_query", "rtmh_server_destroy",
 ]
@@ -258,6 +258,13 @@ class Context:
         if rc != 0:
             raise RuntimeError(f"rtmh_run failed ({rc}): {last_error()}")
         return RunResult(img1, img2, ill, lap, nrm, seis, g1s, ils, st)
+
+    def copy_probe(self, nbytes: int, reps: int = 3) -> float:
+        """Device-to-device copy bandwidth, GB/s counting read + written bytes."""
+        g = C.c_double(0.0)
+        if lib().rtmh_copy_probe(self._h, C.c_double(float(nbytes)), C.c_int(reps), C.byref(g)) != 0:
+            raise RuntimeError(f"rtmh_copy_probe failed: {last_error()}")
+        return float(g.value)
 
     def fd_steps(self, P: Params, ishot, vp0, rho0, fields, nsteps, *, it0=0, with_source=False, coef=None):
         """Advance fields (dict p, vx, vz, p1, p2 of shape (nzp, nxp), float32) in place."""

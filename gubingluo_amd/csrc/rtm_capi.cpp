@@ -89,6 +89,15 @@ extern "C" int rtmh_run_model(rtmh_ctx* ctx, int model_id, const rtmh_params* p,
   return ctx->eng->run_model(model_id, *p, diffcoef, o, out, stats);
 }
 
+extern "C" int rtmh_copy_probe(rtmh_ctx* ctx, double bytes, int reps, double* gbps)
+{
+  if (!ctx || !gbps || bytes <= 0 || reps <= 0) {
+    rtm::set_error("bad argument");
+    return -1;
+  }
+  return ctx->eng->copy_probe((size_t)bytes, reps, gbps);
+}
+
 extern "C" int rtmh_fd_steps(rtmh_ctx* ctx, const rtmh_params* p, int ishot, const float* vp0, const float* rho0,
                              const float* diffcoef, int nsteps, int it0, int with_source, float* pf, float* vxf,
                              float* vzf, float* p1f, float* p2f)

@@ -903,6 +903,36 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   return 0;
 }
 
+int Engine::copy_probe(size_t bytes, int reps, double* gbps)
+{
+  HIPCHK(hipSetDevice(device_));
+  void* a = pool_.get(bytes);
+  void* b = pool_.get(bytes);
+  if (!a || !b) {
+    if (a) pool_.put(a);
+    if (b) pool_.put(b);
+    set_error("out of device memory (copy probe)");
+    return -3;
+  }
+  HIPCHK(hipMemsetAsync(a, 0, bytes, st_));
+  HIPCHK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, st_));   // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st_));
+  for (int r = 0; r < reps; r++) HIPCHK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, st_));
+  HIPCHK(hipEventRecord(e1, st_));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pool_.put(a);
+  pool_.put(b);
+  *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  return 0;
+}
+
 int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const float* rho0, const float* coef,
                      int nsteps, int it0, int with_source, float* pf, float* vxf, float* vzf, float* p1f, float* p2f)
 {

@@ -93,6 +93,9 @@ class Engine {
   int fd_steps(const rtmh_params& P, int ishot, const float* vp0, const float* rho0, const float* coef, int nsteps,
                int it0, int with_source, float* pf, float* vxf, float* vzf, float* p1f, float* p2f);
 
+  // device-to-device copy bandwidth (read + write bytes per second / 1e9)
+  int copy_probe(size_t bytes, int reps, double* gbps);
+
   // bytes one slot of this geometry needs on the device
   static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt);
   // build the device descriptor of one slot inside `base` (uploads: host

@@ -189,7 +189,8 @@ struct Geo {
   static constexpr int G = G_, RPT = RPT_;
   static constexpr int TZ = G * RPT;                   // tile rows
   static constexpr int NT = 64 * G;                    // threads
-  static constexpr int PW = 104;                       // p row: columns x0-20 .. x0+83
+  static constexpr int PW = 128;                       // p row: columns x0-20 .. x0+83 (pitch a
+                                                       // multiple of 64 for stride-64 paired reads)
   static constexpr int VXW = 84;                       // vx row: columns x0-10 .. x0+73
   static constexpr int ZR = TZ + 19;                   // vz rows z0-10 .. z0+TZ+8
   static constexpr int KZ = (ZR + G - 1) / G;          // vz rows per thread (consecutive)
@@ -220,6 +221,18 @@ __device__ __forceinline__ void bst(rsrc_t r, float v, int voff, int soff)
 {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
 }
+// An LDS pointer the compiler must materialise as one register: the taps of
+// a stencil then use the small immediate offsets of that base instead of a
+// separate address add per access (large constant row offsets do not fit
+// the 8-bit offsets of the paired LDS reads)
+typedef __attribute__((address_space(3))) const float lds_cfloat_t;
+__device__ __forceinline__ const float* lds_base(const float* p)
+{
+  unsigned a = (unsigned)(uintptr_t)(lds_cfloat_t*)p;
+  asm volatile("" : "+v"(a));
+  return (const float*)(lds_cfloat_t*)(uintptr_t)a;
+}
+
 // load one dword per lane into LDS: lane i writes lds_row[i] (lds_row is
 // wave-uniform)
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -470,7 +483,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       const int j = g * RPT + r, iz = z0 + j;
-      const float* __restrict__ Pr = PS + (j + 19) * PW + lx + 20;   // p at column ix
+      const float* __restrict__ Pr = lds_base(PS + (j + 19) * PW + lx + 11) + 9;   // p at column ix
 #define PCA(i) Pr[(i)]
       const int svx = strips ? soff_vx(strip_row_lr(s, iz), sc) : -1;
       VEL_CELL(FK, gen, strips, svx, vxv, rrx,
@@ -486,7 +499,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     for (int k = 0; k < KH; k++) {
       const int h = k * NT + tid;
       const int iz = z0 + hrow[k], jx = x0 - 10 + hcol[k];
-      const float* __restrict__ Pr = PS + (hrow[k] + 19) * PW + hcol[k] + 10;   // p at column jx
+      const float* __restrict__ Pr = lds_base(PS + (hrow[k] + 19) * PW + hcol[k] + 1) + 9;   // p at column jx
       hvn[k] = 0.0f;
       if (h < 20 * TZ) {
         const bool hv = jx >= 0 && jx <= nxp - 2;
@@ -539,7 +552,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     const int iz = z0 + j;                             // wave-uniform
     const int so = srow0 + j * pr4;
     float p = po[r];
-    const float* __restrict__ VXr = VXL + j * VXW + lx + 10;
+    const float* __restrict__ VXr = lds_base(VXL + j * VXW + lx) + 10;
     const bool incell = ix < nxp && iz < nzp;
     if (FK == FK_RCN) {
       const bool inner = xin && iz >= npml && iz < nzp - npml;

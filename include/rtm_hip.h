@@ -150,6 +150,11 @@ int rtmh_fd_steps(rtmh_ctx *ctx, const rtmh_params *p, int ishot, const float *v
                   const float *diffcoef, int nsteps, int it0, int with_source, float *pf, float *vxf, float *vzf,
                   float *p1f, float *p2f);
 
+/* Achievable-bandwidth probe (no counterpart in the reference; measurement
+ * only): `reps` device-to-device copies of `bytes` on the context's stream,
+ * timed with HIP events; *gbps = (read + written bytes) / average time. */
+int rtmh_copy_probe(rtmh_ctx *ctx, double bytes, int reps, double *gbps);
+
 
 /* ------------------------------------------------------------------ */
 /* 3. drop-in driver helpers (the host side of RTM_GPU_velstr.cpp)     */
