@@ -17,12 +17,17 @@
 //   q1, q2                : split pressure pair in PML cells (in place; the
 //                           source field's in the forward phase, the
 //                           receiver field's in the reverse phase)
-//   mrho, mvv             : the shot's model window (rho, v^2), edge
-//                           replicated into the PML and the margins
+//   mib, mkk              : the shot's model window as 1/rho and rho*v^2
+//                           (the factors of the velocity and pressure
+//                           updates), edge replicated into the PML and the
+//                           margins
 //   vx(iz,ix) sits at x=(ix+1/2)dx and uses columns 0..nxp-2; vz(iz,ix) at
 //   z=(iz+1/2)dz uses rows 0..nzp-2 (kernel.cu:215-216 array shapes).
 // Damping factors are stored with the same margins (index -RTM_MXL.. of the
-// column arrays, -RTM_MZT.. of the row arrays; 1.0 outside the grid).
+// column arrays, -RTM_MZT.. of the row arrays; 1.0 outside the grid) as
+// a = c1/c2 and b = 1/c2 of the reference's c1 = 1 - dt*d/2, c2 = 2 - c1,
+// so that the damped update (c1*v - u)/c2 is fma(a, v, -(u*b)); a = b = 1
+// off the absorbing layer.
 // Imaging accumulators g1, illum: nzi x nxl, pitch nxl.
 // Boundary store: nt steps x strip_len values, packed per step as
 //   vx lr [2][nzi][10] | vz tb [2][10][nxl] | p lr [2][nzi][10] | p tb [2][10][nxl]
@@ -67,10 +72,11 @@ struct ShotDev {
   float *sp[2], *svx[2], *svz[2];
   float *rp[2], *rvx[2], *rvz[2];
   float *q1, *q2;
-  float *mrho, *mvv;
+  float *mib, *mkk;
   float *g1, *illum;
-  // per-column / per-row damping factors c1 = 1-dt*d/2, c2 = 2-c1
-  const float *c1x, *c2x, *c1xv, *c2xv, *c1z, *c2z, *c1zv, *c2zv;
+  // per-column / per-row damping factors a = c1/c2, b = 1/c2 at p nodes
+  // (x, z) and at the velocity half nodes (xv, zv)
+  const float *ax, *bx, *axv, *bxv, *az, *bz, *azv, *bzv;
   const float* coef;           // [100], row r-1 = order 2r
   // source: a = dt*src inside the box [sx0,sx1] x [sz0,sz1]
   int sx0, sx1, sz0, sz1;

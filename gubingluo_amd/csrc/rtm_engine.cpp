@@ -128,14 +128,17 @@ void taper_row(std::vector<float>& t, int nxl, int nwin)
   }
 }
 
-// PML profile (kernel.cu:174-269) and the damping factors of kernel.cu:592-593
-static void damping(std::vector<float>& c1, std::vector<float>& c2, int n, int npml, float vpmax, float h,
+// PML profile (kernel.cu:174-269) and the damping factors of kernel.cu:592-593,
+// c1 = 1 - dt*d/2 and c2 = 2 - c1, kept as a = c1/c2 and b = 1/c2 (each
+// rounded once): the kernels evaluate (c1*v - u)/c2 as fma(a, v, -(u*b)),
+// which off the layer (a = b = 1) is v - u
+static void damping(std::vector<float>& a, std::vector<float>& b, int n, int npml, float vpmax, float h,
                     bool half, float dt)
 {
   const float R = 1.0e-10f;
   const float d = npml * h;
   const float d0 = (float)(1.5 * (double)vpmax * (double)logf(1 / R) / (double)d);
-  c1.resize(n); c2.resize(n);
+  a.resize(n); b.resize(n);
   for (int i = 0; i < n; i++) {
     float dd = 0.0f;
     if (i < npml) {
@@ -145,8 +148,10 @@ static void damping(std::vector<float>& c1, std::vector<float>& c2, int n, int n
       float t = half ? (float)((double)(float)(i - n + npml + 1) - 0.5) : (float)(i - n + npml + 1);
       t = t / (float)npml; t = t * t; dd = d0 * t;
     }
-    c1[i] = (float)(1.0 - 0.5 * (double)dt * (double)dd);
-    c2[i] = (float)(2.0 - (double)c1[i]);
+    const float c1 = (float)(1.0 - 0.5 * (double)dt * (double)dd);
+    const float c2 = (float)(2.0 - (double)c1);
+    a[i] = c1 / c2;
+    b[i] = 1.0f / c2;
   }
 }
 
@@ -159,12 +164,12 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
   float m = colmax_vv[g.nx1];
   for (int ix = g.nx1; ix <= g.nx2; ix++) m = colmax_vv[ix] > m ? colmax_vv[ix] : m;
   s.vpmax = sqrtf(m);
-  damping(s.c1x, s.c2x, nxp, npml, s.vpmax, P.dx, false, P.dt);
-  damping(s.c1xv, s.c2xv, nxp - 1, npml, s.vpmax, P.dx, true, P.dt);
-  damping(s.c1z, s.c2z, nzp, npml, s.vpmax, P.dz, false, P.dt);
-  damping(s.c1zv, s.c2zv, nzp - 1, npml, s.vpmax, P.dz, true, P.dt);
-  s.c1xv.push_back(1.0f); s.c2xv.push_back(1.0f);            // pad to nxp / nzp
-  s.c1zv.push_back(1.0f); s.c2zv.push_back(1.0f);
+  damping(s.ax, s.bx, nxp, npml, s.vpmax, P.dx, false, P.dt);
+  damping(s.axv, s.bxv, nxp - 1, npml, s.vpmax, P.dx, true, P.dt);
+  damping(s.az, s.bz, nzp, npml, s.vpmax, P.dz, false, P.dt);
+  damping(s.azv, s.bzv, nzp - 1, npml, s.vpmax, P.dz, true, P.dt);
+  s.axv.push_back(1.0f); s.bxv.push_back(1.0f);              // pad to nxp / nzp
+  s.azv.push_back(1.0f); s.bzv.push_back(1.0f);
   // Gaussian source (kernel.cu:23-35): bounding box of its non-zero values
   const float dx2 = P.dx * P.dx, dz2 = P.dz * P.dz;
   int x0 = nxp, x1 = -1, z0 = nzp, z1 = -1;
@@ -339,7 +344,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   // small arrays
   const SmallLayout L = small_layout(s);
   std::vector<char> img(L.total, 0);
-  const std::vector<float>* cv[8] = {&s.c1x, &s.c2x, &s.c1xv, &s.c2xv, &s.c1z, &s.c2z, &s.c1zv, &s.c2zv};
+  const std::vector<float>* cv[8] = {&s.ax, &s.bx, &s.axv, &s.bxv, &s.az, &s.bz, &s.azv, &s.bzv};
   const int pad[8] = {RTM_MXL, RTM_MXL, RTM_MXL, RTM_MXL, RTM_MZT, RTM_MZT, RTM_MZT, RTM_MZT};
   const int len[8] = {d.pitch, d.pitch, d.pitch, d.pitch, d.frows, d.frows, d.frows, d.frows};
   for (int i = 0; i < 8; i++) {
@@ -351,7 +356,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   memcpy(img.data() + L.cut, s.cut.data(), s.cut.size() * 4);
   memcpy(img.data() + L.taper, s.taper.data(), s.taper.size() * 4);
   uploads.push_back({base, std::move(img)});
-  const float** cp[8] = {&d.c1x, &d.c2x, &d.c1xv, &d.c2xv, &d.c1z, &d.c2z, &d.c1zv, &d.c2zv};
+  const float** cp[8] = {&d.ax, &d.bx, &d.axv, &d.bxv, &d.az, &d.bz, &d.azv, &d.bzv};
   for (int i = 0; i < 8; i++) *cp[i] = (const float*)(base + L.c[i]) + pad[i];
   d.sx0 = s.sx0; d.sx1 = s.sx1; d.sz0 = s.sz0; d.sz1 = s.sz1;
   d.sa = (const float*)(base + L.sa);
@@ -361,7 +366,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   const size_t F = align256((size_t)d.frows * d.pitch * 4);
   float** fp[kSlotFields] = {&d.sp[0], &d.sp[1], &d.svx[0], &d.svx[1], &d.svz[0], &d.svz[1],
                              &d.rp[0], &d.rp[1], &d.rvx[0], &d.rvx[1], &d.rvz[0], &d.rvz[1],
-                             &d.q1, &d.q2, &d.mrho, &d.mvv};
+                             &d.q1, &d.q2, &d.mib, &d.mkk};
   for (int i = 0; i < kSlotFields; i++) { *fp[i] = (float*)(base + o) + d.forg; o += F; }
   const size_t I = align256((size_t)d.nzi * d.nxl * 4);
   d.g1 = (float*)(base + o); o += I;

@@ -21,7 +21,8 @@ struct ShotSetup {
   short hk_nt = 0, hk_dt = 0;
   float hk_dx = 0.f;
   float vpmax = 0.f;
-  std::vector<float> c1x, c2x, c1xv, c2xv, c1z, c2z, c1zv, c2zv;  // damping factors
+  // damping a = c1/c2, b = 1/c2 at p nodes (x, z) and velocity nodes (xv, zv)
+  std::vector<float> ax, bx, axv, bxv, az, bz, azv, bzv;
   int sx0 = 0, sx1 = -1, sz0 = 0, sz1 = -1;                          // source box
   std::vector<float> sa;                                              // dt*src in the box
   std::vector<int> cut;                                               // mute

@@ -268,17 +268,19 @@ __device__ __forceinline__ bool any_lane(bool c) { return __builtin_amdgcn_ballo
 enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
 
 // New velocity of one cell from the p window Pr (Pr[i] = p at +i along the
-// derivative), old value vo and density rh.  Forward-type fields (source in
-// the forward phase, receiver in the reverse phase):
-//   v = (c1*v - D)/c2, D = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho
-//   (kernel.cu:588-594); c1 = c2 = 1 off the absorbing layer reduces it to
-//   v - D (kernel.cu:462); radius min(10, distance to the grid edge)
+// derivative), old value vo and reciprocal density ib.  Forward-type fields
+// (source in the forward phase, receiver in the reverse phase):
+//   v = (c1*v - u)/c2, u = (sum c_i (p[+i] - p[-i+1]) * dt/h) / rho
+//   (kernel.cu:588-594), evaluated as u = (s*dt/h)*ib, v = fma(a, v, -(u*b))
+//   with a = c1/c2, b = 1/c2; off the absorbing layer (a = b = 1) that is
+//   v - u (kernel.cu:462); radius min(10, distance to the grid edge)
 //   (cuda_norderx/z, kernel.cu:299-317).  VALID false: the velocity does
 //   not exist there (its value is never read); the damping arrays carry 1
 //   outside the grid.
 // Reconstruction: strip cells restored from the store of step it+1
-// (kernel.cu:1294-1347), interior cells v + D (kernel.cu:318-388).
-#define VEL_CELL(FK, gen, strips, SOFF, VALID, RAD, INTERIOR, P_AT, VO, RH, DTH, C1V, C2V, OUT)            \
+// (kernel.cu:1294-1347), interior cells v + u as fma(s*dt/h, ib, v)
+// (kernel.cu:318-388).
+#define VEL_CELL(FK, gen, strips, SOFF, VALID, RAD, INTERIOR, P_AT, VO, IB, DTH, AV, BV, OUT)               \
   do {                                                                                                     \
     float v_ = 0.0f;                                                                                       \
     if (FK == FK_RCN) {                                                                                    \
@@ -288,39 +290,34 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
       } else if (!(gen) || (INTERIOR)) {                                                                   \
         float acc = 0.0f;                                                                                  \
         _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
-        acc = acc * (DTH);                                                                                 \
-        acc = acc / (RH);                                                                                  \
-        v_ = (VO) + acc;                                                                                   \
+        v_ = __builtin_fmaf(acc * (DTH), (IB), (VO));                                                      \
       }                                                                                                    \
     } else if (!(gen)) {                                                                                   \
       float acc = 0.0f;                                                                                    \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
-      acc = acc * (DTH);                                                                                   \
-      acc = acc / (RH);                                                                                    \
-      v_ = (VO) - acc;                                                                                     \
+      v_ = (VO) - (acc * (DTH)) * (IB);                                                                    \
     } else {                                                                                               \
       const int rr_ = (VALID) ? (RAD) : RTM_R;                                                             \
       float acc = 0.0f;                                                                                    \
       STENCIL_SUM(acc, rr_, c10, ctab, P_AT(i), P_AT(1 - i));                                              \
-      acc = acc * (DTH);                                                                                   \
-      acc = acc / (RH);                                                                                    \
-      const float c1_ = (C1V), c2_ = (C2V);                                                                \
-      /* c1 = c2 = 1 gives fma(1, v, -D) / 1 == v - D exactly */                                           \
-      if (any_lane(c1_ != 1.0f || c2_ != 1.0f)) v_ = __builtin_fmaf(c1_, (VO), -acc) / c2_;                 \
-      else v_ = (VO) - acc;                                                                                \
+      const float u_ = (acc * (DTH)) * (IB);                                                               \
+      const float a_ = (AV), b_ = (BV);                                                                    \
+      /* a = b = 1 gives fma(1, v, -u) == v - u exactly */                                                 \
+      if (any_lane(a_ != 1.0f || b_ != 1.0f)) v_ = __builtin_fmaf(a_, (VO), -(u_ * b_));                   \
+      else v_ = (VO) - u_;                                                                                 \
     }                                                                                                      \
     OUT = v_;                                                                                              \
   } while (0)
 
 // ------------------------------------------------------------ pressure --
-struct DivV { float dxv, dzv; };
+struct DivV { float sx, sz; };
 
+// Staggered differences of vx along x and vz along z (kernel.cu:520-534):
 // VXr points at vx of the cell's column in its LDS row; cz is the column
 // window of vz rows; r the thread-local row
 template <int NW>
 __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float (&cz)[NW], int r, int rad,
-                                      const float* c10, const float* __restrict__ cf, float dtx, float dtz,
-                                      float rho, float vv)
+                                      const float* c10, const float* __restrict__ cf)
 {
   float sx = 0.0f, sz = 0.0f;
   if (rad == RTM_R) {
@@ -340,9 +337,14 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
     }
   }
   DivV d;
-  d.dxv = sx * dtx; d.dxv = d.dxv * rho; d.dxv = d.dxv * vv;     // kernel.cu:536
-  d.dzv = sz * dtz; d.dzv = d.dzv * rho; d.dzv = d.dzv * vv;     // kernel.cu:537
+  d.sx = sx; d.sz = sz;
   return d;
+}
+
+// p - rho v^2 (sx dt/dx + sz dt/dz) (kernel.cu:536-539), kk = rho v^2
+__device__ __forceinline__ float pres_sum(const DivV& d, float dtx, float dtz)
+{
+  return __builtin_fmaf(d.sx, dtx, d.sz * dtz);
 }
 
 // One field through one time step on a tile: p of the current level pc (with
@@ -377,7 +379,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   const int srow0 = (z0 + RTM_MZT) * pr4;             // byte offset of tile row 0
   const int vcol = (ix + RTM_MXL) * 4;                // byte offset of the lane's column
   const rsrc_t Rp = field_rsrc(pc, s), Rvx = field_rsrc(vxc, s), Rvz = field_rsrc(vzc, s);
-  const rsrc_t Rrho = field_rsrc(s.mrho, s);
+  const rsrc_t Rib = field_rsrc(s.mib, s);
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
@@ -404,52 +406,51 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       }
     }
   }
-  float vzo[KZ], rz[KZ];
+  float vzo[KZ], ibz[KZ];
 #pragma unroll
   for (int k = 0; k < KZ; k++) {
     const int kz = g * KZ + k;
     vzo[k] = 0.0f;
-    rz[k] = 1.0f;
+    ibz[k] = 1.0f;
     if (kz < ZR) {
       const int so = srow0 + (kz - 10) * pr4;
       vzo[k] = bld(Rvz, vcol, so);
-      rz[k] = bld(Rrho, vcol, so);
+      ibz[k] = bld(Rib, vcol, so);
     }
   }
-  float vxo[RPT], rho[RPT];
+  float vxo[RPT], ibx[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
     const int so = srow0 + (g * RPT + r) * pr4;
     vxo[r] = bld(Rvx, vcol, so);
-    rho[r] = bld(Rrho, vcol, so);
+    ibx[r] = bld(Rib, vcol, so);
   }
-  float hvo[KH], hrh[KH];
+  float hvo[KH], hib[KH];
   int hrow[KH], hcol[KH];
 #pragma unroll
   for (int k = 0; k < KH; k++) {
     const int h = k * NT + tid;
     hvo[k] = 0.0f;
-    hrh[k] = 1.0f;
+    hib[k] = 1.0f;
     hrow[k] = h / 20;
     const int hc = h - hrow[k] * 20;
     hcol[k] = hc < 10 ? hc : hc + 64;                  // VXL column: x = x0 - 10 + hcol
     if (h < 20 * TZ) {
       const int off = srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4;
       hvo[k] = bld(Rvx, off, 0);
-      hrh[k] = bld(Rrho, off, 0);
+      hib[k] = bld(Rib, off, 0);
     }
   }
   // per-lane column data of the generic path: existence, radius, damping
-  bool vxv = true, dampx = false;
+  bool vxv = true;
   int rrx = RTM_R;
-  float c1xv = 1.0f, c2xv = 1.0f;
+  float axv = 1.0f, bxv = 1.0f;
   StripCols sc{-1, -1, -1};
   if (gen) {
     vxv = ix <= nxp - 2;
     rrx = vxv ? imin(RTM_R, imin(ix + 1, nxp - 1 - ix)) : RTM_R;
-    c1xv = GCF(s.c1xv)[ix];
-    c2xv = GCF(s.c2xv)[ix];
-    dampx = !(c1xv == 1.0f && c2xv == 1.0f);
+    axv = GCF(s.axv)[ix];
+    bxv = GCF(s.bxv)[ix];
   }
   if (strips) sc = strip_cols(s, ix);
   __builtin_amdgcn_s_waitcnt(0);                       // the LDS writes of the staging loads
@@ -469,8 +470,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #define PZA(i) cw[k + 9 + (i)]
       const int svz = strips ? soff_vz(strip_row_vz(s, iz), sc) : -1;
       VEL_CELL(FK, gen, strips, svz, (iz >= 0 && iz <= nzp - 2), imin(RTM_R, imin(iz + 1, nzp - 1 - iz)),
-               (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], rz[k], dtz,
-               GCF(s.c1zv)[iz], GCF(s.c2zv)[iz], vzn_[k]);
+               (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], ibz[k], dtz,
+               GCF(s.azv)[iz], GCF(s.bzv)[iz], vzn_[k]);
 #undef PZA
       if (kz < ZR && kz >= 10 && kz < 10 + TZ && ix < nxp && iz <= nzp - 2) {
         bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
@@ -487,8 +488,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #define PCA(i) Pr[(i)]
       const int svx = strips ? soff_vx(strip_row_lr(s, iz), sc) : -1;
       VEL_CELL(FK, gen, strips, svx, vxv, rrx,
-               (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], rho[r], dtx,
-               c1xv, c2xv, vxn_[r]);
+               (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], ibx[r], dtx,
+               axv, bxv, vxn_[r]);
 #undef PCA
       if (iz < nzp && ix <= nxp - 2) {
         bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
@@ -505,21 +506,23 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         const bool hv = jx >= 0 && jx <= nxp - 2;
 #define PCA(i) Pr[(i)]
         VEL_CELL(FK, gen, strips, strip_vx(s, iz, jx), hv, (hv ? imin(RTM_R, imin(jx + 1, nxp - 1 - jx)) : RTM_R),
-                 (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hrh[k], dtx,
-                 GCF(s.c1xv)[jx], GCF(s.c2xv)[jx], hvn[k]);
+                 (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hib[k], dtx,
+                 GCF(s.axv)[jx], GCF(s.bxv)[jx], hvn[k]);
 #undef PCA
       }
     }
   }
 
-  // own old pressure out of LDS; model of the own cells
-  float po[RPT], vv[RPT];
-  const rsrc_t Rvv = field_rsrc(s.mvv, s);
+  // own old pressure out of LDS; bulk modulus rho v^2 of the own cells
+  // (loaded here rather than with the first batch: held across the velocity
+  // phase its registers cost a wave of occupancy, which is slower)
+  float po[RPT], kk[RPT];
+  const rsrc_t Rkk = field_rsrc(s.mkk, s);
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
     const int j = g * RPT + r;
     po[r] = PS[(j + 19) * PW + 20 + lx];
-    vv[r] = bld(Rvv, vcol, srow0 + j * pr4);
+    kk[r] = bld(Rkk, vcol, srow0 + j * pr4);
   }
   __syncthreads();                                     // PS consumed
 #pragma unroll
@@ -541,8 +544,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   const bool xin = ix >= npml && ix < nxp - npml;
   const bool xring = ix == 0 || ix == nxp - 1;
   const int radx = imin(RTM_R, imin(ix, nxp - 1 - ix));
-  float c1x = 1.0f, c2x = 1.0f;
-  if (FK != FK_RCN && gen && ix < nxp) { c1x = GCF(s.c1x)[ix]; c2x = GCF(s.c2x)[ix]; }
+  float ax = 1.0f, bx = 1.0f;
+  if (FK != FK_RCN && gen && ix < nxp) { ax = GCF(s.ax)[ix]; bx = GCF(s.bx)[ix]; }
   const bool xsrc = (tl.kind & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
   const rsrc_t Rpn = field_rsrc(pn, s);
 
@@ -562,8 +565,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         p = store_get(s, it, sp_);                     // kernel.cu:1348-1400
         wr = true;
       } else if (!gen || inner) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab, dtx, dtz, rho[r], vv[r]);
-        p = p + (d.dxv + d.dzv);                       // kernel.cu:389-429
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
+        p = __builtin_fmaf(kk[r], pres_sum(d, dtx, dtz), p);   // kernel.cu:389-429
         wr = true;
       }
       if (wr && xsrc && iz >= s.sz0 && iz <= s.sz1)    // source removal (rtm_model.cu:437)
@@ -571,8 +574,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       if (wr) bst(Rpn, p, vcol, so);
     } else {
       if (!gen) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab, dtx, dtz, rho[r], vv[r]);
-        p = p - (d.dxv + d.dzv);                       // kernel.cu:539
+        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
+        p = __builtin_fmaf(-kk[r], pres_sum(d, dtx, dtz), p);  // kernel.cu:539
       } else {
         // the outer ring keeps its value (only the source is added there);
         // cells off the grid are computed with radius 10 and dropped
@@ -580,15 +583,16 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         const bool upd = zupd && incell && !xring;
         const int rad = upd ? imin(radx, imin(iz, nzp - 1 - iz)) : RTM_R;
         DivV d;
-        if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, ctab, dtx, dtz, rho[r], vv[r]);
-        else d = div_v(VXr, cz, r, rad, c10, ctab, dtx, dtz, rho[r], vv[r]);
+        if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, ctab);
+        else d = div_v(VXr, cz, r, rad, c10, ctab);
         const bool split = upd && !(xin && iz >= npml && iz < nzp - npml);
-        float pu = p - (d.dxv + d.dzv);
+        float pu = __builtin_fmaf(-kk[r], pres_sum(d, dtx, dtz), p);
         if (any_lane(split) && split) {                // split pair, p = p1 + p2 (kernel.cu:780-783)
           const rsrc_t R1 = field_rsrc(s.q1, s), R2 = field_rsrc(s.q2, s);
           float q1 = bld(R1, vcol, so), q2 = bld(R2, vcol, so);
-          q1 = __builtin_fmaf(c1x, q1, -d.dxv) / c2x;
-          q2 = __builtin_fmaf(GCF(s.c1z)[iz], q2, -d.dzv) / GCF(s.c2z)[iz];
+          const float dxv = (d.sx * dtx) * kk[r], dzv = (d.sz * dtz) * kk[r];
+          q1 = __builtin_fmaf(ax, q1, -(dxv * bx));   // (c1 q - d)/c2 (kernel.cu:774-779)
+          q2 = __builtin_fmaf(GCF(s.az)[iz], q2, -(dzv * GCF(s.bz)[iz]));
           bst(R1, q1, vcol, so);
           bst(R2, q2, vcol, so);
           pu = q1 + q2;
@@ -751,8 +755,9 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
       const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
       const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
       const size_t m = (size_t)r * s.mpitch + c;
-      (s.mrho - s.forg)[e] = s.rho[m];
-      (s.mvv - s.forg)[e] = s.vv[m];
+      const float rh = s.rho[m];
+      (s.mib - s.forg)[e] = 1.0f / rh;
+      (s.mkk - s.forg)[e] = rh * s.vv[m];
     }
   } else {
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }

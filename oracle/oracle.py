@@ -149,7 +149,14 @@ class ShotResult:
     geom: Geom
 
 
-def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, literal=False, forward_only=False,
+def _variant(f64, korder):
+    """Suffix of the arithmetic form: f32 reference order, k32 kernel order, f64."""
+    if f64 and korder:
+        raise ValueError("the kernel-order form is FP32 only")
+    return "f64" if f64 else ("k32" if korder else "f32")
+
+
+def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal=False, forward_only=False,
              nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False) -> ShotResult:
     g = geometry(P, ishot)
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
@@ -164,7 +171,7 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, literal=False, forwar
     vp0 = np.ascontiguousarray(vp0, np.float32)
     rho0 = np.ascontiguousarray(rho0, np.float32)
     si = None if seis_in is None else np.ascontiguousarray(seis_in, np.float32)
-    fn = lib().orc_shot_f64 if f64 else lib().orc_shot_f32
+    fn = getattr(lib(), "orc_shot_" + _variant(f64, korder))
     rc = fn(C.byref(P), C.c_int(ishot), _fp(vp0), _fp(rho0), _fp(coef), C.byref(o), _fp(seis), _fp(si),
             _fp(g1), _fp(il), _fp(sf), _fp(sr))
     if rc != 0:
@@ -173,11 +180,11 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, literal=False, forwar
 
 
 def fd_steps(P, ishot, vp0, rho0, fields_, nsteps, *, it0=0, with_source=False, literal=False,
-             f64=False, coef=None):
+             f64=False, korder=False, coef=None):
     """Advance given wavefields nsteps forward steps in place.
     fields_: dict with p, vx, vz, pl1, pl2, pr1, pr2, pt1, pt2, pb1, pb2 (reference shapes)."""
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
-    fn = lib().orc_fd_steps_f64 if f64 else lib().orc_fd_steps_f32
+    fn = getattr(lib(), "orc_fd_steps_" + _variant(f64, korder))
     names = ["p", "vx", "vz", "pl1", "pl2", "pr1", "pr2", "pt1", "pt2", "pb1", "pb2"]
     rt = np.float64 if f64 else np.float32
     for n in names:
@@ -199,14 +206,14 @@ def alloc_fields(g, npml, f64=False):
                 pb1=np.zeros((npml, nxl), rt), pb2=np.zeros((npml, nxl), rt))
 
 
-def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False):
+def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, korder=False):
     """Whole rtm_model() job: returns (image1, image2s, illum) as float32 (nz, nx)."""
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     img1 = np.zeros((P.nz, P.nx), np.float32)
     img2 = np.zeros((P.nz, P.nx), np.float32)
     ill = np.zeros((P.nz, P.nx), np.float32)
     done = C.c_int(0)
-    fn = lib().orc_rtm_f64 if f64 else lib().orc_rtm_f32
+    fn = getattr(lib(), "orc_rtm_" + _variant(f64, korder))
     rc = fn(C.byref(P), _fp(np.ascontiguousarray(vp0, np.float32)), _fp(np.ascontiguousarray(rho0, np.float32)),
             _fp(coef), C.c_int(int(literal)), C.c_int(1 if (stack64 or f64) else 0),
             _fp(img1), _fp(img2), _fp(ill), C.byref(done))

@@ -24,6 +24,17 @@
  *   Laplacian filter ........................ image_function.cpp:44-79
  *   multi-shot normalisation ................ kernel.cu:1580-1590, image_function.cpp:11-20
  *
+ * Three arithmetic forms of the same discrete algorithm:
+ *   _f32  reference order: every FP32 operation of the reference's kernels in
+ *         source order (the parity yardstick for the seismogram and images);
+ *   _k32  kernel order: FP32 with the operation order of the HIP kernels --
+ *         velocity u = (s*dt/h) * (1/rho), pressure p -= (rho*v^2)*(sx*dt/dx +
+ *         sz*dt/dz) as fma, damping fma(c1/c2, q, -(d * (1/c2))); per-cell 1/rho
+ *         and rho*v^2 and per-node c1/c2, 1/c2 are rounded once.  A checking
+ *         aid: the GPU is compared with it bit for bit, and it is compared with
+ *         _f32 within the stated tolerances;
+ *   _f64  the reference order in FP64 (error yardstick of reduced precision).
+ *
  * Two pressure/PML forms:
  *   literal      -- the kernels' own thread ranges (valid only for npml == 32,
  *                   where the 32-wide PML blocks tile the absorbing layer);
@@ -291,18 +302,28 @@ ORC_EXPORT void orc_coef_table(float *out)
 #define SFX(n) n##_f32
 #define FMA fmaf
 #define ORACLE_IS_F64 0
+#define ORACLE_KORDER 0
+#include "rtm_oracle_impl.h"
+#undef SFX
+#undef ORACLE_KORDER
+
+#define SFX(n) n##_k32
+#define ORACLE_KORDER 1
 #include "rtm_oracle_impl.h"
 #undef RT
 #undef SFX
 #undef FMA
 #undef ORACLE_IS_F64
+#undef ORACLE_KORDER
 
 #define RT double
 #define SFX(n) n##_f64
 #define FMA fma
 #define ORACLE_IS_F64 1
+#define ORACLE_KORDER 0
 #include "rtm_oracle_impl.h"
 #undef RT
 #undef SFX
 #undef FMA
 #undef ORACLE_IS_F64
+#undef ORACLE_KORDER

@@ -34,6 +34,9 @@ typedef struct {
     RT dt, dtx, dtz;
     RT coef[100];                       /* diffcoef[(r-1)*10 + i-1]                  */
     RT *rho, *vv;                       /* padded window: density, v^2 (nzp*nxp)     */
+    RT *ib, *kk;                        /* kernel order: 1/rho, rho*v^2 (nzp*nxp)    */
+    RT *ax, *bx, *az, *bz;              /* kernel order: c1/c2, 1/c2 at p nodes      */
+    RT *axv, *bxv, *azv, *bzv;          /* ... at vx / vz half nodes                 */
     RT *ddx, *ddz, *ddxv, *ddzv;        /* pml profiles: nxp, nzp, nxp-1, nzp-1      */
     RT *c1x, *c2x, *c1z, *c2z;          /* derived damping factors at p nodes        */
     RT *c1xv, *c2xv, *c1zv, *c2zv;      /* ... at vx / vz half nodes                 */
@@ -80,7 +83,9 @@ static void SFX(wf_free)(SFX(wfield) *w)
 
 static void SFX(med_free)(SFX(medium) *m)
 {
-    free(m->rho); free(m->vv);
+    free(m->rho); free(m->vv); free(m->ib); free(m->kk);
+    free(m->ax); free(m->bx); free(m->az); free(m->bz);
+    free(m->axv); free(m->bxv); free(m->azv); free(m->bzv);
     free(m->ddx); free(m->ddz); free(m->ddxv); free(m->ddzv);
     free(m->c1x); free(m->c2x); free(m->c1z); free(m->c2z);
     free(m->c1xv); free(m->c2xv); free(m->c1zv); free(m->c2zv);
@@ -185,6 +190,19 @@ static void SFX(med_setup)(SFX(medium) *m, const orc_geom *g, const orc_params *
     SFX(damp_factors)(m->ddz, nzp, P->dt, m->c1z, m->c2z);
     SFX(damp_factors)(m->ddxv, nxp - 1, P->dt, m->c1xv, m->c2xv);
     SFX(damp_factors)(m->ddzv, nzp - 1, P->dt, m->c1zv, m->c2zv);
+    /* kernel order: per-cell reciprocal density and bulk modulus, damping as
+     * ratio and reciprocal (c1 = c2 = 1 gives a = b = 1, so the damped form
+     * fma(a, v, -(u*b)) is v - u off the absorbing layer) */
+    m->ib = (RT *)malloc(nxz * sizeof(RT)); m->kk = (RT *)malloc(nxz * sizeof(RT));
+    for (size_t i = 0; i < nxz; i++) { m->ib[i] = (RT)1 / m->rho[i]; m->kk[i] = m->rho[i] * m->vv[i]; }
+    m->ax = (RT *)malloc(nxp * sizeof(RT)); m->bx = (RT *)malloc(nxp * sizeof(RT));
+    m->az = (RT *)malloc(nzp * sizeof(RT)); m->bz = (RT *)malloc(nzp * sizeof(RT));
+    m->axv = (RT *)malloc(nxp * sizeof(RT)); m->bxv = (RT *)malloc(nxp * sizeof(RT));
+    m->azv = (RT *)malloc(nzp * sizeof(RT)); m->bzv = (RT *)malloc(nzp * sizeof(RT));
+    for (int i = 0; i < nxp; i++) { m->ax[i] = m->c1x[i] / m->c2x[i]; m->bx[i] = (RT)1 / m->c2x[i]; }
+    for (int i = 0; i < nzp; i++) { m->az[i] = m->c1z[i] / m->c2z[i]; m->bz[i] = (RT)1 / m->c2z[i]; }
+    for (int i = 0; i < nxp - 1; i++) { m->axv[i] = m->c1xv[i] / m->c2xv[i]; m->bxv[i] = (RT)1 / m->c2xv[i]; }
+    for (int i = 0; i < nzp - 1; i++) { m->azv[i] = m->c1zv[i] / m->c2zv[i]; m->bzv[i] = (RT)1 / m->c2zv[i]; }
 
     /* order maps, kernel.cu:271-317 (closed forms; memset 0 elsewhere) */
     m->norder = (int *)calloc(nxz, sizeof(int));
@@ -243,8 +261,13 @@ static void SFX(gen_vel)(SFX(wfield) *w, const SFX(medium) *m)
             const RT *c = &m->coef[(r - 1) * 10];
             RT s = 0;
             for (int i = 1; i <= r; i++) s = FMA(c[i - 1], PX(w, iz, ix + i) - PX(w, iz, ix - i + 1), s);
+#if ORACLE_KORDER
+            const RT u = (s * dtx) * m->ib[(size_t)iz * nxp + ix];
+            VX(w, iz, ix) = FMA(m->axv[ix], VX(w, iz, ix), -(u * m->bxv[ix]));
+#else
             s = s * dtx; s = s / m->rho[(size_t)iz * nxp + ix];
             VX(w, iz, ix) = FMA(m->c1xv[ix], VX(w, iz, ix), -s) / m->c2xv[ix];
+#endif
         }
     }
 #pragma omp parallel for schedule(static)
@@ -254,8 +277,13 @@ static void SFX(gen_vel)(SFX(wfield) *w, const SFX(medium) *m)
         for (int ix = 0; ix < nxp; ix++) {
             RT s = 0;
             for (int i = 1; i <= r; i++) s = FMA(c[i - 1], PX(w, iz + i, ix) - PX(w, iz - i + 1, ix), s);
+#if ORACLE_KORDER
+            const RT u = (s * dtz) * m->ib[(size_t)iz * nxp + ix];
+            VZ(w, iz, ix) = FMA(m->azv[iz], VZ(w, iz, ix), -(u * m->bzv[iz]));
+#else
             s = s * dtz; s = s / m->rho[(size_t)iz * nxp + ix];
             VZ(w, iz, ix) = FMA(m->c1zv[iz], VZ(w, iz, ix), -s) / m->c2zv[iz];
+#endif
         }
     }
 }
@@ -290,9 +318,21 @@ static void SFX(gen_pres)(SFX(wfield) *w, const SFX(medium) *m)
                 sz = FMA(c[i - 1], VZ(w, iz + i - 1, ix) - VZ(w, iz - i, ix), sz);
             }
             size_t id = (size_t)iz * nxp + ix;
+            int inner = (ix >= npml && ix < nxp - npml && iz >= npml && iz < nzp - npml);
+#if ORACLE_KORDER
+            if (inner) {
+                w->p[id] = FMA(-m->kk[id], FMA(sx, dtx, sz * dtz), w->p[id]);
+            } else {
+                const RT dxv = (sx * dtx) * m->kk[id], dzv = (sz * dtz) * m->kk[id];
+                RT *q1, *q2;
+                SFX(split_ptr)(w, iz, ix, &q1, &q2);
+                *q1 = FMA(m->ax[ix], *q1, -(dxv * m->bx[ix]));
+                *q2 = FMA(m->az[iz], *q2, -(dzv * m->bz[iz]));
+                w->p[id] = *q1 + *q2;
+            }
+#else
             RT dxv = sx * dtx; dxv = dxv * m->rho[id]; dxv = dxv * m->vv[id];
             RT dzv = sz * dtz; dzv = dzv * m->rho[id]; dzv = dzv * m->vv[id];
-            int inner = (ix >= npml && ix < nxp - npml && iz >= npml && iz < nzp - npml);
             if (inner) {
                 w->p[id] = w->p[id] - (dxv + dzv);
             } else {
@@ -302,6 +342,7 @@ static void SFX(gen_pres)(SFX(wfield) *w, const SFX(medium) *m)
                 *q2 = FMA(m->c1z[iz], *q2, -dzv) / m->c2z[iz];
                 w->p[id] = *q1 + *q2;
             }
+#endif
         }
     }
 }
@@ -520,8 +561,12 @@ static void SFX(rev_vel)(SFX(wfield) *w, const SFX(medium) *m)
         for (int ix = npml; ix < nxp - npml - 1; ix++) {
             RT s = 0;
             for (int i = 1; i <= R10; i++) s = FMA(c10[i - 1], PX(w, iz, ix + i) - PX(w, iz, ix - i + 1), s);
+#if ORACLE_KORDER
+            VX(w, iz, ix) = FMA(s * m->dtx, m->ib[(size_t)iz * nxp + ix], VX(w, iz, ix));
+#else
             s = s * m->dtx; s = s / m->rho[(size_t)iz * nxp + ix];
             VX(w, iz, ix) += s;
+#endif
         }
     /* cuda_backward_vz: rows [npml, nzp-npml-2], cols [npml, nxp-npml) */
 #pragma omp parallel for schedule(static)
@@ -529,8 +574,12 @@ static void SFX(rev_vel)(SFX(wfield) *w, const SFX(medium) *m)
         for (int ix = npml; ix < nxp - npml; ix++) {
             RT s = 0;
             for (int i = 1; i <= R10; i++) s = FMA(c10[i - 1], PX(w, iz + i, ix) - PX(w, iz - i + 1, ix), s);
+#if ORACLE_KORDER
+            VZ(w, iz, ix) = FMA(s * m->dtz, m->ib[(size_t)iz * nxp + ix], VZ(w, iz, ix));
+#else
             s = s * m->dtz; s = s / m->rho[(size_t)iz * nxp + ix];
             VZ(w, iz, ix) += s;
+#endif
         }
 }
 
@@ -547,9 +596,13 @@ static void SFX(rev_pres)(SFX(wfield) *w, const SFX(medium) *m)
                 sz = FMA(c10[i - 1], VZ(w, iz + i - 1, ix) - VZ(w, iz - i, ix), sz);
             }
             size_t id = (size_t)iz * nxp + ix;
+#if ORACLE_KORDER
+            w->p[id] = FMA(m->kk[id], FMA(sx, m->dtx, sz * m->dtz), w->p[id]);
+#else
             RT dxv = sx * m->dtx; dxv = dxv * m->rho[id]; dxv = dxv * m->vv[id];
             RT dzv = sz * m->dtz; dzv = dzv * m->rho[id]; dzv = dzv * m->vv[id];
             w->p[id] = w->p[id] + (dxv + dzv);
+#endif
         }
 }
 #undef PX
@@ -570,7 +623,7 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     if (orc_shot_geometry(P, ishot, &g) != 0) return -1;
     const int nxp = g.nxp, nzp = g.nzp, npml = P->npml, nt = P->nt;
     const int nxl = g.nxl, nzi = P->nz;
-    if (o->literal && npml != 32) return -2;
+    if (o->literal && (npml != 32 || ORACLE_KORDER)) return -2;
     if (npml < 10) return -3;
     SFX(medium) m;
     SFX(med_setup)(&m, &g, P, vp0, rho0, diffcoef, o->vvpad, o->rhopad);
@@ -675,7 +728,7 @@ ORC_EXPORT int SFX(orc_fd_steps)(const orc_params *P, int ishot, const float *vp
 {
     orc_geom g;
     if (orc_shot_geometry(P, ishot, &g) != 0) return -1;
-    if (literal && P->npml != 32) return -2;
+    if (literal && (P->npml != 32 || ORACLE_KORDER)) return -2;
     SFX(medium) m;
     SFX(med_setup)(&m, &g, P, vp0, rho0, diffcoef, NULL, NULL);
     SFX(wfield) w;

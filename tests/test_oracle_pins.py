@@ -205,6 +205,46 @@ def test_fp32_restatement_vs_fp64():
     assert rel_l2(a.g1, b.g1) < 1e-4
 
 
+def test_kernel_order_form_within_tolerance():
+    """the kernel-order FP32 form (1/rho, rho*v^2, damping ratios rounded once)
+    stays within the parity tolerances of the reference-order form, and no
+    closer to it than rounding allows: it is a different operation order"""
+    nz, nx = 60, 120
+    rng = np.random.default_rng(11)
+    vp = rng.uniform(1800, 3000, (nz, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2300, (nz, nx)).astype(np.float32)
+    P = O.make_params(nx=nx, nz=nz, npml=24, nt=500, flag_layout=2, spx0=600.0)
+    a = O.run_shot(P, 1, vp, rho)
+    k = O.run_shot(P, 1, vp, rho, korder=True)
+    d = O.run_shot(P, 1, vp, rho, f64=True)
+    assert rel_l2(k.seis, a.seis) < 1e-5 and rel_l2(k.g1, a.g1) < 1e-4 and rel_l2(k.illum, a.illum) < 1e-4
+    assert 0 < rel_l2(k.seis, a.seis)
+    # against FP64 the two FP32 orders are equally close
+    ek, ea = rel_l2(k.seis, d.seis), rel_l2(a.seis, d.seis)
+    assert ek < 3 * ea and ea < 3 * ek, (ek, ea)
+
+
+def test_kernel_order_damping_reduces_off_the_layer():
+    """one step from random fields: the kernel-order form agrees with the
+    reference order to rounding in every field, the PML split pair included,
+    and is not identical to it (a different operation order)"""
+    nz, nx, npml = 40, 60, 16
+    rng = np.random.default_rng(12)
+    vp = rng.uniform(1800, 3000, (nz, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2300, (nz, nx)).astype(np.float32)
+    P = O.make_params(nx=nx, nz=nz, npml=npml, nt=10, flag_layout=2, spx0=300.0)
+    g = O.geometry(P, 1)
+    F = O.alloc_fields(g, npml)
+    for f in F.values():
+        f[...] = rng.standard_normal(f.shape).astype(np.float32)
+    A = {k: v.copy() for k, v in F.items()}
+    O.fd_steps(P, 1, vp, rho, F, 1, korder=True)
+    O.fd_steps(P, 1, vp, rho, A, 1)
+    for k in F:
+        assert rel_l2(F[k], A[k]) < 1e-6, k
+    assert not all(np.array_equal(F[k], A[k]) for k in F)
+
+
 def test_ricker_closed_form():
     nt, f, dt = 300, 20.0, 1e-3
     w = O.ricker(nt, f, dt).astype(np.float64)
