@@ -479,6 +479,37 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       }
     }
   }
+
+  // operands of the pressure phase, fetched now so that one memory round trip
+  // for all rows overlaps the vx computation (inside the row loop each row
+  // would wait for its own): rho v^2 of the own cells, and per cell either
+  // the PML split pair (forward-type passes, damped cells) or the imaging
+  // accumulators (receiver pass, interior cells) -- never both
+  const bool xin = ix >= npml && ix < nxp - npml;
+  const bool xring = ix == 0 || ix == nxp - 1;
+  float kk[RPT], qa[RPT], qb[RPT];
+  {
+    const rsrc_t Rkk = field_rsrc(s.mkk, s);
+    const rsrc_t R1 = field_rsrc(s.q1, s), R2 = field_rsrc(s.q2, s);
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int j = g * RPT + r, iz = z0 + j;
+      const int so = srow0 + j * pr4;
+      kk[r] = bld(Rkk, vcol, so);
+      qa[r] = qb[r] = 0.0f;
+      const bool incell = ix < nxp && iz < nzp;
+      const bool inner = xin && iz >= npml && iz < nzp - npml;
+      if (FK != FK_RCN && gen && incell && iz > 0 && iz < nzp - 1 && !xring && !inner) {
+        qa[r] = bld(R1, vcol, so);
+        qb[r] = bld(R2, vcol, so);
+      }
+      if (FK == FK_RCV && incell && inner) {
+        const int ii = (iz - npml) * s.nxl + ix - npml;
+        qa[r] = GCF(s.g1)[ii];
+        qb[r] = GCF(s.illum)[ii];
+      }
+    }
+  }
   {
     const rsrc_t Rvxn = field_rsrc(vxn, s);
 #pragma unroll
@@ -513,17 +544,10 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     }
   }
 
-  // own old pressure out of LDS; bulk modulus rho v^2 of the own cells
-  // (loaded here rather than with the first batch: held across the velocity
-  // phase its registers cost a wave of occupancy, which is slower)
-  float po[RPT], kk[RPT];
-  const rsrc_t Rkk = field_rsrc(s.mkk, s);
+  // own old pressure out of LDS
+  float po[RPT];
 #pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int j = g * RPT + r;
-    po[r] = PS[(j + 19) * PW + 20 + lx];
-    kk[r] = bld(Rkk, vcol, srow0 + j * pr4);
-  }
+  for (int r = 0; r < RPT; r++) po[r] = PS[(g * RPT + r + 19) * PW + 20 + lx];
   __syncthreads();                                     // PS consumed
 #pragma unroll
   for (int k = 0; k < KZ; k++) {
@@ -541,8 +565,6 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   float cz[RPT + 19];
 #pragma unroll
   for (int m = 0; m < RPT + 19; m++) cz[m] = VZL[(g * RPT + m) * 64 + lx];
-  const bool xin = ix >= npml && ix < nxp - npml;
-  const bool xring = ix == 0 || ix == nxp - 1;
   const int radx = imin(RTM_R, imin(ix, nxp - 1 - ix));
   float ax = 1.0f, bx = 1.0f;
   if (FK != FK_RCN && gen && ix < nxp) { ax = GCF(s.ax)[ix]; bx = GCF(s.bx)[ix]; }
@@ -589,7 +611,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         float pu = __builtin_fmaf(-kk[r], pres_sum(d, dtx, dtz), p);
         if (any_lane(split) && split) {                // split pair, p = p1 + p2 (kernel.cu:780-783)
           const rsrc_t R1 = field_rsrc(s.q1, s), R2 = field_rsrc(s.q2, s);
-          float q1 = bld(R1, vcol, so), q2 = bld(R2, vcol, so);
+          float q1 = qa[r], q2 = qb[r];
           const float dxv = (d.sx * dtx) * kk[r], dzv = (d.sz * dtz) * kk[r];
           q1 = __builtin_fmaf(ax, q1, -(dxv * bx));   // (c1 q - d)/c2 (kernel.cu:774-779)
           q2 = __builtin_fmaf(GCF(s.az)[iz], q2, -(dzv * GCF(s.bz)[iz]));
@@ -625,8 +647,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           bst(Rpn, p, vcol, so);
           if (xin && iz >= npml && iz < nzp - npml) {  // imaging (kernel.cu:1401-1416)
             const int ii = (iz - npml) * s.nxl + ix - npml;
-            GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, GCF(s.g1)[ii]);
-            GF(s.illum)[ii] = __builtin_fmaf(ps[r], ps[r], GCF(s.illum)[ii]);
+            GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, qa[r]);
+            GF(s.illum)[ii] = __builtin_fmaf(ps[r], ps[r], qb[r]);
           }
         }
       }
