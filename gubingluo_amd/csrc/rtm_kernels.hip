@@ -201,6 +201,9 @@ struct Geo {
   static constexpr int LDS_A = PZR * PW;               // p rows z0-19.. (x halo on the tile rows)
   static constexpr int LDS_B = TZ * VXW + ZR * 64;     // VXL | VZL
   static constexpr int LDS = LDS_A > LDS_B ? LDS_A : LDS_B;
+  // damping factors of the tile (after LDS; written once per pass): a, b at
+  // the vz rows, at the vx columns x0-10..x0+73, and at the own rows
+  static constexpr int LDS_D = 2 * ZR + 2 * VXW + 2 * TZ;
 };
 
 // ------------------------------------------------------- buffer access --
@@ -367,6 +370,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   constexpr int G = Q::G, RPT = Q::RPT, TZ = Q::TZ, NT = Q::NT, KZ = Q::KZ, KP = Q::KP, KH = Q::KH;
   constexpr int PW = Q::PW, VXW = Q::VXW, ZR = Q::ZR;
   float* PS = lds;                                     // p rows z0-19.., columns x0-20..x0+83
+  float* DMP = lds + Q::LDS;                           // damping table (damped passes of gen tiles)
+  constexpr int DZA = 0, DZB = ZR, DXA = 2 * ZR, DXB = 2 * ZR + VXW, DPA = 2 * ZR + 2 * VXW, DPB = DPA + TZ;
   float* VXL = lds;                                    // after the second barrier
   float* VZL = lds + TZ * VXW;
   const int tid = threadIdx.x, lx = tid & 63;
@@ -441,16 +446,25 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       hib[k] = bld(Rib, off, 0);
     }
   }
-  // per-lane column data of the generic path: existence, radius, damping
+  // per-lane column data of the generic path: existence, radius, damping;
+  // the damping of the tile's rows and halo columns goes to an LDS table, so
+  // that the compute phases wait for no small loads
   bool vxv = true;
   int rrx = RTM_R;
-  float axv = 1.0f, bxv = 1.0f;
+  float axv = 1.0f, bxv = 1.0f, ax = 1.0f, bx = 1.0f;
   StripCols sc{-1, -1, -1};
   if (gen) {
     vxv = ix <= nxp - 2;
     rrx = vxv ? imin(RTM_R, imin(ix + 1, nxp - 1 - ix)) : RTM_R;
+  }
+  if (FK != FK_RCN && gen) {
     axv = GCF(s.axv)[ix];
     bxv = GCF(s.bxv)[ix];
+    if (ix < nxp) { ax = GCF(s.ax)[ix]; bx = GCF(s.bx)[ix]; }
+    // rows z0-10.. and columns x0-10.. lie within the arrays' margins
+    if (tid < ZR) { DMP[DZA + tid] = GCF(s.azv)[z0 - 10 + tid]; DMP[DZB + tid] = GCF(s.bzv)[z0 - 10 + tid]; }
+    if (tid < VXW) { DMP[DXA + tid] = GCF(s.axv)[x0 - 10 + tid]; DMP[DXB + tid] = GCF(s.bxv)[x0 - 10 + tid]; }
+    if (tid < TZ) { DMP[DPA + tid] = GCF(s.az)[z0 + tid]; DMP[DPB + tid] = GCF(s.bz)[z0 + tid]; }
   }
   if (strips) sc = strip_cols(s, ix);
   __builtin_amdgcn_s_waitcnt(0);                       // the LDS writes of the staging loads
@@ -471,7 +485,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       const int svz = strips ? soff_vz(strip_row_vz(s, iz), sc) : -1;
       VEL_CELL(FK, gen, strips, svz, (iz >= 0 && iz <= nzp - 2), imin(RTM_R, imin(iz + 1, nzp - 1 - iz)),
                (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], ibz[k], dtz,
-               GCF(s.azv)[iz], GCF(s.bzv)[iz], vzn_[k]);
+               DMP[DZA + kz], DMP[DZB + kz], vzn_[k]);
 #undef PZA
       if (kz < ZR && kz >= 10 && kz < 10 + TZ && ix < nxp && iz <= nzp - 2) {
         bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
@@ -538,7 +552,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #define PCA(i) Pr[(i)]
         VEL_CELL(FK, gen, strips, strip_vx(s, iz, jx), hv, (hv ? imin(RTM_R, imin(jx + 1, nxp - 1 - jx)) : RTM_R),
                  (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hib[k], dtx,
-                 GCF(s.axv)[jx], GCF(s.bxv)[jx], hvn[k]);
+                 DMP[DXA + hcol[k]], DMP[DXB + hcol[k]], hvn[k]);
 #undef PCA
       }
     }
@@ -566,8 +580,6 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #pragma unroll
   for (int m = 0; m < RPT + 19; m++) cz[m] = VZL[(g * RPT + m) * 64 + lx];
   const int radx = imin(RTM_R, imin(ix, nxp - 1 - ix));
-  float ax = 1.0f, bx = 1.0f;
-  if (FK != FK_RCN && gen && ix < nxp) { ax = GCF(s.ax)[ix]; bx = GCF(s.bx)[ix]; }
   const bool xsrc = (tl.kind & TF_SRC) && ix >= s.sx0 && ix <= s.sx1;
   const rsrc_t Rpn = field_rsrc(pn, s);
 
@@ -614,7 +626,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           float q1 = qa[r], q2 = qb[r];
           const float dxv = (d.sx * dtx) * kk[r], dzv = (d.sz * dtz) * kk[r];
           q1 = __builtin_fmaf(ax, q1, -(dxv * bx));   // (c1 q - d)/c2 (kernel.cu:774-779)
-          q2 = __builtin_fmaf(GCF(s.az)[iz], q2, -(dzv * GCF(s.bz)[iz]));
+          q2 = __builtin_fmaf(DMP[DPA + j], q2, -(dzv * DMP[DPB + j]));
           bst(R1, q1, vcol, so);
           bst(R2, q2, vcol, so);
           pu = q1 + q2;
@@ -665,7 +677,7 @@ template <int G, int RPT>
 __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
 {
   using Q = Geo<G, RPT>;
-  __shared__ float lds[Q::LDS];
+  __shared__ float lds[Q::LDS + Q::LDS_D];
   __shared__ float ctab[100];                          // coefficient table (reduced radii near the edges)
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
@@ -688,7 +700,7 @@ template <int G, int RPT>
 __global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
 {
   using Q = Geo<G, RPT>;
-  __shared__ float lds[Q::LDS];
+  __shared__ float lds[Q::LDS + Q::LDS_D];
   __shared__ float ctab[100];
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
