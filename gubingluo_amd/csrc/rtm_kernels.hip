@@ -251,21 +251,31 @@ __device__ __forceinline__ bool any_lane(bool c) { return __builtin_amdgcn_ballo
 
 // radius-r staggered difference a[i] - b[i] with the coefficient row of
 // order 2r; the fma chain runs i = 1..r as the reference loops
-// (kernel.cu:459-460).  The reduced-radius loop is unrolled with a
-// predicate so that the operand windows stay in registers (a runtime index
-// would put them in scratch memory); cf is the coefficient table in LDS
-// (per-lane rows); row r-1 = 9 of the table is c10, so lanes at full radius
-// give the same sums in either branch.
+// (kernel.cu:459-460).  cf is the coefficient table in LDS with each row
+// zero beyond its order, so a reduced-radius lane runs the same ten steps
+// with its own row and no predicate: the extra steps add 0*(a-b) = +-0,
+// which leaves the sum unchanged (the operands past the radius are finite
+// cells of the staged halo or the zero margins).  Row r-1 = 9 is c10, so
+// lanes at full radius give the same sums in either branch.  The reduced
+// branch writes the difference as -(b - a), the same value: were it the
+// same expression as the full branch's, the compiler would hoist the ten
+// differences above the branch and hold them in registers across it.
 #define STENCIL_SUM(acc, r, c10, cf, A, B)                                                        \
   do {                                                                                            \
     if (!any_lane((r) != RTM_R)) {                                                                \
       _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], (A) - (B), acc); \
     } else {                                                                                      \
-      const float* cr_ = (cf) + ((r) - 1) * RTM_R;                                                \
-      _Pragma("unroll") for (int i = 1; i <= RTM_R; i++)                                          \
-        if (i <= (r)) acc = __builtin_fmaf(cr_[i - 1], (A) - (B), acc);                           \
+      const float* cr_ = lds_base((cf) + ((r) - 1) * RTM_R); /* read per use, not hoisted */      \
+      _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(cr_[i - 1], -((B) - (A)), acc); \
     }                                                                                             \
   } while (0)
+
+// entry t of the coefficient table as the kernels hold it: row t/10 (order
+// 2(t/10+1)) zero beyond its t/10+1 coefficients
+__device__ __forceinline__ float ctab_entry(const ShotDev& s, int t)
+{
+  return (t % RTM_R) <= (t / RTM_R) ? GCF(s.coef)[t] : 0.0f;
+}
 
 // field kinds of one pass
 enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
@@ -330,13 +340,11 @@ __device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float
       sz = __builtin_fmaf(c10[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
     }
   } else {
-    const float* c = cf + (rad - 1) * RTM_R;           // coefficient table in LDS
+    const float* c = lds_base(cf + (rad - 1) * RTM_R); // zero-padded coefficient row in LDS
 #pragma unroll
     for (int i = 1; i <= RTM_R; i++) {
-      if (i <= rad) {
-        sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
-        sz = __builtin_fmaf(c[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
-      }
+      sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
+      sz = __builtin_fmaf(c[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
     }
   }
   DivV d;
@@ -683,7 +691,7 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
-  if (threadIdx.x < 100) ctab[threadIdx.x] = GCF(s.coef)[threadIdx.x];
+  if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;
   const bool odd = it & 1;                             // (a runtime index into s would spill it to scratch)
   float none[Q::RPT], pnew[Q::RPT];
@@ -706,7 +714,7 @@ __global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   const ShotDev s = a.shots[tl.slot];
-  if (threadIdx.x < 100) ctab[threadIdx.x] = GCF(s.coef)[threadIdx.x];
+  if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;
   const bool odd = it & 1;
   float ps[Q::RPT], pr[Q::RPT], none[Q::RPT];
