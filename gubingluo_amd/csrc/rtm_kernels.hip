@@ -172,14 +172,23 @@ __device__ __forceinline__ void store_put(const ShotDev& s, int it, int off, flo
   }
 }
 
-__device__ __forceinline__ float store_get(const ShotDev& s, int it, int off)
+// A stored strip value is fetched with the first loads of a pass (raw: the
+// fp32 value, or the 16-bit code in the low bits) and decoded where it is
+// used, with the four block scales of its step from a small LDS table: the
+// compute phases then wait for no store reads.
+__device__ __forceinline__ float store_raw(const ShotDev& s, int it, int off)
 {
   const long long k = (long long)it * s.strip_len + off;
   if (s.store_fmt == ST_F32) return GCF(s.store)[k];
-  const float sc = GCF(s.sscale)[(long long)it * 4 + strip_block(s, off)];
-  if (s.store_fmt == ST_F16) return __half2float(__ushort_as_half(((const gushort*)s.store)[k])) * sc;
-  const unsigned short h = ((const gushort*)s.store)[k];
-  return __uint_as_float(((unsigned int)h) << 16) * sc;
+  return __uint_as_float((unsigned int)((const gushort*)s.store)[k]);
+}
+__device__ __forceinline__ float store_decode(const ShotDev& s, int off, float raw, const float* sc4)
+{
+  if (s.store_fmt == ST_F32) return raw;
+  const float sc = sc4[strip_block(s, off)];
+  const unsigned int h = __float_as_uint(raw);
+  if (s.store_fmt == ST_F16) return __half2float(__ushort_as_half((unsigned short)h)) * sc;
+  return __uint_as_float(h << 16) * sc;
 }
 
 // ------------------------------------------------------------ tile shape --
@@ -291,15 +300,15 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
 //   not exist there (its value is never read); the damping arrays carry 1
 //   outside the grid.
 // Reconstruction: strip cells restored from the store of step it+1
-// (kernel.cu:1294-1347), interior cells v + u as fma(s*dt/h, ib, v)
-// (kernel.cu:318-388).
+// (kernel.cu:1294-1347; VO then holds the raw stored value), interior cells
+// v + u as fma(s*dt/h, ib, v) (kernel.cu:318-388).
 #define VEL_CELL(FK, gen, strips, SOFF, VALID, RAD, INTERIOR, P_AT, VO, IB, DTH, AV, BV, OUT)               \
   do {                                                                                                     \
     float v_ = 0.0f;                                                                                       \
     if (FK == FK_RCN) {                                                                                    \
       const int so_ = (strips) ? (SOFF) : -1;                                                              \
       if (so_ >= 0) {                                                                                      \
-        v_ = store_get(s, it + 1, so_);                                                                    \
+        v_ = store_decode(s, so_, (VO), DMP);                                                              \
       } else if (!(gen) || (INTERIOR)) {                                                                   \
         float acc = 0.0f;                                                                                  \
         _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
@@ -454,13 +463,36 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       hib[k] = bld(Rib, off, 0);
     }
   }
+  // reconstruction of a strip tile: the stored value of a strip cell
+  // replaces its old velocity, fetched now with the rest (after the old
+  // value's load, which it overwrites) and decoded where it is used
+  StripCols sc{-1, -1, -1};
+  if (strips) sc = strip_cols(s, ix);
+  const bool rstrips = FK == FK_RCN && strips;
+  if (rstrips) {
+#pragma unroll
+    for (int k = 0; k < KZ; k++) {
+      const int kz = g * KZ + k;
+      const int st = kz < ZR ? soff_vz(strip_row_vz(s, z0 - 10 + kz), sc) : -1;
+      if (st >= 0) vzo[k] = store_raw(s, it + 1, st);
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int st = soff_vx(strip_row_lr(s, z0 + g * RPT + r), sc);
+      if (st >= 0) vxo[r] = store_raw(s, it + 1, st);
+    }
+#pragma unroll
+    for (int k = 0; k < KH; k++) {
+      const int st = k * NT + tid < 20 * TZ ? strip_vx(s, z0 + hrow[k], x0 - 10 + hcol[k]) : -1;
+      if (st >= 0) hvo[k] = store_raw(s, it + 1, st);
+    }
+  }
   // per-lane column data of the generic path: existence, radius, damping;
   // the damping of the tile's rows and halo columns goes to an LDS table, so
   // that the compute phases wait for no small loads
   bool vxv = true;
   int rrx = RTM_R;
   float axv = 1.0f, bxv = 1.0f, ax = 1.0f, bx = 1.0f;
-  StripCols sc{-1, -1, -1};
   if (gen) {
     vxv = ix <= nxp - 2;
     rrx = vxv ? imin(RTM_R, imin(ix + 1, nxp - 1 - ix)) : RTM_R;
@@ -474,7 +506,10 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     if (tid < VXW) { DMP[DXA + tid] = GCF(s.axv)[x0 - 10 + tid]; DMP[DXB + tid] = GCF(s.bxv)[x0 - 10 + tid]; }
     if (tid < TZ) { DMP[DPA + tid] = GCF(s.az)[z0 + tid]; DMP[DPB + tid] = GCF(s.bz)[z0 + tid]; }
   }
-  if (strips) sc = strip_cols(s, ix);
+  // 16-bit store: block scales of the steps the reconstruction reads (it+1:
+  // velocities, it: pressure) in the damping table's place (unused there)
+  if (rstrips && s.store_fmt != ST_F32 && tid < 8)
+    DMP[tid] = GCF(s.sscale)[(long long)(tid < 4 ? it + 1 : it) * 4 + (tid & 3)];
   __builtin_amdgcn_s_waitcnt(0);                       // the LDS writes of the staging loads
   __syncthreads();                                     // p staged
 
@@ -529,6 +564,10 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         const int ii = (iz - npml) * s.nxl + ix - npml;
         qa[r] = GCF(s.g1)[ii];
         qb[r] = GCF(s.illum)[ii];
+      }
+      if (rstrips) {                                   // reconstruction: stored p of strip cells
+        const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
+        if (sp_ >= 0) qa[r] = store_raw(s, it, sp_);
       }
     }
   }
@@ -604,7 +643,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       const int sp_ = strips ? soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc) : -1;
       bool wr = false;
       if (sp_ >= 0) {
-        p = store_get(s, it, sp_);                     // kernel.cu:1348-1400
+        p = store_decode(s, sp_, qa[r], DMP + 4);      // kernel.cu:1348-1400
         wr = true;
       } else if (!gen || inner) {
         const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
