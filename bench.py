@@ -10,7 +10,9 @@ filtered).  Data: modelled by the same code from the synthetic model.
 
 A "step" is the migration of one batch of --shots-per-step shots (forward
 modelling with boundary saving, reverse reconstruction + receiver
-propagation + imaging, FP64 stacking).  The velocity model is resident in
+propagation + imaging, FP64 stacking); by default the whole 200-shot survey,
+all shots in flight in one launch batch (about 110 GB of slots), so that
+the per-step launches carry enough tiles to keep the tail short.  The velocity model is resident in
 HBM before the timed region (rtmh_model_create); every step runs through the
 C ABI (rtmh_run_model), which returns after its stream has drained.
 
@@ -164,7 +166,7 @@ def run_survey(args, G, synth):
         recs.append((r.stats, dt, shots))
         s = r.stats
         log(f"[bench] step {k}: {len(shots)} shots {dt:.2f} s, {s.cell_updates / dt / 1e9:.1f} Gcell/s "
-            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f}, tz {s.tile_z})")
+            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f}, tz {s.tile_z}/{s.tile_z_rev})")
     elapsed = time.perf_counter() - t0
     cells = sum(s.cell_updates for s, _, _ in recs)
     nshots = sum(len(sh) for _, _, sh in recs)
@@ -177,7 +179,7 @@ def run_survey(args, G, synth):
         {"workload": "plain RTM, 200-shot survey, 1024x512 dx=10 m, PML 50, Ricker 15 Hz, dt=0.8 ms "
                      f"nt={P.nt}, shots every 50 m, split-spread 301 receivers, fp32, 3 imaging conditions",
          "shots_per_step": S, "batch": recs[0][0].batch, "graph_chunk": args.chunk,
-         "tile_z": recs[0][0].tile_z, "store": args.store},
+         "tile_z": recs[0][0].tile_z, "tile_z_rev": recs[0][0].tile_z_rev, "store": args.store},
         {"shots_per_hour": round(nshots / elapsed * 3600.0, 1),
          "p50_shot_latency_s": round(float(np.percentile(lat, 50)), 3),
          "p99_shot_latency_s": round(float(np.percentile(lat, 99)), 3),
@@ -222,7 +224,7 @@ def run_batched(args, G, synth):
         recs.append((r.stats, dt, shots))
         s = r.stats
         log(f"[bench] step {k}: {len(shots)} shots {dt:.2f} s, {s.cell_updates / dt / 1e9:.1f} Gcell/s "
-            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f}, tz {s.tile_z})")
+            f"(gpu {s.gpu_ms:.0f} ms, fwd {s.fwd_ms:.0f} rev {s.rev_ms:.0f}, tz {s.tile_z}/{s.tile_z_rev})")
     elapsed = time.perf_counter() - t0
     cells = sum(s.cell_updates for s, _, _ in recs)
     nshots = sum(len(sh) for _, _, sh in recs)
@@ -233,7 +235,8 @@ def run_batched(args, G, synth):
         "synthetic: seeded smooth-random layered model, data modelled in the same run",
         {"workload": f"batched RTM, 2048x768 dx=10 m, 256 shots every 80 m, per-shot windows U[400,1600] columns "
                      f"(drawn {min(widths)}..{max(widths)}, clipped at the model edges), nt={P.nt}",
-         "shots_per_step": S, "batch": S, "graph_chunk": args.chunk, "tile_z": recs[0][0].tile_z},
+         "shots_per_step": S, "batch": S, "graph_chunk": args.chunk, "tile_z": recs[0][0].tile_z,
+         "tile_z_rev": recs[0][0].tile_z_rev},
         {"shots_per_hour": round(nshots / elapsed * 3600.0, 1), "hbm_pct_roofline":
          round(100.0 * roof["achieved"] / HBM_PEAK_GBS, 1), "roofline": roof, "cpu_baseline": None})
     ctx.model_destroy(mid)
@@ -355,10 +358,11 @@ def run_serve(args, G, synth):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (survey default 2, others 10)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="survey", choices=["survey", "batched", "precision", "serve"])
-    ap.add_argument("--shots-per-step", type=int, default=0, help="shots per step (survey default 20)")
+    ap.add_argument("--shots-per-step", type=int, default=0,
+                    help="shots per step (survey default 200: the whole survey in one launch batch)")
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: all shots of the step)")
     ap.add_argument("--store", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--chunk", type=int, default=50, help="time steps per replayed hipGraph (0: plain launches)")
@@ -382,9 +386,11 @@ def main():
     import gubingluo_amd as G
     from gubingluo_amd import synth
 
+    if args.steps is None:
+        args.steps = 2 if args.workload == "survey" else 10
     if args.workload == "survey":
         if not args.shots_per_step:
-            args.shots_per_step = 20
+            args.shots_per_step = 200
         out = run_survey(args, G, synth)
     elif args.workload == "batched":
         out = run_batched(args, G, synth)

@@ -471,9 +471,14 @@ void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tile
         // strips (vx, p: columns n-R..n-1 and nxp-n-1..nxp-n+9 over interior
         // rows; vz, p: rows n-R..n-1 and nzp-n-1..nzp-n+9 over interior
         // columns), against the velocity halo of the tile
-        if (hit(10, 10, n, nzp - n - 1, n - R, n - 1) || hit(10, 10, n, nzp - n - 1, nxp - n - 1, nxp - n + R - 1) ||
-            hit(10, 10, n - R, n - 1, n, nxp - n - 1) || hit(10, 10, nzp - n - 1, nzp - n + R - 1, n, nxp - n - 1))
-          f |= TF_STRIP;
+        auto strips_hit = [&](int h) {
+          return hit(h, h, n, nzp - n - 1, n - R, n - 1) || hit(h, h, n, nzp - n - 1, nxp - n - 1, nxp - n + R - 1) ||
+                 hit(h, h, n - R, n - 1, n, nxp - n - 1) || hit(h, h, nzp - n - 1, nzp - n + R - 1, n, nxp - n - 1);
+        };
+        if (strips_hit(10)) f |= TF_STRIP;
+        // own cells only (forward saves; the p blocks lie within these
+        // rectangles: columns n-R..n-1, nxp-n..nxp-n+9; rows n-R..n-1, nzp-n..nzp-n+9)
+        if (strips_hit(0)) f |= TF_SAVE;
         if (d.sx1 >= d.sx0 && hit(0, 0, d.sz0, d.sz1, d.sx0, d.sx1)) f |= TF_SRC;
         if (d.nrec > 0 && hit(0, 0, n, n, d.rcol0, d.rcol0 + d.nrec - 1)) f |= TF_REC;
         // undamped full-radius interior for every velocity the step computes
@@ -653,12 +658,12 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
   double budget = opt.mem_budget > 0 ? opt.mem_budget : 0.8 * (double)(freeb + pool_.held());
-  const int bmax = opt.batch > 0 ? opt.batch : 64;
+  const int bmax = opt.batch > 0 ? opt.batch : nsh;   // 0: as many as the budget holds
 
   hipEvent_t e0, e1, e2;
   HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2));
   double fwd_ms = 0, rev_ms = 0, cells = 0, algb = 0, step_fwd = 0, step_rev = 0;
-  int last_tz = tz;
+  int last_tz = tz, last_tz_r = tz;
   if (stats) memset(stats, 0, sizeof(*stats));
   int nbatches = 0, maxb = 0;
   std::vector<double> img_tmp;  // for temp-image checkpoints
@@ -692,20 +697,59 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     // values, so the tuning steps are part of the pass)
     int maxp = 0, maxz = 0;
     for (int j = 0; j < nb; j++) { maxp = std::max(maxp, ds[j].nxp); maxz = std::max(maxz, ds[j].nzp); }
+    // the forward and reverse step kernels differ, so each pass has its own
+    // tile height (key bit 62: reverse)
     const long long tkey = ((long long)(maxp / 64) << 40) | ((long long)(maxz / 32) << 20) | nb;
+    const long long rkey = tkey | (1LL << 62);
     const int cands[3] = {16, 32, 64};
-    bool tune = false;
+    bool tune = false, tune_r = false;
+    int tz_r = tz;
     if (opt.tile_z == 0) {
+      const bool can = nt >= 3 * kTuneSteps + 2 && !opt.profile;
       auto ct = tuned_.find(tkey);
       if (ct != tuned_.end()) tz = ct->second;
-      else tune = nt >= 3 * kTuneSteps + 2 && !opt.profile;
+      else tune = can;
+      auto cr = tuned_.find(rkey);
+      if (cr != tuned_.end()) tz_r = cr->second;
+      else tune_r = can;
     }
     std::vector<Tile> tiles, tcand[3];
     size_t tcap = 0;
-    if (tune)
-      for (int c = 0; c < 3; c++) { build_tiles(ds, cands[c], tcand[c]); tcap = std::max(tcap, tcand[c].size()); }
+    for (int c = 0; c < 3; c++) {
+      build_tiles(ds, cands[c], tcand[c]);
+      tcap = std::max(tcap, tcand[c].size());
+    }
     build_tiles(ds, tz, tiles);
     tcap = std::max(tcap, tiles.size());
+    // the first 3 x kTuneSteps steps of a pass, kTuneSteps with each candidate
+    // height, timed with events; every candidate computes the same cells, so
+    // these are steps of the pass.  Returns the fastest height.
+    auto tune_pass = [&](int phase) -> int {
+      hipEvent_t te[4];
+      for (auto& e : te) HIPCHK(hipEventCreate(&e));
+      HIPCHK(hipEventRecord(te[0], st_));
+      for (int c = 0; c < 3; c++) {
+        HIPCHK(hipMemcpyAsync(d_tiles_, tcand[c].data(), tcand[c].size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
+        LaunchArgs a{d_ds_, d_tiles_, (int)tcand[c].size(), 0};
+        for (int s2 = 0; s2 < kTuneSteps; s2++) {
+          a.step = c * kTuneSteps + s2;
+          HIPCHK(rtm_launch_step(a, phase, cands[c], phase == PH_FWD ? 1 : 0, st_));
+          if (phase == PH_FWD && opt.store_fmt != ST_F32) HIPCHK(rtm_launch_quant(d_ds_, d_slots_, nb, a.step, st_));
+        }
+        HIPCHK(hipEventRecord(te[c + 1], st_));
+      }
+      HIPCHK(rtm_launch_bump(d_ds_, d_slots_, nb, 3 * kTuneSteps, st_));
+      HIPCHK(hipEventSynchronize(te[3]));
+      float best = 1e30f;
+      int pick = cands[0];
+      for (int c = 0; c < 3; c++) {
+        float m = 0;
+        HIPCHK(hipEventElapsedTime(&m, te[c], te[c + 1]));
+        if (m < best) { best = m; pick = cands[c]; }
+      }
+      for (auto& e : te) (void)hipEventDestroy(e);
+      return pick;
+    };
     std::vector<int> slots(nb);
     for (int j = 0; j < nb; j++) slots[j] = j;
     if (ensure_cb(nb, tcap) != 0) { for (void* p : mem) pool_.put(p); rc = -3; break; }
@@ -726,28 +770,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     HIPCHK(hipEventRecord(e0, st_));
     int fwd_done = 0;
     if (tune) {
-      hipEvent_t te[4];
-      for (auto& e : te) HIPCHK(hipEventCreate(&e));
-      HIPCHK(hipEventRecord(te[0], st_));
-      for (int c = 0; c < 3; c++) {
-        HIPCHK(hipMemcpyAsync(d_tiles, tcand[c].data(), tcand[c].size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
-        LaunchArgs a{d_ds, d_tiles, (int)tcand[c].size(), 0};
-        for (int s2 = 0; s2 < kTuneSteps; s2++) {
-          a.step = c * kTuneSteps + s2;
-          HIPCHK(rtm_launch_step(a, PH_FWD, cands[c], 1, st_));
-          if (opt.store_fmt != ST_F32) HIPCHK(rtm_launch_quant(d_ds, d_slots, nb, a.step, st_));
-        }
-        HIPCHK(hipEventRecord(te[c + 1], st_));
-      }
-      HIPCHK(rtm_launch_bump(d_ds, d_slots, nb, 3 * kTuneSteps, st_));
-      HIPCHK(hipEventSynchronize(te[3]));
-      float best = 1e30f;
-      for (int c = 0; c < 3; c++) {
-        float m = 0;
-        HIPCHK(hipEventElapsedTime(&m, te[c], te[c + 1]));
-        if (m < best) { best = m; tz = cands[c]; }
-      }
-      for (auto& e : te) (void)hipEventDestroy(e);
+      tz = tune_pass(PH_FWD);
       tuned_[tkey] = tz;
       fwd_done = 3 * kTuneSteps;
       build_tiles(ds, tz, tiles);
@@ -810,9 +833,20 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     }
     if (opt.verbose) printf("source wavefiled prepared over...\nBegin backward......\n");
     HIPCHK(hipEventRecord(e1, st_));
-    rc = run_pass(PH_REV, nt - 1, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt, opt.graph_chunk,
-                  (opt.profile && stats) ? stats : nullptr);
+    int rev_done = 0;
+    if (tune_r) {
+      tz_r = tune_pass(PH_REV);
+      tuned_[rkey] = tz_r;
+      rev_done = 3 * kTuneSteps;
+    }
+    if (tune_r || tz_r != tz) {                        // (tuning left the last candidate's tiles)
+      build_tiles(ds, tz_r, tiles);
+      HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
+    }
+    rc = run_pass(PH_REV, nt - 1 - rev_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz_r, opt.store_fmt,
+                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr);
     last_tz = tz;
+    last_tz_r = tz_r;
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e2, st_));
     HIPCHK(hipStreamSynchronize(st_));
@@ -902,6 +936,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats->fwd_ms = fwd_ms; stats->rev_ms = rev_ms; stats->gpu_ms = fwd_ms + rev_ms;
     stats->cell_updates = cells; stats->nshots = nsh; stats->batches = nbatches; stats->tile_z = last_tz;
+    stats->tile_z_rev = last_tz_r;
     stats->batch = maxb; stats->hbm_bytes_alg = algb;
     stats->step_bytes_fwd = step_fwd; stats->step_bytes_rev = step_rev;
   }

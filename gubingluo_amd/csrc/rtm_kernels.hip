@@ -207,6 +207,7 @@ struct Geo {
                                                        // thread's window may run past TZ+38)
   static constexpr int KP = (TZ + 38 + G - 1) / G;     // p rows staged per thread
   static constexpr int KH = (20 * TZ + NT - 1) / NT;   // halo vx cells per thread
+  static_assert((20 * TZ) % 64 == 0, "halo cells fill whole waves");
   static constexpr int LDS_A = PZR * PW;               // p rows z0-19.. (x halo on the tile rows)
   static constexpr int LDS_B = TZ * VXW + ZR * 64;     // VXL | VZL
   static constexpr int LDS = LDS_A > LDS_B ? LDS_A : LDS_B;
@@ -306,13 +307,15 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
   do {                                                                                                     \
     float v_ = 0.0f;                                                                                       \
     if (FK == FK_RCN) {                                                                                    \
-      const int so_ = (strips) ? (SOFF) : -1;                                                              \
-      if (so_ >= 0) {                                                                                      \
-        v_ = store_decode(s, so_, (VO), DMP);                                                              \
-      } else if (!(gen) || (INTERIOR)) {                                                                   \
-        float acc = 0.0f;                                                                                  \
-        _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
-        v_ = __builtin_fmaf(acc * (DTH), (IB), (VO));                                                      \
+      /* computed for every cell and then selected (no divergent branch): */                               \
+      /* cells off the interior stay 0, strip cells take the stored value */                               \
+      float acc = 0.0f;                                                                                    \
+      _Pragma("unroll") for (int i = 1; i <= RTM_R; i++) acc = __builtin_fmaf(c10[i - 1], P_AT(i) - P_AT(1 - i), acc); \
+      v_ = __builtin_fmaf(acc * (DTH), (IB), (VO));                                                        \
+      if ((gen) && !(INTERIOR)) v_ = 0.0f;                                                                 \
+      if (strips) {                                                                                        \
+        const int so_ = (SOFF);                                                                            \
+        if (so_ >= 0) v_ = store_decode(s, so_, (VO), DMP);                                                \
       }                                                                                                    \
     } else if (!(gen)) {                                                                                   \
       float acc = 0.0f;                                                                                    \
@@ -323,10 +326,9 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
       float acc = 0.0f;                                                                                    \
       STENCIL_SUM(acc, rr_, c10, ctab, P_AT(i), P_AT(1 - i));                                              \
       const float u_ = (acc * (DTH)) * (IB);                                                               \
-      const float a_ = (AV), b_ = (BV);                                                                    \
-      /* a = b = 1 gives fma(1, v, -u) == v - u exactly */                                                 \
-      if (any_lane(a_ != 1.0f || b_ != 1.0f)) v_ = __builtin_fmaf(a_, (VO), -(u_ * b_));                   \
-      else v_ = (VO) - u_;                                                                                 \
+      /* one form for every cell of the tile: a = b = 1 (undamped) gives */                                \
+      /* fma(1, v, -u) == v - u exactly, cheaper than a branch per cell */                                 \
+      v_ = __builtin_fmaf((AV), (VO), -(u_ * (BV)));                                                       \
     }                                                                                                      \
     OUT = v_;                                                                                              \
   } while (0)
@@ -394,9 +396,14 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   const int tid = threadIdx.x, lx = tid & 63;
   const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int x0 = tl.x0, z0 = tl.z0, nxp = s.nxp, nzp = s.nzp, npml = s.npml;
+  // halo vx cell k*NT + tid exists for the whole wave or none of it (20*TZ
+  // is a multiple of 64): a wave-uniform test
+  auto halo_on = [&](int k) { return k * NT + g * 64 < 20 * TZ; };
   const int ix = x0 + lx;
   const bool gen = (tl.kind & TF_GEN) != 0;
-  const bool strips = (tl.kind & TF_STRIP) != 0;
+  // strip cells: the reconstruction restores those of the tile and its halo,
+  // the forward pass saves its own
+  const bool strips = (tl.kind & (FK == FK_RCN ? TF_STRIP : TF_SAVE)) != 0;
   const int pr4 = s.pitch * 4;
   const int srow0 = (z0 + RTM_MZT) * pr4;             // byte offset of tile row 0
   const int vcol = (ix + RTM_MXL) * 4;                // byte offset of the lane's column
@@ -411,19 +418,21 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   //         lds modifier: lane i of a wave writes its dword to base + 4 i),
   //         old velocities and density of every velocity cell into registers
   {
-    // wings: lanes 0..19 load columns x0-20.. and x0+64.. with two
-    // instructions of uniform LDS base (the base of an lds load is one value
-    // per wave, so the two wings must not share an instruction)
-    const int wl = (x0 - 20 + lx + RTM_MXL) * 4, wr = (x0 + 64 + lx + RTM_MXL) * 4;
+    // tile rows: columns x0-20..x0+83 as two full-wave loads (x0-20..x0+43
+    // and x0+20..x0+83; the middle 24 columns arrive twice with the same
+    // values), so that no load is issued by part of a wave; halo rows:
+    // columns x0..x0+63
+    const int ca = (x0 - 20 + lx + RTM_MXL) * 4, cb = (x0 + 20 + lx + RTM_MXL) * 4;
 #pragma unroll
     for (int m = 0; m < KP; m++) {
       const int k = g + G * m;
       if (k < TZ + 38) {
         const int so = srow0 + (k - 19) * pr4;
-        bld_lds(Rp, PS + k * PW + 20, vcol, so);
-        if (k >= 19 && k < 19 + TZ && lx < 20) {
-          bld_lds(Rp, PS + k * PW, wl, so);
-          bld_lds(Rp, PS + k * PW + 84, wr, so);
+        if (k >= 19 && k < 19 + TZ) {
+          bld_lds(Rp, PS + k * PW, ca, so);
+          bld_lds(Rp, PS + k * PW + 40, cb, so);
+        } else {
+          bld_lds(Rp, PS + k * PW + 20, vcol, so);
         }
       }
     }
@@ -457,7 +466,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     hrow[k] = h / 20;
     const int hc = h - hrow[k] * 20;
     hcol[k] = hc < 10 ? hc : hc + 64;                  // VXL column: x = x0 - 10 + hcol
-    if (h < 20 * TZ) {
+    if (halo_on(k)) {
       const int off = srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4;
       hvo[k] = bld(Rvx, off, 0);
       hib[k] = bld(Rib, off, 0);
@@ -483,7 +492,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
-      const int st = k * NT + tid < 20 * TZ ? strip_vx(s, z0 + hrow[k], x0 - 10 + hcol[k]) : -1;
+      const int st = halo_on(k) ? strip_vx(s, z0 + hrow[k], x0 - 10 + hcol[k]) : -1;
       if (st >= 0) hvo[k] = store_raw(s, it + 1, st);
     }
   }
@@ -530,9 +539,13 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
                (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], ibz[k], dtz,
                DMP[DZA + kz], DMP[DZB + kz], vzn_[k]);
 #undef PZA
-      if (kz < ZR && kz >= 10 && kz < 10 + TZ && ix < nxp && iz <= nzp - 2) {
-        bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
-        if (FK == FK_SRC && save && strips && svz >= 0) store_put(s, it, svz, vzn_[k]);
+      if (kz >= 10 && kz < 10 + TZ) {                  // own rows (wave-uniform)
+        auto out = [&]() {
+          bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
+          if (FK == FK_SRC && save && strips && svz >= 0) store_put(s, it, svz, vzn_[k]);
+        };
+        if (!gen) out();                               // every lane's cell exists
+        else if (ix < nxp && iz <= nzp - 2) out();
       }
     }
   }
@@ -583,18 +596,19 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
                (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], ibx[r], dtx,
                axv, bxv, vxn_[r]);
 #undef PCA
-      if (iz < nzp && ix <= nxp - 2) {
+      auto out = [&]() {
         bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
         if (FK == FK_SRC && save && strips && svx >= 0) store_put(s, it, svx, vxn_[r]);
-      }
+      };
+      if (!gen) out();
+      else if (iz < nzp && ix <= nxp - 2) out();
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
-      const int h = k * NT + tid;
       const int iz = z0 + hrow[k], jx = x0 - 10 + hcol[k];
       const float* __restrict__ Pr = lds_base(PS + (hrow[k] + 19) * PW + hcol[k] + 1) + 9;   // p at column jx
       hvn[k] = 0.0f;
-      if (h < 20 * TZ) {
+      if (halo_on(k)) {
         const bool hv = jx >= 0 && jx <= nxp - 2;
 #define PCA(i) Pr[(i)]
         VEL_CELL(FK, gen, strips, strip_vx(s, iz, jx), hv, (hv ? imin(RTM_R, imin(jx + 1, nxp - 1 - jx)) : RTM_R),
@@ -619,7 +633,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   for (int r = 0; r < RPT; r++) VXL[(g * RPT + r) * VXW + lx + 10] = vxn_[r];
 #pragma unroll
   for (int k = 0; k < KH; k++)
-    if (k * NT + tid < 20 * TZ) VXL[hrow[k] * VXW + hcol[k]] = hvn[k];
+    if (halo_on(k)) VXL[hrow[k] * VXW + hcol[k]] = hvn[k];
   __syncthreads();                                     // new velocities staged
 
   // ---- 3. new pressure of the own cells
@@ -640,19 +654,23 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     const bool incell = ix < nxp && iz < nzp;
     if (FK == FK_RCN) {
       const bool inner = xin && iz >= npml && iz < nzp - npml;
-      const int sp_ = strips ? soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc) : -1;
-      bool wr = false;
-      if (sp_ >= 0) {
-        p = store_decode(s, sp_, qa[r], DMP + 4);      // kernel.cu:1348-1400
-        wr = true;
-      } else if (!gen || inner) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
-        p = __builtin_fmaf(kk[r], pres_sum(d, dtx, dtz), p);   // kernel.cu:389-429
-        wr = true;
+      // computed for every cell and then selected (no divergent branch):
+      // interior cells take the update, strip cells the stored value
+      const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
+      const float pc = __builtin_fmaf(kk[r], pres_sum(d, dtx, dtz), p);   // kernel.cu:389-429
+      bool wr = !gen || inner;
+      if (wr) p = pc;
+      if (strips) {
+        const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
+        if (sp_ >= 0) {
+          p = store_decode(s, sp_, qa[r], DMP + 4);    // kernel.cu:1348-1400
+          wr = true;
+        }
       }
-      if (wr && xsrc && iz >= s.sz0 && iz <= s.sz1)    // source removal (rtm_model.cu:437)
+      if ((tl.kind & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && wr && xsrc)   // source removal (rtm_model.cu:437)
         p = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
-      if (wr) bst(Rpn, p, vcol, so);
+      if (!gen) bst(Rpn, p, vcol, so);                 // every lane's cell is interior
+      else if (wr) bst(Rpn, p, vcol, so);
     } else {
       if (!gen) {
         const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
@@ -681,9 +699,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         if (upd) p = pu;
       }
       if (FK == FK_SRC) {
-        if (xsrc && iz >= s.sz0 && iz <= s.sz1)        // kernel.cu:36-47
+        if ((tl.kind & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && xsrc)   // kernel.cu:36-47
           p = __builtin_fmaf(GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
-        if (incell) {
+        auto out = [&]() {
           bst(Rpn, p, vcol, so);
           if ((tl.kind & TF_REC) && s.record && iz == npml) {   // kernel.cu:48-71
             const int ir = ix - s.rcol0;
@@ -693,7 +711,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
             const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
             if (sp_ >= 0) store_put(s, it, sp_, p);
           }
-        }
+        };
+        if (!gen) out();
+        else if (incell) out();
       } else {                                         // receiver wavefield
         if ((tl.kind & TF_REC) && iz == npml) {        // Dirichlet insertion (kernel.cu:72-95)
           const int ir = ix - s.rcol0;
@@ -702,13 +722,17 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
             p = s.dt * rv;
           }
         }
-        if (incell) {
+        auto image = [&]() {                           // imaging (kernel.cu:1401-1416)
+          const int ii = (iz - npml) * s.nxl + ix - npml;
+          GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, qa[r]);
+          GF(s.illum)[ii] = __builtin_fmaf(ps[r], ps[r], qb[r]);
+        };
+        if (!gen) {                                    // every lane's cell is interior
           bst(Rpn, p, vcol, so);
-          if (xin && iz >= npml && iz < nzp - npml) {  // imaging (kernel.cu:1401-1416)
-            const int ii = (iz - npml) * s.nxl + ix - npml;
-            GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, qa[r]);
-            GF(s.illum)[ii] = __builtin_fmaf(ps[r], ps[r], qb[r]);
-          }
+          image();
+        } else if (incell) {
+          bst(Rpn, p, vcol, so);
+          if (xin && iz >= npml && iz < nzp - npml) image();
         }
       }
     }

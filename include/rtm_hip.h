@@ -107,6 +107,8 @@ typedef struct rtmh_stats {
   int kern_calls[4];
   double step_bytes_fwd;    /* algorithmic bytes of one forward step of the batch */
   double step_bytes_rev;    /* ... of one reverse step                            */
+  int tile_z_rev;           /* tile height of the reverse pass (tile_z: forward)  */
+  int _pad0;
 } rtmh_stats;
 
 /* last error message of this thread ("" if none) */
