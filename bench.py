@@ -97,16 +97,18 @@ def roofline_of(recs, P, copy_gbs=None):
     """Dominant kernel: the reverse time step k_rev (one launch per step:
     reconstruction, receiver wavefield, imaging), timed by HIP events on the
     engine's stream around every reverse pass of the timed region; the
-    algorithmic bytes per launch come from the per-cell figures (engine
-    stats step_bytes_rev)."""
+    algorithmic bytes come from the per-cell figures (engine stats
+    rev_bytes_alg: all reverse steps of all batches), so a launch's average
+    is exact also when the batches of a step differ in size."""
     rev_steps = sum(s.batches * (P.nt - 1) for s, _, _ in recs)
     rev_ms = sum(s.rev_ms for s, _, _ in recs)
-    step_bytes = float(np.mean([s.step_bytes_rev for s, _, _ in recs]))
+    # totals over every batch of every step (batches may differ in size)
+    step_bytes = sum(s.rev_bytes_alg for s, _, _ in recs) / rev_steps
     avg_step_s = rev_ms * 1e-3 / rev_steps
     achieved = step_bytes / avg_step_s / 1e9
     fwd_steps = sum(s.batches * P.nt for s, _, _ in recs)
     fwd_ms = sum(s.fwd_ms for s, _, _ in recs)
-    fwd_bytes = float(np.mean([s.step_bytes_fwd for s, _, _ in recs]))
+    fwd_bytes = sum(s.fwd_bytes_alg for s, _, _ in recs) / fwd_steps
     ratio = traffic_ratio("k_rev")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),

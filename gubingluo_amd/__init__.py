@@ -62,7 +62,8 @@ class Stats(C.Structure):
                 ("cell_updates", C.c_double), ("nshots", C.c_int), ("batches", C.c_int), ("tile_z", C.c_int),
                 ("batch", C.c_int), ("hbm_bytes_alg", C.c_double), ("kern_ms", C.c_double * 4),
                 ("kern_calls", C.c_int * 4), ("step_bytes_fwd", C.c_double), ("step_bytes_rev", C.c_double),
-                ("tile_z_rev", C.c_int), ("_pad0", C.c_int)]
+                ("tile_z_rev", C.c_int), ("_pad0", C.c_int), ("fwd_bytes_alg", C.c_double),
+                ("rev_bytes_alg", C.c_double)]
 
 
 class Geom(C.Structure):

@@ -45,12 +45,11 @@
 
 // tile flags (Tile::kind)
 enum RtmTileFlag : int {
-  TF_STRIP = 1,   // the tile or its halo holds boundary-strip cells (reconstruction)
+  TF_STRIP = 1,   // the tile or its halo holds boundary-strip cells (restored by the reconstruction)
   TF_SRC = 2,     // the tile holds source cells
   TF_REC = 4,     // the tile holds receiver cells
   TF_GEN = 8,     // the tile or its halo leaves the undamped interior (per-cell rules)
   TF_INT = 16,    // reverse phase: the tile holds reconstructed cells (interior or strips)
-  TF_SAVE = 32,   // the tile's own cells hold boundary-strip cells (forward: saved)
 };
 
 enum RtmPhase : int { PH_IDLE = 0, PH_FWD = 1, PH_REV = 2 };

@@ -259,7 +259,6 @@ bool GraphKey::operator<(const GraphKey& o) const
   if (tz != o.tz) return tz < o.tz;
   if (phase != o.phase) return phase < o.phase;
   if (save != o.save) return save < o.save;
-  if (quant != o.quant) return quant < o.quant;
   if (k != o.k) return k < o.k;
   if (shots != o.shots) return shots < o.shots;
   return tiles < o.tiles;
@@ -322,7 +321,7 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
   const size_t strip = (size_t)40 * (nzi + s.g.nxl);
   size_t b = small_layout(s).total + kSlotFields * F + 2 * I;
   b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
-  b += align256((size_t)nt * strip * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * strip * 4));
+  b += align256((size_t)nt * strip * store_elem(store_fmt));
   b += align256((size_t)nt * 4 * 4);
   return b;
 }
@@ -375,7 +374,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.strip_len = (long long)40 * (d.nzi + d.nxl);
   d.store_fmt = store_fmt;
   d.store = base + o;
-  o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt) + (store_fmt == ST_F32 ? 0 : 2 * d.strip_len * 4));
+  o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt));
   d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4);
 }
 
@@ -389,8 +388,8 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
   LaunchArgs a{dshots, dtiles, k.ntiles, 0};
   for (int s = 0; s < k.k; s++) {
     a.step = s;
-    (void)rtm_launch_step(a, k.phase, k.tz, k.save, st_);
-    if (k.quant) (void)rtm_launch_quant(dshots, dslots, nslots, s, st_);
+    (void)rtm_launch_step(a, k.phase, k.tz, st_);
+    if (k.save) (void)rtm_launch_save(dshots, dslots, nslots, s, st_);
   }
   (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
   if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) return nullptr;
@@ -407,11 +406,10 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
                      rtmh_stats* prof)
 {
   const int save = phase == PH_FWD ? 1 : 0;
-  const int quant = (phase == PH_FWD && store_fmt != ST_F32) ? 1 : 0;
   const int nslots = (int)slots.size();
   int done = 0;
   if (prof) {
-    // every step kernel (and the quantisation after it) bracketed by its own
+    // every step kernel (and the strip save after it) bracketed by its own
     // event pair on the launch stream
     hipEvent_t ev[3];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -420,22 +418,22 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     for (int s = 0; s < nsteps; s++) {
       a.step = s;
       HIPCHK(hipEventRecord(ev[0], st_));
-      HIPCHK(rtm_launch_step(a, phase, tz, save, st_));
+      HIPCHK(rtm_launch_step(a, phase, tz, st_));
       HIPCHK(hipEventRecord(ev[1], st_));
-      if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, st_));
+      if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, st_));
       HIPCHK(hipEventRecord(ev[2], st_));
       HIPCHK(hipEventSynchronize(ev[2]));
       float m0 = 0, m1 = 0;
       HIPCHK(hipEventElapsedTime(&m0, ev[0], ev[1]));
       HIPCHK(hipEventElapsedTime(&m1, ev[1], ev[2]));
       prof->kern_ms[kb] += m0; prof->kern_calls[kb]++;
-      if (quant) { prof->kern_ms[kb + 1] += m1; prof->kern_calls[kb + 1]++; }
+      if (save) { prof->kern_ms[kb + 1] += m1; prof->kern_calls[kb + 1]++; }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return 0;
   }
   if (chunk > 0 && nsteps >= chunk) {
-    GraphKey key{ntiles, tz, phase, save, quant, chunk, dshots, dtiles};
+    GraphKey key{ntiles, tz, phase, save, chunk, dshots, dtiles};
     hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
     if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
     const int nch = nsteps / chunk;
@@ -445,8 +443,8 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   LaunchArgs a{dshots, dtiles, ntiles, 0};
   for (int s = 0; s < nsteps - done; s++) {
     a.step = s;
-    HIPCHK(rtm_launch_step(a, phase, tz, save, st_));
-    if (quant) HIPCHK(rtm_launch_quant(dshots, dslots, nslots, s, st_));
+    HIPCHK(rtm_launch_step(a, phase, tz, st_));
+    if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, st_));
   }
   return 0;
 }
@@ -471,14 +469,9 @@ void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tile
         // strips (vx, p: columns n-R..n-1 and nxp-n-1..nxp-n+9 over interior
         // rows; vz, p: rows n-R..n-1 and nzp-n-1..nzp-n+9 over interior
         // columns), against the velocity halo of the tile
-        auto strips_hit = [&](int h) {
-          return hit(h, h, n, nzp - n - 1, n - R, n - 1) || hit(h, h, n, nzp - n - 1, nxp - n - 1, nxp - n + R - 1) ||
-                 hit(h, h, n - R, n - 1, n, nxp - n - 1) || hit(h, h, nzp - n - 1, nzp - n + R - 1, n, nxp - n - 1);
-        };
-        if (strips_hit(10)) f |= TF_STRIP;
-        // own cells only (forward saves; the p blocks lie within these
-        // rectangles: columns n-R..n-1, nxp-n..nxp-n+9; rows n-R..n-1, nzp-n..nzp-n+9)
-        if (strips_hit(0)) f |= TF_SAVE;
+        if (hit(10, 10, n, nzp - n - 1, n - R, n - 1) || hit(10, 10, n, nzp - n - 1, nxp - n - 1, nxp - n + R - 1) ||
+            hit(10, 10, n - R, n - 1, n, nxp - n - 1) || hit(10, 10, nzp - n - 1, nzp - n + R - 1, n, nxp - n - 1))
+          f |= TF_STRIP;
         if (d.sx1 >= d.sx0 && hit(0, 0, d.sz0, d.sz1, d.sx0, d.sx1)) f |= TF_SRC;
         if (d.nrec > 0 && hit(0, 0, n, n, d.rcol0, d.rcol0 + d.nrec - 1)) f |= TF_REC;
         // undamped full-radius interior for every velocity the step computes
@@ -662,7 +655,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
 
   hipEvent_t e0, e1, e2;
   HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2));
-  double fwd_ms = 0, rev_ms = 0, cells = 0, algb = 0, step_fwd = 0, step_rev = 0;
+  double fwd_ms = 0, rev_ms = 0, cells = 0, algb = 0, step_fwd = 0, step_rev = 0, fwdb = 0, revb = 0;
   int last_tz = tz, last_tz_r = tz;
   if (stats) memset(stats, 0, sizeof(*stats));
   int nbatches = 0, maxb = 0;
@@ -733,8 +726,8 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         LaunchArgs a{d_ds_, d_tiles_, (int)tcand[c].size(), 0};
         for (int s2 = 0; s2 < kTuneSteps; s2++) {
           a.step = c * kTuneSteps + s2;
-          HIPCHK(rtm_launch_step(a, phase, cands[c], phase == PH_FWD ? 1 : 0, st_));
-          if (phase == PH_FWD && opt.store_fmt != ST_F32) HIPCHK(rtm_launch_quant(d_ds_, d_slots_, nb, a.step, st_));
+          HIPCHK(rtm_launch_step(a, phase, cands[c], st_));
+          if (phase == PH_FWD) HIPCHK(rtm_launch_save(d_ds_, d_slots_, nb, a.step, st_));
         }
         HIPCHK(hipEventRecord(te[c + 1], st_));
       }
@@ -870,6 +863,8 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         step_fwd += 32.0 * nxz + 16.0 * pml + strip;
         step_rev += 32.0 * nxz + 16.0 * pml + inn * (32.0 + 16.0) + strip;
       }
+      fwdb += (double)nt * (32.0 * nxz + 16.0 * pml + strip);
+      revb += (double)(nt - 1) * (32.0 * nxz + 16.0 * pml + inn * (32.0 + 16.0) + strip);
     }
     // ---- stack in shot order (FP64); temp images every 50 shots
     std::vector<StackShotHost> stk(nb);
@@ -939,6 +934,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     stats->tile_z_rev = last_tz_r;
     stats->batch = maxb; stats->hbm_bytes_alg = algb;
     stats->step_bytes_fwd = step_fwd; stats->step_bytes_rev = step_rev;
+    stats->fwd_bytes_alg = fwdb; stats->rev_bytes_alg = revb;
   }
   return 0;
 }
@@ -1037,7 +1033,7 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   LaunchArgs a{d_ds, d_tiles, (int)tiles.size(), 0};
   for (int k = 0; k < nsteps; k++) {
     a.step = k;
-    HIPCHK(rtm_launch_step(a, PH_FWD, tz, 0, st_));
+    HIPCHK(rtm_launch_step(a, PH_FWD, tz, st_));
   }
   HIPCHK(hipStreamSynchronize(st_));
   for (int k = 0; k < 5; k++)

@@ -61,7 +61,7 @@ struct SlotMem {
 };
 
 struct GraphKey {
-  int ntiles, tz, phase, save, quant, k;
+  int ntiles, tz, phase, save, k;          // save: k_save after every (forward) step
   const void* shots;
   const void* tiles;
   bool operator<(const GraphKey& o) const;

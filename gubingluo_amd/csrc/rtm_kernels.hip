@@ -160,18 +160,6 @@ __device__ __forceinline__ int strip_block(const ShotDev& s, int off)
   return off < a ? 0 : (off < b ? 1 : (off < c ? 2 : 3));
 }
 
-__device__ __forceinline__ void store_put(const ShotDev& s, int it, int off, float v)
-{
-  if (s.store_fmt == ST_F32) {
-    GF(s.store)[(long long)it * s.strip_len + off] = v;
-  } else {
-    // fp32 staging of step it (by parity), scaled and quantised by k_quant
-    // after the step
-    gfloat* stg = GF(s.store) + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
-    stg[off] = v;
-  }
-}
-
 // A stored strip value is fetched with the first loads of a pass (raw: the
 // fp32 value, or the 16-bit code in the low bits) and decoded where it is
 // used, with the four block scales of its step from a small LDS table: the
@@ -381,7 +369,7 @@ __device__ __forceinline__ float pres_sum(const DivV& d, float dtx, float dtz)
 //              columns x0-10..x0-1, x0+64..x0+73 dealt over all threads
 //   p          own rows z0 + g*RPT + r of column x0+lx
 template <class Q, int FK>
-__device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDev& s, const Tile& tl, int it, int save,
+__device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDev& s, const Tile& tl, int it,
                                            const float* pc, const float* vxc, const float* vzc, float* pn, float* vxn,
                                            float* vzn, const float* __restrict__ ctab, const float (&ps)[Q::RPT],
                                            float (&pnew)[Q::RPT])
@@ -401,9 +389,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   auto halo_on = [&](int k) { return k * NT + g * 64 < 20 * TZ; };
   const int ix = x0 + lx;
   const bool gen = (tl.kind & TF_GEN) != 0;
-  // strip cells: the reconstruction restores those of the tile and its halo,
-  // the forward pass saves its own
-  const bool strips = (tl.kind & (FK == FK_RCN ? TF_STRIP : TF_SAVE)) != 0;
+  // strip cells that the reconstruction restores (the tile's and its halo's);
+  // the forward pass saves its strips in k_save after the step
+  const bool strips = FK == FK_RCN && (tl.kind & TF_STRIP) != 0;
   const int pr4 = s.pitch * 4;
   const int srow0 = (z0 + RTM_MZT) * pr4;             // byte offset of tile row 0
   const int vcol = (ix + RTM_MXL) * 4;                // byte offset of the lane's column
@@ -540,12 +528,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
                DMP[DZA + kz], DMP[DZB + kz], vzn_[k]);
 #undef PZA
       if (kz >= 10 && kz < 10 + TZ) {                  // own rows (wave-uniform)
-        auto out = [&]() {
-          bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
-          if (FK == FK_SRC && save && strips && svz >= 0) store_put(s, it, svz, vzn_[k]);
-        };
-        if (!gen) out();                               // every lane's cell exists
-        else if (ix < nxp && iz <= nzp - 2) out();
+        if (!gen) bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);   // every lane's cell exists
+        else if (ix < nxp && iz <= nzp - 2) bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
       }
     }
   }
@@ -596,12 +580,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
                (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], ibx[r], dtx,
                axv, bxv, vxn_[r]);
 #undef PCA
-      auto out = [&]() {
-        bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
-        if (FK == FK_SRC && save && strips && svx >= 0) store_put(s, it, svx, vxn_[r]);
-      };
-      if (!gen) out();
-      else if (iz < nzp && ix <= nxp - 2) out();
+      if (!gen) bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
+      else if (iz < nzp && ix <= nxp - 2) bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
@@ -707,10 +687,6 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
             const int ir = ix - s.rcol0;
             if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
           }
-          if (save && strips) {
-            const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
-            if (sp_ >= 0) store_put(s, it, sp_, p);
-          }
         };
         if (!gen) out();
         else if (incell) out();
@@ -745,7 +721,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 // ================================================================ kernels ==
 // forward time step of the source wavefield (modelling, recording, strips)
 template <int G, int RPT>
-__global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
+__global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a)
 {
   using Q = Geo<G, RPT>;
   __shared__ float lds[Q::LDS + Q::LDS_D];
@@ -760,7 +736,7 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a, int save)
   float none[Q::RPT], pnew[Q::RPT];
 #pragma unroll
   for (int r = 0; r < Q::RPT; r++) none[r] = 0.0f;
-  field_step<Q, FK_SRC>(lds, s, tl, it, save, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
+  field_step<Q, FK_SRC>(lds, s, tl, it, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
                         odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
                         odd ? s.svz[0] : s.svz[1], ctab, none, pnew);
 }
@@ -784,35 +760,92 @@ __global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
 #pragma unroll
   for (int r = 0; r < Q::RPT; r++) ps[r] = none[r] = 0.0f;
   if (tl.kind & TF_INT) {
-    field_step<Q, FK_RCN>(lds, s, tl, it, 0, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
+    field_step<Q, FK_RCN>(lds, s, tl, it, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
                           odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
                           odd ? s.svz[0] : s.svz[1], ctab, none, ps);
     __syncthreads();                                   // LDS reused by the receiver pass
   }
-  field_step<Q, FK_RCV>(lds, s, tl, it, 0, odd ? s.rp[1] : s.rp[0], odd ? s.rvx[1] : s.rvx[0],
+  field_step<Q, FK_RCV>(lds, s, tl, it, odd ? s.rp[1] : s.rp[0], odd ? s.rvx[1] : s.rvx[0],
                         odd ? s.rvz[1] : s.rvz[0], odd ? s.rp[0] : s.rp[1], odd ? s.rvx[0] : s.rvx[1],
                         odd ? s.rvz[0] : s.rvz[1], ctab, ps, pr);
 }
 
-// 16-bit boundary store: one workgroup per (slot, block) of step `it`
-// reduces the max-abs of the block's fp32 staging, keeps the scale
-// max(maxabs / 32768, FLT_MIN) (1 for an all-zero block) and stores q = x / scale
-// (a denormal max-abs, e.g. source tails before the wave arrives, must not
-// underflow the scale to zero: |x| <= 32768 * FLT_MIN then anyway)
-__global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
-                                                 int nslots, int step)
+// Boundary strips of forward step `it` -- the new fields of that step -- into
+// the store, packed as kernel.cu:1186-1293 per step:
+//   vx lr [2][nzi][10] | vz tb [2][10][nxl] | p lr [2][nzi][10] | p tb [2][10][nxl]
+// One workgroup per (slot, block).  fp32 is a copy; the 16-bit forms reduce
+// the block's max-abs, keep the scale max(maxabs / 32768, FLT_MIN) (1 for an
+// all-zero block; a denormal max-abs, e.g. source tails before the wave
+// arrives, must not underflow the scale to zero: |x| <= 32768 * FLT_MIN then
+// anyway) and store x / scale.
+struct StripBlock {
+  const float* f;    // field (cell (0,0)), new level
+  int lr;            // 1: [2][nzi][10] column strips, 0: [2][10][nxl] row strips
+  int lo, n;         // offset of the block in the step's record, element count
+  int a0, a1;        // first column (lr) / row (tb) of side 0 and side 1
+};
+__device__ __forceinline__ StripBlock strip_block_of(const ShotDev& s, int b, int lev)
+{
+  const int n = s.npml, e1 = 2 * RTM_R * s.nzi, e2 = e1 + 2 * RTM_R * s.nxl, e3 = e2 + 2 * RTM_R * s.nzi;
+  // (levels selected, not indexed: a run-time index would put s in scratch)
+  const float* vx = lev ? s.svx[1] : s.svx[0];
+  const float* vz = lev ? s.svz[1] : s.svz[0];
+  const float* p = lev ? s.sp[1] : s.sp[0];
+  StripBlock k;
+  if (b == 0) k = {vx, 1, 0, e1, n - RTM_R, s.nxp - n - 1};                  // strip_vx
+  else if (b == 1) k = {vz, 0, e1, e2 - e1, n - RTM_R, s.nzp - n - 1};        // strip_vz
+  else if (b == 2) k = {p, 1, e2, e3 - e2, n - RTM_R, s.nxp - n};             // strip_p lr
+  else k = {p, 0, e3, (int)s.strip_len - e3, n - RTM_R, s.nzp - n};            // strip_p tb
+  return k;
+}
+// element e of a block: its field value
+__device__ __forceinline__ float strip_elem(const ShotDev& s, const StripBlock& k, int e)
+{
+  if (k.lr) {                                          // e = (side*nzi + r)*10 + c
+    const int q = e / RTM_R, c = e - q * RTM_R;
+    const int side = q >= s.nzi, r = q - side * s.nzi;
+    return k.f[(size_t)(s.npml + r) * s.pitch + (side ? k.a1 : k.a0) + c];
+  }
+  const int q = e / s.nxl, c = e - q * s.nxl;          // e = (side*10 + r)*nxl + c
+  const int side = q >= RTM_R, r = q - side * RTM_R;
+  return k.f[(size_t)((side ? k.a1 : k.a0) + r) * s.pitch + s.npml + c];
+}
+// A step's strip record is small (2*40(nz+nxl) values per shot), so the save
+// is latency-bound: every thread keeps SV loads in flight.  fp32 blocks are
+// split over gridDim.y workgroups; a 16-bit block is one workgroup (its
+// max-abs must be known before any value is stored).
+constexpr int SV = 8;
+__global__ __launch_bounds__(NTHR) void k_save(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                int nslots, int step)
 {
   __shared__ float red[NTHR / 64];
   const int k = blockIdx.x >> 2, b = blockIdx.x & 3;
   if (k >= nslots) return;
   const ShotDev s = shots[slots[k]];
   const int it = s.it_off + s.it_dir * step;
-  const float* stg = (const float*)s.store + (long long)s.nt * s.strip_len / 2 + (long long)(it & 1) * s.strip_len;
-  const int e1 = 2 * RTM_R * s.nzi, e2 = e1 + 2 * RTM_R * s.nxl, e3 = e2 + 2 * RTM_R * s.nzi;
-  const int lo = b == 0 ? 0 : (b == 1 ? e1 : (b == 2 ? e2 : e3));
-  const int hi = b == 0 ? e1 : (b == 1 ? e2 : (b == 2 ? e3 : (int)s.strip_len));
+  const StripBlock blk = strip_block_of(s, b, (it + 1) & 1);
+  const long long base = (long long)it * s.strip_len + blk.lo;
+  if (s.store_fmt == ST_F32) {
+    const int stride = gridDim.y * NTHR * SV;
+    for (int e0 = (blockIdx.y * NTHR + threadIdx.x) * SV; e0 < blk.n; e0 += stride) {
+      float v[SV];
+#pragma unroll
+      for (int u = 0; u < SV; u++) v[u] = e0 + u < blk.n ? strip_elem(s, blk, e0 + u) : 0.0f;
+#pragma unroll
+      for (int u = 0; u < SV; u++)
+        if (e0 + u < blk.n) GF(s.store)[base + e0 + u] = v[u];
+    }
+    return;
+  }
+  if (blockIdx.y != 0) return;
   float m = 0.0f;
-  for (int e = lo + (int)threadIdx.x; e < hi; e += NTHR) m = fmaxf(m, fabsf(stg[e]));
+  for (int e0 = threadIdx.x; e0 < blk.n; e0 += NTHR * SV) {
+    float v[SV];
+#pragma unroll
+    for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < SV; u++) m = fmaxf(m, fabsf(v[u]));
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -822,15 +855,22 @@ __global__ __launch_bounds__(NTHR) void k_quant(const ShotDev* __restrict__ shot
   for (int w = 1; w < NTHR / 64; w++) m = fmaxf(m, red[w]);
   const float sc = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
   if (threadIdx.x == 0) s.sscale[(long long)it * 4 + b] = sc;
-  const long long base = (long long)it * s.strip_len;
-  for (int e = lo + (int)threadIdx.x; e < hi; e += NTHR) {
-    const float q = stg[e] / sc;
-    if (s.store_fmt == ST_F16) {
-      ((__half*)s.store)[base + e] = __float2half_rn(q);
-    } else {
-      unsigned int u = __float_as_uint(q);            // bf16, round to nearest even
-      u += 0x7FFFu + ((u >> 16) & 1u);
-      ((unsigned short*)s.store)[base + e] = (unsigned short)(u >> 16);
+  for (int e0 = threadIdx.x; e0 < blk.n; e0 += NTHR * SV) {
+    float v[SV];
+#pragma unroll
+    for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < SV; u++) {
+      const int e = e0 + u * NTHR;
+      if (e >= blk.n) continue;
+      const float q = v[u] / sc;
+      if (s.store_fmt == ST_F16) {
+        ((__half*)s.store)[base + e] = __float2half_rn(q);
+      } else {
+        unsigned int w = __float_as_uint(q);           // bf16, round to nearest even
+        w += 0x7FFFu + ((w >> 16) & 1u);
+        ((unsigned short*)s.store)[base + e] = (unsigned short)(w >> 16);
+      }
     }
   }
 }
@@ -960,29 +1000,30 @@ __global__ void k_bump(ShotDev* __restrict__ shots, const int* __restrict__ slot
 static inline int grid8(int ntiles) { return ((ntiles + 7) / 8) * 8; }
 
 template <int G, int RPT>
-static hipError_t launch_step_t(const LaunchArgs& a, int phase, int save, hipStream_t st)
+static hipError_t launch_step_t(const LaunchArgs& a, int phase, hipStream_t st)
 {
   const dim3 g(grid8(a.ntiles)), b(64 * G);
-  if (phase == PH_FWD) hipLaunchKernelGGL((k_fwd<G, RPT>), g, b, 0, st, a, save);
+  if (phase == PH_FWD) hipLaunchKernelGGL((k_fwd<G, RPT>), g, b, 0, st, a);
   else hipLaunchKernelGGL((k_rev<G, RPT>), g, b, 0, st, a);
   return hipGetLastError();
 }
 
-hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st)
+hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t st)
 {
   if (a.ntiles <= 0) return hipSuccess;
   switch (tz) {
-    case 16: return launch_step_t<4, 4>(a, phase, save, st);
-    case 32: return launch_step_t<4, 8>(a, phase, save, st);
-    case 64: return launch_step_t<8, 8>(a, phase, save, st);
+    case 16: return launch_step_t<4, 4>(a, phase, st);
+    case 32: return launch_step_t<4, 8>(a, phase, st);
+    case 64: return launch_step_t<8, 8>(a, phase, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st)
+hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st)
 {
   if (nslots <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_quant, dim3(4 * nslots), dim3(NTHR), 0, st, shots, slots, nslots, step);
+  // fp32 blocks of up to 2*10*nz values: 4 workgroups each (16-bit: 1 works)
+  hipLaunchKernelGGL(k_save, dim3(4 * nslots, 4), dim3(NTHR), 0, st, shots, slots, nslots, step);
   return hipGetLastError();
 }
 

@@ -4,12 +4,13 @@
 #include "rtm_dev.h"
 
 // one time step of the tiles in `a` (one launch): phase PH_FWD (source
-// modelling; save=1 writes the boundary strips) or PH_REV (reconstruction,
-// receiver wavefield, imaging); tz the tile height (16, 32 or 64)
-hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, int save, hipStream_t st);
-// 16-bit boundary store: scale and quantise the strips of step `step` of the
-// listed slots (one workgroup per slot and strip block)
-hipError_t rtm_launch_quant(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st);
+// modelling) or PH_REV (reconstruction, receiver wavefield, imaging); tz the
+// tile height (16, 32 or 64)
+hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t st);
+// after a forward step: its boundary strips into the store of the listed
+// slots (fp32 copy, or 16-bit with per-step block scales; one workgroup per
+// slot and strip block)
+hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st);
 hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st);
 hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st);
 hipError_t rtm_launch_stack(const void* stack_shots, int nsh, int nx, int nz, double* i1, double* i2, double* il,

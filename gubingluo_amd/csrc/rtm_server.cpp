@@ -319,7 +319,7 @@ void Server::rebuild_tiles()
   for (int i = 0; i < (int)slots_.size(); i++) {
     if (slots_[i].phase == PH_FWD) {
       fw.push_back(hds_[i]); fid.push_back(i);
-      if (opt_.store_fmt != ST_F32) qslots_.push_back(i);
+      qslots_.push_back(i);                           // strips saved after each forward step
     } else if (slots_[i].phase == PH_REV) {
       rv.push_back(hds_[i]); rid.push_back(i);
     }
@@ -486,13 +486,13 @@ void Server::loop()
     if (err_) { cv_done_.notify_all(); std::this_thread::sleep_for(std::chrono::milliseconds(5)); continue; }
     if (ntf_ > 0) {
       LaunchArgs a{d_ds_, d_tf_, ntf_, (int)tick_};
-      if (rtm_launch_step(a, PH_FWD, opt_.tile_z, 1, st_) != hipSuccess) fail("forward launch", hipGetLastError());
-      if (nq_ > 0) (void)rtm_launch_quant(d_ds_, d_qs_, nq_, (int)tick_, st_);
+      if (rtm_launch_step(a, PH_FWD, opt_.tile_z, st_) != hipSuccess) fail("forward launch", hipGetLastError());
+      if (nq_ > 0) (void)rtm_launch_save(d_ds_, d_qs_, nq_, (int)tick_, st_);
       launches_ += 2;
     }
     if (ntr_ > 0) {
       LaunchArgs a{d_ds_, d_tr_, ntr_, (int)tick_};
-      if (rtm_launch_step(a, PH_REV, opt_.tile_z, 0, st_) != hipSuccess) fail("reverse launch", hipGetLastError());
+      if (rtm_launch_step(a, PH_REV, opt_.tile_z, st_) != hipSuccess) fail("reverse launch", hipGetLastError());
       launches_ += 2;
     }
     tick_++;

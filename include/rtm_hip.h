@@ -101,14 +101,16 @@ typedef struct rtmh_stats {
   double cell_updates;      /* 2*nt*nxp*nzp + (nt-1)*nz*nxl summed over shots     */
   int nshots, batches, tile_z, batch;
   double hbm_bytes_alg;     /* algorithmic bytes of the propagation passes        */
-  /* profile=1 only: per kernel (0 vel_fwd, 1 pres_fwd, 2 vel_rev, 3 pres_rev)
-   * summed launch time and launch count */
+  /* profile=1 only: per kernel (0 forward step, 1 strip save after it,
+   * 2 reverse step, 3 unused) summed launch time and launch count */
   double kern_ms[4];
   int kern_calls[4];
   double step_bytes_fwd;    /* algorithmic bytes of one forward step of the batch */
   double step_bytes_rev;    /* ... of one reverse step                            */
   int tile_z_rev;           /* tile height of the reverse pass (tile_z: forward)  */
   int _pad0;
+  double fwd_bytes_alg;     /* algorithmic bytes of all forward steps (all batches) */
+  double rev_bytes_alg;     /* ... of all reverse steps                           */
 } rtmh_stats;
 
 /* last error message of this thread ("" if none) */
