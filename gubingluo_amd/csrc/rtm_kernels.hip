@@ -170,10 +170,11 @@ __device__ __forceinline__ float store_raw(const ShotDev& s, int it, int off)
   if (s.store_fmt == ST_F32) return GCF(s.store)[k];
   return __uint_as_float((unsigned int)((const gushort*)s.store)[k]);
 }
-__device__ __forceinline__ float store_decode(const ShotDev& s, int off, float raw, const float* sc4)
+// blk: the strip block of the value (0 vx, 1 vz, 2 p-lr, 3 p-tb)
+__device__ __forceinline__ float store_decode(const ShotDev& s, int blk, float raw, const float* sc4)
 {
   if (s.store_fmt == ST_F32) return raw;
-  const float sc = sc4[strip_block(s, off)];
+  const float sc = sc4[blk];
   const unsigned int h = __float_as_uint(raw);
   if (s.store_fmt == ST_F16) return __half2float(__ushort_as_half((unsigned short)h)) * sc;
   return __uint_as_float(h << 16) * sc;
@@ -302,8 +303,8 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
       v_ = __builtin_fmaf(acc * (DTH), (IB), (VO));                                                        \
       if ((gen) && !(INTERIOR)) v_ = 0.0f;                                                                 \
       if (strips) {                                                                                        \
-        const int so_ = (SOFF);                                                                            \
-        if (so_ >= 0) v_ = store_decode(s, so_, (VO), DMP);                                                \
+        const int sb_ = (SOFF);  /* strip block of a strip cell, -1 elsewhere */                            \
+        if (sb_ >= 0) v_ = store_decode(s, sb_, (VO), DMP);                                                \
       }                                                                                                    \
     } else if (!(gen)) {                                                                                   \
       float acc = 0.0f;                                                                                    \
@@ -462,28 +463,34 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   }
   // reconstruction of a strip tile: the stored value of a strip cell
   // replaces its old velocity, fetched now with the rest (after the old
-  // value's load, which it overwrites) and decoded where it is used
+  // value's load, which it overwrites) and decoded where it is used; which
+  // cells are strip cells is kept as bits of sm (vz rows, vx rows, halo vx,
+  // p rows, p rows in the top/bottom block), so the offsets are computed once
   StripCols sc{-1, -1, -1};
   if (strips) sc = strip_cols(s, ix);
   const bool rstrips = FK == FK_RCN && strips;
+  constexpr int SMX = KZ, SMH = KZ + RPT, SMP = KZ + RPT + KH, SMT = KZ + RPT + KH + RPT;
+  static_assert(SMT + RPT <= 64, "strip bits");
+  unsigned long long sm = 0;
   if (rstrips) {
 #pragma unroll
     for (int k = 0; k < KZ; k++) {
       const int kz = g * KZ + k;
       const int st = kz < ZR ? soff_vz(strip_row_vz(s, z0 - 10 + kz), sc) : -1;
-      if (st >= 0) vzo[k] = store_raw(s, it + 1, st);
+      if (st >= 0) { vzo[k] = store_raw(s, it + 1, st); sm |= 1ull << k; }
     }
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       const int st = soff_vx(strip_row_lr(s, z0 + g * RPT + r), sc);
-      if (st >= 0) vxo[r] = store_raw(s, it + 1, st);
+      if (st >= 0) { vxo[r] = store_raw(s, it + 1, st); sm |= 1ull << (SMX + r); }
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
       const int st = halo_on(k) ? strip_vx(s, z0 + hrow[k], x0 - 10 + hcol[k]) : -1;
-      if (st >= 0) hvo[k] = store_raw(s, it + 1, st);
+      if (st >= 0) { hvo[k] = store_raw(s, it + 1, st); sm |= 1ull << (SMH + k); }
     }
   }
+  auto sbit = [&](int b) { return ((sm >> b) & 1ull) != 0; };
   // per-lane column data of the generic path: existence, radius, damping;
   // the damping of the tile's rows and halo columns goes to an LDS table, so
   // that the compute phases wait for no small loads
@@ -522,8 +529,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       const int kz = g * KZ + k;
       const int iz = z0 - 10 + kz;                     // wave-uniform
 #define PZA(i) cw[k + 9 + (i)]
-      const int svz = strips ? soff_vz(strip_row_vz(s, iz), sc) : -1;
-      VEL_CELL(FK, gen, strips, svz, (iz >= 0 && iz <= nzp - 2), imin(RTM_R, imin(iz + 1, nzp - 1 - iz)),
+      VEL_CELL(FK, gen, strips, (sbit(k) ? 1 : -1), (iz >= 0 && iz <= nzp - 2), imin(RTM_R, imin(iz + 1, nzp - 1 - iz)),
                (ix >= npml && ix < nxp - npml && iz >= npml && iz <= nzp - npml - 2), PZA, vzo[k], ibz[k], dtz,
                DMP[DZA + kz], DMP[DZB + kz], vzn_[k]);
 #undef PZA
@@ -563,8 +569,13 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         qb[r] = GCF(s.illum)[ii];
       }
       if (rstrips) {                                   // reconstruction: stored p of strip cells
-        const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
-        if (sp_ >= 0) qa[r] = store_raw(s, it, sp_);
+        const int lr = strip_row_lr(s, iz);
+        const int sp_ = soff_p(lr, strip_row_ptb(s, iz), sc);
+        if (sp_ >= 0) {
+          qa[r] = store_raw(s, it, sp_);
+          sm |= 1ull << (SMP + r);
+          if (lr < 0) sm |= 1ull << (SMT + r);         // top/bottom block (rows outside n..nzp-n-1)
+        }
       }
     }
   }
@@ -575,8 +586,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       const int j = g * RPT + r, iz = z0 + j;
       const float* __restrict__ Pr = lds_base(PS + (j + 19) * PW + lx + 11) + 9;   // p at column ix
 #define PCA(i) Pr[(i)]
-      const int svx = strips ? soff_vx(strip_row_lr(s, iz), sc) : -1;
-      VEL_CELL(FK, gen, strips, svx, vxv, rrx,
+      VEL_CELL(FK, gen, strips, (sbit(SMX + r) ? 0 : -1), vxv, rrx,
                (ix >= npml && ix <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, vxo[r], ibx[r], dtx,
                axv, bxv, vxn_[r]);
 #undef PCA
@@ -591,7 +601,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       if (halo_on(k)) {
         const bool hv = jx >= 0 && jx <= nxp - 2;
 #define PCA(i) Pr[(i)]
-        VEL_CELL(FK, gen, strips, strip_vx(s, iz, jx), hv, (hv ? imin(RTM_R, imin(jx + 1, nxp - 1 - jx)) : RTM_R),
+        VEL_CELL(FK, gen, strips, (sbit(SMH + k) ? 0 : -1), hv, (hv ? imin(RTM_R, imin(jx + 1, nxp - 1 - jx)) : RTM_R),
                  (jx >= npml && jx <= nxp - npml - 2 && iz >= npml && iz < nzp - npml), PCA, hvo[k], hib[k], dtx,
                  DMP[DXA + hcol[k]], DMP[DXB + hcol[k]], hvn[k]);
 #undef PCA
@@ -640,12 +650,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       const float pc = __builtin_fmaf(kk[r], pres_sum(d, dtx, dtz), p);   // kernel.cu:389-429
       bool wr = !gen || inner;
       if (wr) p = pc;
-      if (strips) {
-        const int sp_ = soff_p(strip_row_lr(s, iz), strip_row_ptb(s, iz), sc);
-        if (sp_ >= 0) {
-          p = store_decode(s, sp_, qa[r], DMP + 4);    // kernel.cu:1348-1400
-          wr = true;
-        }
+      if (strips && sbit(SMP + r)) {
+        p = store_decode(s, sbit(SMT + r) ? 3 : 2, qa[r], DMP + 4);   // kernel.cu:1348-1400
+        wr = true;
       }
       if ((tl.kind & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && wr && xsrc)   // source removal (rtm_model.cu:437)
         p = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
@@ -744,7 +751,7 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a)
 // reverse time step: source reconstruction, then the receiver wavefield and
 // the imaging products of the tile
 template <int G, int RPT>
-__global__ __launch_bounds__(64 * G) void k_rev(LaunchArgs a)
+__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(G == 4 && RPT == 4 ? 5 : 1))) void k_rev(LaunchArgs a)
 {
   using Q = Geo<G, RPT>;
   __shared__ float lds[Q::LDS + Q::LDS_D];
