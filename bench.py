@@ -115,7 +115,7 @@ def roofline_of(recs, P, copy_gbs=None):
             "traffic": None if ratio is None else round(ratio * step_bytes),
             "traffic_source": None if ratio is None else
             f"HBM/algorithmic bytes {ratio} of k_rev from rocprofv3 FETCH_SIZE+WRITE_SIZE "
-            "(profiles/k_fwd_k_rev_cfg2_20shots_traffic.md) times this run's algorithmic bytes",
+            "(profiles/k_fwd_k_rev_cfg2_200shots_traffic.md) times this run's algorithmic bytes",
             "copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
             "kernel": "k_rev (reverse time step: reconstruction + receiver wavefield + imaging)",
             "avg_launch_us": round(avg_step_s * 1e6, 2), "alg_bytes_per_launch": step_bytes,
