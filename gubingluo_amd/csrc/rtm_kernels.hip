@@ -833,14 +833,16 @@ __global__ __launch_bounds__(NTHR) void k_save(const ShotDev* __restrict__ shots
   const StripBlock blk = strip_block_of(s, b, (it + 1) & 1);
   const long long base = (long long)it * s.strip_len + blk.lo;
   if (s.store_fmt == ST_F32) {
+    // lane-contiguous elements: every load and store of a wave covers 64
+    // consecutive values of the record
     const int stride = gridDim.y * NTHR * SV;
-    for (int e0 = (blockIdx.y * NTHR + threadIdx.x) * SV; e0 < blk.n; e0 += stride) {
+    for (int e0 = blockIdx.y * NTHR * SV + threadIdx.x; e0 < blk.n; e0 += stride) {
       float v[SV];
 #pragma unroll
-      for (int u = 0; u < SV; u++) v[u] = e0 + u < blk.n ? strip_elem(s, blk, e0 + u) : 0.0f;
+      for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
 #pragma unroll
       for (int u = 0; u < SV; u++)
-        if (e0 + u < blk.n) GF(s.store)[base + e0 + u] = v[u];
+        if (e0 + u * NTHR < blk.n) GF(s.store)[base + e0 + u * NTHR] = v[u];
     }
     return;
   }
@@ -1020,7 +1022,10 @@ hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t s
   if (a.ntiles <= 0) return hipSuccess;
   switch (tz) {
     case 16: return launch_step_t<4, 4>(a, phase, st);
-    case 32: return launch_step_t<4, 8>(a, phase, st);
+    // 32 rows: the forward step as 8 waves x 4 rows (76 registers, 6 waves
+    // per SIMD), the reverse step as 4 waves x 8 rows (its 8 x 4 form needs
+    // 91 registers and loses 15 %)
+    case 32: return phase == PH_FWD ? launch_step_t<8, 4>(a, phase, st) : launch_step_t<4, 8>(a, phase, st);
     case 64: return launch_step_t<8, 8>(a, phase, st);
     default: return hipErrorInvalidValue;
   }
