@@ -87,7 +87,7 @@ struct ShotDev {
   const int* cut;              // mute: samples it <= cut[ir] are zero
   // boundary store
   void* store;
-  float* sscale;               // nt x 4 scales (16-bit formats)
+  float* sscale;               // nt x 4 scales (16-bit formats), then 8 max-abs accumulators
   long long strip_len;         // values per step
   int store_fmt, _pad4;
 };

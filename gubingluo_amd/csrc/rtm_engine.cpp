@@ -259,6 +259,7 @@ bool GraphKey::operator<(const GraphKey& o) const
   if (tz != o.tz) return tz < o.tz;
   if (phase != o.phase) return phase < o.phase;
   if (save != o.save) return save < o.save;
+  if (fmt != o.fmt) return fmt < o.fmt;
   if (k != o.k) return k < o.k;
   if (shots != o.shots) return shots < o.shots;
   return tiles < o.tiles;
@@ -322,7 +323,7 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
   size_t b = small_layout(s).total + kSlotFields * F + 2 * I;
   b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
   b += align256((size_t)nt * strip * store_elem(store_fmt));
-  b += align256((size_t)nt * 4 * 4);
+  b += align256((size_t)nt * 4 * 4 + 8 * 4);         // scales + 16-bit max-abs accumulators
   return b;
 }
 
@@ -375,7 +376,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.store_fmt = store_fmt;
   d.store = base + o;
   o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt));
-  d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4);
+  d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4 + 8 * 4);
 }
 
 hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
@@ -389,7 +390,7 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
   for (int s = 0; s < k.k; s++) {
     a.step = s;
     (void)rtm_launch_step(a, k.phase, k.tz, st_);
-    if (k.save) (void)rtm_launch_save(dshots, dslots, nslots, s, st_);
+    if (k.save) (void)rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_);
   }
   (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
   if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) return nullptr;
@@ -420,7 +421,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
       HIPCHK(hipEventRecord(ev[0], st_));
       HIPCHK(rtm_launch_step(a, phase, tz, st_));
       HIPCHK(hipEventRecord(ev[1], st_));
-      if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, st_));
+      if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
       HIPCHK(hipEventRecord(ev[2], st_));
       HIPCHK(hipEventSynchronize(ev[2]));
       float m0 = 0, m1 = 0;
@@ -433,7 +434,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     return 0;
   }
   if (chunk > 0 && nsteps >= chunk) {
-    GraphKey key{ntiles, tz, phase, save, chunk, dshots, dtiles};
+    GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, dshots, dtiles};
     hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
     if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
     const int nch = nsteps / chunk;
@@ -444,7 +445,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   for (int s = 0; s < nsteps - done; s++) {
     a.step = s;
     HIPCHK(rtm_launch_step(a, phase, tz, st_));
-    if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, st_));
+    if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
   }
   return 0;
 }
@@ -727,7 +728,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         for (int s2 = 0; s2 < kTuneSteps; s2++) {
           a.step = c * kTuneSteps + s2;
           HIPCHK(rtm_launch_step(a, phase, cands[c], st_));
-          if (phase == PH_FWD) HIPCHK(rtm_launch_save(d_ds_, d_slots_, nb, a.step, st_));
+          if (phase == PH_FWD) HIPCHK(rtm_launch_save(d_ds_, d_slots_, nb, a.step, opt.store_fmt, st_));
         }
         HIPCHK(hipEventRecord(te[c + 1], st_));
       }

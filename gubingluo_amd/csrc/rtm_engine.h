@@ -61,7 +61,7 @@ struct SlotMem {
 };
 
 struct GraphKey {
-  int ntiles, tz, phase, save, k;          // save: k_save after every (forward) step
+  int ntiles, tz, phase, save, fmt, k;     // save: strip save after every (forward) step, fmt its store format
   const void* shots;
   const void* tiles;
   bool operator<(const GraphKey& o) const;
@@ -120,7 +120,7 @@ class Engine {
   std::map<GraphKey, hipGraphExec_t> graphs_;
   std::map<int, Model> models_;
   std::map<long long, int> tuned_;      // geometry class -> tile height
-  static constexpr int kTuneSteps = 8;
+  static constexpr int kTuneSteps = 12;
   mutable std::mutex models_mu_;
   int next_model_ = 1;
   // persistent control block: captured graphs reference these addresses

@@ -817,38 +817,54 @@ __device__ __forceinline__ float strip_elem(const ShotDev& s, const StripBlock& 
   const int side = q >= RTM_R, r = q - side * RTM_R;
   return k.f[(size_t)((side ? k.a1 : k.a0) + r) * s.pitch + s.npml + c];
 }
-// A step's strip record is small (2*40(nz+nxl) values per shot), so the save
-// is latency-bound: every thread keeps SV loads in flight.  fp32 blocks are
-// split over gridDim.y workgroups; a 16-bit block is one workgroup (its
-// max-abs must be known before any value is stored).
+// A step's strip record is small (2*40(nz+nxl) values per shot), so a save
+// is latency-bound: every thread keeps SV loads in flight, and each block is
+// split over gridDim.y workgroups with lane-contiguous elements (every load
+// and store of a wave covers 64 consecutive values of the record).
 constexpr int SV = 8;
+__device__ __forceinline__ int save_it(const ShotDev& s, int step)
+{
+  return step < 0 ? -step - 1 : s.it_off + s.it_dir * step;   // step < 0: absolute -step-1
+}
+// 16-bit forms: per slot two max-abs accumulators (step parity) after the
+// nt x 4 scales, cleared by the phase start and by the previous step's save
+__device__ __forceinline__ unsigned int* save_acc(const ShotDev& s) { return (unsigned int*)(s.sscale + (size_t)s.nt * 4); }
+
+// fp32: copy block b of the slot's step into the store
 __global__ __launch_bounds__(NTHR) void k_save(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
                                                 int nslots, int step)
+{
+  const int k = blockIdx.x >> 2, b = blockIdx.x & 3;
+  if (k >= nslots) return;
+  const ShotDev s = shots[slots[k]];
+  const int it = save_it(s, step);
+  const StripBlock blk = strip_block_of(s, b, (it + 1) & 1);
+  const long long base = (long long)it * s.strip_len + blk.lo;
+  const int stride = gridDim.y * NTHR * SV;
+  for (int e0 = blockIdx.y * NTHR * SV + threadIdx.x; e0 < blk.n; e0 += stride) {
+    float v[SV];
+#pragma unroll
+    for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < SV; u++)
+      if (e0 + u * NTHR < blk.n) GF(s.store)[base + e0 + u * NTHR] = v[u];
+  }
+}
+
+// 16-bit, pass 1: max-abs of the workgroup's share of block b, one atomic
+// max per workgroup (non-negative floats order as their bit patterns)
+__global__ __launch_bounds__(NTHR) void k_save_max(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                    int nslots, int step)
 {
   __shared__ float red[NTHR / 64];
   const int k = blockIdx.x >> 2, b = blockIdx.x & 3;
   if (k >= nslots) return;
   const ShotDev s = shots[slots[k]];
-  const int it = s.it_off + s.it_dir * step;
+  const int it = save_it(s, step);
   const StripBlock blk = strip_block_of(s, b, (it + 1) & 1);
-  const long long base = (long long)it * s.strip_len + blk.lo;
-  if (s.store_fmt == ST_F32) {
-    // lane-contiguous elements: every load and store of a wave covers 64
-    // consecutive values of the record
-    const int stride = gridDim.y * NTHR * SV;
-    for (int e0 = blockIdx.y * NTHR * SV + threadIdx.x; e0 < blk.n; e0 += stride) {
-      float v[SV];
-#pragma unroll
-      for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
-#pragma unroll
-      for (int u = 0; u < SV; u++)
-        if (e0 + u * NTHR < blk.n) GF(s.store)[base + e0 + u * NTHR] = v[u];
-    }
-    return;
-  }
-  if (blockIdx.y != 0) return;
+  const int stride = gridDim.y * NTHR * SV;
   float m = 0.0f;
-  for (int e0 = threadIdx.x; e0 < blk.n; e0 += NTHR * SV) {
+  for (int e0 = blockIdx.y * NTHR * SV + threadIdx.x; e0 < blk.n; e0 += stride) {
     float v[SV];
 #pragma unroll
     for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
@@ -859,12 +875,34 @@ __global__ __launch_bounds__(NTHR) void k_save(const ShotDev* __restrict__ shots
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  m = red[0];
-#pragma unroll
-  for (int w = 1; w < NTHR / 64; w++) m = fmaxf(m, red[w]);
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NTHR / 64; w++) m = fmaxf(m, red[w]);
+    atomicMax(save_acc(s) + (it & 1) * 4 + b, __float_as_uint(m));
+  }
+}
+
+// 16-bit, pass 2: the scale max(maxabs / 32768, FLT_MIN) (1 for an all-zero
+// block; a denormal max-abs, e.g. source tails before the wave arrives, must
+// not underflow the scale to zero: |x| <= 32768 * FLT_MIN then anyway), and
+// the workgroup's share of the block stored as x / scale
+__global__ __launch_bounds__(NTHR) void k_save_q(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                  int nslots, int step)
+{
+  const int k = blockIdx.x >> 2, b = blockIdx.x & 3;
+  if (k >= nslots) return;
+  const ShotDev s = shots[slots[k]];
+  const int it = save_it(s, step);
+  const StripBlock blk = strip_block_of(s, b, (it + 1) & 1);
+  const long long base = (long long)it * s.strip_len + blk.lo;
+  unsigned int* acc = save_acc(s);
+  const float m = __uint_as_float(acc[(it & 1) * 4 + b]);
   const float sc = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
-  if (threadIdx.x == 0) s.sscale[(long long)it * 4 + b] = sc;
-  for (int e0 = threadIdx.x; e0 < blk.n; e0 += NTHR * SV) {
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
+    s.sscale[(long long)it * 4 + b] = sc;
+    acc[((it + 1) & 1) * 4 + b] = 0u;                  // the next step's accumulator
+  }
+  const int stride = gridDim.y * NTHR * SV;
+  for (int e0 = blockIdx.y * NTHR * SV + threadIdx.x; e0 < blk.n; e0 += stride) {
     float v[SV];
 #pragma unroll
     for (int u = 0; u < SV; u++) v[u] = e0 + u * NTHR < blk.n ? strip_elem(s, blk, e0 + u * NTHR) : 0.0f;
@@ -903,6 +941,7 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
     for (int k = 0; k < 14; k++)
       if (k >= first) (f[k] - s.forg)[e] = 0.0f;
   if (phase == PH_FWD) {
+    if (s.store_fmt != ST_F32 && g0 < 8) save_acc(s)[g0] = 0u;   // 16-bit max-abs accumulators
     for (long long e = g0; e < n; e += gs) {
       const int row = (int)(e / s.pitch), col = (int)(e - (long long)row * s.pitch);
       const int iz = row - RTM_MZT, ix = col - RTM_MXL;
@@ -1031,11 +1070,18 @@ hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t s
   }
 }
 
-hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st)
+hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, int store_fmt,
+                           hipStream_t st)
 {
   if (nslots <= 0) return hipSuccess;
-  // fp32 blocks of up to 2*10*nz values: 4 workgroups each (16-bit: 1 works)
-  hipLaunchKernelGGL(k_save, dim3(4 * nslots, 4), dim3(NTHR), 0, st, shots, slots, nslots, step);
+  // blocks of up to 2*10*nz values: 4 workgroups each (fp32); the 16-bit
+  // passes use 8 (they run on the few-shot launches of the served form too)
+  if (store_fmt == ST_F32) {
+    hipLaunchKernelGGL(k_save, dim3(4 * nslots, 4), dim3(NTHR), 0, st, shots, slots, nslots, step);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_save_max, dim3(4 * nslots, 8), dim3(NTHR), 0, st, shots, slots, nslots, step);
+  hipLaunchKernelGGL(k_save_q, dim3(4 * nslots, 8), dim3(NTHR), 0, st, shots, slots, nslots, step);
   return hipGetLastError();
 }
 

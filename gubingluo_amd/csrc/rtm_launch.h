@@ -10,7 +10,8 @@ hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t s
 // after a forward step: its boundary strips into the store of the listed
 // slots (fp32 copy, or 16-bit with per-step block scales; one workgroup per
 // slot and strip block)
-hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, hipStream_t st);
+hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, int store_fmt,
+                           hipStream_t st);
 hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st);
 hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st);
 hipError_t rtm_launch_stack(const void* stack_shots, int nsh, int nx, int nz, double* i1, double* i2, double* il,

@@ -487,7 +487,7 @@ void Server::loop()
     if (ntf_ > 0) {
       LaunchArgs a{d_ds_, d_tf_, ntf_, (int)tick_};
       if (rtm_launch_step(a, PH_FWD, opt_.tile_z, st_) != hipSuccess) fail("forward launch", hipGetLastError());
-      if (nq_ > 0) (void)rtm_launch_save(d_ds_, d_qs_, nq_, (int)tick_, st_);
+      if (nq_ > 0) (void)rtm_launch_save(d_ds_, d_qs_, nq_, (int)tick_, opt_.store_fmt, st_);
       launches_ += 2;
     }
     if (ntr_ > 0) {
