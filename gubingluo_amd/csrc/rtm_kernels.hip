@@ -51,6 +51,19 @@ namespace {
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
+// The slot descriptor of a step kernel, read through the constant address
+// space: no kernel writes a descriptor while a step runs (k_bump advances the
+// time origin between launches), so its fields may come from scalar loads and
+// be loaded again where needed.  A by-value copy instead has to stay live in
+// scalar registers across the whole step (later stores could alias the
+// source), which overflows them into vector lanes: about a thousand
+// v_readlane reloads in the reverse step.
+typedef __attribute__((address_space(4))) const ShotDev const_shot_t;
+__device__ __forceinline__ const ShotDev& shot_of(const ShotDev* shots, int slot)
+{
+  return *(const ShotDev*)((const_shot_t*)shots + slot);
+}
+
 // blockIdx -> tile: workgroups are dealt to the 8 XCDs round-robin, so give
 // each XCD a contiguous run of the tile table (neighbouring tiles of one shot
 // share their halos through that XCD's L2).
@@ -736,7 +749,7 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a)
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
-  const ShotDev s = a.shots[tl.slot];
+  const ShotDev& s = shot_of(a.shots, tl.slot);
   if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;
   const bool odd = it & 1;                             // (a runtime index into s would spill it to scratch)
@@ -759,7 +772,7 @@ __global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(G == 4 &
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
-  const ShotDev s = a.shots[tl.slot];
+  const ShotDev& s = shot_of(a.shots, tl.slot);
   if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;
   const bool odd = it & 1;
