@@ -211,7 +211,7 @@ struct Geo {
   static constexpr int KH = (20 * TZ + NT - 1) / NT;   // halo vx cells per thread
   static_assert((20 * TZ) % 64 == 0, "halo cells fill whole waves");
   static constexpr int LDS_A = PZR * PW;               // p rows z0-19.. (x halo on the tile rows)
-  static constexpr int LDS_B = TZ * VXW + ZR * 64;     // VXL | VZL
+  static constexpr int LDS_B = TZ * VXW + ZR * 64 + TZ * 64;   // VXL | VZL | SXL
   static constexpr int LDS = LDS_A > LDS_B ? LDS_A : LDS_B;
   // damping factors of the tile (after LDS; written once per pass): a, b at
   // the vz rows, at the vx columns x0-10..x0+73, and at the own rows
@@ -227,6 +227,11 @@ typedef __amdgpu_buffer_rsrc_t rsrc_t;
 __device__ __forceinline__ rsrc_t field_rsrc(const float* f00, const ShotDev& s)
 {
   return __builtin_amdgcn_make_buffer_rsrc((void*)(f00 - s.forg), (short)0, s.frows * s.pitch * 4, 0x00020000);
+}
+// imaging accumulators: nzi x nxl, pitch nxl, addressed from interior cell (0,0)
+__device__ __forceinline__ rsrc_t image_rsrc(float* a, const ShotDev& s)
+{
+  return __builtin_amdgcn_make_buffer_rsrc((void*)a, (short)0, s.nzi * s.nxl * 4, 0x00020000);
 }
 __device__ __forceinline__ float bld(rsrc_t r, int voff, int soff)
 {
@@ -338,31 +343,23 @@ enum : int { FK_SRC = 0, FK_RCV = 1, FK_RCN = 2 };
 // ------------------------------------------------------------ pressure --
 struct DivV { float sx, sz; };
 
-// Staggered differences of vx along x and vz along z (kernel.cu:520-534):
-// VXr points at vx of the cell's column in its LDS row; cz is the column
-// window of vz rows; r the thread-local row
+// Staggered difference of vz along z (kernel.cu:520-534) of thread-local row
+// r from the column window cz of vz rows; the x half (sx) comes from the row
+// segments of the pressure phase
 template <int NW>
-__device__ __forceinline__ DivV div_v(const float* __restrict__ VXr, const float (&cz)[NW], int r, int rad,
-                                      const float* c10, const float* __restrict__ cf)
+__device__ __forceinline__ float div_z(const float (&cz)[NW], int r, int rad, const float* c10,
+                                       const float* __restrict__ cf)
 {
-  float sx = 0.0f, sz = 0.0f;
+  float sz = 0.0f;
   if (rad == RTM_R) {
 #pragma unroll
-    for (int i = 1; i <= RTM_R; i++) {
-      sx = __builtin_fmaf(c10[i - 1], VXr[i - 1] - VXr[-i], sx);
-      sz = __builtin_fmaf(c10[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
-    }
+    for (int i = 1; i <= RTM_R; i++) sz = __builtin_fmaf(c10[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
   } else {
-    const float* c = lds_base(cf + (rad - 1) * RTM_R); // zero-padded coefficient row in LDS
+    const float* c = lds_base(cf + (rad - 1) * RTM_R);
 #pragma unroll
-    for (int i = 1; i <= RTM_R; i++) {
-      sx = __builtin_fmaf(c[i - 1], VXr[i - 1] - VXr[-i], sx);
-      sz = __builtin_fmaf(c[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
-    }
+    for (int i = 1; i <= RTM_R; i++) sz = __builtin_fmaf(c[i - 1], cz[r + 9 + i] - cz[r + 10 - i], sz);
   }
-  DivV d;
-  d.sx = sx; d.sz = sz;
-  return d;
+  return sz;
 }
 
 // p - rho v^2 (sx dt/dx + sz dt/dz) (kernel.cu:536-539), kk = rho v^2
@@ -409,6 +406,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   const int pr4 = s.pitch * 4;
   const int srow0 = (z0 + RTM_MZT) * pr4;             // byte offset of tile row 0
   const int vcol = (ix + RTM_MXL) * 4;                // byte offset of the lane's column
+  const int icol = (ix - npml) * 4;                    // ... in the imaging accumulators
   const rsrc_t Rp = field_rsrc(pc, s), Rvx = field_rsrc(vxc, s), Rvz = field_rsrc(vzc, s);
   const rsrc_t Rib = field_rsrc(s.mib, s);
   float c10[RTM_R];
@@ -577,9 +575,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         qb[r] = bld(R2, vcol, so);
       }
       if (FK == FK_RCV && incell && inner) {
-        const int ii = (iz - npml) * s.nxl + ix - npml;
-        qa[r] = GCF(s.g1)[ii];
-        qb[r] = GCF(s.illum)[ii];
+        const int io = (iz - npml) * s.nxl * 4;        // wave-uniform row offset
+        qa[r] = bld(image_rsrc(s.g1, s), icol, io);
+        qb[r] = bld(image_rsrc(s.illum, s), icol, io);
       }
       if (rstrips) {                                   // reconstruction: stored p of strip cells
         const int lr = strip_row_lr(s, iz);
@@ -639,7 +637,55 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     if (halo_on(k)) VXL[hrow[k] * VXW + hcol[k]] = hvn[k];
   __syncthreads();                                     // new velocities staged
 
-  // ---- 3. new pressure of the own cells
+  // ---- 3. x-differences of the new vx by row segments: thread tid takes
+  //         RPT consecutive cells of one tile row and reads their vx window
+  //         once (RPT + 19 values instead of 20 per cell), and parks the sums
+  //         in SXL for the pressure update.  The radius is the cell's own
+  //         (the same rule as the pressure update's); a segment of a wave
+  //         with any reduced-radius cell takes the zero-padded coefficient
+  //         rows, which give the radius-10 sums bit for bit.
+  float* SXL = lds + TZ * VXW + ZR * 64;
+  {
+    constexpr int NSEG = 64 / RPT;                     // NSEG * TZ == NT
+    const int j = tid / NSEG, c0 = (tid - j * NSEG) * RPT;
+    const int iz = z0 + j;
+    float w[RPT + 19];                                 // vx at columns x0 + c0 - 10 ..
+#pragma unroll
+    for (int m = 0; m < RPT + 19; m++) w[m] = VXL[j * VXW + c0 + m];
+    int radc[RPT];
+    bool red = false;
+#pragma unroll
+    for (int c = 0; c < RPT; c++) {
+      radc[c] = RTM_R;
+      if (FK != FK_RCN && gen) {
+        const int jx = x0 + c0 + c;
+        const bool upd = iz > 0 && iz < nzp - 1 && jx < nxp && jx != 0 && jx != nxp - 1;
+        if (upd) radc[c] = imin(imin(RTM_R, imin(jx, nxp - 1 - jx)), imin(iz, nzp - 1 - iz));
+        red = red || radc[c] != RTM_R;
+      }
+    }
+    if (FK == FK_RCN || !gen || !any_lane(red)) {
+#pragma unroll
+      for (int c = 0; c < RPT; c++) {
+        float sx = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) sx = __builtin_fmaf(c10[i - 1], w[c + 9 + i] - w[c + 10 - i], sx);
+        SXL[j * 64 + c0 + c] = sx;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < RPT; c++) {
+        const float* cr = lds_base(ctab + (radc[c] - 1) * RTM_R);
+        float sx = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= RTM_R; i++) sx = __builtin_fmaf(cr[i - 1], w[c + 9 + i] - w[c + 10 - i], sx);
+        SXL[j * 64 + c0 + c] = sx;
+      }
+    }
+  }
+  __syncthreads();                                     // x-differences staged
+
+  // ---- 4. new pressure of the own cells
   float cz[RPT + 19];
 #pragma unroll
   for (int m = 0; m < RPT + 19; m++) cz[m] = VZL[(g * RPT + m) * 64 + lx];
@@ -653,13 +699,15 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     const int iz = z0 + j;                             // wave-uniform
     const int so = srow0 + j * pr4;
     float p = po[r];
-    const float* __restrict__ VXr = lds_base(VXL + j * VXW + lx) + 10;
+    const float sxl = SXL[j * 64 + lx];
     const bool incell = ix < nxp && iz < nzp;
     if (FK == FK_RCN) {
       const bool inner = xin && iz >= npml && iz < nzp - npml;
       // computed for every cell and then selected (no divergent branch):
       // interior cells take the update, strip cells the stored value
-      const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
+      DivV d;
+      d.sx = sxl;
+      d.sz = div_z(cz, r, RTM_R, c10, ctab);
       const float pc = __builtin_fmaf(kk[r], pres_sum(d, dtx, dtz), p);   // kernel.cu:389-429
       bool wr = !gen || inner;
       if (wr) p = pc;
@@ -673,7 +721,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       else if (wr) bst(Rpn, p, vcol, so);
     } else {
       if (!gen) {
-        const DivV d = div_v(VXr, cz, r, RTM_R, c10, ctab);
+        DivV d;
+        d.sx = sxl;
+        d.sz = div_z(cz, r, RTM_R, c10, ctab);
         p = __builtin_fmaf(-kk[r], pres_sum(d, dtx, dtz), p);  // kernel.cu:539
       } else {
         // the outer ring keeps its value (only the source is added there);
@@ -682,8 +732,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         const bool upd = zupd && incell && !xring;
         const int rad = upd ? imin(radx, imin(iz, nzp - 1 - iz)) : RTM_R;
         DivV d;
-        if (!any_lane(rad != RTM_R)) d = div_v(VXr, cz, r, RTM_R, c10, ctab);
-        else d = div_v(VXr, cz, r, rad, c10, ctab);
+        d.sx = sxl;
+        if (!any_lane(rad != RTM_R)) d.sz = div_z(cz, r, RTM_R, c10, ctab);
+        else d.sz = div_z(cz, r, rad, c10, ctab);
         const bool split = upd && !(xin && iz >= npml && iz < nzp - npml);
         float pu = __builtin_fmaf(-kk[r], pres_sum(d, dtx, dtz), p);
         if (any_lane(split) && split) {                // split pair, p = p1 + p2 (kernel.cu:780-783)
@@ -719,9 +770,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           }
         }
         auto image = [&]() {                           // imaging (kernel.cu:1401-1416)
-          const int ii = (iz - npml) * s.nxl + ix - npml;
-          GF(s.g1)[ii] = __builtin_fmaf(ps[r], p, qa[r]);
-          GF(s.illum)[ii] = __builtin_fmaf(ps[r], ps[r], qb[r]);
+          const int io = (iz - npml) * s.nxl * 4;
+          bst(image_rsrc(s.g1, s), __builtin_fmaf(ps[r], p, qa[r]), icol, io);
+          bst(image_rsrc(s.illum, s), __builtin_fmaf(ps[r], ps[r], qb[r]), icol, io);
         };
         if (!gen) {                                    // every lane's cell is interior
           bst(Rpn, p, vcol, so);
