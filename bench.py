@@ -115,7 +115,7 @@ def roofline_of(recs, P, copy_gbs=None):
             "traffic": None if ratio is None else round(ratio * step_bytes),
             "traffic_source": None if ratio is None else
             f"HBM/algorithmic bytes {ratio} of k_rev from rocprofv3 FETCH_SIZE+WRITE_SIZE "
-            "(profiles/k_fwd_k_rev_cfg2_200shots_traffic.md) times this run's algorithmic bytes",
+            "(profiles/k_fwd_k_rev_survey200_traffic.md) times this run's algorithmic bytes",
             "copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
             "kernel": "k_rev (reverse time step: reconstruction + receiver wavefield + imaging)",
             "avg_launch_us": round(avg_step_s * 1e6, 2), "alg_bytes_per_launch": step_bytes,
@@ -135,12 +135,12 @@ def base_line(args, value, unit, elapsed, dtype, data, config, extra):
 
 # ------------------------------------------------------------------ survey --
 def run_survey(args, G, synth):
-    P, vp, rho = synth.cfg2(nshot=200)
+    P, vp, rho = synth.survey200(nshot=200)
     if args.nt:
         P.nt = args.nt
     S = args.shots_per_step
     store = {"f32": G.STORE_F32, "f16": G.STORE_F16, "bf16": G.STORE_BF16}[args.store]
-    log(f"[bench] cfg2 survey: {P.nx}x{P.nz} nt={P.nt} npml={P.npml}, {S} shots/step, "
+    log(f"[bench] 200-shot survey: {P.nx}x{P.nz} nt={P.nt} npml={P.npml}, {S} shots/step, "
         f"steps={args.steps} warmup={args.warmup} store={args.store} chunk={args.chunk}")
     ctx = G.Context(0)
     copy_gbs = ctx.copy_probe(args.copy_mib << 20, 3) if args.copy_mib else None
@@ -198,11 +198,11 @@ def run_survey(args, G, synth):
 
 # ----------------------------------------------------------------- batched --
 def run_batched(args, G, synth):
-    P, vp, rho, om = synth.cfg3()
+    P, vp, rho, om = synth.batched_ragged()
     if args.nt:
         P.nt = args.nt
     S = args.shots_per_step or 64
-    log(f"[bench] cfg3 batched: {P.nx}x{P.nz} nt={P.nt}, {S} shots/step (ragged windows), steps={args.steps}")
+    log(f"[bench] batched survey: {P.nx}x{P.nz} nt={P.nt}, {S} shots/step (ragged windows), steps={args.steps}")
     ctx = G.Context(0)
     mid = ctx.model_create(vp, rho)
 
@@ -248,7 +248,7 @@ def run_batched(args, G, synth):
 
 # --------------------------------------------------------------- precision --
 def run_precision(args, G, synth):
-    P, vp, rho = synth.cfg4()
+    P, vp, rho = synth.precision_grid()
     if args.nt:
         P.nt = args.nt
     S = args.shots_per_step or 8
@@ -299,8 +299,8 @@ def run_precision(args, G, synth):
 
 # ------------------------------------------------------------------- serve --
 def run_serve(args, G, synth):
-    models = synth.cfg5_models()
-    reqs = synth.cfg5_stream(args.requests, models)
+    models = synth.serve_models()
+    reqs = synth.serve_stream(args.requests, models)
     ctx = G.Context(0)
     mids = [ctx.model_create(v, r) for v, r in models]
     total_shots = sum(P.nshot for _, _, P in reqs)

@@ -449,29 +449,6 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       ibz[k] = bld(Rib, vcol, so);
     }
   }
-  float vxo[RPT], ibx[RPT];
-#pragma unroll
-  for (int r = 0; r < RPT; r++) {
-    const int so = srow0 + (g * RPT + r) * pr4;
-    vxo[r] = bld(Rvx, vcol, so);
-    ibx[r] = bld(Rib, vcol, so);
-  }
-  float hvo[KH], hib[KH];
-  int hrow[KH], hcol[KH];
-#pragma unroll
-  for (int k = 0; k < KH; k++) {
-    const int h = k * NT + tid;
-    hvo[k] = 0.0f;
-    hib[k] = 1.0f;
-    hrow[k] = h / 20;
-    const int hc = h - hrow[k] * 20;
-    hcol[k] = hc < 10 ? hc : hc + 64;                  // VXL column: x = x0 - 10 + hcol
-    if (halo_on(k)) {
-      const int off = srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4;
-      hvo[k] = bld(Rvx, off, 0);
-      hib[k] = bld(Rib, off, 0);
-    }
-  }
   // reconstruction of a strip tile: the stored value of a strip cell
   // replaces its old velocity, fetched now with the rest (after the old
   // value's load, which it overwrites) and decoded where it is used; which
@@ -489,16 +466,6 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       const int kz = g * KZ + k;
       const int st = kz < ZR ? soff_vz(strip_row_vz(s, z0 - 10 + kz), sc) : -1;
       if (st >= 0) { vzo[k] = store_raw(s, it + 1, st); sm |= 1ull << k; }
-    }
-#pragma unroll
-    for (int r = 0; r < RPT; r++) {
-      const int st = soff_vx(strip_row_lr(s, z0 + g * RPT + r), sc);
-      if (st >= 0) { vxo[r] = store_raw(s, it + 1, st); sm |= 1ull << (SMX + r); }
-    }
-#pragma unroll
-    for (int k = 0; k < KH; k++) {
-      const int st = halo_on(k) ? strip_vx(s, z0 + hrow[k], x0 - 10 + hcol[k]) : -1;
-      if (st >= 0) { hvo[k] = store_raw(s, it + 1, st); sm |= 1ull << (SMH + k); }
     }
   }
   auto sbit = [&](int b) { return ((sm >> b) & 1ull) != 0; };
@@ -527,6 +494,41 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     DMP[tid] = GCF(s.sscale)[(long long)(tid < 4 ? it + 1 : it) * 4 + (tid & 3)];
   __builtin_amdgcn_s_waitcnt(0);                       // the LDS writes of the staging loads
   __syncthreads();                                     // p staged
+  float vxo[RPT], ibx[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int so = srow0 + (g * RPT + r) * pr4;
+    vxo[r] = bld(Rvx, vcol, so);
+    ibx[r] = bld(Rib, vcol, so);
+  }
+  float hvo[KH], hib[KH];
+  int hrow[KH], hcol[KH];
+#pragma unroll
+  for (int k = 0; k < KH; k++) {
+    const int h = k * NT + tid;
+    hvo[k] = 0.0f;
+    hib[k] = 1.0f;
+    hrow[k] = h / 20;
+    const int hc = h - hrow[k] * 20;
+    hcol[k] = hc < 10 ? hc : hc + 64;                  // VXL column: x = x0 - 10 + hcol
+    if (halo_on(k)) {
+      const int off = srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4;
+      hvo[k] = bld(Rvx, off, 0);
+      hib[k] = bld(Rib, off, 0);
+    }
+  }
+  if (rstrips) {
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      const int st = soff_vx(strip_row_lr(s, z0 + g * RPT + r), sc);
+      if (st >= 0) { vxo[r] = store_raw(s, it + 1, st); sm |= 1ull << (SMX + r); }
+    }
+#pragma unroll
+    for (int k = 0; k < KH; k++) {
+      const int st = halo_on(k) ? strip_vx(s, z0 + hrow[k], x0 - 10 + hcol[k]) : -1;
+      if (st >= 0) { hvo[k] = store_raw(s, it + 1, st); sm |= 1ull << (SMH + k); }
+    }
+  }
 
   // ---- 2. new velocities
   float vzn_[KZ], vxn_[RPT], hvn[KH];

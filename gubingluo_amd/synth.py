@@ -48,7 +48,7 @@ def layered_random(nx, nz, dx, dz, seed=0, nlayers=10, vmin=1500.0, vmax=4500.0,
     return v.astype(np.float32)
 
 
-def cfg1():
+def single_shot_anomaly():
     """1 shot, 400x200, dx=10, npml 40, Ricker 20 Hz at x=2000 m z=20 m, dt=1 ms nt=2000,
     400 surface receivers (full array), cross-correlation image."""
     vp = two_layer_anomaly()
@@ -59,7 +59,7 @@ def cfg1():
     return P, vp, rho
 
 
-def cfg2(nshot=200, seed=2):
+def survey200(nshot=200, seed=2):
     """200 shots every 50 m, 1024x512, dx=10, npml 50, Ricker 15 Hz, dt=0.8 ms nt=4000,
     split-spread 301 receivers (layout 1, offsetmax 3000 m)."""
     nx, nz, dx = 1024, 512, 10.0
@@ -71,7 +71,7 @@ def cfg2(nshot=200, seed=2):
     return P, vp, rho
 
 
-def cfg3(nshot=256, seed=3, window_seed=33):
+def batched_ragged(nshot=256, seed=3, window_seed=33):
     """Batched RTM: 2048x768, dx=10, npml 50, Ricker 15 Hz, dt=0.8 ms nt=4000, 256 shots every
     80 m, split-spread windows (layout 1) whose width is drawn per shot from U[400, 1600] columns.
     Returns (P, vp, rho, offsetmax per shot)."""
@@ -88,7 +88,7 @@ def cfg3(nshot=256, seed=3, window_seed=33):
     return P, vp, rho, offsetmax
 
 
-def cfg4(nshot=128, seed=4):
+def precision_grid(nshot=128, seed=4):
     """Precision variant: 2048x1024, dx=dz=7.5 m, npml 50, Ricker 25 Hz, dt=0.6 ms nt=6000,
     128 shots every 120 m, full-aperture receivers (layout 2)."""
     nx, nz, dx = 2048, 1024, 7.5
@@ -100,23 +100,23 @@ def cfg4(nshot=128, seed=4):
     return P, vp, rho
 
 
-CFG5_NX = (256, 512, 768, 1024, 1280, 1536, 1792, 2048)
-CFG5_NZ = (256, 384, 512, 640, 768, 896, 1024)
+SERVE_NX = (256, 512, 768, 1024, 1280, 1536, 1792, 2048)
+SERVE_NZ = (256, 384, 512, 640, 768, 896, 1024)
 
 
-def cfg5_models(nmodels=8, seed=5):
+def serve_models(nmodels=8, seed=5):
     """8 resident models with grids drawn from 256x256 .. 2048x1024 (dx=dz=10 m)."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(nmodels):
-        nx = int(rng.choice(CFG5_NX))
-        nz = int(rng.choice(CFG5_NZ))
+        nx = int(rng.choice(SERVE_NX))
+        nz = int(rng.choice(SERVE_NZ))
         vp = layered_random(nx, nz, 10.0, 10.0, seed=100 + k)
         out.append((vp, np.full_like(vp, 2000.0)))
     return out
 
 
-def cfg5_stream(nreq, models, rate=4.0, seed=55):
+def serve_stream(nreq, models, rate=4.0, seed=55):
     """Poisson arrivals (rate per second) of RTM requests: model index, nt ~ U[1000, 6000],
     1..32 shots spread evenly over the model, split-spread 3000 m (layout 1), 15 Hz, dt=0.8 ms."""
     rng = np.random.default_rng(seed)

@@ -19,21 +19,21 @@ def test_more_than_one_gpu_exits_with_message():
     assert "one GPU" in r.stderr
 
 
-def test_cfg2_survey_geometry():
-    P, vp, rho = synth.cfg2(nshot=200)
+def test_survey200_geometry():
+    P, vp, rho = synth.survey200(nshot=200)
     assert vp.shape == (512, 1024) and vp.min() >= 1500.0 and vp.max() <= 4725.0
     g = G.geometry(P, 100)
     assert g.nrec == 301 and g.nxl == 301 and g.nzp == 612       # split spread, layout 1
 
 
-def test_cfg3_ragged_windows():
-    P, vp, rho, om = synth.cfg3()
+def test_batched_ragged_windows():
+    P, vp, rho, om = synth.batched_ragged()
     assert vp.shape == (768, 2048) and len(om) == 256
     NX = np.floor(om / P.dx + 1).astype(int)
     widths = 2 * ((NX - 1) // 2) + 1                               # layout-1 windows are odd
     assert NX.min() >= 400 and NX.max() <= 1600
     # seeded: the same draw every time
-    assert np.array_equal(om, synth.cfg3()[3])
+    assert np.array_equal(om, synth.batched_ragged()[3])
     # a window per shot through the same geometry rule
     Pk = G.Params.from_buffer_copy(P)
     Pk.offsetmax = float(om[127])
@@ -41,15 +41,15 @@ def test_cfg3_ragged_windows():
     assert g.nxl == int(widths[127])
 
 
-def test_cfg4_and_cfg5_shapes():
-    P, vp, rho = synth.cfg4()
+def test_precision_grid_and_serve_shapes():
+    P, vp, rho = synth.precision_grid()
     assert vp.shape == (1024, 2048) and P.nt == 6000 and abs(P.dt - 6e-4) < 1e-9 and P.dx == 7.5
     assert G.geometry(P, 1).nxl == 2048
-    models = synth.cfg5_models()
+    models = synth.serve_models()
     assert len(models) == 8
     for vpm, _ in models:
         assert 256 <= vpm.shape[0] <= 1024 and 256 <= vpm.shape[1] <= 2048
-    reqs = synth.cfg5_stream(50, models)
+    reqs = synth.serve_stream(50, models)
     t = np.array([r[0] for r in reqs])
     assert np.all(np.diff(t) > 0)
     assert all(1000 <= r[2].nt <= 6000 and 1 <= r[2].nshot <= 32 for r in reqs)

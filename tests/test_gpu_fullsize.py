@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def survey():
-    P, vp, rho = synth.cfg2(nshot=200)
+    P, vp, rho = synth.survey200(nshot=200)
     return {n: getattr(P, n) for n, _ in P._fields_}, vp, rho
 
 
