@@ -4,6 +4,7 @@
 #include "rtm_engine.h"
 
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
 #include <string.h>
 
@@ -170,22 +171,35 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
   damping(s.azv, s.bzv, nzp - 1, npml, s.vpmax, P.dz, true, P.dt);
   s.axv.push_back(1.0f); s.bxv.push_back(1.0f);              // pad to nxp / nzp
   s.azv.push_back(1.0f); s.bzv.push_back(1.0f);
-  // Gaussian source (kernel.cu:23-35): bounding box of its non-zero values
+  // Gaussian source (kernel.cu:23-35): bounding box of its non-zero values.
+  // expf(-a) is 0 in float for a > 104 (below half the smallest denormal), so
+  // only the rectangle around the source where alp^2 dist <= 120 (plus two
+  // cells) can hold non-zero values; evaluating the whole window instead
+  // (245 K expf for a 401 x 612 window) dominated the host set-up of a batch
   const float dx2 = P.dx * P.dx, dz2 = P.dz * P.dz;
   int x0 = nxp, x1 = -1, z0 = nzp, z1 = -1;
-  std::vector<float> src((size_t)nxp * nzp);
-  for (int iz = 0; iz < nzp; iz++)
-    for (int ix = 0; ix < nxp; ix++) {
+  int bx0 = 0, bx1 = nxp - 1, bz0 = 0, bz1 = nzp - 1;
+  const double a2 = (double)P.alp * (double)P.alp;
+  if (a2 > 0.0) {
+    const double r2 = 120.0 / a2;
+    const double rxd = std::ceil(std::sqrt(r2 / dx2)) + 2.0, rzd = std::ceil(std::sqrt(r2 / dz2)) + 2.0;
+    if (rxd < nxp) { bx0 = std::max(0, g.nsx - (int)rxd); bx1 = std::min(nxp - 1, g.nsx + (int)rxd); }
+    if (rzd < nzp) { bz0 = std::max(0, g.nsz - (int)rzd); bz1 = std::min(nzp - 1, g.nsz + (int)rzd); }
+  }
+  const int bw = std::max(0, bx1 - bx0 + 1), bh = std::max(0, bz1 - bz0 + 1);
+  std::vector<float> src((size_t)bw * bh);
+  for (int iz = bz0; iz <= bz1; iz++)
+    for (int ix = bx0; ix <= bx1; ix++) {
       const float dist = (ix - g.nsx) * (ix - g.nsx) * dx2 + (iz - g.nsz) * (iz - g.nsz) * dz2;
       const float v = P.amp * expf(-P.alp * P.alp * dist);
-      src[(size_t)iz * nxp + ix] = v;
+      src[(size_t)(iz - bz0) * bw + ix - bx0] = v;
       if (v != 0.0f) { x0 = imin(x0, ix); x1 = imax(x1, ix); z0 = imin(z0, iz); z1 = imax(z1, iz); }
     }
   s.sa.clear();
   if (x1 >= x0) {
     s.sx0 = x0; s.sx1 = x1; s.sz0 = z0; s.sz1 = z1;
     for (int iz = z0; iz <= z1; iz++)
-      for (int ix = x0; ix <= x1; ix++) s.sa.push_back(P.dt * src[(size_t)iz * nxp + ix]);
+      for (int ix = x0; ix <= x1; ix++) s.sa.push_back(P.dt * src[(size_t)(iz - bz0) * bw + ix - bx0]);
   } else {
     s.sx0 = 0; s.sx1 = -1; s.sz0 = 0; s.sz1 = -1;
     s.sa.push_back(0.0f);
