@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/rtm_hip.h"
@@ -99,6 +100,102 @@ extern "C" void rtmh_get_constant(const float* vp, float fdom, float dx0, float 
     g->dt = (float)((int)(dt_max * 10000.0) * 0.0001);
     g->nt = (int)(t / g->dt);
   }
+}
+
+// pmlvelsmooth1d (Velocity_function.cpp:9-103).  The relaxation updates the
+// working copy in place, so the sweep order below is part of the result: the
+// left/right layers column by column, then the top/bottom layers, then the
+// edge rows/columns (their 4-point average includes the cell itself), then the
+// four corners.  The change test compares the sweep's result with the window
+// before it; fabs of a float is the float overload in the reference's C++.
+extern "C" void rtmh_pml_smooth(float* w, int nx, int nz, int npml)
+{
+  auto at = [nx](std::vector<float>& a, int iz, int ix) -> float& { return a[(size_t)iz * nx + ix]; };
+  for (int ix = 0; ix < npml; ix++)
+    for (int iz = 1; iz < nz - 1; iz++) {
+      w[(size_t)iz * nx + ix] = w[(size_t)iz * nx + 2 * npml - ix - 1];
+      w[(size_t)iz * nx + nx - npml + ix] = w[(size_t)iz * nx + nx - npml - ix - 1];
+    }
+  for (int iz = 0; iz < npml; iz++)
+    for (int ix = 1; ix < nx - 1; ix++) {
+      w[(size_t)iz * nx + ix] = w[(size_t)(2 * npml - iz - 1) * nx + ix];
+      w[(size_t)(nz - npml + iz) * nx + ix] = w[(size_t)(nz - npml - iz - 1) * nx + ix];
+    }
+  const size_t n = (size_t)nx * nz;
+  std::vector<float> v(w, w + n);
+  // (float)(0.25 * (double)sum): the reference's double literal
+  auto avg = [](float a, float b, float c, float d) { return (float)(0.25 * (double)(a + b + c + d)); };
+  float err = 1.0f;
+  for (int sweep = 0; sweep <= 1000 && (double)err > 0.0005; sweep++) {
+    for (int ix = 1; ix < npml; ix++)
+      for (int iz = 1; iz < nz - 1; iz++) {
+        const int xr = ix + nx - npml - 1;
+        at(v, iz, ix) = avg(at(v, iz, ix + 1), at(v, iz, ix - 1), at(v, iz + 1, ix), at(v, iz - 1, ix));
+        at(v, iz, xr) = avg(at(v, iz, xr + 1), at(v, iz, xr - 1), at(v, iz + 1, xr), at(v, iz - 1, xr));
+      }
+    for (int ix = 1; ix < nx - 1; ix++)
+      for (int iz = 1; iz < npml; iz++) {
+        const int zb = iz + nz - npml - 1;
+        at(v, iz, ix) = avg(at(v, iz, ix + 1), at(v, iz, ix - 1), at(v, iz + 1, ix), at(v, iz - 1, ix));
+        at(v, zb, ix) = avg(at(v, zb, ix + 1), at(v, zb, ix - 1), at(v, zb + 1, ix), at(v, zb - 1, ix));
+      }
+    for (int iz = 1; iz < nz - 1; iz++) {
+      at(v, iz, 0) = avg(at(v, iz, 0), at(v, iz, 1), at(v, iz + 1, 0), at(v, iz - 1, 0));
+      at(v, iz, nx - 1) = avg(at(v, iz, nx - 1), at(v, iz, nx - 2), at(v, iz + 1, nx - 1), at(v, iz - 1, nx - 1));
+    }
+    for (int ix = 1; ix < nx - 1; ix++) {
+      at(v, 0, ix) = avg(at(v, 0, ix), at(v, 1, ix), at(v, 0, ix - 1), at(v, 0, ix + 1));
+      at(v, nz - 1, ix) = avg(at(v, nz - 1, ix), at(v, nz - 2, ix), at(v, nz - 1, ix - 1), at(v, nz - 1, ix + 1));
+    }
+    at(v, 0, 0) = avg(at(v, 0, 0), at(v, 1, 0), at(v, 0, 1), at(v, 1, 1));
+    at(v, nz - 1, 0) = avg(at(v, nz - 1, 0), at(v, nz - 2, 0), at(v, nz - 1, 1), at(v, nz - 2, 1));
+    at(v, 0, nx - 1) = avg(at(v, 0, nx - 1), at(v, 0, nx - 2), at(v, 1, nx - 1), at(v, 1, nx - 2));
+    at(v, nz - 1, nx - 1) = avg(at(v, nz - 1, nx - 1), at(v, nz - 2, nx - 1), at(v, nz - 1, nx - 2), at(v, nz - 2, nx - 2));
+    float worst = -1.0e+5f;
+    for (size_t i = 0; i < n; i++) {
+      const float rel = fabsf(w[i] - v[i]) / fabsf(w[i]);
+      if (rel > worst) worst = rel;
+    }
+    err = worst;
+    std::copy(v.begin(), v.end(), w);
+  }
+}
+
+// velsmooth1d (Velocity_function.cpp:199-313).  The weights depend only on
+// the offset, so they are tabulated once (the same exp arguments, so the same
+// values); each cell's weighted sum runs over the window in row order in
+// double, as the reference's, and is divided by the sum of the weights.
+extern "C" void rtmh_gauss_smooth(float* w, int nz, int nx, int nsp)
+{
+  if (nsp <= 0) return;
+  const int k = 2 * nsp + 1, nxe = nx + 2 * nsp, nze = nz + 2 * nsp;
+  const double den = 2.0 * nsp / 3.0 * nsp / 3.0;
+  std::vector<double> wt((size_t)k * k);
+  for (int i = 0; i < k; i++)
+    for (int j = 0; j < k; j++) {
+      const double d1 = i - nsp, d2 = j - nsp;
+      wt[(size_t)i * k + j] = exp(-(d1 * d1 + d2 * d2) / den);
+    }
+  double norm = 0;
+  for (double x : wt) norm += x;                      // row order, as the reference sums it
+  // edge-extended copy: each extended cell takes the nearest window cell
+  std::vector<float> e((size_t)nze * nxe);
+  for (int iz = 0; iz < nze; iz++) {
+    const int sz = std::min(std::max(iz - nsp, 0), nz - 1);
+    for (int ix = 0; ix < nxe; ix++) e[(size_t)iz * nxe + ix] = w[(size_t)sz * nx + std::min(std::max(ix - nsp, 0), nx - 1)];
+  }
+  std::vector<float> out((size_t)nz * nx);
+  for (int iz = 0; iz < nz; iz++)
+    for (int ix = 0; ix < nx; ix++) {
+      double s = 0;
+      for (int i = 0; i < k; i++) {
+        const float* row = &e[(size_t)(iz + i) * nxe + ix];
+        const double* wr = &wt[(size_t)i * k];
+        for (int j = 0; j < k; j++) s += wr[j] * row[j];
+      }
+      out[(size_t)iz * nx + ix] = (float)s;
+    }
+  for (size_t i = 0; i < out.size(); i++) w[i] = (float)(out[i] / norm);
 }
 
 // interpvel1 (Velocity_function.cpp:1946-1977): linear refinement by lx, lz

@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
+#include <thread>
 
 #include "host_io.h"
 #include "rtm_launch.h"
@@ -156,14 +158,37 @@ static void damping(std::vector<float>& a, std::vector<float>& b, int n, int npm
   }
 }
 
-void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s)
+void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, ShotSetup& s)
+{
+  const rtmh_geom& g = s.g;
+  const int nxp = g.nxp, nzp = g.nzp;
+  s.wvv.resize((size_t)nxp * nzp);
+  s.wrho.resize((size_t)nxp * nzp);
+  for (int iz = 0; iz < nzp; iz++)
+    for (int ix = 0; ix < nxp; ix++) {
+      const float v = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, P.npml, iz, ix);
+      s.wvv[(size_t)iz * nxp + ix] = v * v;                    // v^2 (Velocity_function.cpp:2059)
+      s.wrho[(size_t)iz * nxp + ix] = win_at(rho0, P.nx, P.nz, g.nx1, g.nxl, P.npml, iz, ix);
+    }
+  if (P.medium_flag) {                                         // rtm_model.cu:204-206
+    rtmh_pml_smooth(s.wvv.data(), nxp, nzp, P.npml);
+    rtmh_pml_smooth(s.wrho.data(), nxp, nzp, P.npml);
+  }
+}
+
+void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s, const float* wvv)
 {
   const rtmh_geom& g = s.g;
   const int npml = P.npml, nxp = g.nxp, nzp = g.nzp;
   // per-shot vmax over the padded window (rtm_model.cu:207): edge replication
-  // adds no new values, so the window's column maxima of v^2 suffice
+  // adds no new values, so the window's column maxima of v^2 suffice; a
+  // smoothed window gives its own maximum
   float m = colmax_vv[g.nx1];
   for (int ix = g.nx1; ix <= g.nx2; ix++) m = colmax_vv[ix] > m ? colmax_vv[ix] : m;
+  if (wvv) {
+    m = wvv[0];
+    for (size_t i = 0; i < (size_t)nxp * nzp; i++) m = wvv[i] > m ? wvv[i] : m;
+  }
   s.vpmax = sqrtf(m);
   damping(s.ax, s.bx, nxp, npml, s.vpmax, P.dx, false, P.dt);
   damping(s.axv, s.bxv, nxp - 1, npml, s.vpmax, P.dx, true, P.dt);
@@ -210,13 +235,17 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
   const int nw = (int)(P.direct / (P.fdom * dt));
   const int tlength = (int)(P.direct0 / dt);
   const int coff = (P.flag_layout == 3) ? g.noffset : 0;
-  const float vvs = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, g.nsz, g.nsx);
-  const float vvsrc = vvs * vvs;
+  // v^2 of the padded window the forward pass uses (the mute kernels read d_vp)
+  auto vv_at = [&](int iz, int ix) {
+    if (wvv) return wvv[(size_t)iz * nxp + ix];
+    const float v = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, iz, ix);
+    return v * v;
+  };
+  const float vvsrc = vv_at(g.nsz, g.nsx);
   s.cut.assign(g.nrec > 0 ? g.nrec : 1, -1);
   for (int ir = 0; ir < g.nrec; ir++) {
     const int col = ir + npml + coff;
-    const float vr = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, npml, col);
-    const float vvr = vr * vr;
+    const float vvr = vv_at(npml, col);
     const float vtemp = (float)(0.482 * (1.0 / (double)sqrtf(vvr) + 1.0 / (double)sqrtf(vvsrc)));
     float dist = (col - g.nsx) * (col - g.nsx) * dx2 + (npml - g.nsz) * (npml - g.nsz) * dz2;
     dist = sqrtf(dist);
@@ -227,6 +256,17 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
     c = c < P.nt ? c : P.nt;
     s.cut[ir] = c;
   }
+}
+
+// fn(0..n-1) on up to 16 host threads (independent items)
+static void parallel_for(int n, const std::function<void(int)>& fn)
+{
+  const int nth = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+  if (nth <= 1) { for (int k = 0; k < n; k++) fn(k); return; }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; t++)
+    th.emplace_back([&, t]() { for (int k = t; k < n; k += nth) fn(k); });
+  for (auto& x : th) x.join();
 }
 
 // ------------------------------------------------------------------ pool ----
@@ -338,6 +378,7 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
   b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
   b += align256((size_t)nt * strip * store_elem(store_fmt));
   b += align256((size_t)nt * 4 * 4 + 8 * 4);         // scales + 16-bit max-abs accumulators
+  if (!s.wvv.empty()) b += 2 * align256((size_t)s.g.nxp * s.g.nzp * 4);   // smoothed windows
   return b;
 }
 
@@ -391,6 +432,16 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.store = base + o;
   o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt));
   d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4 + 8 * 4);
+  if (!s.wvv.empty()) {
+    // the shot's own (smoothed) padded windows as the model the slot reads
+    const size_t wb = (size_t)g.nxp * g.nzp * 4;
+    float* wv = (float*)(base + o); o += align256(wb);
+    float* wr = (float*)(base + o); o += align256(wb);
+    uploads.push_back({wv, std::vector<char>((const char*)s.wvv.data(), (const char*)s.wvv.data() + wb)});
+    uploads.push_back({wr, std::vector<char>((const char*)s.wrho.data(), (const char*)s.wrho.data() + wb)});
+    d.vv = wv; d.rho = wr;
+    d.mpitch = g.nxp; d.mz0 = 0; d.mzn = g.nzp; d.mx0 = 0; d.mxs = 0; d.mxn = g.nxp;
+  }
 }
 
 hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
@@ -511,10 +562,7 @@ int params_check(const rtmh_params& P, std::string& why)
   if (!(P.dx > 0 && P.dz > 0 && P.dt > 0)) { why = "dx, dz, dt must be > 0"; return -1; }
   if (P.flag_layout < 1 || P.flag_layout > 4) { why = "flag_layout must be 1..4"; return -1; }
   if (P.dshot <= 0) { why = "dshot must be > 0"; return -1; }
-  if (P.medium_flag != 0 || P.ismth != 0) {
-    why = "medium_flag / ismth model smoothing is not implemented";
-    return -1;
-  }
+  if (P.ismth < 0) { why = "ismth must be >= 0"; return -1; }
   return 0;
 }
 
@@ -526,6 +574,7 @@ int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, in
   m.nx = nx; m.nz = nz;
   const size_t nm = (size_t)nx * nz;
   m.vp0.assign(vp0, vp0 + nm);
+  m.rho0.assign(rho0, rho0 + nm);
   std::vector<float> vv(nm);
   // v^2 as extractvel1 stores it (Velocity_function.cpp:2059)
   for (size_t i = 0; i < nm; i++) vv[i] = vp0[i] * vp0[i];
@@ -627,12 +676,19 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   const size_t nm = (size_t)nx * nz;
   // per-shot set-up, taper window state carried in shot order (rtm_model.cu:70, 465-466)
   std::vector<ShotSetup> setup(nsh);
+  std::vector<rtmh_params> pk(nsh, P);
+  for (int k = 0; k < nsh; k++) {
+    if (opt.shot_offsetmax) pk[k].offsetmax = opt.shot_offsetmax[k];
+    if (!shot_geometry(pk[k], shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
+  }
+  // model smoothing: each shot gets its own padded windows (smoothed on the
+  // host as the reference does, shots in parallel); the slots then read them
+  // instead of the resident model
+  const bool smooth = P.medium_flag != 0 || P.ismth > 0;
+  if (smooth) parallel_for(nsh, [&](int k) { shot_windows(pk[k], vp0, M->rho0.data(), setup[k]); });
   int nwin = 50;
   for (int k = 0; k < nsh; k++) {
-    rtmh_params Pk = P;
-    if (opt.shot_offsetmax) Pk.offsetmax = opt.shot_offsetmax[k];
-    if (!shot_geometry(Pk, shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
-    shot_setup(Pk, vp0, M->colmax.data(), setup[k]);
+    shot_setup(pk[k], vp0, M->colmax.data(), setup[k], smooth ? setup[k].wvv.data() : nullptr);
     while (2 * nwin >= setup[k].g.nxl) nwin -= 3;
     setup[k].nwin = nwin;
     taper_row(setup[k].taper, setup[k].g.nxl, nwin);
@@ -831,6 +887,23 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       }
     }
     if (rc) { for (void* p : mem) pool_.put(p); break; }
+
+    // velsmooth1d of each shot's window before its reverse pass (rtm_model.cu:
+    // 402-405): reconstruction and receiver propagation use the smoothed model
+    if (P.ismth > 0) {
+      parallel_for(nb, [&](int j) {
+        ShotSetup& s = setup[k0 + j];
+        rtmh_gauss_smooth(s.wvv.data(), s.g.nzp, s.g.nxp, P.ismth);
+        rtmh_gauss_smooth(s.wrho.data(), s.g.nzp, s.g.nxp, P.ismth);
+      });
+      for (int j = 0; j < nb; j++) {
+        const ShotSetup& s = setup[k0 + j];
+        const size_t wb = s.wvv.size() * 4;
+        HIPCHK(hipMemcpyAsync((void*)ds[j].vv, s.wvv.data(), wb, hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync((void*)ds[j].rho, s.wrho.data(), wb, hipMemcpyHostToDevice, st_));
+        HIPCHK(rtm_launch_model_window(d_ds, j, 512, st_));
+      }
+    }
 
     // ---- reverse pass: reconstruction + receiver propagation + imaging
     for (int j = 0; j < nb; j++) { ds[j].phase = PH_REV; ds[j].it_off = nt - 2; ds[j].it_dir = -1; }

@@ -28,13 +28,22 @@ struct ShotSetup {
   std::vector<int> cut;                                               // mute
   std::vector<float> taper;                                           // nxl
   int nwin = 0;
+  // padded v^2 and density windows (nzp x nxp) of a job with model smoothing
+  // (medium_flag / ismth); empty otherwise (the slot then reads the resident
+  // model)
+  std::vector<float> wvv, wrho;
 };
 
 // shot window and padded geometry (rtm_model.cu:122-153 and the layout
 // variants); returns false if the window is empty or the layout unknown
 bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s);
-// host set-up of one shot given the job's v^2 column maxima
-void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s);
+// host set-up of one shot given the job's v^2 column maxima; with a padded v^2
+// window (wvv, nzp x nxp) vpmax and the mute come from it instead
+void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s,
+                const float* wvv = nullptr);
+// the padded windows of a shot (extractvel1 / extractrho1 + extendvel1) and,
+// with medium_flag, their absorbing layers smoothed (pmlvelsmooth1d)
+void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, ShotSetup& s);
 // Ricker of kernel.cu:13-22
 void ricker(std::vector<float>& w, int nt, float fdom, float dt);
 // taper row (kernel.cu:1591-1604)
@@ -80,6 +89,7 @@ class Engine {
     float* d_rho = nullptr;
     float* d_vv = nullptr;
     std::vector<float> vp0;     // host copy: mute velocities
+    std::vector<float> rho0;    // host copy: smoothed windows
     std::vector<float> colmax;  // host: column maxima of v^2 (per-shot vmax)
     size_t bytes = 0;
   };

@@ -993,6 +993,31 @@ __global__ __launch_bounds__(NTHR) void k_save_q(const ShotDev* __restrict__ sho
 // into the PML, extendvel1 Velocity_function.cpp:2011-2030, and on into the
 // margins).  REV: zero both levels of the source velocities (rtm_model.cu:
 // 366-372), the receiver set, the PML pair and the imaging accumulators.
+// the slot's model window (1/rho and rho v^2 in the field layout, margins
+// included) from its model source: the resident model, or the shot's own
+// (smoothed) padded window
+__device__ void model_window(const ShotDev& s, long long g0, long long gs, long long n)
+{
+  for (long long e = g0; e < n; e += gs) {
+    const int row = (int)(e / s.pitch), col = (int)(e - (long long)row * s.pitch);
+    const int iz = row - RTM_MZT, ix = col - RTM_MXL;
+    const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
+    const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
+    const size_t m = (size_t)r * s.mpitch + c;
+    const float rh = s.rho[m];
+    (s.mib - s.forg)[e] = 1.0f / rh;
+    (s.mkk - s.forg)[e] = rh * s.vv[m];
+  }
+}
+
+// model window rebuilt before the reverse pass (after velsmooth1d)
+__global__ __launch_bounds__(NTHR) void k_model_window(const ShotDev* __restrict__ shots, int slot)
+{
+  const ShotDev s = shots[slot];
+  const long long n = (long long)s.frows * s.pitch;
+  model_window(s, (long long)blockIdx.x * NTHR + threadIdx.x, (long long)gridDim.x * NTHR, n);
+}
+
 __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__ shots, int slot, int phase)
 {
   const ShotDev s = shots[slot];
@@ -1008,16 +1033,7 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
       if (k >= first) (f[k] - s.forg)[e] = 0.0f;
   if (phase == PH_FWD) {
     if (s.store_fmt != ST_F32 && g0 < 8) save_acc(s)[g0] = 0u;   // 16-bit max-abs accumulators
-    for (long long e = g0; e < n; e += gs) {
-      const int row = (int)(e / s.pitch), col = (int)(e - (long long)row * s.pitch);
-      const int iz = row - RTM_MZT, ix = col - RTM_MXL;
-      const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
-      const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
-      const size_t m = (size_t)r * s.mpitch + c;
-      const float rh = s.rho[m];
-      (s.mib - s.forg)[e] = 1.0f / rh;
-      (s.mkk - s.forg)[e] = rh * s.vv[m];
-    }
+    model_window(s, g0, gs, n);
   } else {
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
   }
@@ -1154,6 +1170,12 @@ hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, i
 hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st)
 {
   hipLaunchKernelGGL(k_phase_init, dim3(nblocks), dim3(NTHR), 0, st, shots, slot, phase);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_model_window(const ShotDev* shots, int slot, int nblocks, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_model_window, dim3(nblocks), dim3(NTHR), 0, st, shots, slot);
   return hipGetLastError();
 }
 

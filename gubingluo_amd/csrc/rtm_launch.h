@@ -13,6 +13,8 @@ hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t s
 hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, int store_fmt,
                            hipStream_t st);
 hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st);
+// the slot's 1/rho and rho v^2 rebuilt from its model source (after smoothing)
+hipError_t rtm_launch_model_window(const ShotDev* shots, int slot, int nblocks, hipStream_t st);
 hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st);
 hipError_t rtm_launch_stack(const void* stack_shots, int nsh, int nx, int nz, double* i1, double* i2, double* il,
                             hipStream_t st);

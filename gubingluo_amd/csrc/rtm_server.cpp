@@ -173,6 +173,10 @@ int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* i
   std::string why;
   if (params_check(P, why) != 0) { set_error(why); return -1; }
   if (P.mode != 0) { set_error("served requests model their data (mode 0)"); return -1; }
+  if (P.medium_flag != 0 || P.ismth != 0) {
+    set_error("model smoothing (medium_flag, ismth) is not served; run the job with rtmh_run_model");
+    return -1;
+  }
   const Engine::Model* M = eng_->model(model_id);
   if (!M || M->nx != P.nx || M->nz != P.nz) { set_error("no such model or size mismatch"); return -1; }
   auto r = std::make_unique<SrvRequest>();

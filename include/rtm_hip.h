@@ -195,6 +195,19 @@ void rtmh_get_constant(const float *vp, float fdom, float dx0, float dz0, int nx
 /* interpvel1 (Velocity_function.cpp:1946-1977). */
 void rtmh_interp_model(const float *vp0, float *vp, int nx0, int nz0, int nx, int nz, int lx, int lz);
 
+/* Model smoothing of a shot's padded window (v^2 or density; nz rows of nx,
+ * x fastest), in place.  rtmh_pml_smooth replaces pmlvelsmooth1d
+ * (Velocity_function.cpp:9-103; applied when medium_flag != 0, rtm_model.cu:
+ * 204-206): the absorbing layer mirrored from its neighbour cells, then
+ * relaxed with 4-point averages until a sweep changes no cell by more than
+ * 5e-4 (relative) or 1001 sweeps ran.  rtmh_gauss_smooth replaces
+ * velsmooth1d (Velocity_function.cpp:199-313; applied before the reverse
+ * pass when ismth > 0, rtm_model.cu:402-405): a (2 nsp + 1)^2 Gaussian
+ * window (sigma nsp/3) over the edge-extended window, normalised.  Both give
+ * the reference's values bit for bit. */
+void rtmh_pml_smooth(float *w, int nx, int nz, int npml);
+void rtmh_gauss_smooth(float *w, int nz, int nx, int nsp);
+
 /* selfdoc banner (RTM_GPU_velstr.cpp:155-182) on stdout. */
 void rtmh_selfdoc(const rtmh_parfile *pf, int nx0, int nz0, const rtmh_grid *g);
 

@@ -181,6 +181,124 @@ ORC_EXPORT void orc_window_pad(float *out, const float *in, int nx, int nz, int 
         }
 }
 
+/* pmlvelsmooth1d (Velocity_function.cpp:9-103), applied to the padded v^2 and
+ * density windows when medium_flag is set (rtm_model.cu:204-206): the
+ * absorbing layer is mirrored from the cells next to it, then relaxed in
+ * place with 4-point averages in the reference's sweep order until the
+ * largest relative change of a sweep is <= 5e-4 or 1001 sweeps have run.
+ * w is nz x nx, x fastest.  The reference is C++: fabs of a float is the
+ * float overload, so the change ratio is a float division. */
+ORC_EXPORT void orc_pml_smooth(float *w, int nx, int nz, int npml)
+{
+    for (int ix = 0; ix < npml; ix++)
+        for (int iz = 1; iz < nz - 1; iz++) {
+            w[(size_t)iz * nx + ix] = w[(size_t)iz * nx + 2 * npml - ix - 1];
+            w[(size_t)iz * nx + nx - npml + ix] = w[(size_t)iz * nx + nx - npml - ix - 1];
+        }
+    for (int iz = 0; iz < npml; iz++)
+        for (int ix = 1; ix < nx - 1; ix++) {
+            w[(size_t)iz * nx + ix] = w[(size_t)(2 * npml - iz - 1) * nx + ix];
+            w[(size_t)(nz - npml + iz) * nx + ix] = w[(size_t)(nz - npml - iz - 1) * nx + ix];
+        }
+    const size_t n = (size_t)nx * nz;
+    float *v = (float *)malloc(n * sizeof(float));
+    memcpy(v, w, n * sizeof(float));
+#define SV(iz, ix) v[(size_t)(iz) * nx + (ix)]
+    float err = 1.0f;
+    int num = 0;
+    const float maxit = 1000;
+    while (num <= maxit && err > 0.0005) {
+        for (int ix = 1; ix < npml; ix++)
+            for (int iz = 1; iz < nz - 1; iz++) {
+                SV(iz, ix) = 0.25 * (SV(iz, ix + 1) + SV(iz, ix - 1) + SV(iz + 1, ix) + SV(iz - 1, ix));
+                SV(iz, ix + nx - npml - 1) = 0.25 * (SV(iz, ix + nx - npml) + SV(iz, ix + nx - npml - 2) +
+                                                     SV(iz + 1, ix + nx - npml - 1) + SV(iz - 1, ix + nx - npml - 1));
+            }
+        for (int ix = 1; ix < nx - 1; ix++)
+            for (int iz = 1; iz < npml; iz++) {
+                SV(iz, ix) = 0.25 * (SV(iz, ix + 1) + SV(iz, ix - 1) + SV(iz + 1, ix) + SV(iz - 1, ix));
+                SV(iz + nz - npml - 1, ix) = 0.25 * (SV(iz + nz - npml - 1, ix + 1) + SV(iz + nz - npml - 1, ix - 1) +
+                                                     SV(iz + nz - npml, ix) + SV(iz + nz - npml - 2, ix));
+            }
+        for (int iz = 1; iz < nz - 1; iz++) {
+            SV(iz, 0) = 0.25 * (SV(iz, 0) + SV(iz, 1) + SV(iz + 1, 0) + SV(iz - 1, 0));
+            SV(iz, nx - 1) = 0.25 * (SV(iz, nx - 1) + SV(iz, nx - 2) + SV(iz + 1, nx - 1) + SV(iz - 1, nx - 1));
+        }
+        for (int ix = 1; ix < nx - 1; ix++) {
+            SV(0, ix) = 0.25 * (SV(0, ix) + SV(1, ix) + SV(0, ix - 1) + SV(0, ix + 1));
+            SV(nz - 1, ix) = 0.25 * (SV(nz - 1, ix) + SV(nz - 2, ix) + SV(nz - 1, ix - 1) + SV(nz - 1, ix + 1));
+        }
+        SV(0, 0) = 0.25 * (SV(0, 0) + SV(1, 0) + SV(0, 1) + SV(1, 1));
+        SV(nz - 1, 0) = 0.25 * (SV(nz - 1, 0) + SV(nz - 2, 0) + SV(nz - 1, 1) + SV(nz - 2, 1));
+        SV(0, nx - 1) = 0.25 * (SV(0, nx - 1) + SV(0, nx - 2) + SV(1, nx - 1) + SV(1, nx - 2));
+        SV(nz - 1, nx - 1) = 0.25 * (SV(nz - 1, nx - 1) + SV(nz - 2, nx - 1) + SV(nz - 1, nx - 2) + SV(nz - 2, nx - 2));
+        float m = -1.0e+5f;
+        for (size_t i = 0; i < n; i++) {
+            const float r = fabsf(w[i] - v[i]) / fabsf(w[i]);
+            m = r > m ? r : m;
+        }
+        err = m;
+        memcpy(w, v, n * sizeof(float));
+        num++;
+    }
+#undef SV
+    free(v);
+}
+
+/* velsmooth1d (Velocity_function.cpp:199-313), applied to the padded v^2 and
+ * density windows before the reverse pass when ismth > 0 (rtm_model.cu:
+ * 402-405): the window extended by nsp cells (edge values), a (2 nsp + 1)^2
+ * Gaussian window with sigma = nsp/3 summed in double in row order, then
+ * divided by the sum of the weights.  w is n1 (z) x n2 (x), x fastest. */
+ORC_EXPORT void orc_gauss_smooth(float *w, int n1, int n2, int nsp)
+{
+    const int n1e = n1 + 2 * nsp, n2e = n2 + 2 * nsp;
+    float *b = (float *)calloc((size_t)n1e * n2e, sizeof(float));
+    float *a = (float *)calloc((size_t)n1 * n2, sizeof(float));
+#define SB(i, j) b[(size_t)(i) * n2e + (j)]
+#define SA(i, j) a[(size_t)(i) * n2 + (j)]
+    for (int i1 = 0; i1 < n1; i1++)
+        for (int i2 = 0; i2 < n2; i2++) SB(i1 + nsp, i2 + nsp) = w[(size_t)i1 * n2 + i2];
+    for (int i1 = 0; i1 < n1; i1++)
+        for (int i2 = 0; i2 < nsp; i2++) {
+            SB(i1 + nsp, i2) = w[(size_t)i1 * n2];
+            SB(i1 + nsp, i2 + n2 + nsp) = w[(size_t)i1 * n2 + n2 - 1];
+        }
+    for (int i1 = 0; i1 < nsp; i1++)
+        for (int i2 = 0; i2 < n2; i2++) {
+            SB(i1, i2 + nsp) = w[i2];
+            SB(nsp + n1 + i1, i2 + nsp) = w[(size_t)(n1 - 1) * n2 + i2];
+        }
+    for (int i1 = 0; i1 < nsp; i1++)
+        for (int i2 = 0; i2 < nsp; i2++) {
+            SB(i1, i2) = w[0];
+            SB(i1, nsp + n2 + i2) = w[n2 - 1];
+            SB(i1 + nsp + n1, i2) = w[(size_t)(n1 - 1) * n2];
+            SB(i1 + nsp + n1, i2 + nsp + n2) = w[(size_t)(n1 - 1) * n2 + n2 - 1];
+        }
+    for (int i1 = nsp; i1 < n1 + nsp; i1++)
+        for (int i2 = nsp; i2 < n2 + nsp; i2++) {
+            double s = 0;
+            for (int i11 = i1 - nsp; i11 <= i1 + nsp; i11++)
+                for (int i22 = i2 - nsp; i22 <= i2 + nsp; i22++) {
+                    const double d1 = i11 - i1, d2 = i22 - i2;
+                    const double wt = exp(-(d1 * d1 + d2 * d2) / (2.0 * nsp / 3.0 * nsp / 3.0));
+                    s += wt * SB(i11, i22);
+                }
+            SA(i1 - nsp, i2 - nsp) = (float)s;
+        }
+    double norm = 0;
+    for (int i11 = 0; i11 <= 2.0 * nsp; i11++)
+        for (int i22 = 0; i22 <= 2.0 * nsp; i22++) {
+            const double d1 = i11 - nsp, d2 = i22 - nsp;
+            norm += exp(-(d1 * d1 + d2 * d2) / (2.0 * nsp / 3.0 * nsp / 3.0));
+        }
+    for (size_t i = 0; i < (size_t)n1 * n2; i++) w[i] = (float)(a[i] / norm);
+#undef SA
+#undef SB
+    free(a); free(b);
+}
+
 /* cuda_mute1/2/3 (kernel.cu:96-173): last muted sample per receiver.
  * vvpad: padded v^2 window of the shot (NULL: built from vp0). */
 ORC_EXPORT void orc_mute_cuts(const orc_params *P, const orc_geom *g, const float *unused,

@@ -34,6 +34,7 @@ typedef struct {
     RT dt, dtx, dtz;
     RT coef[100];                       /* diffcoef[(r-1)*10 + i-1]                  */
     RT *rho, *vv;                       /* padded window: density, v^2 (nzp*nxp)     */
+    float *rhf, *vvf;                   /* the same windows in float, as smoothed    */
     RT *ib, *kk;                        /* kernel order: 1/rho, rho*v^2 (nzp*nxp)    */
     RT *ax, *bx, *az, *bz;              /* kernel order: c1/c2, 1/c2 at p nodes      */
     RT *axv, *bxv, *azv, *bzv;          /* ... at vx / vz half nodes                 */
@@ -83,7 +84,7 @@ static void SFX(wf_free)(SFX(wfield) *w)
 
 static void SFX(med_free)(SFX(medium) *m)
 {
-    free(m->rho); free(m->vv); free(m->ib); free(m->kk);
+    free(m->rho); free(m->vv); free(m->ib); free(m->kk); free(m->rhf); free(m->vvf);
     free(m->ax); free(m->bx); free(m->az); free(m->bz);
     free(m->axv); free(m->bxv); free(m->azv); free(m->bzv);
     free(m->ddx); free(m->ddz); free(m->ddxv); free(m->ddzv);
@@ -162,6 +163,11 @@ static void SFX(med_setup)(SFX(medium) *m, const orc_geom *g, const orc_params *
         orc_window_pad(vvf, vp0, P->nx, P->nz, g->nx1, g->nx2, npml, 1);
         orc_window_pad(rhf, rho0, P->nx, P->nz, g->nx1, g->nx2, npml, 0);
     }
+    /* pml layers smooth (rtm_model.cu:204-206), before vpmax */
+    if (P->medium_flag) {
+        orc_pml_smooth(vvf, nxp, nzp, npml);
+        orc_pml_smooth(rhf, nxp, nzp, npml);
+    }
     m->vv = (RT *)malloc(nxz * sizeof(RT));
     m->rho = (RT *)malloc(nxz * sizeof(RT));
     float vmax2 = vvf[0];
@@ -174,7 +180,7 @@ static void SFX(med_setup)(SFX(medium) *m, const orc_geom *g, const orc_params *
 #else
     RT vpmax = sqrtf(vmax2);                    /* rtm_model.cu:207 */
 #endif
-    free(vvf); free(rhf);
+    m->vvf = vvf; m->rhf = rhf;
 
     m->ddx = (RT *)malloc(nxp * sizeof(RT));  m->ddz = (RT *)malloc(nzp * sizeof(RT));
     m->ddxv = (RT *)malloc((nxp - 1) * sizeof(RT)); m->ddzv = (RT *)malloc((nzp - 1) * sizeof(RT));
@@ -656,11 +662,21 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
         for (size_t i = 0; i < (size_t)nt * g.nrec; i++) rec[i] = (RT)seis_in[i];
     /* direct-wave mute (kernel.cu:96-173) */
     int *cut = (int *)malloc(sizeof(int) * (g.nrec > 0 ? g.nrec : 1));
-    orc_mute_cuts(P, &g, NULL, vp0, o->vvpad, cut);
+    orc_mute_cuts(P, &g, NULL, vp0, m.vvf, cut);      /* the model of the forward pass */
     for (int ir = 0; ir < g.nrec; ir++)
         for (int it = 0; it < nt && it <= cut[ir]; it++) rec[(size_t)it * g.nrec + ir] = 0;
     free(cut);
 
+    /* velsmooth1d of the window before the reverse pass (rtm_model.cu:402-405):
+     * reconstruction and receiver propagation use the smoothed model */
+    if (P->ismth > 0) {
+        orc_gauss_smooth(m.vvf, nzp, nxp, P->ismth);
+        orc_gauss_smooth(m.rhf, nzp, nxp, P->ismth);
+        for (size_t i = 0; i < (size_t)nxp * nzp; i++) {
+            m.vv[i] = (RT)m.vvf[i]; m.rho[i] = (RT)m.rhf[i];
+            m.ib[i] = (RT)1 / m.rho[i]; m.kk[i] = m.rho[i] * m.vv[i];
+        }
+    }
     /* reverse: ps <- P_{nt-1}; vxs = vzs = 0 (rtm_model.cu:366-372) */
     memset(S.vx, 0, (size_t)nzp * (nxp - 1) * sizeof(RT));
     memset(S.vz, 0, (size_t)(nzp - 1) * nxp * sizeof(RT));

@@ -34,6 +34,30 @@ def test_window_pad():
         assert np.array_equal(O.window_pad(rho, nx1, nx2, npml, False), d[f"rho_{nx1}_{nx2}"])
 
 
+def _smooth(lib, fn, w, *args):
+    out = np.ascontiguousarray(w, np.float32).copy()
+    getattr(lib, fn)(G._fp(out), *[C.c_int(int(x)) for x in args])
+    return out
+
+
+def test_model_smoothing():
+    """pmlvelsmooth1d / velsmooth1d outputs of the reference itself: the
+    oracle's restatement and the library's host functions equal them."""
+    d = load("smoothing.npz")
+    base, npml, nsp = d["base"], int(d["npml"]), int(d["nsp"])
+    nzp, nxp = base.shape
+    for lib, pml, gauss in ((O.lib(), "orc_pml_smooth", "orc_gauss_smooth"),
+                            (G.lib(), "rtmh_pml_smooth", "rtmh_gauss_smooth")):
+        assert np.array_equal(_smooth(lib, pml, base, nxp, nzp, npml), d["pml"]), pml
+        assert np.array_equal(_smooth(lib, gauss, base, nzp, nxp, nsp), d["smooth"]), gauss
+    # the absorbing layer is relaxed, the interior untouched; the Gaussian
+    # filter preserves a constant window
+    pml = d["pml"]
+    assert np.array_equal(pml[npml:nzp - npml, npml:nxp - npml], base[npml:nzp - npml, npml:nxp - npml])
+    const = np.full((13, 17), 2500.0, np.float32)
+    assert np.allclose(_smooth(G.lib(), "rtmh_gauss_smooth", const, 13, 17, 4), const, rtol=1e-6, atol=0)
+
+
 def test_interp_model():
     d = load("interpvel.npz")
     vp0 = d["vp0"]
@@ -118,3 +142,12 @@ def test_live_cross_check_with_reference_helpers():
         a = img.copy()
         L.ref_laplacian(C.c_int(nx), C.c_int(nz), C.c_int(9), C.c_int(11), G._fp(a))
         assert np.array_equal(O.laplacian(img, 9, 11), a)
+        # model smoothing on a padded window of random size, layer and radius
+        wz, wx, wp, ns = int(rng.integers(22, 40)), int(rng.integers(22, 50)), int(rng.integers(2, 10)), int(rng.integers(1, 6))
+        win = rng.uniform(1500, 4500, (wz, wx)).astype(np.float32)
+        ref_pml = _smooth(L, "ref_pmlvelsmooth", win, wx, wz, wp)
+        ref_gau = _smooth(L, "ref_velsmooth", win, wz, wx, ns)
+        for lib, pml, gauss in ((O.lib(), "orc_pml_smooth", "orc_gauss_smooth"),
+                                (G.lib(), "rtmh_pml_smooth", "rtmh_gauss_smooth")):
+            assert np.array_equal(_smooth(lib, pml, win, wx, wz, wp), ref_pml), (pml, wz, wx, wp)
+            assert np.array_equal(_smooth(lib, gauss, win, wz, wx, ns), ref_gau), (gauss, wz, wx, ns)

@@ -175,3 +175,26 @@ def test_ragged_windows_vs_oracle(gpu_ctx):
         assert r.shot_g1[k].shape == o.g1.shape
         assert np.array_equal(r.shot_seis[k], o.seis)
         assert np.array_equal(r.shot_g1[k], o.g1)
+
+
+@pytest.mark.parametrize("medium_flag,ismth", [(1, 0), (0, 2), (1, 3)])
+def test_model_smoothing_vs_oracle(gpu_ctx, medium_flag, ismth):
+    """medium_flag: the absorbing layer of each shot's window relaxed before
+    the forward pass (pmlvelsmooth1d, also changing vpmax and the damping);
+    ismth: the window Gaussian-smoothed before the reverse pass (velsmooth1d).
+    Two shots with different windows in one batch."""
+    vp, rho = small_model(nz=60, nx=200, seed=12)
+    kw = job(nx=200, nz=60, nshot=2, dshot=1, flag_layout=1, npml=32, spx0=700.0, dspx=500.0, offsetmax=800.0,
+             nt=260, medium_flag=medium_flag, ismth=ismth)
+    P, Po = G.make_params(**kw), O.make_params(**kw)
+    print(f"[smoothing] medium_flag={medium_flag} ismth={ismth}", flush=True)
+    r = gpu_ctx.run(P, vp, rho, batch=2, want_shots=True)
+    plain = gpu_ctx.run(G.make_params(**dict(kw, medium_flag=0, ismth=0)), vp, rho, batch=2, want_shots=True)
+    for k in range(2):
+        o = O.run_shot(Po, k + 1, vp, rho, korder=True)
+        assert np.array_equal(r.shot_seis[k], o.seis), k
+        assert np.array_equal(r.shot_g1[k], o.g1), k
+        assert np.array_equal(r.shot_illum[k], o.illum), k
+        o32 = O.run_shot(Po, k + 1, vp, rho)
+        assert rel_l2(r.shot_seis[k], o32.seis) <= 1e-5 and rel_l2(r.shot_g1[k], o32.g1) <= 1e-4
+        assert not np.array_equal(r.shot_g1[k], plain.shot_g1[k])      # the smoothing took effect
