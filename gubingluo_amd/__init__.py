@@ -47,7 +47,8 @@ class Params(C.Structure):
 class Options(C.Structure):
     _fields_ = [("batch", C.c_int), ("store_fmt", C.c_int), ("graph_chunk", C.c_int), ("tile_z", C.c_int),
                 ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int),
-                ("shot_offsetmax", C.c_void_p)]
+                ("shot_offsetmax", C.c_void_p), ("checkpoint", C.c_char_p), ("resume", C.c_int),
+                ("_pad_opt", C.c_int)]
 
 
 class Outputs(C.Structure):
@@ -202,8 +203,11 @@ class Context:
 
     def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
             graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
-            want_post=True, seis_in=None, profile=False, shot_offsetmax=None) -> RunResult:
-        """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model."""
+            want_post=True, seis_in=None, profile=False, shot_offsetmax=None, checkpoint=None,
+            resume=False) -> RunResult:
+        """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model.
+        checkpoint: path written after every batch (FP64 stacks, shots done); resume=True
+        continues a job from it."""
         coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
         nz, nx = P.nz, P.nx
         if model is None:
@@ -248,8 +252,9 @@ class Context:
             out.shot_seis_in = C.cast(pa, C.c_void_p)
         som = None if shot_offsetmax is None else np.ascontiguousarray(shot_offsetmax, np.float32)
         keep.append(som)
+        ck = None if checkpoint is None else os.fsencode(checkpoint)
         opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
-                      int(profile), _fp(som))
+                      int(profile), _fp(som), ck, int(resume), 0)
         st = Stats()
         if model is None:
             rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),

@@ -258,6 +258,72 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
   }
 }
 
+// ------------------------------------------------------------ checkpoint ----
+// File: "RTMCKPT1", job fingerprint, nx, nz, shots done, then the three FP64
+// stacks (image1, image2s, illum accumulators), nz*nx each.
+struct CkptHead {
+  char magic[8];
+  uint64_t fp;
+  int32_t nx, nz, done, version;
+};
+
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n)
+{
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 1099511628211ULL; }
+  return h;
+}
+
+// everything that changes a shot's image except which shots run: parameters
+// but nshot, the model, the coefficients, the boundary precision, the apertures
+uint64_t Engine::job_fingerprint(const rtmh_params& P, const Model& M, const float* coef, const rtmh_options& opt,
+                                 int nsh)
+{
+  rtmh_params q = P;
+  q.nshot = 0;
+  uint64_t h = 1469598103934665603ULL;
+  h = fnv1a(h, &q, sizeof(q));
+  h = fnv1a(h, &M.nx, sizeof(M.nx));
+  h = fnv1a(h, &M.nz, sizeof(M.nz));
+  h = fnv1a(h, M.vp0.data(), M.vp0.size() * 4);
+  h = fnv1a(h, M.rho0.data(), M.rho0.size() * 4);
+  h = fnv1a(h, coef, 100 * 4);
+  h = fnv1a(h, &opt.store_fmt, sizeof(opt.store_fmt));
+  if (opt.shot_offsetmax) h = fnv1a(h, opt.shot_offsetmax, (size_t)nsh * 4);
+  return h;
+}
+
+// 1: read, 0: no file, -1: another job's or damaged
+int Engine::read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int& done, std::vector<double>& st3)
+{
+  FILE* f = fopen(path, "rb");
+  if (!f) return 0;
+  CkptHead h;
+  const size_t n = (size_t)nx * nz * 3;
+  st3.resize(n);
+  const bool ok = fread(&h, sizeof(h), 1, f) == 1 && memcmp(h.magic, "RTMCKPT1", 8) == 0 && h.fp == fp &&
+                  h.nx == nx && h.nz == nz && h.done >= 0 && fread(st3.data(), 8, n, f) == n;
+  fclose(f);
+  if (!ok) return -1;
+  done = h.done;
+  return 1;
+}
+
+int Engine::write_checkpoint(const char* path, uint64_t fp, int nx, int nz, int done, const std::vector<double>& st3)
+{
+  CkptHead h;
+  memcpy(h.magic, "RTMCKPT1", 8);
+  h.fp = fp; h.nx = nx; h.nz = nz; h.done = done; h.version = 1;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return -1;
+  bool ok = fwrite(&h, sizeof(h), 1, f) == 1 && fwrite(st3.data(), 8, st3.size(), f) == st3.size();
+  ok = (fflush(f) == 0) && ok;
+  ok = (fclose(f) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), path) != 0) { remove(tmp.c_str()); return -1; }
+  return 0;
+}
+
 // fn(0..n-1) on up to 16 host threads (independent items)
 static void parallel_for(int n, const std::function<void(int)>& fn)
 {
@@ -718,6 +784,26 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   HIPCHK(rtm_launch_zero_f64(d_il, (long long)nm, st_));
   HIPCHK(hipStreamSynchronize(st_));
 
+  // checkpoint / resume: the stacks of the shots done so far, in job order
+  const uint64_t job_fp = job_fingerprint(P, *M, coef, opt, nsh);
+  int k_start = 0;
+  if (opt.checkpoint && opt.resume) {
+    std::vector<double> st3;
+    int done = 0;
+    const int cr = read_checkpoint(opt.checkpoint, job_fp, nx, nz, done, st3);
+    if (cr < 0) {
+      set_error(std::string("checkpoint ") + opt.checkpoint + " belongs to another job or is damaged");
+      pool_.put(common);
+      return -5;
+    }
+    if (cr == 1) {
+      k_start = std::min(done, nsh);
+      double* dst[3] = {d_i1, d_i2, d_il};
+      for (int i = 0; i < 3; i++)
+        HIPCHK(hipMemcpy(dst[i], st3.data() + (size_t)i * nm, nm * 8, hipMemcpyHostToDevice));
+    }
+  }
+
   // memory budget and batch size
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
@@ -733,7 +819,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   std::vector<double> img_tmp;  // for temp-image checkpoints
   int rc = 0;
 
-  for (int k0 = 0; k0 < nsh && rc == 0;) {
+  for (int k0 = k_start; k0 < nsh && rc == 0;) {
     // greedy batch of consecutive shots within the budget
     int k1 = k0;
     double used = 0;
@@ -985,6 +1071,16 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     if (opt.verbose)
       for (int j = 0; j < nb; j++) printf("%d shot finished: %g (s)\n", shots[k0 + j], 0.0);
     for (void* p : mem) pool_.put(p);
+    if (opt.checkpoint) {
+      std::vector<double> st3(3 * nm);
+      const double* src[3] = {d_i1, d_i2, d_il};
+      for (int i = 0; i < 3; i++) HIPCHK(hipMemcpy(st3.data() + (size_t)i * nm, src[i], nm * 8, hipMemcpyDeviceToHost));
+      if (write_checkpoint(opt.checkpoint, job_fp, nx, nz, k1, st3) != 0) {
+        set_error(std::string("cannot write checkpoint ") + opt.checkpoint);
+        rc = -6;
+        break;
+      }
+    }
     k0 = k1;
   }
   if (rc != 0) { pool_.put(common); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); return rc; }

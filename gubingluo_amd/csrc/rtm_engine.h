@@ -96,6 +96,11 @@ class Engine {
   int model_create(int nx, int nz, const float* vp0, const float* rho0, int* id);
   int model_destroy(int id);
   const Model* model(int id) const;
+  // checkpoint file of a job (FP64 stacks after the shots done)
+  static uint64_t job_fingerprint(const rtmh_params& P, const Model& M, const float* coef, const rtmh_options& opt,
+                                  int nsh);
+  static int read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int& done, std::vector<double>& st3);
+  static int write_checkpoint(const char* path, uint64_t fp, int nx, int nz, int done, const std::vector<double>& st3);
 
   int run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
           rtmh_outputs* out, rtmh_stats* stats);

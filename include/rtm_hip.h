@@ -81,6 +81,18 @@ typedef struct rtmh_options {
   int profile;        /* 1: plain launches timed one by one with HIP events       */
   const float *shot_offsetmax; /* per shot (job order) aperture overriding offsetmax
                                   (ragged windows); NULL: offsetmax for every shot */
+  /* Checkpoint / resume (no counterpart in the reference, whose temp images
+   * every 50 shots are fp32 snapshots it never reads back): after every batch
+   * the FP64 image stacks and the number of shots done are written to
+   * `checkpoint` (atomically: a temporary file renamed over it), with a
+   * fingerprint of the job (parameters except nshot, model, coefficients,
+   * boundary precision, per-shot apertures).  With resume = 1 a job whose
+   * fingerprint matches continues after the shots done, and its images equal
+   * those of an uninterrupted run bit for bit; a checkpoint of another job is
+   * an error, a missing file starts from the first shot. */
+  const char *checkpoint;      /* NULL: no checkpoint */
+  int resume;
+  int _pad_opt;
 } rtmh_options;
 
 /* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */

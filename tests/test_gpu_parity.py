@@ -198,3 +198,21 @@ def test_model_smoothing_vs_oracle(gpu_ctx, medium_flag, ismth):
         o32 = O.run_shot(Po, k + 1, vp, rho)
         assert rel_l2(r.shot_seis[k], o32.seis) <= 1e-5 and rel_l2(r.shot_g1[k], o32.g1) <= 1e-4
         assert not np.array_equal(r.shot_g1[k], plain.shot_g1[k])      # the smoothing took effect
+
+
+def test_checkpoint_resume_equals_uninterrupted(gpu_ctx, tmp_path):
+    """a job stopped after 3 of 5 shots and resumed from its checkpoint gives the
+    uninterrupted job's images bit for bit; another job's checkpoint is refused"""
+    vp, rho = small_model(nz=50, nx=160, seed=13)
+    kw = job(nx=160, nz=50, nshot=5, dshot=1, flag_layout=1, offsetmax=600.0, spx0=300.0, dspx=120.0, nt=220)
+    print("[checkpoint] 5 shots vs 3 + resume", flush=True)
+    full = gpu_ctx.run(G.make_params(**kw), vp, rho, batch=2)
+    ck = str(tmp_path / "job.ckpt")
+    first = gpu_ctx.run(G.make_params(**dict(kw, nshot=3)), vp, rho, batch=2, checkpoint=ck)
+    assert first.stats.batches == 2
+    rest = gpu_ctx.run(G.make_params(**kw), vp, rho, batch=2, checkpoint=ck, resume=True)
+    assert rest.stats.batches == 1                           # shots 4 and 5 only
+    for name in ("image1", "image2s", "illum", "image_lap", "image_norm"):
+        assert np.array_equal(getattr(rest, name), getattr(full, name)), name
+    with pytest.raises(RuntimeError, match="another job"):
+        gpu_ctx.run(G.make_params(**dict(kw, fdom=25.0)), vp, rho, batch=2, checkpoint=ck, resume=True)
