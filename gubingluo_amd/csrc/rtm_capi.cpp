@@ -142,6 +142,10 @@ extern "C" void rtm_model(int idevice, int nshot, int startshot, int dshot, int 
   o.graph_chunk = 50;
   o.write_files = 1;
   o.verbose = 1;
+  // checkpoint / resume for the drop-in, whose signature is the reference's:
+  // RTMH_CHECKPOINT names the file (written after every batch, resumed from)
+  const char* ck = getenv("RTMH_CHECKPOINT");
+  if (ck && *ck) { o.checkpoint = ck; o.resume = 1; }
   rtmh_outputs out;
   memset(&out, 0, sizeof(out));
   out.image1 = image1;

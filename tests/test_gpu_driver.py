@@ -108,3 +108,13 @@ def test_rtm_gpu_driver_images(gpu_ctx, tmp_path):
     i1, i2, il = O.rtm(O.make_params(**kw), vp, rho, stack64=True, korder=True)
     assert np.array_equal(out["Image1"], i1) and np.array_equal(out["Image2s"], i2) and np.array_equal(out["Illum"], il)
     assert np.abs(i1).max() > 0
+    # RTMH_CHECKPOINT: the first run writes the checkpoint, the second finds every
+    # shot done and writes the same images from the stacks it read back
+    env = dict(os.environ, RTMH_CHECKPOINT=str(tmp_path / "job.ckpt"))
+    for rep in range(2):
+        r = subprocess.run([EXE], cwd=str(tmp_path), capture_output=True, text=True, timeout=150, env=env)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert (tmp_path / "job.ckpt").stat().st_size > 3 * nx * nz * 8
+        for n in ("Image1", "Image2s", "Illum"):
+            assert np.array_equal(read_su(str(tmp_path / "output" / f"0{n}.su"), nz, nx), out[n]), (rep, n)
+    assert "shot finished" not in r.stdout                          # the second run ran no shot
