@@ -48,14 +48,18 @@ class Options(C.Structure):
     _fields_ = [("batch", C.c_int), ("store_fmt", C.c_int), ("graph_chunk", C.c_int), ("tile_z", C.c_int),
                 ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int),
                 ("shot_offsetmax", C.c_void_p), ("checkpoint", C.c_char_p), ("resume", C.c_int),
-                ("_pad_opt", C.c_int)]
+                ("ic_extra", C.c_int)]
+
+
+# the other imaging conditions (rtmh_outputs order)
+IC_IMAGES = ("image3s", "image4s", "image5s", "image3m", "image4m", "image5m", "imagetrue")
 
 
 class Outputs(C.Structure):
     _fields_ = [("image1", C.c_void_p), ("image2s", C.c_void_p), ("illum", C.c_void_p),
                 ("image_lap", C.c_void_p), ("image_norm", C.c_void_p),
                 ("shot_seis", C.c_void_p), ("shot_g1", C.c_void_p), ("shot_illum", C.c_void_p),
-                ("shot_seis_in", C.c_void_p)]
+                ("shot_seis_in", C.c_void_p)] + [(n, C.c_void_p) for n in IC_IMAGES]
 
 
 class Stats(C.Structure):
@@ -159,6 +163,7 @@ class RunResult:
     shot_g1: list | None
     shot_illum: list | None
     stats: Stats
+    ic: dict | None = None    # ic_extra: image3s, image4s, image5s, image3m, image4m, image5m, imagetrue
 
 
 class Context:
@@ -204,7 +209,7 @@ class Context:
     def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
             graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
             want_post=True, seis_in=None, profile=False, shot_offsetmax=None, checkpoint=None,
-            resume=False) -> RunResult:
+            resume=False, ic_extra=False) -> RunResult:
         """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model.
         checkpoint: path written after every batch (FP64 stacks, shots done); resume=True
         continues a job from it."""
@@ -220,6 +225,10 @@ class Context:
         lap = np.zeros((nz, nx), np.float32) if want_post else None
         nrm = np.zeros((nz, nx), np.float32) if want_post else None
         out = Outputs(_fp(img1), _fp(img2), _fp(ill), _fp(lap), _fp(nrm), None, None, None, None)
+        ic = {n: np.zeros((nz, nx), np.float32) for n in IC_IMAGES} if ic_extra else None
+        if ic is not None:
+            for n, a in ic.items():
+                setattr(out, n, _fp(a))
         shots = shot_list(P)
         keep = []
         seis = g1s = ils = None
@@ -254,7 +263,7 @@ class Context:
         keep.append(som)
         ck = None if checkpoint is None else os.fsencode(checkpoint)
         opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
-                      int(profile), _fp(som), ck, int(resume), 0)
+                      int(profile), _fp(som), ck, int(resume), int(ic_extra))
         st = Stats()
         if model is None:
             rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),
@@ -264,7 +273,7 @@ class Context:
                                       C.byref(st))
         if rc != 0:
             raise RuntimeError(f"rtmh_run failed ({rc}): {last_error()}")
-        return RunResult(img1, img2, ill, lap, nrm, seis, g1s, ils, st)
+        return RunResult(img1, img2, ill, lap, nrm, seis, g1s, ils, st, ic)
 
     def copy_probe(self, nbytes: int, reps: int = 3) -> float:
         """Device-to-device copy bandwidth, GB/s counting read + written bytes."""

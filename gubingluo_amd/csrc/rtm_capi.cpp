@@ -146,11 +146,17 @@ extern "C" void rtm_model(int idevice, int nshot, int startshot, int dshot, int 
   // RTMH_CHECKPOINT names the file (written after every batch, resumed from)
   const char* ck = getenv("RTMH_CHECKPOINT");
   if (ck && *ck) { o.checkpoint = ck; o.resume = 1; }
+  o.ic_extra = 1;                                      // every image the reference computes
+  const size_t n = (size_t)nx * nz;
+  std::vector<float> imagetrue(n, 0.0f);
   rtmh_outputs out;
   memset(&out, 0, sizeof(out));
   out.image1 = image1;
   out.image2s = image2s;
   out.illum = illum;
+  out.image3s = image3s; out.image4s = image4s; out.image5s = image5s;
+  out.image3m = image3m; out.image4m = image4m; out.image5m = image5m;
+  out.imagetrue = imagetrue.data();
   rtmh_stats st;
   memset(&st, 0, sizeof(st));
   const int rc = rtmh_run(ctx, &P, vp0, rho0, diffcoef, &o, &out, &st);
@@ -160,12 +166,7 @@ extern "C" void rtm_model(int idevice, int nshot, int startshot, int dshot, int 
     exit(1);
   }
   rtmh_destroy(ctx);
-  const size_t n = (size_t)nx * nz;
-  memcpy(image2m, image1, n * sizeof(float));
-  float* zeros[] = {image3s, image4s, image5s, image3m, image4m, image5m};
-  for (float* z : zeros)
-    if (z) memset(z, 0, n * sizeof(float));
-  std::vector<float> imagetrue(n, 0.0f);
+  memcpy(image2m, image1, n * sizeof(float));          // the reference's normalisation calls are commented out
   char path[64];
   snprintf(path, sizeof(path), "./output/%dImagetrue.su", idevice);
   rtmio_write_image(imagetrue.data(), nz, nx, dx, path);

@@ -90,6 +90,10 @@ struct ShotDev {
   float* sscale;               // nt x 4 scales (16-bit formats), then 8 max-abs accumulators
   long long strip_len;         // values per step
   int store_fmt, _pad4;
+  // the other imaging conditions (null when off): ud du lr rl g4 (nzi x nxl
+  // each, pitch nxl), then 61 bins of g31 and 61 of g32
+  float* ic;
+  float dx, dz;                // grid spacings (central differences of the energy norm)
 };
 
 struct Tile {

@@ -289,17 +289,18 @@ uint64_t Engine::job_fingerprint(const rtmh_params& P, const Model& M, const flo
   h = fnv1a(h, M.rho0.data(), M.rho0.size() * 4);
   h = fnv1a(h, coef, 100 * 4);
   h = fnv1a(h, &opt.store_fmt, sizeof(opt.store_fmt));
+  h = fnv1a(h, &opt.ic_extra, sizeof(opt.ic_extra));
   if (opt.shot_offsetmax) h = fnv1a(h, opt.shot_offsetmax, (size_t)nsh * 4);
   return h;
 }
 
 // 1: read, 0: no file, -1: another job's or damaged
-int Engine::read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int& done, std::vector<double>& st3)
+int Engine::read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int nstk, int& done, std::vector<double>& st3)
 {
   FILE* f = fopen(path, "rb");
   if (!f) return 0;
   CkptHead h;
-  const size_t n = (size_t)nx * nz * 3;
+  const size_t n = (size_t)nx * nz * nstk;
   st3.resize(n);
   const bool ok = fread(&h, sizeof(h), 1, f) == 1 && memcmp(h.magic, "RTMCKPT1", 8) == 0 && h.fp == fp &&
                   h.nx == nx && h.nz == nz && h.done >= 0 && fread(st3.data(), 8, n, f) == n;
@@ -381,6 +382,7 @@ bool GraphKey::operator<(const GraphKey& o) const
   if (save != o.save) return save < o.save;
   if (fmt != o.fmt) return fmt < o.fmt;
   if (k != o.k) return k < o.k;
+  if (ic != o.ic) return ic < o.ic;
   if (shots != o.shots) return shots < o.shots;
   return tiles < o.tiles;
 }
@@ -433,7 +435,7 @@ static SmallLayout small_layout(const ShotSetup& s)
   return L;
 }
 
-size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
+size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic)
 {
   const size_t pitch = pitch_of(s.g.nxp), nzp = s.g.nzp;
   const size_t F = align256((size_t)frows_of((int)nzp) * pitch * 4);
@@ -445,14 +447,16 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt)
   b += align256((size_t)nt * strip * store_elem(store_fmt));
   b += align256((size_t)nt * 4 * 4 + 8 * 4);         // scales + 16-bit max-abs accumulators
   if (!s.wvv.empty()) b += 2 * align256((size_t)s.g.nxp * s.g.nzp * 4);   // smoothed windows
+  if (ic) b += align256((size_t)(5 + 2 * 61) * nzi * s.g.nxl * 4);         // other imaging conditions
   return b;
 }
 
 void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
                          std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                         const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch)
+                         const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch, bool ic)
 {
   memset(&d, 0, sizeof(d));
+  d.dx = P.dx; d.dz = P.dz;
   const rtmh_geom& g = s.g;
   d.nxp = g.nxp; d.nzp = g.nzp; d.npml = P.npml; d.pitch = pitch_of(g.nxp);
   d.nxl = g.nxl; d.nzi = P.nz; d.nt = P.nt; d.phase = PH_IDLE;
@@ -508,6 +512,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
     d.vv = wv; d.rho = wr;
     d.mpitch = g.nxp; d.mz0 = 0; d.mzn = g.nzp; d.mx0 = 0; d.mxs = 0; d.mxn = g.nxp;
   }
+  if (ic) { d.ic = (float*)(base + o); o += align256((size_t)(5 + 2 * 61) * d.nzi * d.nxl * 4); }
 }
 
 hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
@@ -522,6 +527,7 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
     a.step = s;
     (void)rtm_launch_step(a, k.phase, k.tz, st_);
     if (k.save) (void)rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_);
+    if (k.ic) (void)rtm_launch_ic(dshots, dslots, nslots, s, k.ic, st_);
   }
   (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
   if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) return nullptr;
@@ -538,6 +544,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
                      rtmh_stats* prof)
 {
   const int save = phase == PH_FWD ? 1 : 0;
+  const int ic = phase == PH_REV ? ic_cells_ : 0;          // the other imaging conditions after each reverse step
   const int nslots = (int)slots.size();
   int done = 0;
   if (prof) {
@@ -553,6 +560,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
       HIPCHK(rtm_launch_step(a, phase, tz, st_));
       HIPCHK(hipEventRecord(ev[1], st_));
       if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
+      if (ic) HIPCHK(rtm_launch_ic(dshots, dslots, nslots, s, ic, st_));
       HIPCHK(hipEventRecord(ev[2], st_));
       HIPCHK(hipEventSynchronize(ev[2]));
       float m0 = 0, m1 = 0;
@@ -565,7 +573,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     return 0;
   }
   if (chunk > 0 && nsteps >= chunk) {
-    GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, dshots, dtiles};
+    GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, ic, dshots, dtiles};
     hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
     if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
     const int nch = nsteps / chunk;
@@ -577,6 +585,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     a.step = s;
     HIPCHK(rtm_launch_step(a, phase, tz, st_));
     if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
+    if (ic) HIPCHK(rtm_launch_ic(dshots, dslots, nslots, s, ic, st_));
   }
   return 0;
 }
@@ -751,6 +760,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   // host as the reference does, shots in parallel); the slots then read them
   // instead of the resident model
   const bool smooth = P.medium_flag != 0 || P.ismth > 0;
+  const bool icx = opt.ic_extra != 0;
   if (smooth) parallel_for(nsh, [&](int k) { shot_windows(pk[k], vp0, M->rho0.data(), setup[k]); });
   int nwin = 50;
   for (int k = 0; k < nsh; k++) {
@@ -764,7 +774,8 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
 
   // device: model, coefficients, wavelet, FP64 stacks
   const size_t common_bytes = align256(100 * 4) + align256((size_t)nt * 4) +
-                              3 * align256(nm * 8) + 3 * align256(nm * 4);
+                              3 * align256(nm * 8) + 3 * align256(nm * 4) +
+                              (icx ? 7 * align256(nm * 8) + 7 * align256(nm * 4) + align256(nsh * sizeof(StackIcHost)) : 0);
   char* common = (char*)pool_.get(common_bytes);
   if (!common) { set_error("out of device memory (job)"); return -3; }
   const float* d_rho = M->d_rho;
@@ -777,12 +788,26 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   float* d_f1 = (float*)((char*)d_il + align256(nm * 8));
   float* d_f2 = (float*)((char*)d_f1 + align256(nm * 4));
   float* d_f3 = (float*)((char*)d_f2 + align256(nm * 4));
+  // the other imaging conditions: 7 FP64 stacks, their fp32 images, stack entries
+  double* d_i7 = icx ? (double*)((char*)d_f3 + align256(nm * 4)) : nullptr;
+  float* d_f7 = icx ? (float*)((char*)d_i7 + 7 * align256(nm * 8)) : nullptr;
+  StackIcHost* d_stkic = icx ? (StackIcHost*)((char*)d_f7 + 7 * align256(nm * 4)) : nullptr;
+  auto i7 = [&](int k) { return d_i7 + (size_t)k * nm; };     // contiguous: k_stack_ic indexes k*nx*nz
+  auto f7 = [&](int k) { return d_f7 + (size_t)k * nm; };
+  if (icx)
+    for (int k = 0; k < 7; k++) HIPCHK(rtm_launch_zero_f64(i7(k), (long long)nm, st_));
+  // every FP64 stack (checkpoint order)
+  std::vector<double*> stacks = {nullptr, nullptr, nullptr};
   HIPCHK(hipMemcpyAsync(d_coef, coef, 100 * 4, hipMemcpyHostToDevice, st_));
   HIPCHK(hipMemcpyAsync(d_wav, wav.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st_));
   HIPCHK(rtm_launch_zero_f64(d_i1, (long long)nm, st_));
   HIPCHK(rtm_launch_zero_f64(d_i2, (long long)nm, st_));
   HIPCHK(rtm_launch_zero_f64(d_il, (long long)nm, st_));
   HIPCHK(hipStreamSynchronize(st_));
+  stacks = {d_i1, d_i2, d_il};
+  if (icx)
+    for (int k = 0; k < 7; k++) stacks.push_back(i7(k));
+  const int nstk = (int)stacks.size();
 
   // checkpoint / resume: the stacks of the shots done so far, in job order
   const uint64_t job_fp = job_fingerprint(P, *M, coef, opt, nsh);
@@ -790,7 +815,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   if (opt.checkpoint && opt.resume) {
     std::vector<double> st3;
     int done = 0;
-    const int cr = read_checkpoint(opt.checkpoint, job_fp, nx, nz, done, st3);
+    const int cr = read_checkpoint(opt.checkpoint, job_fp, nx, nz, nstk, done, st3);
     if (cr < 0) {
       set_error(std::string("checkpoint ") + opt.checkpoint + " belongs to another job or is damaged");
       pool_.put(common);
@@ -798,9 +823,8 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     }
     if (cr == 1) {
       k_start = std::min(done, nsh);
-      double* dst[3] = {d_i1, d_i2, d_il};
-      for (int i = 0; i < 3; i++)
-        HIPCHK(hipMemcpy(dst[i], st3.data() + (size_t)i * nm, nm * 8, hipMemcpyHostToDevice));
+      for (int i = 0; i < nstk; i++)
+        HIPCHK(hipMemcpy(stacks[i], st3.data() + (size_t)i * nm, nm * 8, hipMemcpyHostToDevice));
     }
   }
 
@@ -824,7 +848,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     int k1 = k0;
     double used = 0;
     while (k1 < nsh && k1 - k0 < bmax) {
-      const double b = (double)slot_bytes(setup[k1], nt, opt.store_fmt);
+      const double b = (double)slot_bytes(setup[k1], nt, opt.store_fmt, icx);
       if (k1 > k0 && used + b > budget) break;
       used += b;
       k1++;
@@ -836,9 +860,9 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     std::vector<ShotDev> ds(nb);
     std::vector<std::pair<void*, std::vector<char>>> uploads;
     for (int j = 0; j < nb; j++) {
-      mem[j] = pool_.get(slot_bytes(setup[k0 + j], nt, opt.store_fmt));
+      mem[j] = pool_.get(slot_bytes(setup[k0 + j], nt, opt.store_fmt, icx));
       if (!mem[j]) { set_error("out of device memory (shot slots)"); rc = -3; break; }
-      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, d_rho, d_vv, nx);
+      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, d_rho, d_vv, nx, icx);
       ds[j].record = (P.mode == 0) ? 1 : 0;
     }
     if (rc != 0) { for (void* p : mem) pool_.put(p); break; }
@@ -885,6 +909,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
           a.step = c * kTuneSteps + s2;
           HIPCHK(rtm_launch_step(a, phase, cands[c], st_));
           if (phase == PH_FWD) HIPCHK(rtm_launch_save(d_ds_, d_slots_, nb, a.step, opt.store_fmt, st_));
+          else if (ic_cells_) HIPCHK(rtm_launch_ic(d_ds_, d_slots_, nb, a.step, ic_cells_, st_));
         }
         HIPCHK(hipEventRecord(te[c + 1], st_));
       }
@@ -974,6 +999,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     }
     if (rc) { for (void* p : mem) pool_.put(p); break; }
 
+    // the other imaging conditions of this batch: interior cells of its widest shot
+    ic_cells_ = 0;
+    if (icx)
+      for (int j = 0; j < nb; j++) ic_cells_ = std::max(ic_cells_, nz * setup[k0 + j].g.nxl);
+
     // velsmooth1d of each shot's window before its reverse pass (rtm_model.cu:
     // 402-405): reconstruction and receiver propagation use the smoothed model
     if (P.ismth > 0) {
@@ -1044,8 +1074,14 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     std::vector<StackShotHost> stk(nb);
     for (int j = 0; j < nb; j++) stk[j] = StackShotHost{ds[j].g1, ds[j].illum, (const float*)((const char*)mem[j] + small_layout(setup[k0 + j]).taper), setup[k0 + j].g.nx1, setup[k0 + j].g.nxl};
     HIPCHK(hipMemcpyAsync(d_stk, stk.data(), nb * sizeof(StackShotHost), hipMemcpyHostToDevice, st_));
+    if (icx) {
+      std::vector<StackIcHost> sic(nb);
+      for (int j = 0; j < nb; j++) sic[j] = StackIcHost{ds[j].ic, stk[j].illum, stk[j].taper, stk[j].nx1, stk[j].nxl};
+      HIPCHK(hipMemcpyAsync(d_stkic, sic.data(), nb * sizeof(StackIcHost), hipMemcpyHostToDevice, st_));
+    }
     for (int j = 0; j < nb; j++) {
       HIPCHK(rtm_launch_stack(d_stk + j, 1, nx, nz, d_i1, d_i2, d_il, st_));
+      if (icx) HIPCHK(rtm_launch_stack_ic(d_stkic + j, nx, nz, d_i7, st_));
       const int ishot = shots[k0 + j];
       if (opt.write_files && (ishot - 1) % 50 == 0) {
         std::vector<double> a(nm);
@@ -1059,11 +1095,18 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
           snprintf(path, sizeof(path), "./output/%stemp.su", o.name);
           rtmio_write_image(f.data(), nz, nx, P.dx, path);
         }
-        const char* zeros[] = {"Image3s", "Image4s", "Image5s", "Image3m", "Image4m", "Image5m", "Imagetrue"};
-        for (const char* zn : zeros) {
+        const char* icn[] = {"Image3s", "Image4s", "Image5s", "Image3m", "Image4m", "Image5m", "Imagetrue"};
+        for (int k = 0; k < 7; k++) {
+          const float* img = zero.data();
+          if (icx) {
+            HIPCHK(hipMemcpyAsync(a.data(), i7(k), nm * 8, hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            for (size_t i = 0; i < nm; i++) f[i] = (float)a[i];
+            img = f.data();
+          }
           char path[64];
-          snprintf(path, sizeof(path), "./output/%stemp.su", zn);
-          rtmio_write_image(zero.data(), nz, nx, P.dx, path);
+          snprintf(path, sizeof(path), "./output/%stemp.su", icn[k]);
+          rtmio_write_image(img, nz, nx, P.dx, path);
         }
       }
     }
@@ -1071,10 +1114,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     if (opt.verbose)
       for (int j = 0; j < nb; j++) printf("%d shot finished: %g (s)\n", shots[k0 + j], 0.0);
     for (void* p : mem) pool_.put(p);
+    ic_cells_ = 0;
     if (opt.checkpoint) {
-      std::vector<double> st3(3 * nm);
-      const double* src[3] = {d_i1, d_i2, d_il};
-      for (int i = 0; i < 3; i++) HIPCHK(hipMemcpy(st3.data() + (size_t)i * nm, src[i], nm * 8, hipMemcpyDeviceToHost));
+      std::vector<double> st3(nstk * nm);
+      for (int i = 0; i < nstk; i++)
+        HIPCHK(hipMemcpy(st3.data() + (size_t)i * nm, stacks[i], nm * 8, hipMemcpyDeviceToHost));
       if (write_checkpoint(opt.checkpoint, job_fp, nx, nz, k1, st3) != 0) {
         set_error(std::string("cannot write checkpoint ") + opt.checkpoint);
         rc = -6;
@@ -1089,6 +1133,14 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   HIPCHK(rtm_launch_round(d_i1, d_f1, (long long)nm, st_));
   HIPCHK(rtm_launch_round(d_i2, d_f2, (long long)nm, st_));
   HIPCHK(rtm_launch_round(d_il, d_f3, (long long)nm, st_));
+  if (out && icx) {
+    float* dst[7] = {out->image3s, out->image4s, out->image5s, out->image3m, out->image4m, out->image5m, out->imagetrue};
+    for (int k = 0; k < 7; k++) {
+      if (!dst[k]) continue;
+      HIPCHK(rtm_launch_round(i7(k), f7(k), (long long)nm, st_));
+      HIPCHK(hipMemcpyAsync(dst[k], f7(k), nm * 4, hipMemcpyDeviceToHost, st_));
+    }
+  }
   if (out) {
     if (out->image1) HIPCHK(hipMemcpyAsync(out->image1, d_f1, nm * 4, hipMemcpyDeviceToHost, st_));
     if (out->image2s) HIPCHK(hipMemcpyAsync(out->image2s, d_f2, nm * 4, hipMemcpyDeviceToHost, st_));

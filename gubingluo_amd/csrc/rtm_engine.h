@@ -71,6 +71,7 @@ struct SlotMem {
 
 struct GraphKey {
   int ntiles, tz, phase, save, fmt, k;     // save: strip save after every (forward) step, fmt its store format
+  int ic;                                  // cells of the other imaging conditions after every reverse step (0: off)
   const void* shots;
   const void* tiles;
   bool operator<(const GraphKey& o) const;
@@ -99,7 +100,7 @@ class Engine {
   // checkpoint file of a job (FP64 stacks after the shots done)
   static uint64_t job_fingerprint(const rtmh_params& P, const Model& M, const float* coef, const rtmh_options& opt,
                                   int nsh);
-  static int read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int& done, std::vector<double>& st3);
+  static int read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int nstk, int& done, std::vector<double>& st3);
   static int write_checkpoint(const char* path, uint64_t fp, int nx, int nz, int done, const std::vector<double>& st3);
 
   int run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
@@ -113,12 +114,13 @@ class Engine {
   int copy_probe(size_t bytes, int reps, double* gbps);
 
   // bytes one slot of this geometry needs on the device
-  static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt);
+  static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic = false);
   // build the device descriptor of one slot inside `base` (uploads: host
   // images of its small arrays to copy to the device)
   static void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
                           std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                          const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch);
+                          const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch,
+                          bool ic = false);
   // offset of the taper row inside a slot block
   static size_t taper_offset(const ShotSetup& s);
   std::mutex& models_mutex() { return models_mu_; }
@@ -146,6 +148,7 @@ class Engine {
   Tile* d_tiles_ = nullptr;
   int* d_slots_ = nullptr;
   struct StackShotHost* d_stk_ = nullptr;
+  int ic_cells_ = 0;                    // the other imaging conditions of the running batch: interior cells (0: off)
   int ensure_cb(int nslots, size_t ntiles);
 
   int run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,

@@ -1036,6 +1036,8 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
     model_window(s, g0, gs, n);
   } else {
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
+    if (s.ic)
+      for (long long e = g0; e < ni * (5 + 2 * 61); e += gs) s.ic[e] = 0.f;
   }
 }
 
@@ -1076,6 +1078,112 @@ __global__ __launch_bounds__(NTHR) void k_stack(const StackShot* __restrict__ sh
     a3 += (double)il;
   }
   img1[e] = a1; img2s[e] = a2; ill[e] = a3;
+}
+
+// ------------------------------------------------- other imaging conditions --
+// After reverse step `step` (kernel.cu:1417-1529, one thread per interior
+// cell of a slot): wavefield decomposition, Poynting-vector angle bins and the
+// energy norm, from the new fields (level (it+1)&1) and the fields before the
+// step (level it&1).  The reference's quirks are kept: vx and vxs are read at
+// the p index (stride nxp in an array of rows nxp-1, kernel.cu:1431,1468) and
+// the source Poynting vector takes the receiver's vz (kernel.cu:1469).  Each
+// expression is evaluated as nvcc contracts it by default (a*b + c*d ->
+// fma(a, b, c*d); accumulations -> fma), with the float / double mixing of
+// the reference's double literals.  Off the hot path: launched only when the
+// job asks for these images.
+__global__ __launch_bounds__(NTHR) void k_ic(const ShotDev* __restrict__ shots, const int* __restrict__ slots, int step)
+{
+  const ShotDev& s = shot_of(shots, slots[blockIdx.y]);
+  if (!s.ic) return;
+  const int nxl = s.nxl, nn = s.nzi * nxl;
+  const int e = blockIdx.x * NTHR + threadIdx.x;
+  if (e >= nn) return;
+  const int iz = e / nxl + s.npml, ix = e - (e / nxl) * nxl + s.npml;   // padded cell
+  const int nxp = s.nxp, pitch = s.pitch;
+  const int it = s.it_off + s.it_dir * step;
+  const bool odd = it & 1;                              // the step read level it&1, wrote the other
+  const float* psn = odd ? s.sp[0] : s.sp[1];
+  const float* pso = odd ? s.sp[1] : s.sp[0];
+  const float* prn = odd ? s.rp[0] : s.rp[1];
+  const float* pro = odd ? s.rp[1] : s.rp[0];
+  const float* vxs_f = odd ? s.svx[0] : s.svx[1];
+  const float* vzs_f = odd ? s.svz[0] : s.svz[1];
+  const float* vxr_f = odd ? s.rvx[0] : s.rvx[1];
+  const float* vzr_f = odd ? s.rvz[0] : s.rvz[1];
+  const long long id = (long long)iz * pitch + ix;
+  // the p index read in an array of rows nxp-1: row q/(nxp-1), column
+  // q%(nxp-1); past that array's end (windows taller than about npml*nxp, where
+  // the reference reads out of bounds) the value is taken as 0
+  const int q = iz * nxp + ix, qr = q / (nxp - 1), qc = q - qr * (nxp - 1);
+  const bool qin = qr < s.nzp;
+  const long long iq = qin ? (long long)qr * pitch + qc : 0;
+  const float ps = psn[id], p = prn[id];
+  const float vxs = qin ? vxs_f[iq] : 0.0f, vzs = vzs_f[id], vxr = qin ? vxr_f[iq] : 0.0f, vzr = vzr_f[id];
+  const int mr = imin(imax(iz - s.mz0, 0), s.mzn - 1), mc = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
+  const float vv = s.vv[(size_t)mr * s.mpitch + mc];
+  float* ic = s.ic;
+  // cuda_wavefield_decomposition (kernel.cu:1417-1450)
+  const float vtxs = -ps * vxs, vtzs = -ps * vzs, vtxr = -p * vxr, vtzr = -p * vzr;
+  const float psu = vtzs < 0 ? ps : 0.0f, psd = vtzs > 0 ? ps : 0.0f, psl = vtxs < 0 ? ps : 0.0f, psr = vtxs > 0 ? ps : 0.0f;
+  const float pu = vtzr < 0 ? p : 0.0f, pd = vtzr > 0 ? p : 0.0f, pl = vtxr < 0 ? p : 0.0f, pr = vtxr > 0 ? p : 0.0f;
+  ic[e] = __builtin_fmaf(psu, pd, ic[e]);
+  ic[nn + e] = __builtin_fmaf(psd, pu, ic[nn + e]);
+  ic[2 * nn + e] = __builtin_fmaf(psl, pr, ic[2 * nn + e]);
+  ic[3 * nn + e] = __builtin_fmaf(psr, pl, ic[3 * nn + e]);
+  // cuda_poynting (kernel.cu:1452-1488)
+  const float ksx = ps * vxs, ksz = ps * vzr, krx = p * vxr, krz = p * vzr;
+  const float qd = sqrtf(__builtin_fmaf(ksx, ksx, ksz * ksz) * __builtin_fmaf(krx, krx, krz * krz));
+  float w = (float)((double)__builtin_fmaf(ksx, krx, ksz * krz) / (qd > 0 ? (double)qd : 1.0e-6));
+  if (w >= -1.0 && w <= 1.0) {
+    const double pi = 3.1415926535897932;
+    w = (float)(0.5 * (double)acosf(w));
+    w = (float)(pi / 2.0 - (double)w);
+    if (w > 0 && w <= pi / 3.0) {
+      const float dw = (float)(pi / 180);
+      const size_t ib = (size_t)(int)(w / dw) * nn + e;
+      float* g31 = ic + 5 * (size_t)nn;
+      float* g32 = ic + 66 * (size_t)nn;
+      g31[ib] = __builtin_fmaf(ps * p, cosf(w), g31[ib]);
+      g32[ib] = (float)((double)g32[ib] + (double)(ps * p) * 16.0 * pi * (double)sqrtf(vv) / (double)sinf(w));
+    }
+  }
+  // cuda_energynorm (kernel.cu:1489-1529)
+  const float psx = (float)(0.5 * (double)(psn[id + 1] - psn[id - 1]) / (double)s.dx);
+  const float psz = (float)(0.5 * (double)(psn[id + pitch] - psn[id - pitch]) / (double)s.dz);
+  const float px = (float)(0.5 * (double)(prn[id + 1] - prn[id - 1]) / (double)s.dx);
+  const float pz = (float)(0.5 * (double)(prn[id + pitch] - prn[id - pitch]) / (double)s.dz);
+  const float t = -(ps - pso[id]) * (p - pro[id]) / (s.dt * s.dt * vv);
+  ic[4 * nn + e] = ic[4 * nn + e] + __builtin_fmaf(psz, pz, __builtin_fmaf(psx, px, t));
+}
+
+// one shot's other images into the FP64 stacks (kernel.cu:1530-1579): g2 as
+// the four decomposed products summed, g3 / g3_true as the 61 bins summed,
+// each tapered, then normalised by the tapered illumination (image3s..5s)
+// or not (image3m..5m, imagetrue)
+struct StackIc { const float* ic; const float* illum; const float* taper; int nx1, nxl; };
+__global__ __launch_bounds__(NTHR) void k_stack_ic(const StackIc* __restrict__ sh, int nx, int nz, double* __restrict__ i7)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e >= (long long)nx * nz) return;
+  const int iz = (int)(e / nx), ix = (int)(e - (long long)iz * nx);
+  const StackIc q = *sh;
+  const int c = ix - q.nx1;
+  if (c < 0 || c >= q.nxl) return;
+  const size_t nn = (size_t)nz * q.nxl, ip = (size_t)iz * q.nxl + c;
+  const float* ic = q.ic;
+  const float t = q.taper[c];
+  float g3 = 0.0f, g3t = 0.0f;
+  for (int iw = 0; iw <= 60; iw++) { g3 += ic[(5 + iw) * nn + ip]; g3t += ic[(66 + iw) * nn + ip]; }
+  const float g2 = ((ic[ip] + ic[nn + ip]) + ic[2 * nn + ip]) + ic[3 * nn + ip];
+  const float il = q.illum[ip] * t;
+  const float g[3] = {g2 * t, g3 * t, ic[4 * nn + ip] * t};
+  const size_t n = (size_t)nx * nz;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    i7[k * n + e] += (double)g[k] / ((double)il + 1.0e-6);
+    i7[(3 + k) * n + e] += (double)g[k];
+  }
+  i7[6 * n + e] += (double)(g3t * t);
 }
 
 // final products: fp32 images from the FP64 stacks, the Laplacian-filtered
@@ -1183,6 +1291,21 @@ hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t
 {
   if (nrec <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_seed_receivers, dim3((nrec + NTHR - 1) / NTHR), dim3(NTHR), 0, st, shots, slot);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_ic(const ShotDev* shots, const int* slots, int nslots, int step, int max_cells, hipStream_t st)
+{
+  if (nslots <= 0 || max_cells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ic, dim3((unsigned)((max_cells + NTHR - 1) / NTHR), (unsigned)nslots), dim3(NTHR), 0, st,
+                     shots, slots, step);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_stack_ic(const void* sh, int nx, int nz, double* i7, hipStream_t st)
+{
+  const long long n = (long long)nx * nz;
+  hipLaunchKernelGGL(k_stack_ic, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, (const StackIc*)sh, nx, nz, i7);
   return hipGetLastError();
 }
 

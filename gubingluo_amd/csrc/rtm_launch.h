@@ -19,6 +19,11 @@ hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t
 hipError_t rtm_launch_stack(const void* stack_shots, int nsh, int nx, int nz, double* i1, double* i2, double* il,
                             hipStream_t st);
 hipError_t rtm_launch_round(const double* a, float* o, long long n, hipStream_t st);
+// the other imaging conditions after reverse step `step` of every slot listed
+hipError_t rtm_launch_ic(const ShotDev* shots, const int* slots, int nslots, int step, int max_cells, hipStream_t st);
+// their FP64 stacks: i7 = image3s, image4s, image5s, image3m, image4m, image5m, imagetrue
+struct StackIcHost { const float* ic; const float* illum; const float* taper; int nx1, nxl; };
+hipError_t rtm_launch_stack_ic(const void* stack_shot, int nx, int nz, double* i7, hipStream_t st);
 hipError_t rtm_launch_laplace(const float* im, float* out, int nx, int nz, int dxi, int dzi, hipStream_t st);
 hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, long long n, hipStream_t st);
 hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st);

@@ -92,7 +92,10 @@ typedef struct rtmh_options {
    * an error, a missing file starts from the first shot. */
   const char *checkpoint;      /* NULL: no checkpoint */
   int resume;
-  int _pad_opt;
+  /* 1: also the decomposition, Poynting and energy-norm imaging conditions
+   * (image3s/4s/5s, image3m/4m/5m, imagetrue; kernel.cu:1417-1555), one more
+   * launch per reverse step.  The drop-in rtm_model sets it. */
+  int ic_extra;
 } rtmh_options;
 
 /* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */
@@ -104,6 +107,9 @@ typedef struct rtmh_outputs {
   float **shot_g1;          /* per shot: [nz*nxl] cross-correlation, untapered    */
   float **shot_illum;       /* per shot: [nz*nxl] illumination, untapered         */
   const float **shot_seis_in; /* per shot: [nt*nrec] data to migrate (mode 1)     */
+  /* with ic_extra: decomposition (3), Poynting (4) and energy-norm (5)
+   * images normalised by the illumination (s) or not (m), and imagetrue */
+  float *image3s, *image4s, *image5s, *image3m, *image4m, *image5m, *imagetrue;
 } rtmh_outputs;
 
 typedef struct rtmh_stats {

@@ -39,7 +39,8 @@ class Geom(C.Structure):
 
 class ShotOpts(C.Structure):
     _fields_ = [("literal", C.c_int), ("forward_only", C.c_int), ("nt_limit", C.c_int),
-                ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p), ("exact_reverse", C.c_int)]
+                ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p), ("exact_reverse", C.c_int),
+                ("ic_out", C.c_void_p)]
 
 
 _lib = None
@@ -157,7 +158,7 @@ def _variant(f64, korder):
 
 
 def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal=False, forward_only=False,
-             nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False) -> ShotResult:
+             nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False, extra_ic=False) -> ShotResult:
     g = geometry(P, ishot)
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     rt = np.float64 if f64 else np.float32
@@ -166,8 +167,9 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal
     il = np.zeros((P.nz, g.nxl), rt)
     sf = np.zeros((g.nzp, g.nxp), np.float32) if snap_it >= 0 else None
     sr = np.zeros((g.nzp, g.nxp), np.float32) if snap_it >= 0 else None
+    ic = np.zeros((4, P.nz, g.nxl), np.float32) if extra_ic else None
     o = ShotOpts(literal=int(literal), forward_only=int(forward_only), nt_limit=int(nt_limit),
-                 snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse))
+                 snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse), ic_out=_fp(ic))
     vp0 = np.ascontiguousarray(vp0, np.float32)
     rho0 = np.ascontiguousarray(rho0, np.float32)
     si = None if seis_in is None else np.ascontiguousarray(seis_in, np.float32)
@@ -176,7 +178,25 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal
             _fp(g1), _fp(il), _fp(sf), _fp(sr))
     if rc != 0:
         raise RuntimeError(f"oracle shot failed rc={rc}")
-    return ShotResult(seis[:, :g.nrec], g1, il, sf, sr, g)
+    res = ShotResult(seis[:, :g.nrec], g1, il, sf, sr, g)
+    if ic is not None:
+        res.ic = {"g2": ic[0], "g3": ic[1], "g4": ic[2], "g3_true": ic[3]}
+    return res
+
+
+def ic_cells(ps, vxs, vzs, p, vxr, vzr, ps_pre, p_pre, vv, npml, dx, dz, dt, korder=True):
+    """One step of the decomposition / Poynting / energy-norm imaging conditions on
+    given fields (reference shapes); returns the accumulators ud, du, lr, rl, g4
+    (nzi, nxl) and g31, g32 (61, nzi, nxl), starting from zero."""
+    nzp, nxp = ps.shape
+    nzi, nxl = nzp - 2 * npml, nxp - 2 * npml
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    icb = np.zeros((5 + 2 * 61, nzi, nxl), np.float32)
+    args = [f(a) for a in (ps, vxs, vzs, p, vxr, vzr, ps_pre, p_pre, vv)]
+    getattr(lib(), "orc_ic_cells_" + _variant(False, korder))(C.c_int(nxp), C.c_int(nzp), C.c_int(npml),
+                                                             *[_fp(a) for a in args], C.c_float(dx), C.c_float(dz),
+                                                             C.c_float(dt), _fp(icb))
+    return dict(ud=icb[0], du=icb[1], lr=icb[2], rl=icb[3], g4=icb[4], g31=icb[5:66], g32=icb[66:127])
 
 
 def fd_steps(P, ishot, vp0, rho0, fields_, nsteps, *, it0=0, with_source=False, literal=False,

@@ -88,6 +88,9 @@ typedef struct {
     int exact_reverse;    /* checking aid, not reference semantics: remove the
                              source at it+1 from interior cells only, so the
                              reverse pass inverts the forward one */
+    float *ic_out;        /* optional: the other imaging conditions of the shot,
+                             4 x nz x nxl (g2 decomposition sum, g3 Poynting
+                             theta sum, g4 energy norm, g3_true), untapered */
 } orc_shot_opts;
 
 /* ------------------------------------------------------------------ */

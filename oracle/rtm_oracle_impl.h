@@ -619,6 +619,91 @@ static void SFX(rev_pres)(SFX(wfield) *w, const SFX(medium) *m)
 /* one shot: forward (record + boundary save), mute, reverse + imaging    */
 /* --------------------------------------------------------------------- */
 /* opts: see orc_shot_opts in rtm_oracle.c */
+/* The other imaging conditions of one reverse step on the interior cells
+ * (kernel.cu:1417-1529), after the cross-correlation, with the reference's
+ * quirks: vx / vxs are read at the p index (stride nxp instead of nxp-1,
+ * kernel.cu:1431,1468) and the source Poynting vector takes the receiver's vz
+ * (kernel.cu:1469).  Operation order as nvcc contracts it by default
+ * (a*b + c*d -> fma(a, b, c*d); accumulations -> fma); the float / double
+ * mixing of the double literals (pi, eps, 0.5, 16.0) as written.
+ * icb: ud du lr rl g4 (nz x nxl each), then 61 bins of g31, 61 of g32. */
+static void SFX(extra_ic)(const SFX(wfield) *S, const SFX(wfield) *R, const SFX(medium) *m, const RT *ps_pre,
+                          const RT *p_pre, RT *icb, float dx, float dz, float dt)
+{
+    const int nxp = m->nxp, nzp = m->nzp, npml = m->npml, nxl = m->nxl;
+    const size_t nn = (size_t)m->nzi * nxl;
+    RT *ud = icb, *du = icb + nn, *lr = icb + 2 * nn, *rl = icb + 3 * nn, *g4 = icb + 4 * nn;
+    RT *g31 = icb + 5 * nn, *g32 = icb + 66 * nn;
+    const float dw = (float)(ORC_PI / 180);
+    for (int iz = npml; iz < nzp - npml; iz++)
+        for (int ix = npml; ix < nxp - npml; ix++) {
+            const size_t id = (size_t)iz * nxp + ix, ip = (size_t)(iz - npml) * nxl + ix - npml;
+            const RT ps = S->p[id], p = R->p[id];
+            const int qin = id < (size_t)nzp * (nxp - 1);   /* past the vx array's end: 0 */
+            const RT vxs = qin ? S->vx[id] : 0, vzs = S->vz[id], vxr = qin ? R->vx[id] : 0, vzr = R->vz[id];
+            /* cuda_wavefield_decomposition (kernel.cu:1417-1450) */
+            const RT vtxs = -ps * vxs, vtzs = -ps * vzs, vtxr = -p * vxr, vtzr = -p * vzr;
+            const RT psu = vtzs < 0 ? ps : 0, psd = vtzs > 0 ? ps : 0, psl = vtxs < 0 ? ps : 0, psr = vtxs > 0 ? ps : 0;
+            const RT pu = vtzr < 0 ? p : 0, pd = vtzr > 0 ? p : 0, pl = vtxr < 0 ? p : 0, pr = vtxr > 0 ? p : 0;
+            ud[ip] = FMA(psu, pd, ud[ip]);
+            du[ip] = FMA(psd, pu, du[ip]);
+            lr[ip] = FMA(psl, pr, lr[ip]);
+            rl[ip] = FMA(psr, pl, rl[ip]);
+            /* cuda_poynting (kernel.cu:1452-1488) */
+            const RT ksx = ps * vxs, ksz = ps * vzr, krx = p * vxr, krz = p * vzr;
+            const RT vv = m->vv[id];
+#if ORACLE_IS_F64
+            const double q = sqrt(FMA(ksx, ksx, ksz * ksz) * FMA(krx, krx, krz * krz));
+            double w = FMA(ksx, krx, ksz * krz) / (q > 0 ? q : 1.0e-6);
+            if (w >= -1.0 && w <= 1.0) {
+                w = ORC_PI / 2.0 - 0.5 * acos(w);
+                if (w > 0 && w <= ORC_PI / 3.0) {
+                    const size_t ib = (size_t)(int)(w / (ORC_PI / 180)) * nn + ip;
+                    g31[ib] = FMA(ps * p, cos(w), g31[ib]);
+                    g32[ib] += ps * p * 16.0 * ORC_PI * sqrt(vv) / sin(w);
+                }
+            }
+#else
+            const float q = sqrtf(fmaf(ksx, ksx, ksz * ksz) * fmaf(krx, krx, krz * krz));
+            float w = (float)((double)fmaf(ksx, krx, ksz * krz) / (q > 0 ? (double)q : 1.0e-6));
+            if (w >= -1.0 && w <= 1.0) {
+                w = (float)(0.5 * (double)acosf(w));
+                w = (float)(ORC_PI / 2.0 - (double)w);
+                if (w > 0 && w <= ORC_PI / 3.0) {
+                    const size_t ib = (size_t)(int)(w / dw) * nn + ip;
+                    g31[ib] = fmaf(ps * p, cosf(w), g31[ib]);
+                    g32[ib] = (float)((double)g32[ib] +
+                                      (double)(ps * p) * 16.0 * ORC_PI * (double)sqrtf(vv) / (double)sinf(w));
+                }
+            }
+#endif
+            (void)dw;
+            /* cuda_energynorm (kernel.cu:1489-1529): central differences of the
+             * new fields, time differences against the fields before the step */
+            const RT psx = (RT)(0.5 * (double)(S->p[id + 1] - S->p[id - 1]) / (double)dx);
+            const RT psz = (RT)(0.5 * (double)(S->p[id + nxp] - S->p[id - nxp]) / (double)dz);
+            const RT px = (RT)(0.5 * (double)(R->p[id + 1] - R->p[id - 1]) / (double)dx);
+            const RT pz = (RT)(0.5 * (double)(R->p[id + nxp] - R->p[id - nxp]) / (double)dz);
+            const RT t = -(ps - ps_pre[id]) * (p - p_pre[id]) / ((RT)dt * (RT)dt * vv);
+            g4[ip] = g4[ip] + FMA(psz, pz, FMA(psx, px, t));
+        }
+}
+
+/* one step of the other imaging conditions on given fields (reference shapes:
+ * p nzp*nxp, vx nzp*(nxp-1), vz (nzp-1)*nxp, vv nzp*nxp), for the pins */
+ORC_EXPORT void SFX(orc_ic_cells)(int nxp, int nzp, int npml, const RT *ps, const RT *vxs, const RT *vzs, const RT *p,
+                                  const RT *vxr, const RT *vzr, const RT *ps_pre, const RT *p_pre, const RT *vv,
+                                  float dx, float dz, float dt, RT *icb)
+{
+    SFX(wfield) S, R;
+    SFX(medium) m;
+    memset(&S, 0, sizeof(S)); memset(&R, 0, sizeof(R)); memset(&m, 0, sizeof(m));
+    S.p = (RT *)ps; S.vx = (RT *)vxs; S.vz = (RT *)vzs;
+    R.p = (RT *)p; R.vx = (RT *)vxr; R.vz = (RT *)vzr;
+    m.nxp = nxp; m.nzp = nzp; m.npml = npml; m.nxl = nxp - 2 * npml; m.nzi = nzp - 2 * npml; m.vv = (RT *)vv;
+    SFX(extra_ic)(&S, &R, &m, ps_pre, p_pre, icb, dx, dz, dt);
+}
+
 static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const float *rho0,
                          const float *diffcoef, const orc_shot_opts *o,
                          float *seis_out, const float *seis_in,
@@ -681,11 +766,22 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     memset(S.vx, 0, (size_t)nzp * (nxp - 1) * sizeof(RT));
     memset(S.vz, 0, (size_t)(nzp - 1) * nxp * sizeof(RT));
     SFX(wf_alloc)(&Rw, nxp, nzp, npml);
+    const size_t nn = (size_t)nzi * nxl;
+    RT *icb = NULL, *ps_pre = NULL, *p_pre = NULL;
+    if (o->ic_out) {
+        icb = (RT *)calloc(nn * (5 + 2 * 61), sizeof(RT));
+        ps_pre = (RT *)malloc((size_t)nxp * nzp * sizeof(RT));
+        p_pre = (RT *)malloc((size_t)nxp * nzp * sizeof(RT));
+    }
     RT *g1 = (RT *)calloc((size_t)nzi * nxl, sizeof(RT));
     RT *il = (RT *)calloc((size_t)nzi * nxl, sizeof(RT));
     /* seed receiver wavefield with the last sample (rtm_model.cu:410) */
     for (int ir = 0; ir < g.nrec; ir++) Rw.p[rrow + g.rcol0 + ir] = m.dt * rec[(size_t)(nt - 1) * g.nrec + ir];
     for (int it = nt - 2; it >= 0; it--) {
+        if (icb) {                                     /* d_ps_pre, d_p_pre (rtm_model.cu:419-420) */
+            memcpy(ps_pre, S.p, (size_t)nxp * nzp * sizeof(RT));
+            memcpy(p_pre, Rw.p, (size_t)nxp * nzp * sizeof(RT));
+        }
         /* source wavefield: vel strips of step it+1, interior vel update,
          * p strips of step it, interior p update, source removal */
         SFX(strips_io)(&S, store + (size_t)(it + 1) * slen, 0, 1);
@@ -712,8 +808,21 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
                 g1[idi] = FMA(S.p[idp], Rw.p[idp], g1[idi]);
                 il[idi] = FMA(S.p[idp], S.p[idp], il[idi]);
             }
+        if (icb) SFX(extra_ic)(&S, &Rw, &m, ps_pre, p_pre, icb, P->dx, P->dz, P->dt);
         if (snap_rec && it == o->snap_it)
             for (size_t i = 0; i < (size_t)nxp * nzp; i++) snap_rec[i] = (float)S.p[i];
+    }
+    if (icb) {
+        /* cuda_stack_udlr, cuda_stack_theta (kernel.cu:1530-1555) */
+        for (size_t i = 0; i < nn; i++) {
+            RT g3 = 0, g3t = 0;
+            for (int iw = 0; iw <= 60; iw++) { g3 += icb[(5 + iw) * nn + i]; g3t += icb[(66 + iw) * nn + i]; }
+            o->ic_out[i] = (float)(((icb[i] + icb[nn + i]) + icb[2 * nn + i]) + icb[3 * nn + i]);
+            o->ic_out[nn + i] = (float)g3;
+            o->ic_out[2 * nn + i] = (float)icb[4 * nn + i];
+            o->ic_out[3 * nn + i] = (float)g3t;
+        }
+        free(icb); free(ps_pre); free(p_pre);
     }
     memcpy(g1_out, g1, (size_t)nzi * nxl * sizeof(RT));
     memcpy(illum_out, il, (size_t)nzi * nxl * sizeof(RT));

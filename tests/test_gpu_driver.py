@@ -91,18 +91,21 @@ def test_rtm_gpu_driver_images(gpu_ctx, tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "Prestack Reverse Time Migration by Bingluo Gu from UPC." in r.stdout
     out = {n: read_su(str(tmp_path / "output" / f"0{n}.su"), nz, nx)
-           for n in ("Image1", "Image2s", "Illum", "Image2m", "Image3s", "Image1lap", "Imagenorm")}
+           for n in ("Image1", "Image2s", "Illum", "Image2m", "Image3s", "Image4s", "Image5m", "Image1lap", "Imagenorm")}
+    true = read_su(str(tmp_path / "output" / "0Imagetrue.su"), nz, nx)
     for k in (1, 3, 5):
         assert (tmp_path / "seisReal" / f"{k}seisreal.su").stat().st_size > 0
     # the same job through the library's own entry point and through the oracle
     kw = dict(nshot=nshot, startshot=1, dshot=2, flag_layout=1, nx=nx, nz=nz, nt=nt, dx=10.0, dz=10.0, dt=1e-3,
               npml=32, fdom=20.0, spx0=300.0, spz0=20.0, dspx=120.0, offset=0.0, offsetmax=600.0)
-    lib = gpu_ctx.run(G.make_params(**kw), vp, rho)
+    lib = gpu_ctx.run(G.make_params(**kw), vp, rho, ic_extra=True)   # the drop-in computes every image
     assert np.array_equal(out["Image1"], lib.image1)
     assert np.array_equal(out["Image2s"], lib.image2s)
     assert np.array_equal(out["Illum"], lib.illum)
     assert np.array_equal(out["Image2m"], lib.image1)          # image2m = image1 in the reference
-    assert not out["Image3s"].any()                             # imaging conditions not built: zeros
+    assert np.array_equal(out["Image3s"], lib.ic["image3s"]) and out["Image3s"].any()
+    assert np.array_equal(out["Image4s"], lib.ic["image4s"]) and np.array_equal(out["Image5m"], lib.ic["image5m"])
+    assert np.array_equal(true, lib.ic["imagetrue"])
     assert np.array_equal(out["Image1lap"], lib.image_lap)
     assert np.array_equal(out["Imagenorm"], lib.image_norm)
     i1, i2, il = O.rtm(O.make_params(**kw), vp, rho, stack64=True, korder=True)

@@ -216,3 +216,40 @@ def test_checkpoint_resume_equals_uninterrupted(gpu_ctx, tmp_path):
         assert np.array_equal(getattr(rest, name), getattr(full, name)), name
     with pytest.raises(RuntimeError, match="another job"):
         gpu_ctx.run(G.make_params(**dict(kw, fdom=25.0)), vp, rho, batch=2, checkpoint=ck, resume=True)
+
+
+def test_other_imaging_conditions_vs_oracle(gpu_ctx):
+    """decomposition, Poynting and energy-norm images (ic_extra): the kernel-order
+    oracle's per-shot products stacked as the engine stacks them (taper, FP64,
+    illumination-normalised or not).  Decomposition and energy norm bit for bit;
+    the Poynting images within 1e-3 (acosf / cosf / sinf of the device library
+    against the host's)"""
+    nz, nx = 50, 160
+    vp, rho = small_model(nz=nz, nx=nx, seed=14)
+    kw = job(nx=nx, nz=nz, nshot=5, dshot=2, flag_layout=1, offsetmax=600.0, spx0=300.0, dspx=120.0, nt=230)
+    P, Po = G.make_params(**kw), O.make_params(**kw)
+    print("[imaging conditions] 3 shots layout 1", flush=True)
+    r = gpu_ctx.run(P, vp, rho, batch=2, ic_extra=True)
+    base = gpu_ctx.run(P, vp, rho, batch=2)
+    assert np.array_equal(r.image1, base.image1) and np.array_equal(r.illum, base.illum)
+    st = {k: np.zeros((nz, nx)) for k in G.IC_IMAGES}
+    nwin = 50
+    for ishot in (1, 3, 5):
+        o = O.run_shot(Po, ishot, vp, rho, korder=True, extra_ic=True)
+        g, nxl = o.geom, o.geom.nxl
+        while 2 * nwin >= nxl:
+            nwin -= 3
+        t = O.taper_row(nxl, nwin)
+        il = (o.illum * t).astype(np.float64)
+        cols = slice(g.nx1, g.nx1 + nxl)
+        for k, name in (("3", "g2"), ("4", "g3"), ("5", "g4")):
+            gk = (o.ic[name] * t).astype(np.float64)
+            st[f"image{k}s"][:, cols] += gk / (il + 1e-6)
+            st[f"image{k}m"][:, cols] += gk
+        st["imagetrue"][:, cols] += (o.ic["g3_true"] * t).astype(np.float64)
+    for k in ("image3s", "image3m", "image5s", "image5m"):
+        assert np.array_equal(r.ic[k], st[k].astype(np.float32)), (k, rel_l2(r.ic[k], st[k]))
+    for k in ("image4s", "image4m", "imagetrue"):
+        assert rel_l2(r.ic[k], st[k]) <= 1e-3, (k, rel_l2(r.ic[k], st[k]))
+    for k in G.IC_IMAGES:
+        assert np.abs(r.ic[k]).max() > 0, k

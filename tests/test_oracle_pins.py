@@ -290,3 +290,60 @@ def test_laplacian_textbook():
                        (a[2:, 1:-1] + a[:-2, 1:-1] - 2 * a[1:-1, 1:-1]) / 25.0
     assert np.max(np.abs(out - want)) < 1e-5 * np.max(np.abs(want))
     assert np.all(out[0] == 0) and np.all(out[-1] == 0) and np.all(out[:, 0] == 0) and np.all(out[:, -1] == 0)
+
+
+def _ic_fields(nzp=24, nxp=26, **val):
+    """constant fields of the reference shapes"""
+    full = lambda v: np.full((nzp, nxp), v, np.float32)
+    return dict(ps=full(val.get("ps", 0.0)), vxs=np.full((nzp, nxp - 1), val.get("vxs", 0.0), np.float32),
+                vzs=np.full((nzp - 1, nxp), val.get("vzs", 0.0), np.float32), p=full(val.get("p", 0.0)),
+                vxr=np.full((nzp, nxp - 1), val.get("vxr", 0.0), np.float32),
+                vzr=np.full((nzp - 1, nxp), val.get("vzr", 0.0), np.float32),
+                ps_pre=full(val.get("ps_pre", 0.0)), p_pre=full(val.get("p_pre", 0.0)), vv=full(val.get("vv", 4.0e6)))
+
+
+def test_imaging_conditions_decomposition_signs():
+    """kernel.cu:1417-1450: with the source field moving one way in z and x
+    (-ps*vz < 0, -ps*vx < 0) and the receiver field the other way in z only, the
+    up-down product alone collects ps*p"""
+    f = _ic_fields(ps=2.0, vxs=1.0, vzs=1.0, p=3.0, vxr=1.0, vzr=-1.0)
+    r = O.ic_cells(**f, npml=6, dx=10.0, dz=10.0, dt=1e-3)
+    assert np.all(r["ud"] == 6.0)
+    assert not r["du"].any() and not r["lr"].any() and not r["rl"].any()
+    f = _ic_fields(ps=2.0, vxs=-1.0, vzs=-1.0, p=3.0, vxr=1.0, vzr=1.0)   # every direction reversed
+    r = O.ic_cells(**f, npml=6, dx=10.0, dz=10.0, dt=1e-3)
+    assert np.all(r["du"] == 6.0) and np.all(r["rl"] == 6.0) and not r["ud"].any() and not r["lr"].any()
+
+
+def test_imaging_conditions_poynting_angle_bin():
+    """kernel.cu:1452-1488 with ps = p = 1: source vector (vxs, vzr), receiver
+    vector (vxr, vzr) at cos = 0.3, so theta = pi/2 - acos(0.3)/2 = 0.9377 rad
+    falls in bin 53 of width pi/180; g31 += cos(theta), g32 += 16 pi v / sin(theta)"""
+    c = (-2.0 + np.sqrt(4.0 - 4 * 0.82 * 0.82)) / (2 * 0.82)      # (c + 1)/sqrt(2 (c^2 + 1)) = 0.3
+    f = _ic_fields(ps=1.0, vxs=1.0, p=1.0, vxr=c, vzr=1.0, vv=4.0e6)
+    r = O.ic_cells(**f, npml=6, dx=10.0, dz=10.0, dt=1e-3)
+    th = np.pi / 2 - 0.5 * np.arccos(0.3)
+    assert int(th / (np.pi / 180)) == 53
+    assert np.allclose(r["g31"][53], np.cos(th), rtol=1e-5, atol=0)
+    assert np.allclose(r["g32"][53], 16 * np.pi * 2000.0 / np.sin(th), rtol=1e-5, atol=0)
+    others = np.delete(np.arange(61), 53)
+    assert not r["g31"][others].any() and not r["g32"][others].any()
+    # parallel vectors: theta = pi/2, outside (0, pi/3]: no contribution
+    r = O.ic_cells(**_ic_fields(ps=1.0, vxs=1.0, p=1.0, vxr=1.0, vzr=1.0), npml=6, dx=10.0, dz=10.0, dt=1e-3)
+    assert not r["g31"].any() and not r["g32"].any()
+
+
+def test_imaging_conditions_energy_norm_linear_fields():
+    """kernel.cu:1489-1529: for ps = a x + b z and p = c x + d z the central
+    differences are exact, so g4 = -(dps)(dp)/(dt^2 v^2) + a c + b d"""
+    nzp, nxp, dx, dz, dt = 24, 26, 10.0, 7.5, 1e-3
+    z, x = np.meshgrid(np.arange(nzp) * dz, np.arange(nxp) * dx, indexing="ij")
+    a, b, c, d = 0.25, -0.5, 0.75, 0.125
+    f = _ic_fields(nzp, nxp, vv=2.25e6)
+    f["ps"] = (a * x + b * z).astype(np.float32)
+    f["p"] = (c * x + d * z).astype(np.float32)
+    f["ps_pre"] = f["ps"] - np.float32(0.5)
+    f["p_pre"] = f["p"] + np.float32(0.25)
+    r = O.ic_cells(**f, npml=6, dx=dx, dz=dz, dt=dt)
+    want = -(0.5 * -0.25) / (dt * dt * 2.25e6) + a * c + b * d
+    assert np.allclose(r["g4"], want, rtol=2e-5, atol=0)
