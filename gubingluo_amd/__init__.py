@@ -26,6 +26,7 @@ EXPORTED = [
     "rtmh_run_model", "rtmh_fd_steps", "rtmh_copy_probe", "rtmh_coef_table", "rtmh_read_parfile",
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
     "rtmh_server_create", "rtmh_server_submit", "rtmh_server_wait", "rtmh_server_query", "rtmh_server_destroy",
+    "rtmh_scan_gather", "rtmh_read_gather", "rtmh_interp_seis", "rtmh_gather_geometry",
 ]
 
 
@@ -48,7 +49,7 @@ class Options(C.Structure):
     _fields_ = [("batch", C.c_int), ("store_fmt", C.c_int), ("graph_chunk", C.c_int), ("tile_z", C.c_int),
                 ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int),
                 ("shot_offsetmax", C.c_void_p), ("checkpoint", C.c_char_p), ("resume", C.c_int),
-                ("ic_extra", C.c_int)]
+                ("ic_extra", C.c_int), ("shot_field", C.c_void_p)]
 
 
 # the other imaging conditions (rtmh_outputs order)
@@ -148,6 +149,48 @@ def geometry(P: Params, ishot: int) -> Geom:
     return g
 
 
+def gather_geometry(P: Params, ishot: int, field) -> Geom:
+    """Window of a recorded gather, field = (sx, offset, offsetmax) (rtm_real.cu:121-136)."""
+    g = Geom()
+    f = np.ascontiguousarray(field, np.float32)
+    if lib().rtmh_gather_geometry(C.byref(P), C.c_int(ishot), _fp(f), C.byref(g)) != 0:
+        raise ValueError(f"gather {ishot} outside the model")
+    return g
+
+
+def scan_gather(path: str, sgyflag: int = 0, max_shots: int = 4096):
+    """Shot table of a gather file (scanshotgather): (table[4][nshots], endianflag)."""
+    prof = np.zeros((4, max_shots), np.int32)
+    e, n = C.c_int(0), C.c_int(0)
+    rc = lib().rtmh_scan_gather(os.fsencode(path), C.c_int(sgyflag), C.byref(e), _fp(prof), C.c_int(max_shots),
+                                C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"rtmh_scan_gather failed ({rc}): {last_error()}")
+    return prof[:, :n.value].copy(), e.value
+
+
+def read_gather(path: str, itraceall: int, ntrace: int, nt0: int, ncols: int, sgyflag: int = 0,
+                endianflag: int = 0) -> np.ndarray:
+    out = np.zeros((nt0, ncols), np.float32)
+    rc = lib().rtmh_read_gather(os.fsencode(path), C.c_int(sgyflag), C.c_int(endianflag), C.c_int(itraceall),
+                                C.c_int(ntrace), C.c_int(nt0), _fp(out), C.c_int(ncols))
+    if rc != 0:
+        raise RuntimeError(f"rtmh_read_gather failed ({rc}): {last_error()}")
+    return out
+
+
+def interp_seis(seisobs, nx: int, nt: int, dt: float, dt0: float) -> np.ndarray:
+    """Interpseis2d: seisobs[nt0][nx0] at dt0 onto [nt][nx] at dt."""
+    a = np.ascontiguousarray(seisobs, np.float32)
+    nt0, nx0 = a.shape
+    out = np.zeros((nt, nx), np.float32)
+    rc = lib().rtmh_interp_seis(_fp(out), _fp(a), C.c_int(nx), C.c_int(nx0), C.c_int(nt), C.c_int(nt0),
+                                C.c_float(dt), C.c_float(dt0))
+    if rc != 0:
+        raise ValueError(f"rtmh_interp_seis failed ({rc}): {last_error()}")
+    return out
+
+
 def shot_list(P: Params):
     return list(range(P.startshot, P.nshot + 1, P.dshot))
 
@@ -209,10 +252,11 @@ class Context:
     def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
             graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
             want_post=True, seis_in=None, profile=False, shot_offsetmax=None, checkpoint=None,
-            resume=False, ic_extra=False) -> RunResult:
+            resume=False, ic_extra=False, shot_field=None) -> RunResult:
         """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model.
         checkpoint: path written after every batch (FP64 stacks, shots done); resume=True
-        continues a job from it."""
+        continues a job from it.  shot_field: per shot (sx, offset, offsetmax) of recorded
+        gathers (the geometry of rtm_real)."""
         coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
         nz, nx = P.nz, P.nx
         if model is None:
@@ -239,7 +283,7 @@ class Context:
                 if shot_offsetmax is not None:
                     Pk = Params.from_buffer_copy(P)
                     Pk.offsetmax = float(shot_offsetmax[k])
-                geoms.append(geometry(Pk, s))
+                geoms.append(geometry(Pk, s) if shot_field is None else gather_geometry(P, s, shot_field[k]))
         if want_shots:
             seis = [np.zeros((P.nt, g.nrec), np.float32) for g in geoms]
             g1s = [np.zeros((nz, g.nxl), np.float32) for g in geoms]
@@ -261,9 +305,11 @@ class Context:
             out.shot_seis_in = C.cast(pa, C.c_void_p)
         som = None if shot_offsetmax is None else np.ascontiguousarray(shot_offsetmax, np.float32)
         keep.append(som)
+        sf = None if shot_field is None else np.ascontiguousarray(shot_field, np.float32).reshape(-1, 3)
+        keep.append(sf)
         ck = None if checkpoint is None else os.fsencode(checkpoint)
         opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
-                      int(profile), _fp(som), ck, int(resume), int(ic_extra))
+                      int(profile), _fp(som), ck, int(resume), int(ic_extra), _fp(sf))
         st = Stats()
         if model is None:
             rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),

@@ -344,11 +344,37 @@ extern "C" int rtmh_driver_main(int argc, char** argv)
               diffcoef, wavelet.data(), image1, image2s, image3s, image4s, image5s, image2m, image3m, image4m,
               image5m, Illum);
   } else {
-    rtm_real(pf.idevice, pf.nshot, pf.startshot, pf.dshot, pf.medium_flag, pf.flag_layout, pf.sgyflag,
-             pf.endianflag, pf.ismth, nx, nz, g.nt, g.dx, g.dz, g.dt, nx0, nz0, pf.nt0, pf.dx0, pf.dz0, pf.dt0,
-             pf.npml, g.lx, g.lz, pf.tsnap, pf.fdom, pf.spz0, pf.amp, pf.alp, (int)pf.direct, (int)pf.direct0,
-             vp.data(), rho.data(), diffcoef, wavelet.data(), pf.shotgather, nullptr, image1, image2s, image3s,
-             image4s, image5s, image2m, image3m, image4m, image5m, Illum);
+    // shot table of the gather file, scanned once and kept in
+    // shotprofileId.dat (RTM_GPU_velstr.cpp:234-252)
+    const int nall = pf.nshotall > 0 ? pf.nshotall : 1;
+    std::vector<int> prof(4 * (size_t)nall, 0);
+    int* rows[4] = {prof.data(), prof.data() + nall, prof.data() + 2 * (size_t)nall, prof.data() + 3 * (size_t)nall};
+    int endian = pf.endianflag;
+    FILE* tf = fopen("shotprofileId.dat", "r");
+    if (!tf) {
+      snprintf(path, sizeof(path), "./seisReal/%s", pf.shotgather);
+      int found = 0;
+      if (rtmh_scan_gather(path, pf.sgyflag, &endian, prof.data(), nall, &found) != 0) {
+        printf("%s\n", rtmh_last_error());
+        return 1;
+      }
+      tf = fopen("shotprofileId.dat", "w");
+      if (tf) {
+        for (int i = 0; i < nall; i++) fprintf(tf, "%d %d %d %d\n", rows[0][i], rows[1][i], rows[2][i], rows[3][i]);
+        fclose(tf);
+      }
+    } else {
+      for (int i = 0; i < nall; i++)
+        if (fscanf(tf, "%d %d %d %d", &rows[0][i], &rows[1][i], &rows[2][i], &rows[3][i]) != 4) break;
+      fclose(tf);
+    }
+    // the mute's width and minimum length in samples (rtm_real.cu:28-29)
+    const int nw = (int)(pf.direct / (pf.fdom * g.dt)), tlength = (int)(pf.direct0 / g.dt);
+    rtm_real(pf.idevice, pf.nshot, pf.startshot, pf.dshot, pf.medium_flag, pf.flag_layout, pf.sgyflag, endian,
+             pf.ismth, nx, nz, g.nt, g.dx, g.dz, g.dt, nx0, nz0, pf.nt0, pf.dx0, pf.dz0, pf.dt0, pf.npml, g.lx, g.lz,
+             pf.tsnap, pf.fdom, pf.spz0, pf.amp, pf.alp, nw, tlength, vp.data(), rho.data(), diffcoef,
+             wavelet.data(), pf.shotgather, rows, image1, image2s, image3s, image4s, image5s, image2m, image3m,
+             image4m, image5m, Illum);
   }
   // image outputs (RTM_GPU_velstr.cpp:259-297)
   struct { const float* im; const char* name; } outs[] = {

@@ -56,6 +56,14 @@ extern "C" int rtmh_shot_geometry(const rtmh_params* p, int ishot, rtmh_geom* g)
   return 0;
 }
 
+extern "C" int rtmh_gather_geometry(const rtmh_params* p, int ishot, const float* field, rtmh_geom* g)
+{
+  rtm::ShotSetup s;
+  if (!p || !g || !field || !rtm::shot_geometry(*p, ishot, s, field)) return -1;
+  *g = s.g;
+  return 0;
+}
+
 extern "C" int rtmh_run(rtmh_ctx* ctx, const rtmh_params* p, const float* vp0, const float* rho0, const float* diffcoef,
                         const rtmh_options* opt, rtmh_outputs* out, rtmh_stats* stats)
 {
@@ -172,11 +180,3 @@ extern "C" void rtm_model(int idevice, int nshot, int startshot, int dshot, int 
   rtmio_write_image(imagetrue.data(), nz, nx, dx, path);
 }
 
-extern "C" void rtm_real(int, int, int, int, int, int, int, int, int, int, int, int, float, float, float, int, int,
-                         int, float, float, float, int, int, int, float, float, float, float, float, int, int, float*,
-                         float*, float*, float*, char*, int**, float*, float*, float*, float*, float*, float*, float*,
-                         float*, float*, float*)
-{
-  fprintf(stderr, "rtm_real: the field-data path (dataflag=1) is not implemented by librtm_hip\n");
-  exit(2);
-}

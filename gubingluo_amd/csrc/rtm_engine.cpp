@@ -36,13 +36,27 @@ static inline int imax(int a, int b) { return a > b ? a : b; }
 static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // ------------------------------------------------------------ geometry ----
-bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s)
+bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s, const float* field)
 {
   const float dx = P.dx, dz = P.dz;
   const int nx = P.nx, npml = P.npml;
   int nx1, nx2, is = 0, noffset = 0;
   float spx, spz;
-  if (P.flag_layout == 1) {                                  // rtm_model.cu:124-134
+  int layout = P.flag_layout;
+  if (field) {
+    // a recorded gather (rtm_real.cu:121-136): source at sx, receivers from
+    // offset to offsetmax on its right (the receiver rows of layout 3); the
+    // reference does not clamp the window to the model, so a gather reaching
+    // past it is refused here
+    const float sx = field[0], offset = field[1], offsetmax = field[2];
+    if (sx < 0 || offset < 0 || offsetmax < offset) return false;
+    noffset = (int)(offset / dx);
+    nx1 = (int)(sx / dx);
+    nx2 = (int)((sx + offsetmax) / dx);
+    if (nx2 > nx - 1) return false;
+    spx = sx - nx1 * dx;
+    layout = 3;
+  } else if (P.flag_layout == 1) {                                  // rtm_model.cu:124-134
     const int NX = (int)(P.offsetmax / dx + 1);
     const int sp1 = (int)(P.spx0 / dx), np = (int)(P.dspx / dx), nr = (NX - 1) / 2;
     is = sp1 + (ishot - 1) * np;
@@ -73,7 +87,7 @@ bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s)
   s.hk_nt = (short)(unsigned short)(P.nt);
   s.hk_dt = (short)(unsigned short)(P.dt * 1000000);
   s.hk_dx = dx;
-  if (P.flag_layout <= 2) {
+  if (layout <= 2) {
     s.hk_itraceall = (ishot - 1) * nxl + 1;
     s.hk_offset = (int)(nx1 * dx - spx);
     s.hk_sx = (int)spx;
@@ -92,8 +106,8 @@ bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s)
   g.nzp = P.nz + 2 * npml;
   g.nsx = (int)(spx / dx);
   g.nsz = (int)(spz / dz);
-  if (P.flag_layout <= 2) { g.nrec = nxl; g.rcol0 = npml; }
-  else if (P.flag_layout == 3) { g.nrec = nxl - noffset; g.rcol0 = npml + noffset; }
+  if (layout <= 2) { g.nrec = nxl; g.rcol0 = npml; }
+  else if (layout == 3) { g.nrec = nxl - noffset; g.rcol0 = npml + noffset; }
   else { g.nrec = nxl - noffset; g.rcol0 = npml; }
   if (g.nrec < 0) g.nrec = 0;
   return true;
@@ -234,7 +248,7 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
   const float _dt = (float)(1.0 / (double)dt);
   const int nw = (int)(P.direct / (P.fdom * dt));
   const int tlength = (int)(P.direct0 / dt);
-  const int coff = (P.flag_layout == 3) ? g.noffset : 0;
+  const int coff = (g.rcol0 > npml) ? g.noffset : 0;   // layout 3 and recorded gathers (mute2)
   // v^2 of the padded window the forward pass uses (the mute kernels read d_vp)
   auto vv_at = [&](int iz, int ix) {
     if (wvv) return wvv[(size_t)iz * nxp + ix];
@@ -291,6 +305,7 @@ uint64_t Engine::job_fingerprint(const rtmh_params& P, const Model& M, const flo
   h = fnv1a(h, &opt.store_fmt, sizeof(opt.store_fmt));
   h = fnv1a(h, &opt.ic_extra, sizeof(opt.ic_extra));
   if (opt.shot_offsetmax) h = fnv1a(h, opt.shot_offsetmax, (size_t)nsh * 4);
+  if (opt.shot_field) h = fnv1a(h, opt.shot_field, (size_t)nsh * 12);
   return h;
 }
 
@@ -754,7 +769,10 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   std::vector<rtmh_params> pk(nsh, P);
   for (int k = 0; k < nsh; k++) {
     if (opt.shot_offsetmax) pk[k].offsetmax = opt.shot_offsetmax[k];
-    if (!shot_geometry(pk[k], shots[k], setup[k])) { set_error("shot window outside the model"); return -2; }
+    if (!shot_geometry(pk[k], shots[k], setup[k], opt.shot_field ? opt.shot_field + 3 * k : nullptr)) {
+      set_error("shot window outside the model");
+      return -2;
+    }
   }
   // model smoothing: each shot gets its own padded windows (smoothed on the
   // host as the reference does, shots in parallel); the slots then read them

@@ -36,7 +36,7 @@ struct ShotSetup {
 
 // shot window and padded geometry (rtm_model.cu:122-153 and the layout
 // variants); returns false if the window is empty or the layout unknown
-bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s);
+bool shot_geometry(const rtmh_params& P, int ishot, ShotSetup& s, const float* field = nullptr);
 // host set-up of one shot given the job's v^2 column maxima; with a padded v^2
 // window (wvv, nzp x nxp) vpmax and the mute come from it instead
 void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s,

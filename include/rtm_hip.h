@@ -43,10 +43,16 @@ void rtm_model(int idevice, int nshot, int startshot, int dshot, int mode, int m
                float *wavelet, float *image1, float *image2s, float *image3s, float *image4s, float *image5s,
                float *image2m, float *image3m, float *image4m, float *image5m, float *illum);
 
-/* Link-compatible stand-in for rtm_real() (rtm_real.cu:17-22; the caller's
- * prototype at RTM_GPU_velstr.cpp:30-35 differs in three positions).  The
- * field-data path is not implemented: prints a message and exits with
- * status 2. */
+/* Replaces rtm_real() of rtm_real.cu:17-22 (field data), with the caller's
+ * prototype (RTM_GPU_velstr.cpp:30-35), which differs from the callee's in
+ * two positions: nw and tlength arrive as ints, and here they are the
+ * direct-wave mute's width and minimum length in samples, the values
+ * rtm_real.cu:28-29 derives from direct and direct0.  Per shot ishot it reads
+ * shotprofileId[0..3][ishot-1] (sx, traces, first trace, offset; the table of
+ * rtmh_scan_gather), the gather from ./seisReal/<filename_shotgather>,
+ * resamples it onto the modelling grid (rtmh_interp_seis) and migrates it
+ * with the receivers from the offset to the far end on the source's right;
+ * outputs and file side effects as rtm_model(). */
 void rtm_real(int idevice, int nshot, int startshot, int dshot, int medium_flag, int flag_layout, int sgyflag,
               int endianflag, int ismth, int nx, int nz, int nt, float dx, float dz, float dt, int nx0, int nz0,
               int nt0, float dx0, float dz0, float dt0, int npml, int lx, int lz, float tsnap, float fdom,
@@ -54,6 +60,29 @@ void rtm_real(int idevice, int nshot, int startshot, int dshot, int medium_flag,
               float *wavelet, char filename_shotgather[40], int **shotprofileId, float *image1, float *image2s,
               float *image3s, float *image4s, float *image5s, float *image2m, float *image3m, float *image4m,
               float *image5m, float *illum);
+
+/* Shot-gather scan (scanshotgather, RTM_GPU_velstr.cpp:363-454): a new shot
+ * wherever the trace header's sx changes.  prof is [4][max_shots] (rows:
+ * sx, traces, first trace 1-based, offset of the first trace), the layout of
+ * the reference's shotprofileId; *endianflag = 1 when the headers are
+ * big-endian (the reference's probe: ns or dt not positive as read, both
+ * positive byte-swapped).  0 on success, -3 when the file holds more than
+ * max_shots shots. */
+int rtmh_scan_gather(const char *path, int sgyflag, int *endianflag, int *prof, int max_shots, int *nshots);
+
+/* The traces of one shot (rtm_real.cu:379-391): ntrace traces of nt0
+ * samples from trace itraceall (1-based) into seis[nt0][ncols]; traces past
+ * ncols are dropped, columns past ntrace are zero. */
+int rtmh_read_gather(const char *path, int sgyflag, int endianflag, int itraceall, int ntrace, int nt0,
+                     float *seis, int ncols);
+
+/* Interpseis2d (Interp_function.cpp:245-264): seisobs[nt0][nx0] sampled at
+ * dt0 onto seiscal[nt][nx] at dt: f-k trace interpolation by
+ * lx = (nx-1)/(nx0-1) when nx > nx0 (GeneralFKinterp), Kaiser-windowed sinc
+ * in time when nt > nt0 (Sincinterp), a copy otherwise (the reference leaves
+ * the record unset there).  f-k interpolation of an odd nt0 or onto more than
+ * lx*nx0 traces reads past the reference's buffers and is refused (-1). */
+int rtmh_interp_seis(float *seiscal, const float *seisobs, int nx, int nx0, int nt, int nt0, float dt, float dt0);
 
 /* ------------------------------------------------------------------ */
 /* 2. native API                                                       */
@@ -96,6 +125,11 @@ typedef struct rtmh_options {
    * (image3s/4s/5s, image3m/4m/5m, imagetrue; kernel.cu:1417-1555), one more
    * launch per reverse step.  The drop-in rtm_model sets it. */
   int ic_extra;
+  /* per shot (job order) [sx, offset, offsetmax] of a recorded gather
+   * (rtm_real.cu:121-136): window from sx to sx + offsetmax, receivers from
+   * offset on the source's right, direct-wave mute of cuda_mute2; NULL: the
+   * geometry of flag_layout.  The drop-in rtm_real sets it. */
+  const float *shot_field;
 } rtmh_options;
 
 /* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */
@@ -148,6 +182,9 @@ typedef struct rtmh_geom {
   int ishot, nx1, nx2, nxl, nxp, nzp, nsx, nsz, nrec, rcol0, noffset;
 } rtmh_geom;
 int rtmh_shot_geometry(const rtmh_params *p, int ishot, rtmh_geom *g);
+/* Window of a recorded gather, field = [sx, offset, offsetmax]
+ * (rtm_real.cu:121-136); -1 when it leaves the model. */
+int rtmh_gather_geometry(const rtmh_params *p, int ishot, const float *field, rtmh_geom *g);
 
 /* Whole job as rtm_model() does it, with options; returns 0 or a negative
  * status (message in rtmh_last_error()). */

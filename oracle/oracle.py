@@ -28,6 +28,8 @@ class Params(C.Structure):
         ("direct", C.c_float), ("direct0", C.c_float), ("tsnap", C.c_float), ("fdom", C.c_float),
         ("spx0", C.c_float), ("spz0", C.c_float), ("dspx", C.c_float), ("offset", C.c_float),
         ("offsetmax", C.c_float), ("amp", C.c_float), ("alp", C.c_float),
+        # a recorded gather's geometry (rtm_real.cu:121-136) when field != 0
+        ("field", C.c_int), ("fsx", C.c_float), ("foffset", C.c_float), ("foffsetmax", C.c_float),
     ]
 
 

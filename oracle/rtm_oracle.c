@@ -65,6 +65,10 @@ typedef struct {
     float dx, dz, dt;
     int npml;
     float direct, direct0, tsnap, fdom, spx0, spz0, dspx, offset, offsetmax, amp, alp;
+    /* field != 0: the shot is a recorded gather (rtm_real.cu:121-136) with
+       source at fsx and receivers from foffset to foffsetmax on its right */
+    int field;
+    float fsx, foffset, foffsetmax;
 } orc_params;
 
 /* per-shot geometry */
@@ -102,7 +106,18 @@ ORC_EXPORT int orc_shot_geometry(const orc_params *P, int ishot, orc_geom *g)
     int nx1, nx2, is = 0, noffset = 0;
     float spx, spz;
     g->ishot = ishot;
-    if (P->flag_layout == 1) {                                   /* rtm_model.cu:124-134 */
+    int layout = P->flag_layout;
+    if (P->field) {                                              /* rtm_real.cu:121-136 */
+        /* noffset, nx1, nx2 from the gather's sx, offset and offsetmax on
+           the modelling grid; the window is not clamped to the model there,
+           so a gather that leaves it has no defined result */
+        noffset = (int)(P->foffset / dx);
+        nx1 = (int)(P->fsx / dx);
+        nx2 = (int)((P->fsx + P->foffsetmax) / dx);
+        if (P->fsx < 0 || nx2 > nx - 1) return -1;
+        spx = P->fsx - nx1 * dx;
+        layout = 3;                                              /* record2 / insert_record2 / mute2 */
+    } else if (P->flag_layout == 1) {                            /* rtm_model.cu:124-134 */
         int NX = (int)(P->offsetmax / dx + 1);
         int sp1 = (int)(P->spx0 / dx);
         int np = (int)(P->dspx / dx);
@@ -135,7 +150,7 @@ ORC_EXPORT int orc_shot_geometry(const orc_params *P, int ishot, orc_geom *g)
     g->hk_nt = (int)(short)(unsigned short)(P->nt);
     g->hk_dt = (int)(short)(unsigned short)(P->dt * 1000000);
     g->hk_dx = dx;
-    if (P->flag_layout <= 2) {
+    if (layout <= 2) {
         g->hk_itraceall = (ishot - 1) * nxl + 1;
         g->hk_offset = (int)(nx1 * dx - spx);
         g->hk_sx = (int)(spx);
@@ -155,8 +170,8 @@ ORC_EXPORT int orc_shot_geometry(const orc_params *P, int ishot, orc_geom *g)
     g->nsx = (int)(spx / dx);
     g->nsz = (int)(spz / dz);
     /* receivers: record1/insert1 (layouts 1,2), record2 (3), record3 (4) */
-    if (P->flag_layout <= 2) { g->nrec = nxl; g->rcol0 = npml; }
-    else if (P->flag_layout == 3) { g->nrec = nxl - noffset; g->rcol0 = npml + noffset; }
+    if (layout <= 2) { g->nrec = nxl; g->rcol0 = npml; }
+    else if (layout == 3) { g->nrec = nxl - noffset; g->rcol0 = npml + noffset; }
     else { g->nrec = nxl - noffset; g->rcol0 = npml; }
     if (g->nrec < 0) g->nrec = 0;
     return 0;
@@ -320,7 +335,7 @@ ORC_EXPORT void orc_mute_cuts(const orc_params *P, const orc_geom *g, const floa
     float dx2 = P->dx * P->dx, dz2 = P->dz * P->dz;
     int nw = (int)(P->direct / (P->fdom * dt));                 /* rtm_model.cu:22-23 */
     int tlength = (int)(P->direct0 / dt);
-    int coff = (P->flag_layout == 3) ? g->noffset : 0;          /* mute2 shifts by noffset */
+    int coff = (P->flag_layout == 3 || P->field) ? g->noffset : 0;   /* mute2 shifts by noffset */
     for (int ir = 0; ir < g->nrec; ir++) {
         int col = ir + npml + coff;
         float vtemp = (float)(0.482 * (1.0 / (double)sqrtf(vvpad[(size_t)npml * nxp + col]) +
