@@ -190,6 +190,41 @@ void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, Sho
   }
 }
 
+// direct-wave mute (kernel.cu:96-173) as the last muted sample per receiver,
+// from the v^2 window `wvv` (or the resident model's window when null)
+void mute_cuts(const rtmh_params& P, const float* vp0, ShotSetup& s, const float* wvv)
+{
+  const rtmh_geom& g = s.g;
+  const int npml = P.npml, nxp = g.nxp;
+  const float dx2 = P.dx * P.dx, dz2 = P.dz * P.dz;
+  const float dt = P.dt;
+  const float _dt = (float)(1.0 / (double)dt);
+  const int nw = (int)(P.direct / (P.fdom * dt));
+  const int tlength = (int)(P.direct0 / dt);
+  const int coff = (g.rcol0 > npml) ? g.noffset : 0;   // layout 3 and recorded gathers (mute2)
+  // v^2 of the padded window the forward pass uses (the mute kernels read d_vp)
+  auto vv_at = [&](int iz, int ix) {
+    if (wvv) return wvv[(size_t)iz * nxp + ix];
+    const float v = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, iz, ix);
+    return v * v;
+  };
+  const float vvsrc = vv_at(g.nsz, g.nsx);
+  s.cut.assign(g.nrec > 0 ? g.nrec : 1, -1);
+  for (int ir = 0; ir < g.nrec; ir++) {
+    const int col = ir + npml + coff;
+    const float vvr = vv_at(npml, col);
+    const float vtemp = (float)(0.482 * (1.0 / (double)sqrtf(vvr) + 1.0 / (double)sqrtf(vvsrc)));
+    float dist = (col - g.nsx) * (col - g.nsx) * dx2 + (npml - g.nsz) * (npml - g.nsz) * dz2;
+    dist = sqrtf(dist);
+    const int temp = tlength < P.nt ? tlength : P.nt;
+    const int c0 = (int)(dist * vtemp * _dt) + nw;
+    int c = c0 < P.nt ? c0 : P.nt;
+    c = c > temp ? c : temp;
+    c = c < P.nt ? c : P.nt;
+    s.cut[ir] = c;
+  }
+}
+
 void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, ShotSetup& s, const float* wvv)
 {
   const rtmh_geom& g = s.g;
@@ -243,33 +278,7 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
     s.sx0 = 0; s.sx1 = -1; s.sz0 = 0; s.sz1 = -1;
     s.sa.push_back(0.0f);
   }
-  // direct-wave mute (kernel.cu:96-173) as the last muted sample per receiver
-  const float dt = P.dt;
-  const float _dt = (float)(1.0 / (double)dt);
-  const int nw = (int)(P.direct / (P.fdom * dt));
-  const int tlength = (int)(P.direct0 / dt);
-  const int coff = (g.rcol0 > npml) ? g.noffset : 0;   // layout 3 and recorded gathers (mute2)
-  // v^2 of the padded window the forward pass uses (the mute kernels read d_vp)
-  auto vv_at = [&](int iz, int ix) {
-    if (wvv) return wvv[(size_t)iz * nxp + ix];
-    const float v = win_at(vp0, P.nx, P.nz, g.nx1, g.nxl, npml, iz, ix);
-    return v * v;
-  };
-  const float vvsrc = vv_at(g.nsz, g.nsx);
-  s.cut.assign(g.nrec > 0 ? g.nrec : 1, -1);
-  for (int ir = 0; ir < g.nrec; ir++) {
-    const int col = ir + npml + coff;
-    const float vvr = vv_at(npml, col);
-    const float vtemp = (float)(0.482 * (1.0 / (double)sqrtf(vvr) + 1.0 / (double)sqrtf(vvsrc)));
-    float dist = (col - g.nsx) * (col - g.nsx) * dx2 + (npml - g.nsz) * (npml - g.nsz) * dz2;
-    dist = sqrtf(dist);
-    const int temp = tlength < P.nt ? tlength : P.nt;
-    const int c0 = (int)(dist * vtemp * _dt) + nw;
-    int c = c0 < P.nt ? c0 : P.nt;
-    c = c > temp ? c : temp;
-    c = c < P.nt ? c : P.nt;
-    s.cut[ir] = c;
-  }
+  mute_cuts(P, vp0, s, wvv);
 }
 
 // ------------------------------------------------------------ checkpoint ----
@@ -1031,11 +1040,17 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         rtmh_gauss_smooth(s.wrho.data(), s.g.nzp, s.g.nxp, P.ismth);
       });
       for (int j = 0; j < nb; j++) {
-        const ShotSetup& s = setup[k0 + j];
+        ShotSetup& s = setup[k0 + j];
         const size_t wb = s.wvv.size() * 4;
         HIPCHK(hipMemcpyAsync((void*)ds[j].vv, s.wvv.data(), wb, hipMemcpyHostToDevice, st_));
         HIPCHK(hipMemcpyAsync((void*)ds[j].rho, s.wrho.data(), wb, hipMemcpyHostToDevice, st_));
         HIPCHK(rtm_launch_model_window(d_ds, j, 512, st_));
+        if (opt.shot_field) {
+          // a recorded gather is muted after the smoothing (rtm_real.cu:370-403),
+          // so with the smoothed velocities
+          mute_cuts(pk[k0 + j], vp0, s, s.wvv.data());
+          HIPCHK(hipMemcpyAsync((void*)ds[j].cut, s.cut.data(), s.cut.size() * 4, hipMemcpyHostToDevice, st_));
+        }
       }
     }
 

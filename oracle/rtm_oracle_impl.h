@@ -745,12 +745,14 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     }
     if (seis_in)
         for (size_t i = 0; i < (size_t)nt * g.nrec; i++) rec[i] = (RT)seis_in[i];
-    /* direct-wave mute (kernel.cu:96-173) */
+    /* direct-wave mute (kernel.cu:96-173): rtm_model mutes with the model of
+     * the forward pass, rtm_real after the smoothing below (rtm_real.cu:370-403) */
     int *cut = (int *)malloc(sizeof(int) * (g.nrec > 0 ? g.nrec : 1));
-    orc_mute_cuts(P, &g, NULL, vp0, m.vvf, cut);      /* the model of the forward pass */
-    for (int ir = 0; ir < g.nrec; ir++)
-        for (int it = 0; it < nt && it <= cut[ir]; it++) rec[(size_t)it * g.nrec + ir] = 0;
-    free(cut);
+    if (!P->field) {
+        orc_mute_cuts(P, &g, NULL, vp0, m.vvf, cut);
+        for (int ir = 0; ir < g.nrec; ir++)
+            for (int it = 0; it < nt && it <= cut[ir]; it++) rec[(size_t)it * g.nrec + ir] = 0;
+    }
 
     /* velsmooth1d of the window before the reverse pass (rtm_model.cu:402-405):
      * reconstruction and receiver propagation use the smoothed model */
@@ -762,6 +764,12 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
             m.ib[i] = (RT)1 / m.rho[i]; m.kk[i] = m.rho[i] * m.vv[i];
         }
     }
+    if (P->field) {
+        orc_mute_cuts(P, &g, NULL, vp0, m.vvf, cut);
+        for (int ir = 0; ir < g.nrec; ir++)
+            for (int it = 0; it < nt && it <= cut[ir]; it++) rec[(size_t)it * g.nrec + ir] = 0;
+    }
+    free(cut);
     /* reverse: ps <- P_{nt-1}; vxs = vzs = 0 (rtm_model.cu:366-372) */
     memset(S.vx, 0, (size_t)nzp * (nxp - 1) * sizeof(RT));
     memset(S.vz, 0, (size_t)(nzp - 1) * nxp * sizeof(RT));

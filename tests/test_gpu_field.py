@@ -29,17 +29,21 @@ def model(nz, nx, seed):
     return vp, rho
 
 
-def test_gather_migration_vs_oracle(gpu_ctx):
+@pytest.mark.parametrize("medium_flag,ismth", [(0, 0), (1, 2)])
+def test_gather_migration_vs_oracle(gpu_ctx, medium_flag, ismth):
+    """with smoothing a gather is muted with the smoothed velocities
+    (rtm_real.cu:370-403), unlike rtm_model's data"""
     nz, nx, nt = 50, 200, 260
     vp, rho = model(nz, nx, 21)
     fields = [(123.0, 0.0, 600.0), (487.5, 35.0, 905.0), (1000.0, 20.0, 700.0)]
     kw = dict(nshot=3, startshot=1, dshot=1, mode=1, flag_layout=3, nx=nx, nz=nz, nt=nt, dx=10.0, dz=10.0,
-              dt=1e-3, npml=32, fdom=20.0, spz0=20.0, direct=1.5, direct0=0.05)
+              dt=1e-3, npml=32, fdom=20.0, spz0=20.0, direct=1.5, direct0=0.05, medium_flag=medium_flag,
+              ismth=ismth)
     P = G.make_params(**kw)
     rng = np.random.default_rng(22)
     geoms = [G.gather_geometry(P, k + 1, f) for k, f in enumerate(fields)]
     recs = [rng.standard_normal((nt, g.nrec)).astype(np.float32) for g in geoms]
-    print("[field] 3 gathers 200x50 nt 260", flush=True)
+    print(f"[field] 3 gathers 200x50 nt 260 medium_flag {medium_flag} ismth {ismth}", flush=True)
     r = gpu_ctx.run(P, vp, rho, want_shots=True, seis_in=recs, shot_field=fields)
     for k, f in enumerate(fields):
         Po = O.make_params(**dict(kw, field=1, fsx=f[0], foffset=f[1], foffsetmax=f[2]))
