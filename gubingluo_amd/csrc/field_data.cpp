@@ -15,8 +15,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <complex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -152,10 +154,10 @@ void sinc_interp(const float* in, float* out, int nx, int ldin, int ldout, int n
 //     real part, no 1/n scaling.
 // For odd nt0 the reference's first mirrored row is one past its buffer, and
 // for nx > lx*nx0 its output columns are; such inputs are refused.
-int fk_interp(const float* in, int ldin, float* out, int ldout, int nx, int nx0, int nt0, int lx)
+int fk_interp(const float* in, int ldin, float* out, int ldout, int nx, int nx0, int nt0, int lx, std::string& err)
 {
-  if (nt0 % 2 != 0) { set_err("x-interpolation of an odd trace length (nt0) is not supported"); return -1; }
-  if (nx > lx * nx0) { set_err("x-interpolation: more output traces than lx*nx0"); return -1; }
+  if (nt0 % 2 != 0) { err = "x-interpolation of an odd trace length (nt0) is not supported"; return -1; }
+  if (nx > lx * nx0) { err = "x-interpolation: more output traces than lx*nx0"; return -1; }
   const int nf = 1 + nt0 / 2, Nx = lx * nx0, Nt = lx * nt0;
   std::vector<cd> B((size_t)nx0 * nt0), K, C((size_t)Nx * nt0), S((size_t)Nx * Nt, cd(0, 0)),
       Z((size_t)Nx * Nt, cd(0, 0)), H((size_t)Nx * nf), G1((size_t)nf * Nx), G((size_t)Nx * nt0);
@@ -285,22 +287,22 @@ extern "C" int rtmh_scan_gather(const char* path, int sgyflag, int* endianflag, 
   return 0;
 }
 
-extern "C" int rtmh_read_gather(const char* path, int sgyflag, int endianflag, int itraceall, int ntrace, int nt0,
-                                float* seis, int ncols)
+static int read_gather_impl(const char* path, int sgyflag, int endianflag, int itraceall, int ntrace, int nt0, float* seis,
+                            int ncols, std::string& err)
 {
-  if (!path || !seis || itraceall < 1 || ntrace < 0 || nt0 < 1 || ncols < 0) { set_err("bad argument"); return -1; }
+  if (!path || !seis || itraceall < 1 || ntrace < 0 || nt0 < 1 || ncols < 0) { err = "bad argument"; return -1; }
   FILE* fp = fopen(path, "rb");
-  if (!fp) { set_err(std::string("cannot open ") + path); return -1; }
+  if (!fp) { err = std::string("cannot open ") + path; return -1; }
   // rtm_real.cu:379-391: trace itraceall onwards, samples byte-swapped when
   // endianflag (float_to_float); traces past the record's width are dropped
   const long long off = (long long)(itraceall - 1) * (240 + (long long)nt0 * 4) + 3600LL * (sgyflag ? 1 : 0);
-  if (fseek(fp, (long)off, SEEK_SET) != 0) { fclose(fp); set_err("seek past the end of the gather file"); return -2; }
+  if (fseek(fp, (long)off, SEEK_SET) != 0) { fclose(fp); err = "seek past the end of the gather file"; return -2; }
   memset(seis, 0, sizeof(float) * (size_t)nt0 * ncols);
   std::vector<float> t(nt0);
   const int n = ntrace < ncols ? ntrace : ncols;
   for (int ix = 0; ix < n; ix++) {
     fseek(fp, 240L, SEEK_CUR);
-    if (fread(t.data(), 4, nt0, fp) != (size_t)nt0) { fclose(fp); set_err("short gather file"); return -2; }
+    if (fread(t.data(), 4, nt0, fp) != (size_t)nt0) { fclose(fp); err = "short gather file"; return -2; }
     if (endianflag)
       for (float& v : t) { uint32_t u; memcpy(&u, &v, 4); u = bswap32(u); memcpy(&v, &u, 4); }
     for (int it = 0; it < nt0; it++) seis[(size_t)it * ncols + ix] = t[it];
@@ -309,22 +311,31 @@ extern "C" int rtmh_read_gather(const char* path, int sgyflag, int endianflag, i
   return 0;
 }
 
-extern "C" int rtmh_interp_seis(float* seiscal, const float* seisobs, int nx, int nx0, int nt, int nt0, float dt,
-                                float dt0)
+extern "C" int rtmh_read_gather(const char* path, int sgyflag, int endianflag, int itraceall, int ntrace, int nt0,
+                                float* seis, int ncols)
 {
-  if (!seiscal || !seisobs || nx < 1 || nx0 < 1 || nt < 1 || nt0 < 1) { set_err("bad argument"); return -1; }
+  std::string err;
+  const int rc = read_gather_impl(path, sgyflag, endianflag, itraceall, ntrace, nt0, seis, ncols, err);
+  if (rc != 0) set_err(err);
+  return rc;
+}
+
+static int interp_seis_impl(float* seiscal, const float* seisobs, int nx, int nx0, int nt, int nt0, float dt, float dt0,
+                            std::string& err)
+{
+  if (!seiscal || !seisobs || nx < 1 || nx0 < 1 || nt < 1 || nt0 < 1) { err = "bad argument"; return -1; }
   // Interpseis2d (Interp_function.cpp:245-264)
   if (nx > nx0) {
-    if (nx0 < 2) { set_err("x-interpolation needs nx0 >= 2"); return -1; }
+    if (nx0 < 2) { err = "x-interpolation needs nx0 >= 2"; return -1; }
     const int lx = (nx - 1) / (nx0 - 1);
     if (nt > nt0) {
       std::vector<float> tmp((size_t)nt0 * nx);
-      if (fk_interp(seisobs, nx0, tmp.data(), nx, nx, nx0, nt0, lx) != 0) return -1;
+      if (fk_interp(seisobs, nx0, tmp.data(), nx, nx, nx0, nt0, lx, err) != 0) return -1;
       sinc_interp(tmp.data(), seiscal, nx, nx, nx, nt, nt0, dt, dt0);
     } else {
       // the reference writes all nt0 rows into the nt-row record; the first nt are kept
       std::vector<float> tmp((size_t)nt0 * nx);
-      if (fk_interp(seisobs, nx0, tmp.data(), nx, nx, nx0, nt0, lx) != 0) return -1;
+      if (fk_interp(seisobs, nx0, tmp.data(), nx, nx, nx0, nt0, lx, err) != 0) return -1;
       memcpy(seiscal, tmp.data(), sizeof(float) * (size_t)nt * nx);
     }
   } else if (nt > nt0) {
@@ -336,6 +347,15 @@ extern "C" int rtmh_interp_seis(float* seiscal, const float* seisobs, int nx, in
       for (int ix = 0; ix < nx; ix++) seiscal[(size_t)it * nx + ix] = seisobs[(size_t)it * nx0 + ix];
   }
   return 0;
+}
+
+extern "C" int rtmh_interp_seis(float* seiscal, const float* seisobs, int nx, int nx0, int nt, int nt0, float dt,
+                                float dt0)
+{
+  std::string err;
+  const int rc = interp_seis_impl(seiscal, seisobs, nx, nx0, nt, nt0, dt, dt0, err);
+  if (rc != 0) set_err(err);
+  return rc;
 }
 
 // rtm_real.cu:17-581 with the caller's prototype (RTM_GPU_velstr.cpp:30-35):
@@ -361,7 +381,10 @@ extern "C" void rtm_real(int idevice, int nshot, int startshot, int dshot, int m
   std::vector<float> field(3 * (size_t)(nsh > 0 ? nsh : 1));
   std::vector<std::vector<float>> rec(nsh);
   std::vector<const float*> recp(nsh);
-  for (int k = 0; k < nsh; k++) {
+  // per gather: apertures, traces, resampling (host, independent per shot:
+  // up to 16 threads, each shot's error kept with it)
+  std::vector<std::string> err(nsh);
+  auto prep = [&](int k) {
     const int j = shots[k] - 1;
     const int sx = shotprofileId[0][j], ntrace = shotprofileId[1][j], itr = shotprofileId[2][j];
     const int off = shotprofileId[3][j];
@@ -370,20 +393,21 @@ extern "C" void rtm_real(int idevice, int nshot, int startshot, int dshot, int m
     const int noffset0 = (int)(offset / dx0), nx10 = (int)(spx / dx0), nx20 = (int)((spx + offsetmax) / dx0);
     const int noffset = (int)(offset / dx), nx1 = (int)(spx / dx), nx2 = (int)((spx + offsetmax) / dx);
     const int ncol0 = nx20 - nx10 + 1 - noffset0, ncol = nx2 - nx1 + 1 - noffset;
-    if (ncol0 < 1 || ncol < 1) { fprintf(stderr, "rtm_real: shot %d has no receivers\n", shots[k]); exit(1); }
+    if (ncol0 < 1 || ncol < 1) { err[k] = "no receivers"; return; }
     std::vector<float> obs((size_t)nt0 * ncol0);
-    if (rtmh_read_gather(path, sgyflag, endianflag, itr, ntrace, nt0, obs.data(), ncol0) != 0) {
-      fprintf(stderr, "rtm_real: %s\n", rtmh_last_error());
-      exit(1);
-    }
+    if (read_gather_impl(path, sgyflag, endianflag, itr, ntrace, nt0, obs.data(), ncol0, err[k]) != 0) return;
     rec[k].assign((size_t)nt * ncol, 0.0f);
-    if (rtmh_interp_seis(rec[k].data(), obs.data(), ncol, ncol0, nt, nt0, dt, dt0) != 0) {
-      fprintf(stderr, "rtm_real: shot %d: %s\n", shots[k], rtmh_last_error());
-      exit(1);
-    }
+    if (interp_seis_impl(rec[k].data(), obs.data(), ncol, ncol0, nt, nt0, dt, dt0, err[k]) != 0) return;
     recp[k] = rec[k].data();
     field[3 * k] = spx; field[3 * k + 1] = offset; field[3 * k + 2] = offsetmax;
-  }
+  };
+  const int nth = std::max(1, std::min({nsh, 16, (int)std::thread::hardware_concurrency()}));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; t++)
+    th.emplace_back([&, t]() { for (int k = t; k < nsh; k += nth) prep(k); });
+  for (auto& x : th) x.join();
+  for (int k = 0; k < nsh; k++)
+    if (!err[k].empty()) { fprintf(stderr, "rtm_real: shot %d: %s\n", shots[k], err[k].c_str()); exit(1); }
   rtmh_params P;
   memset(&P, 0, sizeof(P));
   P.nshot = nshot; P.startshot = startshot; P.dshot = dshot; P.mode = 1; P.medium_flag = medium_flag;
