@@ -27,6 +27,12 @@ Other workloads (one JSON line each, same keys):
                         grids 256^2..2048x1024, nt 1000-6000, 1-32 shots,
                         FP16 boundaries, continuous batching; against the
                         plain one-shot-per-launch GPU form on the same stream
+  --workload field      field-data path on the survey model: 128 recorded
+                        one-sided gathers (151 traces at 20 m, 2000 samples
+                        at 1.6 ms, seeded random samples) scanned, read and
+                        resampled to 301 traces at 10 m and 4000 samples at
+                        0.8 ms on up to 16 host threads (f-k and sinc), then
+                        migrated; the step is timed whole (file to images)
 
 Output: ONE JSON line on stdout (progress goes to stderr).
 """
@@ -196,6 +202,98 @@ def run_survey(args, G, synth):
     return out
 
 
+# ------------------------------------------------------------------- field --
+def run_field(args, G, synth):
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
+    P, vp, rho = synth.survey200(nshot=128)
+    nsh, ntr, nt0, dt0, dx0 = 128, 151, 2000, 1.6e-3, 20.0
+    P.mode, P.flag_layout, P.nshot, P.startshot, P.dshot = 1, 3, nsh, 1, 1
+    if args.nt:
+        P.nt = args.nt
+        nt0 = P.nt // 2
+    # one SEG-Y gather file: shot k at sx = 500 + 50 k m, traces at offsets 0, 20, ... 3000 m
+    rng = np.random.default_rng(41)
+    hdr = np.dtype([("tracl", "<i4"), ("p0", "V32"), ("offset", "<i4"), ("p1", "V32"), ("sx", "<i4"),
+                    ("p2", "V38"), ("ns", "<i2"), ("dt", "<i2"), ("p3", "V122")])
+    rec = np.zeros(nsh * ntr, dtype=[("h", hdr), ("d", "<f4", (nt0,))])
+    rec["h"]["tracl"] = np.arange(1, nsh * ntr + 1)
+    rec["h"]["offset"] = np.tile(np.arange(ntr) * int(dx0), nsh)
+    rec["h"]["sx"] = np.repeat(500 + 50 * np.arange(nsh), ntr)
+    rec["h"]["ns"], rec["h"]["dt"] = nt0, int(dt0 * 1e6)
+    rec["d"] = rng.standard_normal((nsh * ntr, nt0)).astype(np.float32)
+    tmpd = tempfile.mkdtemp(prefix="rtmh_field_")
+    path = os.path.join(tmpd, "gather.sgy")
+    with open(path, "wb") as f:
+        f.write(b"\0" * 3600)
+        rec.tofile(f)
+    del rec
+    log(f"[bench] field data: {nsh} gathers x {ntr} traces x {nt0} samples at {dx0:.0f} m / {dt0 * 1e3:.1f} ms "
+        f"-> {P.nx}x{P.nz} dx={P.dx:.0f} nt={P.nt} dt={P.dt * 1e3:.1f} ms; steps={args.steps} warmup={args.warmup}")
+    ctx = G.Context(0)
+    mid = ctx.model_create(vp, rho)
+
+    def prepare():
+        tab, endian = G.scan_gather(path, 1)
+
+        def one(k):
+            sx, n, itr, off = (int(v) for v in tab[:, k])
+            offsetmax = np.float32(off + (n - 1) * dx0)
+            n1, n2, no = int(np.float32(sx) / np.float32(P.dx)), int(np.float32(sx + offsetmax) / np.float32(P.dx)), \
+                int(np.float32(off) / np.float32(P.dx))
+            n10 = int(np.float32(sx) / np.float32(dx0))
+            n20 = int(np.float32(np.float32(sx) + offsetmax) / np.float32(dx0))
+            obs = G.read_gather(path, itr, n, nt0, n20 - n10 + 1 - int(np.float32(off) / np.float32(dx0)), 1, endian)
+            return (float(sx), float(off), float(offsetmax)), G.interp_seis(obs, n2 - n1 + 1 - no, P.nt, P.dt, dt0)
+
+        with ThreadPoolExecutor(16) as ex:
+            got = list(ex.map(one, range(tab.shape[1])))
+        return [g[0] for g in got], [g[1] for g in got]
+
+    def step():
+        t0 = time.perf_counter()
+        fields, seis = prepare()
+        t1 = time.perf_counter()
+        r = ctx.run(P, model=mid, seis_in=seis, shot_field=fields, graph_chunk=args.chunk, tile_z=args.tz,
+                    want_post=True)
+        return r, time.perf_counter() - t0, t1 - t0
+
+    for k in range(args.warmup):
+        _, dt, prep = step()
+        log(f"[bench] warmup step {k}: {dt:.2f} s (host preparation {prep:.2f} s)")
+    recs, preps = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r, dt, prep = step()
+        recs.append((r.stats, dt, list(range(1, nsh + 1))))
+        preps.append(prep)
+        s = r.stats
+        log(f"[bench] step {k}: {nsh} gathers {dt:.2f} s (host preparation {prep:.2f} s), "
+            f"{s.cell_updates / dt / 1e9:.1f} Gcell/s whole, {s.cell_updates / (s.gpu_ms * 1e-3) / 1e9:.1f} on the GPU")
+    elapsed = time.perf_counter() - t0
+    cells = sum(s.cell_updates for s, _, _ in recs)
+    gpu_s = sum(s.gpu_ms for s, _, _ in recs) * 1e-3
+    roof = roofline_of(recs, P, None)
+    out = base_line(
+        args, round(cells / elapsed / 1e9, 3), "Gcell/s", elapsed, "f32",
+        "synthetic: seeded random samples in a SEG-Y gather file written before the timed region",
+        {"workload": f"field-data RTM (rtm_real path), survey model 1024x512 dx=10 m, PML 50, {nsh} one-sided "
+                     f"gathers of {ntr} traces at 20 m and {nt0} samples at 1.6 ms resampled to 301 traces "
+                     f"and nt={P.nt} at 0.8 ms, fp32",
+         "shots_per_step": nsh, "graph_chunk": args.chunk, "tile_z": recs[0][0].tile_z,
+         "tile_z_rev": recs[0][0].tile_z_rev},
+        {"shots_per_hour": round(nsh * len(recs) / elapsed * 3600.0, 1),
+         "host_preparation_s_per_step": round(float(np.mean(preps)), 3),
+         "gpu_gcell_s": round(cells / gpu_s / 1e9, 3),
+         "roofline": roof, "cpu_baseline": None})
+    ctx.model_destroy(mid)
+    ctx.close()
+    os.remove(path)
+    os.rmdir(tmpd)
+    return out
+
+
 # ----------------------------------------------------------------- batched --
 def run_batched(args, G, synth):
     P, vp, rho, om = synth.batched_ragged()
@@ -362,7 +460,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (survey default 2, others 10)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="survey", choices=["survey", "batched", "precision", "serve"])
+    ap.add_argument("--workload", default="survey", choices=["survey", "batched", "precision", "serve", "field"])
     ap.add_argument("--shots-per-step", type=int, default=0,
                     help="shots per step (survey default 200: the whole survey in one launch batch)")
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: all shots of the step)")
@@ -389,7 +487,7 @@ def main():
     from gubingluo_amd import synth
 
     if args.steps is None:
-        args.steps = 2 if args.workload == "survey" else 10
+        args.steps = 2 if args.workload in ("survey", "field") else 10
     if args.workload == "survey":
         if not args.shots_per_step:
             args.shots_per_step = 200
@@ -398,6 +496,8 @@ def main():
         out = run_batched(args, G, synth)
     elif args.workload == "precision":
         out = run_precision(args, G, synth)
+    elif args.workload == "field":
+        out = run_field(args, G, synth)
     else:
         out = run_serve(args, G, synth)
     print(json.dumps(out), flush=True)

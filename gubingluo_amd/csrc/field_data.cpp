@@ -31,9 +31,9 @@ void set_err(const std::string& s) { rtm::set_error(s); }
 // Unnormalised complex DFT in double precision, X[k] = sum_j x[j] e^{sign 2 pi i jk/n}:
 // sign -1 is FFTW_FORWARD, +1 FFTW_BACKWARD (no 1/n either way), the
 // conventions of the FFTW calls at Interp_function.cpp:150-225.  Powers of
-// two by an iterative radix-2 transform, other lengths by Bluestein's
-// chirp-z identity jk = (j^2 + k^2 - (k-j)^2)/2 over a power-of-two
-// convolution.
+// two by an iterative radix-2 transform, lengths with prime factors up to 7
+// by mixed-radix Cooley-Tukey, others by Bluestein's chirp-z identity
+// jk = (j^2 + k^2 - (k-j)^2)/2 over a power-of-two convolution.
 void fft_pow2(cd* a, int n, int sign)
 {
   for (int i = 1, j = 0; i < n; i++) {
@@ -55,33 +55,114 @@ void fft_pow2(cd* a, int n, int sign)
   }
 }
 
+struct Chirp {
+  long long key;
+  int m;
+  std::vector<cd> w, Bf;     // e^{sign i pi k^2/n}; FFT of the conjugate chirp, length m
+};
+
+void fft_bluestein(cd* a, int n, int sign)
+{
+  // chirp and its transform per (length, direction), kept per thread
+  thread_local std::vector<Chirp> cache;
+  const long long key = 2LL * n + (sign > 0 ? 1 : 0);
+  const Chirp* c = nullptr;
+  for (auto& e : cache)
+    if (e.key == key) { c = &e; break; }
+  if (!c) {
+    Chirp e;
+    e.key = key;
+    e.m = 1;
+    while (e.m < 2 * n - 1) e.m <<= 1;
+    e.w.resize(n);
+    const long long n2 = 2LL * n;
+    for (int k = 0; k < n; k++) {
+      const long long q = ((long long)k * k) % n2;      // e^{i pi k^2/n} has period 2n in k^2
+      const double ang = sign * M_PI * (double)q / n;
+      e.w[k] = cd(cos(ang), sin(ang));
+    }
+    e.Bf.assign(e.m, cd(0, 0));
+    e.Bf[0] = std::conj(e.w[0]);
+    for (int k = 1; k < n; k++) e.Bf[k] = e.Bf[e.m - k] = std::conj(e.w[k]);
+    fft_pow2(e.Bf.data(), e.m, -1);
+    if (cache.size() >= 8) cache.erase(cache.begin());
+    cache.push_back(std::move(e));
+    c = &cache.back();
+  }
+  const int m = c->m;
+  std::vector<cd> A(m, cd(0, 0));
+  for (int k = 0; k < n; k++) A[k] = a[k] * c->w[k];
+  fft_pow2(A.data(), m, -1);
+  for (int i = 0; i < m; i++) A[i] *= c->Bf[i];
+  fft_pow2(A.data(), m, 1);
+  for (int k = 0; k < n; k++) a[k] = A[k] / (double)m * c->w[k];
+}
+
+// Mixed-radix Cooley-Tukey (decimation in time, radices 2, 3, 5, 7) for
+// lengths whose prime factors are all small, e.g. trace lengths 2000 or 4000:
+// the p sub-transforms of the decimated sequences x[r + p j], then p-point
+// butterflies with twiddles W_n^{rk}.  tw holds e^{sign 2 pi i j/N}, j < N.
+void fft_mixed(const cd* x, int stride, cd* y, int n, int N, const cd* tw)
+{
+  if (n == 1) { y[0] = x[0]; return; }
+  int p = 2;
+  while (n % p) p++;
+  const int m = n / p;
+  for (int r = 0; r < p; r++) fft_mixed(x + (size_t)r * stride, stride * p, y + (size_t)r * m, m, N, tw);
+  const int sn = N / n, sp = N / p;
+  cd t[7], o[7];
+  for (int k = 0; k < m; k++) {
+    for (int r = 0; r < p; r++) t[r] = y[(size_t)r * m + k] * tw[(size_t)((long long)r * k * sn % N)];
+    for (int q = 0; q < p; q++) {
+      cd acc = t[0];
+      for (int r = 1; r < p; r++) acc += t[r] * tw[(size_t)((r * q) % p) * sp];
+      o[q] = acc;
+    }
+    for (int q = 0; q < p; q++) y[(size_t)k + (size_t)q * m] = o[q];
+  }
+}
+
+bool small_factors(int n)
+{
+  for (int p : {2, 3, 5, 7})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
 void fft(cd* a, int n, int sign)
 {
   if (n <= 1) return;
   if ((n & (n - 1)) == 0) { fft_pow2(a, n, sign); return; }
-  int m = 1;
-  while (m < 2 * n - 1) m <<= 1;
-  std::vector<cd> w(n), A(m, cd(0, 0)), B(m, cd(0, 0));
-  const long long n2 = 2LL * n;
-  for (int k = 0; k < n; k++) {
-    const long long q = ((long long)k * k) % n2;        // e^{i pi k^2/n} has period 2n in k^2
-    const double ang = sign * M_PI * (double)q / n;
-    w[k] = cd(cos(ang), sin(ang));
+  if (!small_factors(n)) { fft_bluestein(a, n, sign); return; }
+  // twiddle table per (length, direction), kept per thread
+  thread_local std::vector<std::pair<long long, std::vector<cd>>> cache;
+  const long long key = 2LL * n + (sign > 0 ? 1 : 0);
+  const std::vector<cd>* tw = nullptr;
+  for (auto& c : cache)
+    if (c.first == key) { tw = &c.second; break; }
+  if (!tw) {
+    std::vector<cd> t(n);
+    for (int j = 0; j < n; j++) {
+      const double ang = sign * 2.0 * M_PI * (double)j / n;
+      t[j] = cd(cos(ang), sin(ang));
+    }
+    if (cache.size() >= 8) cache.erase(cache.begin());
+    cache.emplace_back(key, std::move(t));
+    tw = &cache.back().second;
   }
-  for (int k = 0; k < n; k++) A[k] = a[k] * w[k];
-  B[0] = std::conj(w[0]);
-  for (int k = 1; k < n; k++) B[k] = B[m - k] = std::conj(w[k]);
-  fft_pow2(A.data(), m, -1);
-  fft_pow2(B.data(), m, -1);
-  for (int i = 0; i < m; i++) A[i] *= B[i];
-  fft_pow2(A.data(), m, 1);
-  for (int k = 0; k < n; k++) a[k] = A[k] / (double)m * w[k];
+  std::vector<cd> x(a, a + n);
+  fft_mixed(x.data(), 1, a, n, n, tw->data());
 }
 
 // 2-D transform of a row-major n0 x n1 array (fftw_plan_dft_2d(n0, n1, ...))
 void fft2(cd* a, int n0, int n1, int sign)
 {
-  for (int i = 0; i < n0; i++) fft(a + (size_t)i * n1, n1, sign);
+  for (int i = 0; i < n0; i++) {
+    const cd* r = a + (size_t)i * n1;
+    bool zero = true;
+    for (int j = 0; j < n1 && zero; j++) zero = (r[j] == cd(0, 0));
+    if (!zero) fft(a + (size_t)i * n1, n1, sign);        // the transform of zeros is zeros
+  }
   std::vector<cd> col(n0);
   for (int j = 0; j < n1; j++) {
     for (int i = 0; i < n0; i++) col[i] = a[(size_t)i * n1 + j];
@@ -122,20 +203,23 @@ void sinc_interp(const float* in, float* out, int nx, int ldin, int ldout, int n
   }
   std::vector<float> tn(nt0);
   for (int it = 0; it < nt0; it++) tn[it] = it * dt0;
-  for (int ix = 0; ix < nx; ix++) {
-    for (int it = 0; it < nt; it++) {
-      const int it0 = (int)((it + 1) * dt / dt0);
-      const int s0 = it0 - half > 0 ? it0 - half : 0;
-      const int s1 = it0 + half < nt0 - 1 ? it0 + half : nt0 - 1;
-      const int len = s1 - s0 + 1;
-      const int w0 = (s0 <= half) ? nw - len : 0;
-      const float t = it * dt;
+  // the window, its taps and sincs depend on it only; each trace then sums
+  // in*sinc*w in k order
+  float sc[21];
+  for (int it = 0; it < nt; it++) {
+    const int it0 = (int)((it + 1) * dt / dt0);
+    const int s0 = it0 - half > 0 ? it0 - half : 0;
+    const int s1 = it0 + half < nt0 - 1 ? it0 + half : nt0 - 1;
+    const int len = s1 - s0 + 1;
+    const int w0 = (s0 <= half) ? nw - len : 0;
+    const float t = it * dt;
+    for (int k = 0; k < len; k++) {
+      const float x = (t - tn[k + s0]) / dt0;
+      sc[k] = (x == 0.0f) ? 1.0f : (float)(sin(3.1415926535897932 * x) / (3.1415926535897932 * x));
+    }
+    for (int ix = 0; ix < nx; ix++) {
       float sum = 0.0f;
-      for (int k = 0; k < len; k++) {
-        const float x = (t - tn[k + s0]) / dt0;
-        const float sc = (x == 0.0f) ? 1.0f : (float)(sin(3.1415926535897932 * x) / (3.1415926535897932 * x));
-        sum = sum + in[(size_t)(k + s0) * ldin + ix] * sc * wk[k + w0];
-      }
+      for (int k = 0; k < len; k++) sum = sum + in[(size_t)(k + s0) * ldin + ix] * sc[k] * wk[k + w0];
       out[(size_t)it * ldout + ix] = sum;
     }
   }
