@@ -49,7 +49,8 @@ class Options(C.Structure):
     _fields_ = [("batch", C.c_int), ("store_fmt", C.c_int), ("graph_chunk", C.c_int), ("tile_z", C.c_int),
                 ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int),
                 ("shot_offsetmax", C.c_void_p), ("checkpoint", C.c_char_p), ("resume", C.c_int),
-                ("ic_extra", C.c_int), ("shot_field", C.c_void_p)]
+                ("ic_extra", C.c_int), ("shot_field", C.c_void_p), ("seis_wait", C.c_void_p),
+                ("seis_user", C.c_void_p)]
 
 
 # the other imaging conditions (rtmh_outputs order)
@@ -179,11 +180,16 @@ def read_gather(path: str, itraceall: int, ntrace: int, nt0: int, ncols: int, sg
     return out
 
 
-def interp_seis(seisobs, nx: int, nt: int, dt: float, dt0: float) -> np.ndarray:
-    """Interpseis2d: seisobs[nt0][nx0] at dt0 onto [nt][nx] at dt."""
+SEIS_WAIT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int)
+
+
+def interp_seis(seisobs, nx: int, nt: int, dt: float, dt0: float, out=None) -> np.ndarray:
+    """Interpseis2d: seisobs[nt0][nx0] at dt0 onto [nt][nx] at dt (into `out` when given)."""
     a = np.ascontiguousarray(seisobs, np.float32)
     nt0, nx0 = a.shape
-    out = np.zeros((nt, nx), np.float32)
+    if out is None:
+        out = np.zeros((nt, nx), np.float32)
+    assert out.shape == (nt, nx) and out.dtype == np.float32 and out.flags.c_contiguous
     rc = lib().rtmh_interp_seis(_fp(out), _fp(a), C.c_int(nx), C.c_int(nx0), C.c_int(nt), C.c_int(nt0),
                                 C.c_float(dt), C.c_float(dt0))
     if rc != 0:
@@ -252,11 +258,12 @@ class Context:
     def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
             graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
             want_post=True, seis_in=None, profile=False, shot_offsetmax=None, checkpoint=None,
-            resume=False, ic_extra=False, shot_field=None) -> RunResult:
+            resume=False, ic_extra=False, shot_field=None, seis_wait=None) -> RunResult:
         """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model.
         checkpoint: path written after every batch (FP64 stacks, shots done); resume=True
         continues a job from it.  shot_field: per shot (sx, offset, offsetmax) of recorded
-        gathers (the geometry of rtm_real)."""
+        gathers (the geometry of rtm_real).  seis_wait: a SEIS_WAIT callback (user, shot index)
+        the engine calls before it reads shot k of seis_in, after the forward pass."""
         coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
         nz, nx = P.nz, P.nx
         if model is None:
@@ -310,6 +317,9 @@ class Context:
         ck = None if checkpoint is None else os.fsencode(checkpoint)
         opt = Options(batch, store_fmt, graph_chunk, tile_z, int(write_files), int(verbose), float(mem_budget),
                       int(profile), _fp(som), ck, int(resume), int(ic_extra), _fp(sf))
+        if seis_wait is not None:
+            keep.append(seis_wait)
+            opt.seis_wait = C.cast(seis_wait, C.c_void_p)
         st = Stats()
         if model is None:
             rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),

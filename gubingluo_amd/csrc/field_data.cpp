@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <complex>
+#include <condition_variable>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -465,33 +467,56 @@ extern "C" void rtm_real(int idevice, int nshot, int startshot, int dshot, int m
   std::vector<float> field(3 * (size_t)(nsh > 0 ? nsh : 1));
   std::vector<std::vector<float>> rec(nsh);
   std::vector<const float*> recp(nsh);
-  // per gather: apertures, traces, resampling (host, independent per shot:
-  // up to 16 threads, each shot's error kept with it)
+  // per gather: apertures (here), traces and resampling (on up to 16 host
+  // threads, each shot's error kept with it).  The engine reads a record only
+  // after the forward pass of its batch, so the preparation runs beside it:
+  // seis_wait blocks until the shot's record is done.
   std::vector<std::string> err(nsh);
-  auto prep = [&](int k) {
+  std::vector<int> ncol0(nsh), ncol(nsh);
+  for (int k = 0; k < nsh; k++) {
     const int j = shots[k] - 1;
-    const int sx = shotprofileId[0][j], ntrace = shotprofileId[1][j], itr = shotprofileId[2][j];
-    const int off = shotprofileId[3][j];
+    const int sx = shotprofileId[0][j], ntrace = shotprofileId[1][j], off = shotprofileId[3][j];
     // rtm_real.cu:121-136: apertures of the recorded and the modelling grid
     const float offsetmax = (float)(off + (ntrace - 1) * dx0), offset = (float)off, spx = (float)sx;
     const int noffset0 = (int)(offset / dx0), nx10 = (int)(spx / dx0), nx20 = (int)((spx + offsetmax) / dx0);
     const int noffset = (int)(offset / dx), nx1 = (int)(spx / dx), nx2 = (int)((spx + offsetmax) / dx);
-    const int ncol0 = nx20 - nx10 + 1 - noffset0, ncol = nx2 - nx1 + 1 - noffset;
-    if (ncol0 < 1 || ncol < 1) { err[k] = "no receivers"; return; }
-    std::vector<float> obs((size_t)nt0 * ncol0);
-    if (read_gather_impl(path, sgyflag, endianflag, itr, ntrace, nt0, obs.data(), ncol0, err[k]) != 0) return;
-    rec[k].assign((size_t)nt * ncol, 0.0f);
-    if (interp_seis_impl(rec[k].data(), obs.data(), ncol, ncol0, nt, nt0, dt, dt0, err[k]) != 0) return;
+    ncol0[k] = nx20 - nx10 + 1 - noffset0;
+    ncol[k] = nx2 - nx1 + 1 - noffset;
+    if (ncol0[k] < 1 || ncol[k] < 1) { fprintf(stderr, "rtm_real: shot %d has no receivers\n", shots[k]); exit(1); }
+    rec[k].assign((size_t)nt * ncol[k], 0.0f);
     recp[k] = rec[k].data();
     field[3 * k] = spx; field[3 * k + 1] = offset; field[3 * k + 2] = offsetmax;
+  }
+  struct Ready {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<char> done;
+    const std::vector<std::string>* err;
+  } ready;
+  ready.done.assign(nsh, 0);
+  ready.err = &err;
+  auto prep = [&](int k) {
+    const int j = shots[k] - 1;
+    std::vector<float> obs((size_t)nt0 * ncol0[k]);
+    if (read_gather_impl(path, sgyflag, endianflag, shotprofileId[2][j], shotprofileId[1][j], nt0, obs.data(),
+                         ncol0[k], err[k]) == 0)
+      interp_seis_impl(rec[k].data(), obs.data(), ncol[k], ncol0[k], nt, nt0, dt, dt0, err[k]);
+    {
+      std::lock_guard<std::mutex> lk(ready.mu);
+      ready.done[k] = 1;
+    }
+    ready.cv.notify_all();
   };
   const int nth = std::max(1, std::min({nsh, 16, (int)std::thread::hardware_concurrency()}));
   std::vector<std::thread> th;
   for (int t = 0; t < nth; t++)
     th.emplace_back([&, t]() { for (int k = t; k < nsh; k += nth) prep(k); });
-  for (auto& x : th) x.join();
-  for (int k = 0; k < nsh; k++)
-    if (!err[k].empty()) { fprintf(stderr, "rtm_real: shot %d: %s\n", shots[k], err[k].c_str()); exit(1); }
+  auto wait_cb = [](void* user, int k) -> int {
+    Ready* r = (Ready*)user;
+    std::unique_lock<std::mutex> lk(r->mu);
+    r->cv.wait(lk, [&] { return r->done[k] != 0; });
+    return (*r->err)[k].empty() ? 0 : -1;
+  };
   rtmh_params P;
   memset(&P, 0, sizeof(P));
   P.nshot = nshot; P.startshot = startshot; P.dshot = dshot; P.mode = 1; P.medium_flag = medium_flag;
@@ -515,6 +540,8 @@ extern "C" void rtm_real(int idevice, int nshot, int startshot, int dshot, int m
   if (ck && *ck) { o.checkpoint = ck; o.resume = 1; }
   o.ic_extra = 1;
   o.shot_field = field.data();
+  o.seis_wait = wait_cb;
+  o.seis_user = &ready;
   const size_t n = (size_t)nx * nz;
   std::vector<float> imagetrue(n, 0.0f);
   rtmh_outputs out;
@@ -527,6 +554,9 @@ extern "C" void rtm_real(int idevice, int nshot, int startshot, int dshot, int m
   rtmh_stats st;
   memset(&st, 0, sizeof(st));
   const int rc = rtmh_run(ctx, &P, vp, rho, diffcoef, &o, &out, &st);
+  for (auto& x : th) x.join();
+  for (int k = 0; k < nsh; k++)
+    if (!err[k].empty()) { fprintf(stderr, "rtm_real: shot %d: %s\n", shots[k], err[k].c_str()); exit(1); }
   if (rc != 0) {
     fprintf(stderr, "rtm_real: %s\n", rtmh_last_error());
     rtmh_destroy(ctx);

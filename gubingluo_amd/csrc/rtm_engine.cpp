@@ -1009,6 +1009,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         }
       } else {
         rec.assign(nrv, 0.0f);
+        if (opt.seis_wait && opt.seis_wait(opt.seis_user, k0 + j) != 0) {
+          set_error("the input record of shot " + std::to_string(ishot) + " is not available");
+          rc = -4;
+          break;
+        }
         if (out && out->shot_seis_in && out->shot_seis_in[k0 + j]) {
           memcpy(rec.data(), out->shot_seis_in[k0 + j], nrv * 4);
         } else {

@@ -130,6 +130,12 @@ typedef struct rtmh_options {
    * offset on the source's right, direct-wave mute of cuda_mute2; NULL: the
    * geometry of flag_layout.  The drop-in rtm_real sets it. */
   const float *shot_field;
+  /* mode 1: called with the shot's job index before its input record
+   * (rtmh_outputs.shot_seis_in) is read, which happens after the forward
+   * pass of its batch, so that a caller can still be preparing the records
+   * meanwhile; non-zero return aborts the job.  NULL: records are ready. */
+  int (*seis_wait)(void *user, int shot);
+  void *seis_user;
 } rtmh_options;
 
 /* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */

@@ -44,7 +44,10 @@ def test_gather_migration_vs_oracle(gpu_ctx, medium_flag, ismth):
     geoms = [G.gather_geometry(P, k + 1, f) for k, f in enumerate(fields)]
     recs = [rng.standard_normal((nt, g.nrec)).astype(np.float32) for g in geoms]
     print(f"[field] 3 gathers 200x50 nt 260 medium_flag {medium_flag} ismth {ismth}", flush=True)
-    r = gpu_ctx.run(P, vp, rho, want_shots=True, seis_in=recs, shot_field=fields)
+    calls = []
+    cb = G.SEIS_WAIT(lambda user, k: calls.append(k) or 0)
+    r = gpu_ctx.run(P, vp, rho, want_shots=True, seis_in=recs, shot_field=fields, seis_wait=cb)
+    assert calls == [0, 1, 2]                  # each record waited for once, in job order
     for k, f in enumerate(fields):
         Po = O.make_params(**dict(kw, field=1, fsx=f[0], foffset=f[1], foffsetmax=f[2]))
         o = O.run_shot(Po, k + 1, vp, rho, seis_in=recs[k], korder=True)
