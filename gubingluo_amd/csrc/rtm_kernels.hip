@@ -237,9 +237,14 @@ __device__ __forceinline__ float bld(rsrc_t r, int voff, int soff)
 {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
+// Field stores carry the non-temporal hint (cache-policy bit 1, "nt" on
+// gfx950): a value written in one step is next read in the following step,
+// after the whole batch (far more than the L2 and the Infinity Cache hold)
+// has streamed through the caches.  Measured +0.4-0.8 % on the 200-shot
+// survey (three alternations with the cached form on one device).
 __device__ __forceinline__ void bst(rsrc_t r, float v, int voff, int soff)
 {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 2);
 }
 // An LDS pointer the compiler must materialise as one register: the taps of
 // a stencil then use the small immediate offsets of that base instead of a
