@@ -32,8 +32,8 @@ Other workloads (one JSON line each, same keys):
                         at 1.6 ms, seeded random samples) scanned, read and
                         resampled to 301 traces at 10 m and 4000 samples at
                         0.8 ms on 16 host threads (f-k and sinc) beside the
-                        forward pass, then migrated; the step is timed whole
-                        (file to images)
+                        GPU, migrated in batches of 32 (rtm_real's rule); the
+                        step is timed whole (file to images)
 
 Output: ONE JSON line on stdout (progress goes to stderr).
 """
@@ -269,8 +269,10 @@ def run_field(args, G, synth):
                 return 1
 
         cb = G.SEIS_WAIT(wait)
-        r = ctx.run(P, model=mid, seis_in=recs, shot_field=fields, graph_chunk=args.chunk, tile_z=args.tz,
-                    want_post=True, seis_wait=cb)
+        # rtm_real's batch rule: a quarter of the job, at least 32 shots
+        batch = args.batch or (0 if nsh <= 32 else max(32, (nsh + 3) // 4))
+        r = ctx.run(P, model=mid, seis_in=recs, shot_field=fields, batch=batch, graph_chunk=args.chunk,
+                    tile_z=args.tz, want_post=True, seis_wait=cb)
         ex.shutdown()
         return r, time.perf_counter() - t0, max(done) - t0
 
@@ -296,7 +298,7 @@ def run_field(args, G, synth):
         {"workload": f"field-data RTM (rtm_real path), survey model 1024x512 dx=10 m, PML 50, {nsh} one-sided "
                      f"gathers of {ntr} traces at 20 m and {nt0} samples at 1.6 ms resampled to 301 traces "
                      f"and nt={P.nt} at 0.8 ms, fp32",
-         "shots_per_step": nsh, "graph_chunk": args.chunk, "tile_z": recs[0][0].tile_z,
+         "shots_per_step": nsh, "batch": recs[0][0].batch, "graph_chunk": args.chunk, "tile_z": recs[0][0].tile_z,
          "tile_z_rev": recs[0][0].tile_z_rev},
         {"shots_per_hour": round(nsh * len(recs) / elapsed * 3600.0, 1),
          "host_preparation_done_after_s": round(float(np.mean(preps)), 3),
