@@ -27,6 +27,7 @@ EXPORTED = [
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
     "rtmh_server_create", "rtmh_server_submit", "rtmh_server_wait", "rtmh_server_query", "rtmh_server_destroy",
     "rtmh_scan_gather", "rtmh_read_gather", "rtmh_interp_seis", "rtmh_gather_geometry",
+    "rtmh_server_request_bytes",
 ]
 
 
@@ -382,6 +383,15 @@ class Server:
         if lib().rtmh_server_submit(self._h, C.c_int(model), C.byref(P), _fp(coef), C.byref(rid)) != 0:
             raise RuntimeError(f"rtmh_server_submit failed: {last_error()}")
         return int(rid.value)
+
+    def request_bytes(self, model: int, P: Params, coef=None):
+        """(bytes of the job's largest shot while resident, bytes of its request block)."""
+        coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
+        a, b = C.c_double(0.0), C.c_double(0.0)
+        if lib().rtmh_server_request_bytes(self._h, C.c_int(model), C.byref(P), _fp(coef), C.byref(a),
+                                           C.byref(b)) != 0:
+            raise RuntimeError(f"rtmh_server_request_bytes failed: {last_error()}")
+        return float(a.value), float(b.value)
 
     def wait(self, rid: int, P: Params, timeout=0.0):
         """Block until request rid is done; returns (image1, image2s, illum, image_lap, image_norm, stats)."""

@@ -17,12 +17,15 @@
 //   q1, q2                : split pressure pair in PML cells (in place; the
 //                           source field's in the forward phase, the
 //                           receiver field's in the reverse phase)
-//   mib, mkk              : the shot's model window as 1/rho and rho*v^2
-//                           (the factors of the velocity and pressure
-//                           updates), edge replicated into the PML and the
-//                           margins
 //   vx(iz,ix) sits at x=(ix+1/2)dx and uses columns 0..nxp-2; vz(iz,ix) at
 //   z=(iz+1/2)dz uses rows 0..nzp-2 (kernel.cu:215-216 array shapes).
+// The model is not copied per slot: the kernels read 1/rho and rho*v^2 (the
+// factors of the velocity and pressure updates) from the model the slot
+// names -- the resident model, shared by every shot of a batch, or a shot's
+// own smoothed window -- at row clamp(iz-mz0, 0, mzn-1) and column
+// mx0 + clamp(ix-mxs, 0, mxn-1): the window cut and edge replication of
+// extractvel1 / extendvel1 (Velocity_function.cpp:2011-2073) as addressing,
+// continued into the field margins.
 // Damping factors are stored with the same margins (index -RTM_MXL.. of the
 // column arrays, -RTM_MZT.. of the row arrays; 1.0 outside the grid) as
 // a = c1/c2 and b = 1/c2 of the reference's c1 = 1 - dt*d/2, c2 = 2 - c1,
@@ -63,16 +66,17 @@ struct ShotDev {
   int it_off, it_dir;          // time index = it_off + it_dir * step
   int record, rcol0, nrec, _pad0;
   float dt, dtx, dtz, _pad1;
-  // model window source (resident model): value(iz,ix) =
+  // model window source (resident model or smoothed window): value(iz,ix) =
   //   m[clamp(iz-mz0,0,mzn-1)*mpitch + mx0 + clamp(ix-mxs,0,mxn-1)]
   const float* rho;
   const float* vv;             // velocity squared (extractvel1, Velocity_function.cpp:2059)
+  const float* mib;            // 1/rho (rounded once)
+  const float* mkk;            // rho*v^2 (rounded once)
   int mpitch, mz0, mzn, mx0, mxs, mxn, _pad2, _pad3;
   // wavefields
   float *sp[2], *svx[2], *svz[2];
   float *rp[2], *rvx[2], *rvz[2];
   float *q1, *q2;
-  float *mib, *mkk;
   float *g1, *illum;
   // per-column / per-row damping factors a = c1/c2, b = 1/c2 at p nodes
   // (x, z) and at the velocity half nodes (xv, zv)

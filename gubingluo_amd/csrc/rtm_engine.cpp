@@ -334,6 +334,24 @@ int Engine::read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int n
   return 1;
 }
 
+int Engine::resume_point(const char* path, uint64_t fp, int nx, int nz, int nstk, int nsh, int& done,
+                         std::vector<double>& st3, std::string& why)
+{
+  done = 0;
+  const int cr = read_checkpoint(path, fp, nx, nz, nstk, done, st3);
+  if (cr < 0) {
+    why = std::string("checkpoint ") + path + " belongs to another job or is damaged";
+    return -5;
+  }
+  if (cr == 1 && done > nsh) {
+    // its stacks hold shots past this job's end
+    why = std::string("checkpoint ") + path + " holds " + std::to_string(done) + " shots, more than this job's " +
+          std::to_string(nsh);
+    return -5;
+  }
+  return cr;
+}
+
 int Engine::write_checkpoint(const char* path, uint64_t fp, int nx, int nz, int done, const std::vector<double>& st3)
 {
   CkptHead h;
@@ -407,8 +425,11 @@ bool GraphKey::operator<(const GraphKey& o) const
   if (fmt != o.fmt) return fmt < o.fmt;
   if (k != o.k) return k < o.k;
   if (ic != o.ic) return ic < o.ic;
+  if (nslots != o.nslots) return nslots < o.nslots;
+  if (guard != o.guard) return guard < o.guard;
   if (shots != o.shots) return shots < o.shots;
-  return tiles < o.tiles;
+  if (tiles != o.tiles) return tiles < o.tiles;
+  return slots < o.slots;
 }
 
 Engine::Engine(int device) : device_(device)
@@ -420,7 +441,10 @@ Engine::Engine(int device) : device_(device)
 Engine::~Engine()
 {
   drop_graphs();
-  for (auto& kv : models_) { (void)hipFree(kv.second.d_rho); (void)hipFree(kv.second.d_vv); }
+  for (auto& kv : models_) {
+    (void)hipFree(kv.second.d_rho); (void)hipFree(kv.second.d_vv);
+    (void)hipFree(kv.second.d_ib); (void)hipFree(kv.second.d_kk);
+  }
   models_.clear();
   if (cb_) (void)hipFree(cb_);
   pool_.trim();
@@ -436,8 +460,8 @@ void Engine::drop_graphs()
 static size_t store_elem(int fmt) { return fmt == ST_F32 ? 4 : 2; }
 
 // fields of a slot: two levels of p, vx, vz for the source and the receiver
-// wavefield, the PML pair, the model window (rho, v^2)
-static constexpr int kSlotFields = 16;
+// wavefield, the PML pair (the model is read from the shared one)
+static constexpr int kSlotFields = 14;
 
 // small per-slot arrays, in this order, each 256-byte aligned
 struct SmallLayout {
@@ -469,15 +493,15 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic)
   size_t b = small_layout(s).total + kSlotFields * F + 2 * I;
   b += align256((size_t)nt * (s.g.nrec > 0 ? s.g.nrec : 1) * 4);
   b += align256((size_t)nt * strip * store_elem(store_fmt));
-  b += align256((size_t)nt * 4 * 4 + 8 * 4);         // scales + 16-bit max-abs accumulators
-  if (!s.wvv.empty()) b += 2 * align256((size_t)s.g.nxp * s.g.nzp * 4);   // smoothed windows
+  b += align256((size_t)nt * 4 * 4 + 16 * 4);        // scales, 16-bit max-abs accumulators, guard word
+  if (!s.wvv.empty()) b += 4 * align256((size_t)s.g.nxp * s.g.nzp * 4);   // smoothed windows: v^2, rho, 1/rho, rho v^2
   if (ic) b += align256((size_t)(5 + 2 * 61) * nzi * s.g.nxl * 4);         // other imaging conditions
   return b;
 }
 
 void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
                          std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                         const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch, bool ic)
+                         const float* wav_dev, const ModelDev& model, bool ic)
 {
   memset(&d, 0, sizeof(d));
   d.dx = P.dx; d.dz = P.dz;
@@ -488,7 +512,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.it_off = 0; d.it_dir = 1;
   d.record = 1; d.rcol0 = g.rcol0; d.nrec = g.nrec;
   d.dt = P.dt; d.dtx = P.dt / P.dx; d.dtz = P.dt / P.dz;   // rtm_model.cu:73-74
-  d.rho = rho_dev; d.vv = vv_dev; d.mpitch = model_pitch;
+  d.rho = model.rho; d.vv = model.vv; d.mib = model.ib; d.mkk = model.kk; d.mpitch = model.pitch;
   d.mz0 = P.npml; d.mzn = P.nz; d.mx0 = g.nx1; d.mxs = P.npml; d.mxn = g.nxl;
   // small arrays
   const SmallLayout L = small_layout(s);
@@ -515,7 +539,7 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   const size_t F = align256((size_t)d.frows * d.pitch * 4);
   float** fp[kSlotFields] = {&d.sp[0], &d.sp[1], &d.svx[0], &d.svx[1], &d.svz[0], &d.svz[1],
                              &d.rp[0], &d.rp[1], &d.rvx[0], &d.rvx[1], &d.rvz[0], &d.rvz[1],
-                             &d.q1, &d.q2, &d.mib, &d.mkk};
+                             &d.q1, &d.q2};
   for (int i = 0; i < kSlotFields; i++) { *fp[i] = (float*)(base + o) + d.forg; o += F; }
   const size_t I = align256((size_t)d.nzi * d.nxl * 4);
   d.g1 = (float*)(base + o); o += I;
@@ -525,15 +549,18 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
   d.store_fmt = store_fmt;
   d.store = base + o;
   o += align256((size_t)P.nt * d.strip_len * store_elem(store_fmt));
-  d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4 + 8 * 4);
+  d.sscale = (float*)(base + o); o += align256((size_t)P.nt * 4 * 4 + 16 * 4);
   if (!s.wvv.empty()) {
-    // the shot's own (smoothed) padded windows as the model the slot reads
+    // the shot's own (smoothed) padded windows as the model the slot reads;
+    // 1/rho and rho v^2 of them by k_model_prep once they are uploaded
     const size_t wb = (size_t)g.nxp * g.nzp * 4;
     float* wv = (float*)(base + o); o += align256(wb);
     float* wr = (float*)(base + o); o += align256(wb);
+    float* wi = (float*)(base + o); o += align256(wb);
+    float* wk = (float*)(base + o); o += align256(wb);
     uploads.push_back({wv, std::vector<char>((const char*)s.wvv.data(), (const char*)s.wvv.data() + wb)});
     uploads.push_back({wr, std::vector<char>((const char*)s.wrho.data(), (const char*)s.wrho.data() + wb)});
-    d.vv = wv; d.rho = wr;
+    d.vv = wv; d.rho = wr; d.mib = wi; d.mkk = wk;
     d.mpitch = g.nxp; d.mz0 = 0; d.mzn = g.nzp; d.mx0 = 0; d.mxs = 0; d.mxn = g.nxp;
   }
   if (ic) { d.ic = (float*)(base + o); o += align256((size_t)(5 + 2 * 61) * d.nzi * d.nxl * 4); }
@@ -554,6 +581,8 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
     if (k.ic) (void)rtm_launch_ic(dshots, dslots, nslots, s, k.ic, st_);
   }
   (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
+  // the chunk's last step, relative to the advanced time origin
+  if (k.guard) (void)rtm_launch_guard(dshots, dslots, nslots, -1, d_flag_, st_);
   if (hipStreamEndCapture(st_, &g) != hipSuccess || !g) return nullptr;
   hipGraphExec_t ex = nullptr;
   if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
@@ -565,7 +594,7 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
 
 int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
                      const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
-                     rtmh_stats* prof)
+                     rtmh_stats* prof, int guard_every)
 {
   const int save = phase == PH_FWD ? 1 : 0;
   const int ic = phase == PH_REV ? ic_cells_ : 0;          // the other imaging conditions after each reverse step
@@ -597,7 +626,8 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     return 0;
   }
   if (chunk > 0 && nsteps >= chunk) {
-    GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, ic, dshots, dtiles};
+    GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, ic, nslots, guard_every > 0 ? 1 : 0,
+                 dshots, dtiles, dslots};
     hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
     if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
     const int nch = nsteps / chunk;
@@ -605,13 +635,39 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     done = nch * chunk;
   }
   LaunchArgs a{dshots, dtiles, ntiles, 0};
-  for (int s = 0; s < nsteps - done; s++) {
+  const int rest = nsteps - done;
+  for (int s = 0; s < rest; s++) {
     a.step = s;
     HIPCHK(rtm_launch_step(a, phase, tz, st_));
     if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
     if (ic) HIPCHK(rtm_launch_ic(dshots, dslots, nslots, s, ic, st_));
+    if (guard_every > 0 && ((s + 1) % guard_every == 0 || s == rest - 1))
+      HIPCHK(rtm_launch_guard(dshots, dslots, nslots, s, d_flag_, st_));
   }
   return 0;
+}
+
+int Engine::guard_clear()
+{
+  static const unsigned int zero = 0u;
+  HIPCHK(hipMemcpyAsync(d_flag_, &zero, 4, hipMemcpyHostToDevice, st_));
+  return 0;
+}
+
+int Engine::guard_check(const std::vector<ShotDev>& ds, const std::vector<int>& shots, int k0, const char* pass)
+{
+  unsigned int f = 0;
+  HIPCHK(hipMemcpy(&f, d_flag_, 4, hipMemcpyDeviceToHost));
+  if (f < 0x7F800000u) return 0;
+  std::string bad;
+  for (size_t j = 0; j < ds.size(); j++) {
+    unsigned int w = 0;
+    HIPCHK(hipMemcpy(&w, rtm_guard_word(ds[j]), 4, hipMemcpyDeviceToHost));
+    if (w >= 0x7F800000u) bad += (bad.empty() ? "" : ", ") + std::to_string(shots[k0 + j]);
+  }
+  set_error(std::string("the ") + pass + " wavefield of shot(s) " + bad +
+            " is not finite (blow-up: dt above the stability bound for the model, or non-finite model values)");
+  return -7;
 }
 
 // tile table of a batch with the per-tile flags (TF_*) of the cell classes
@@ -665,9 +721,37 @@ int params_check(const rtmh_params& P, std::string& why)
   return 0;
 }
 
+float dt_stable(const rtmh_params& P, const float* coef, float vmax)
+{
+  float sum = 0.0f;                                           // sum1abs(diffcoef, N/2)
+  for (int i = 0; i < RTM_R; i++) sum += fabsf(coef[(RTM_R - 1) * RTM_R + i]);
+  const float hmin = P.dx < P.dz ? P.dx : P.dz;
+  return hmin / (sqrtf(2.0f) * sum * vmax);
+}
+
+int stability_check(const rtmh_params& P, const float* coef, float vmax, std::string& why)
+{
+  if (!(vmax > 0.0f) || !std::isfinite(vmax)) { why = "the model's largest velocity is not a positive number"; return -1; }
+  const float dtm = dt_stable(P, coef, vmax);
+  if (P.dt > dtm) {
+    char b[256];
+    snprintf(b, sizeof(b),
+             "dt = %g s exceeds the stability bound min(dx,dz)/(sqrt(2) sum|c| vmax) = %g s for vmax = %g m/s "
+             "(the driver's get_constant refines dt to meet it)", (double)P.dt, (double)dtm, (double)vmax);
+    why = b;
+    return -1;
+  }
+  return 0;
+}
+
 int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, int* id)
 {
   if (nx < 2 || nz < 2 || !vp0 || !rho0) { set_error("bad model"); return -1; }
+  for (size_t i = 0; i < (size_t)nx * nz; i++)
+    if (!(vp0[i] >= 0.0f) || !(rho0[i] > 0.0f) || !std::isfinite(vp0[i]) || !std::isfinite(rho0[i])) {
+      set_error("bad model: velocity must be finite and >= 0, density finite and > 0 (cell " + std::to_string(i) + ")");
+      return -1;
+    }
   HIPCHK(hipSetDevice(device_));
   Model m;
   m.nx = nx; m.nz = nz;
@@ -683,14 +767,17 @@ int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, in
     for (int iz = 0; iz < nz; iz++) mx = vv[(size_t)iz * nx + ix] > mx ? vv[(size_t)iz * nx + ix] : mx;
     m.colmax[ix] = mx;
   }
-  if (hipMalloc(&m.d_rho, nm * 4) != hipSuccess || hipMalloc(&m.d_vv, nm * 4) != hipSuccess) {
-    (void)hipFree(m.d_rho);
+  if (hipMalloc(&m.d_rho, nm * 4) != hipSuccess || hipMalloc(&m.d_vv, nm * 4) != hipSuccess ||
+      hipMalloc(&m.d_ib, nm * 4) != hipSuccess || hipMalloc(&m.d_kk, nm * 4) != hipSuccess) {
+    (void)hipFree(m.d_rho); (void)hipFree(m.d_vv); (void)hipFree(m.d_ib); (void)hipFree(m.d_kk);
     set_error("out of device memory (model)");
     return -3;
   }
   HIPCHK(hipMemcpy(m.d_rho, rho0, nm * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m.d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice));
-  m.bytes = 2 * nm * 4;
+  HIPCHK(rtm_launch_model_prep(m.d_rho, m.d_vv, m.d_ib, m.d_kk, (long long)nm, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  m.bytes = 4 * nm * 4;
   std::lock_guard<std::mutex> lk(models_mu_);
   const int k = next_model_++;
   models_[k] = std::move(m);
@@ -705,6 +792,8 @@ int Engine::model_destroy(int id)
   if (it == models_.end()) { set_error("no such model"); return -1; }
   (void)hipFree(it->second.d_rho);
   (void)hipFree(it->second.d_vv);
+  (void)hipFree(it->second.d_ib);
+  (void)hipFree(it->second.d_kk);
   models_.erase(it);
   return 0;
 }
@@ -741,12 +830,13 @@ int Engine::ensure_cb(int nslots, size_t ntiles)
   if (cb_slots_ < 64) cb_slots_ = 64;
   cb_tiles_ = ntiles > cb_tiles_ ? ntiles * 2 : cb_tiles_;
   const size_t bytes = align256(cb_slots_ * sizeof(ShotDev)) + align256(cb_tiles_ * sizeof(Tile)) +
-                       align256(cb_slots_ * 4) + align256(cb_slots_ * sizeof(StackShotHost));
+                       align256(cb_slots_ * 4) + align256(cb_slots_ * sizeof(StackShotHost)) + 256;
   HIPCHK(hipMalloc(&cb_, bytes));
   d_ds_ = (ShotDev*)cb_;
   d_tiles_ = (Tile*)((char*)cb_ + align256(cb_slots_ * sizeof(ShotDev)));
   d_slots_ = (int*)((char*)d_tiles_ + align256(cb_tiles_ * sizeof(Tile)));
   d_stk_ = (StackShotHost*)((char*)d_slots_ + align256(cb_slots_ * 4));
+  d_flag_ = (unsigned int*)((char*)d_stk_ + align256(cb_slots_ * sizeof(StackShotHost)));
   return 0;
 }
 
@@ -765,6 +855,13 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   const Model* M = model(model_id);
   if (!M) { set_error("no such model"); return -1; }
   if (M->nx != P.nx || M->nz != P.nz) { set_error("model size differs from nx, nz"); return -1; }
+  {
+    float mv = 0.0f;
+    for (float c : M->colmax) mv = c > mv ? c : mv;
+    if (stability_check(P, coef, sqrtf(mv), why) != 0) { set_error(why); return -1; }
+  }
+  // wavefield guard interval: the reference's ntsnap (rtm_model.cu:21)
+  const int guard_every = std::max(1, (int)(P.tsnap / P.dt));
   HIPCHK(hipSetDevice(device_));
   const float* vp0 = M->vp0.data();
 
@@ -805,8 +902,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
                               (icx ? 7 * align256(nm * 8) + 7 * align256(nm * 4) + align256(nsh * sizeof(StackIcHost)) : 0);
   char* common = (char*)pool_.get(common_bytes);
   if (!common) { set_error("out of device memory (job)"); return -3; }
-  const float* d_rho = M->d_rho;
-  const float* d_vv = M->d_vv;
+  const ModelDev mdev = M->dev();
   float* d_coef = (float*)common;
   float* d_wav = (float*)((char*)d_coef + align256(100 * 4));
   double* d_i1 = (double*)((char*)d_wav + align256((size_t)nt * 4));
@@ -842,14 +938,14 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   if (opt.checkpoint && opt.resume) {
     std::vector<double> st3;
     int done = 0;
-    const int cr = read_checkpoint(opt.checkpoint, job_fp, nx, nz, nstk, done, st3);
+    const int cr = resume_point(opt.checkpoint, job_fp, nx, nz, nstk, nsh, done, st3, why);
     if (cr < 0) {
-      set_error(std::string("checkpoint ") + opt.checkpoint + " belongs to another job or is damaged");
+      set_error(why);
       pool_.put(common);
-      return -5;
+      return cr;
     }
     if (cr == 1) {
-      k_start = std::min(done, nsh);
+      k_start = done;
       for (int i = 0; i < nstk; i++)
         HIPCHK(hipMemcpy(stacks[i], st3.data() + (size_t)i * nm, nm * 8, hipMemcpyHostToDevice));
     }
@@ -889,7 +985,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     for (int j = 0; j < nb; j++) {
       mem[j] = pool_.get(slot_bytes(setup[k0 + j], nt, opt.store_fmt, icx));
       if (!mem[j]) { set_error("out of device memory (shot slots)"); rc = -3; break; }
-      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, d_rho, d_vv, nx, icx);
+      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, mdev, icx);
       ds[j].record = (P.mode == 0) ? 1 : 0;
     }
     if (rc != 0) { for (void* p : mem) pool_.put(p); break; }
@@ -964,11 +1060,17 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     HIPCHK(hipMemcpyAsync(d_ds, ds.data(), nb * sizeof(ShotDev), hipMemcpyHostToDevice, st_));
     HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     HIPCHK(hipMemcpyAsync(d_slots, slots.data(), nb * 4, hipMemcpyHostToDevice, st_));
-    for (int j = 0; j < nb; j++) HIPCHK(rtm_launch_phase_init(d_ds, j, PH_FWD, 512, st_));
+    for (int j = 0; j < nb; j++) {
+      HIPCHK(rtm_launch_phase_init(d_ds, j, PH_FWD, 512, st_));
+      if (smooth)                                    // the factors of the shot's smoothed windows
+        HIPCHK(rtm_launch_model_prep(ds[j].rho, ds[j].vv, (float*)ds[j].mib, (float*)ds[j].mkk,
+                                     (long long)setup[k0 + j].g.nxp * setup[k0 + j].g.nzp, st_));
+    }
     if (opt.verbose)
       for (int j = 0; j < nb; j++) printf("N0. %d shot source wavefield calculating......\n", shots[k0 + j]);
 
     // ---- forward pass: propagation + record + boundary strips
+    if (guard_clear() != 0) { for (void* p : mem) pool_.put(p); rc = -10; break; }
     HIPCHK(hipEventRecord(e0, st_));
     int fwd_done = 0;
     if (tune) {
@@ -979,10 +1081,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     }
     rc = run_pass(PH_FWD, nt - fwd_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt,
-                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr);
+                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every);
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e1, st_));
     HIPCHK(hipStreamSynchronize(st_));
+    if ((rc = guard_check(ds, shots, k0, "forward")) != 0) { for (void* p : mem) pool_.put(p); break; }
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     fwd_ms += ms;
@@ -1049,7 +1152,8 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
         const size_t wb = s.wvv.size() * 4;
         HIPCHK(hipMemcpyAsync((void*)ds[j].vv, s.wvv.data(), wb, hipMemcpyHostToDevice, st_));
         HIPCHK(hipMemcpyAsync((void*)ds[j].rho, s.wrho.data(), wb, hipMemcpyHostToDevice, st_));
-        HIPCHK(rtm_launch_model_window(d_ds, j, 512, st_));
+        HIPCHK(rtm_launch_model_prep(ds[j].rho, ds[j].vv, (float*)ds[j].mib, (float*)ds[j].mkk,
+                                     (long long)s.g.nxp * s.g.nzp, st_));
         if (opt.shot_field) {
           // a recorded gather is muted after the smoothing (rtm_real.cu:370-403),
           // so with the smoothed velocities
@@ -1067,6 +1171,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       HIPCHK(rtm_launch_seed(d_ds, j, ds[j].nrec, st_));
     }
     if (opt.verbose) printf("source wavefiled prepared over...\nBegin backward......\n");
+    if (guard_clear() != 0) { for (void* p : mem) pool_.put(p); rc = -10; break; }
     HIPCHK(hipEventRecord(e1, st_));
     int rev_done = 0;
     if (tune_r) {
@@ -1079,12 +1184,13 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     }
     rc = run_pass(PH_REV, nt - 1 - rev_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz_r, opt.store_fmt,
-                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr);
+                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every);
     last_tz = tz;
     last_tz_r = tz_r;
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e2, st_));
     HIPCHK(hipStreamSynchronize(st_));
+    if ((rc = guard_check(ds, shots, k0, "reverse")) != 0) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventElapsedTime(&ms, e1, e2));
     rev_ms += ms;
 
@@ -1266,7 +1372,7 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   if (!with_source) { s.sx0 = 0; s.sx1 = -1; s.sz0 = 0; s.sz1 = -1; }
   std::vector<float> wav;
   ricker(wav, P.nt, P.fdom, P.dt);
-  const size_t cb = 2 * align256(nm * 4) + align256(400) + align256((size_t)P.nt * 4);
+  const size_t cb = 4 * align256(nm * 4) + align256(400) + align256((size_t)P.nt * 4);
   char* common = (char*)pool_.get(cb);
   const size_t sb = slot_bytes(s, P.nt, ST_F32);
   char* slot = (char*)pool_.get(sb);
@@ -1276,9 +1382,11 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   std::vector<std::pair<void*, std::vector<char>>> up;
   float* d_rho = (float*)common;
   float* d_vv = (float*)(common + align256(nm * 4));
-  float* d_coef = (float*)(common + 2 * align256(nm * 4));
+  float* d_ib = (float*)(common + 2 * align256(nm * 4));
+  float* d_kk = (float*)(common + 3 * align256(nm * 4));
+  float* d_coef = (float*)(common + 4 * align256(nm * 4));
   float* d_wav = (float*)((char*)d_coef + align256(400));
-  layout_slot(P, s, ST_F32, slot, d, up, d_coef, d_wav, d_rho, d_vv, nx);
+  layout_slot(P, s, ST_F32, slot, d, up, d_coef, d_wav, ModelDev{d_rho, d_vv, d_ib, d_kk, nx});
   d.record = 0; d.phase = PH_FWD; d.it_off = it0; d.it_dir = 1;
   std::vector<ShotDev> dv(1, d);
   build_tiles(dv, tz, tiles);
@@ -1287,6 +1395,8 @@ int Engine::fd_steps(const rtmh_params& P, int ishot, const float* vp0, const fl
   Tile* d_tiles = (Tile*)(ctl + align256(sizeof(ShotDev)));
   HIPCHK(hipMemcpy(d_rho, rho0, nm * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d_vv, vv.data(), nm * 4, hipMemcpyHostToDevice));
+  HIPCHK(rtm_launch_model_prep(d_rho, d_vv, d_ib, d_kk, (long long)nm, st_));
+  HIPCHK(hipStreamSynchronize(st_));
   HIPCHK(hipMemcpy(d_coef, coef, 400, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(d_wav, wav.data(), (size_t)P.nt * 4, hipMemcpyHostToDevice));
   for (auto& u : up) HIPCHK(hipMemcpy(u.first, u.second.data(), u.second.size(), hipMemcpyHostToDevice));

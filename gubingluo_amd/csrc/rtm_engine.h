@@ -64,16 +64,31 @@ class DevPool {
 };
 
 // ---- engine -----------------------------------------------------------------
+// a model on the device as the slots read it: density, v^2, and the step
+// kernels' factors 1/rho and rho v^2, rows of `pitch` floats
+struct ModelDev {
+  const float *rho, *vv, *ib, *kk;
+  int pitch;
+};
+
 struct SlotMem {
   void* base = nullptr;
   size_t bytes = 0;
 };
 
+// Everything a captured chunk bakes in by value: the launch shapes (tiles,
+// tile height, slots: k_save's and k_ic's grids and k_bump's count), the
+// kernels chosen (phase, store format, imaging conditions, guard) and the
+// control-block addresses.  Descriptors and tiles are read from the control
+// block at replay, so batches that agree on all of these share a graph.
 struct GraphKey {
   int ntiles, tz, phase, save, fmt, k;     // save: strip save after every (forward) step, fmt its store format
   int ic;                                  // cells of the other imaging conditions after every reverse step (0: off)
+  int nslots;                              // slots of the batch (k_save / k_ic grids, k_bump count)
+  int guard;                               // wavefield guard after the chunk
   const void* shots;
   const void* tiles;
+  const void* slots;
   bool operator<(const GraphKey& o) const;
 };
 
@@ -89,6 +104,9 @@ class Engine {
     int nx = 0, nz = 0;
     float* d_rho = nullptr;
     float* d_vv = nullptr;
+    float* d_ib = nullptr;      // 1/rho
+    float* d_kk = nullptr;      // rho v^2
+    ModelDev dev() const { return ModelDev{d_rho, d_vv, d_ib, d_kk, nx}; }
     std::vector<float> vp0;     // host copy: mute velocities
     std::vector<float> rho0;    // host copy: smoothed windows
     std::vector<float> colmax;  // host: column maxima of v^2 (per-shot vmax)
@@ -102,6 +120,11 @@ class Engine {
                                   int nsh);
   static int read_checkpoint(const char* path, uint64_t fp, int nx, int nz, int nstk, int& done, std::vector<double>& st3);
   static int write_checkpoint(const char* path, uint64_t fp, int nx, int nz, int done, const std::vector<double>& st3);
+  // where a job of nsh shots resumes from its checkpoint: 1 with the shots
+  // done and the stacks, 0 without a file (start at shot 0), -5 (why set) for
+  // another job's or a damaged file, or one with more shots done than nsh
+  static int resume_point(const char* path, uint64_t fp, int nx, int nz, int nstk, int nsh, int& done,
+                          std::vector<double>& st3, std::string& why);
 
   int run(const rtmh_params& P, const float* vp0, const float* rho0, const float* coef, const rtmh_options& opt,
           rtmh_outputs* out, rtmh_stats* stats);
@@ -117,10 +140,10 @@ class Engine {
   static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic = false);
   // build the device descriptor of one slot inside `base` (uploads: host
   // images of its small arrays to copy to the device)
+  // (the slot reads `model`, or its own smoothed windows when s carries them)
   static void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
                           std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                          const float* wav_dev, const float* rho_dev, const float* vv_dev, int model_pitch,
-                          bool ic = false);
+                          const float* wav_dev, const ModelDev& model, bool ic = false);
   // offset of the taper row inside a slot block
   static size_t taper_offset(const ShotSetup& s);
   std::mutex& models_mutex() { return models_mu_; }
@@ -148,12 +171,19 @@ class Engine {
   Tile* d_tiles_ = nullptr;
   int* d_slots_ = nullptr;
   struct StackShotHost* d_stk_ = nullptr;
+  unsigned int* d_flag_ = nullptr;      // wavefield guard of the running pass (largest |p| bits)
   int ic_cells_ = 0;                    // the other imaging conditions of the running batch: interior cells (0: off)
   int ensure_cb(int nslots, size_t ntiles);
 
+  // guard_every > 0: the wavefield guard after every guard_every steps (plain
+  // launches) or after every replayed chunk, and after the last step
   int run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
                const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
-               rtmh_stats* prof);
+               rtmh_stats* prof, int guard_every = 0);
+  // clears the guard flag before a pass; after it, checks the flag and names
+  // the shots whose wavefield is not finite (-7)
+  int guard_clear();
+  int guard_check(const std::vector<ShotDev>& ds, const std::vector<int>& shots, int k0, const char* pass);
   hipGraphExec_t graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
                            int nslots, int store_fmt);
   void drop_graphs();
@@ -163,6 +193,12 @@ class Engine {
 // (tiles carry the index of the descriptor in `ds` plus `slot_base`)
 void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles, const std::vector<int>* slot_ids = nullptr);
 int params_check(const rtmh_params& P, std::string& why);
+// the stability bound of get_constant (RTM_GPU_velstr.cpp:355-361):
+// min(dx, dz) / (sqrt(2) sum|c| vmax), sum over the order-20 coefficient row
+// (coef[90..99]); vmax in m/s
+float dt_stable(const rtmh_params& P, const float* coef, float vmax);
+// 0 when dt is within the bound, -1 with the reason otherwise
+int stability_check(const rtmh_params& P, const float* coef, float vmax, std::string& why);
 
 // thread-local error text for rtmh_last_error()
 void set_error(const std::string& s);

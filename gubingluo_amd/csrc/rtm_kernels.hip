@@ -9,8 +9,10 @@
 // 389-429 reverse) with source (kernel.cu:36-47), recording (48-71),
 // receiver insertion (72-95), boundary strips (1186-1400) and imaging
 // (1401-1416).  The halo velocities are computed again by the neighbouring
-// tiles instead of being read back, so a step reads p, vx, vz and the model
-// once and writes p, vx, vz once per cell.  Fields have two time levels
+// tiles instead of being read back, so a step reads p, vx, vz once and
+// writes them once per cell.  The model (1/rho and rho v^2) is read from one
+// array shared by every shot of a launch (the resident model; clamped
+// indices give each shot its padded window), which stays in the caches.  Fields have two time levels
 // (step it reads level it&1 and writes level (it+1)&1), so a tile's reads of
 // its neighbours' cells never race with their writes.
 //
@@ -233,6 +235,34 @@ __device__ __forceinline__ rsrc_t image_rsrc(float* a, const ShotDev& s)
 {
   return __builtin_amdgcn_make_buffer_rsrc((void*)a, (short)0, s.nzi * s.nxl * 4, 0x00020000);
 }
+// the model a slot reads (mzn rows of mpitch floats)
+__device__ __forceinline__ rsrc_t model_rsrc(const float* m, const ShotDev& s)
+{
+  return __builtin_amdgcn_make_buffer_rsrc((void*)m, (short)0, s.mzn * s.mpitch * 4, 0x00020000);
+}
+// byte offsets into the model of padded row iz (wave-uniform) and column ix:
+// the window cut with edge replication (extractvel1 + extendvel1,
+// Velocity_function.cpp:2011-2073) as a clamp
+__device__ __forceinline__ int model_row(const ShotDev& s, int iz)
+{
+  return imin(imax(iz - s.mz0, 0), s.mzn - 1) * s.mpitch * 4;
+}
+__device__ __forceinline__ int model_col(const ShotDev& s, int ix)
+{
+  return (s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1)) * 4;
+}
+// A load offset past any buffer's end: a buffer load whose lane offset is
+// out of range returns 0 for that lane without fetching memory (the same
+// holds for the loads that write LDS).  Field cells outside the grid's
+// columns -- the margins, and the lanes of a window's last tile column past
+// nxp-1 -- are never written and hold 0, so their loads take this offset:
+// same values, and the part of the last tile column's lines past the window
+// is not fetched.
+#define RTM_OOB ((int)0x80000000)
+__device__ __forceinline__ int col_load(int voff, int col, int nxp)
+{
+  return (col >= 0 && col < nxp) ? voff : (voff | RTM_OOB);
+}
 __device__ __forceinline__ float bld(rsrc_t r, int voff, int soff)
 {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
@@ -411,9 +441,11 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   const int pr4 = s.pitch * 4;
   const int srow0 = (z0 + RTM_MZT) * pr4;             // byte offset of tile row 0
   const int vcol = (ix + RTM_MXL) * 4;                // byte offset of the lane's column
+  const int vld = col_load(vcol, ix, nxp);             // ... for loads
   const int icol = (ix - npml) * 4;                    // ... in the imaging accumulators
   const rsrc_t Rp = field_rsrc(pc, s), Rvx = field_rsrc(vxc, s), Rvz = field_rsrc(vzc, s);
-  const rsrc_t Rib = field_rsrc(s.mib, s);
+  const rsrc_t Rib = model_rsrc(s.mib, s);
+  const int mcol = model_col(s, ix);                   // the lane's model column
   float c10[RTM_R];
 #pragma unroll
   for (int i = 0; i < RTM_R; i++) c10[i] = GCF(s.coef)[90 + i];
@@ -427,7 +459,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     // and x0+20..x0+83; the middle 24 columns arrive twice with the same
     // values), so that no load is issued by part of a wave; halo rows:
     // columns x0..x0+63
-    const int ca = (x0 - 20 + lx + RTM_MXL) * 4, cb = (x0 + 20 + lx + RTM_MXL) * 4;
+    const int ca = col_load((x0 - 20 + lx + RTM_MXL) * 4, x0 - 20 + lx, nxp);
+    const int cb = col_load((x0 + 20 + lx + RTM_MXL) * 4, x0 + 20 + lx, nxp);
 #pragma unroll
     for (int m = 0; m < KP; m++) {
       const int k = g + G * m;
@@ -437,7 +470,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           bld_lds(Rp, PS + k * PW, ca, so);
           bld_lds(Rp, PS + k * PW + 40, cb, so);
         } else {
-          bld_lds(Rp, PS + k * PW + 20, vcol, so);
+          bld_lds(Rp, PS + k * PW + 20, vld, so);
         }
       }
     }
@@ -450,8 +483,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     ibz[k] = 1.0f;
     if (kz < ZR) {
       const int so = srow0 + (kz - 10) * pr4;
-      vzo[k] = bld(Rvz, vcol, so);
-      ibz[k] = bld(Rib, vcol, so);
+      vzo[k] = bld(Rvz, vld, so);
+      ibz[k] = bld(Rib, mcol, model_row(s, z0 - 10 + kz));
     }
   }
   // reconstruction of a strip tile: the stored value of a strip cell
@@ -503,8 +536,8 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
     const int so = srow0 + (g * RPT + r) * pr4;
-    vxo[r] = bld(Rvx, vcol, so);
-    ibx[r] = bld(Rib, vcol, so);
+    vxo[r] = bld(Rvx, vld, so);
+    ibx[r] = bld(Rib, mcol, model_row(s, z0 + g * RPT + r));
   }
   float hvo[KH], hib[KH];
   int hrow[KH], hcol[KH];
@@ -517,9 +550,10 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
     const int hc = h - hrow[k] * 20;
     hcol[k] = hc < 10 ? hc : hc + 64;                  // VXL column: x = x0 - 10 + hcol
     if (halo_on(k)) {
-      const int off = srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4;
+      const int off = col_load(srow0 + hrow[k] * pr4 + (x0 - 10 + hcol[k] + RTM_MXL) * 4, x0 - 10 + hcol[k], nxp);
       hvo[k] = bld(Rvx, off, 0);
-      hib[k] = bld(Rib, off, 0);
+      hib[k] = bld(Rib, model_col(s, x0 - 10 + hcol[k]) +
+                            imin(imax(z0 + hrow[k] - s.mz0, 0), s.mzn - 1) * s.mpitch * 4, 0);
     }
   }
   if (rstrips) {
@@ -567,13 +601,13 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
   const bool xring = ix == 0 || ix == nxp - 1;
   float kk[RPT], qa[RPT], qb[RPT];
   {
-    const rsrc_t Rkk = field_rsrc(s.mkk, s);
+    const rsrc_t Rkk = model_rsrc(s.mkk, s);
     const rsrc_t R1 = field_rsrc(s.q1, s), R2 = field_rsrc(s.q2, s);
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       const int j = g * RPT + r, iz = z0 + j;
       const int so = srow0 + j * pr4;
-      kk[r] = bld(Rkk, vcol, so);
+      kk[r] = bld(Rkk, mcol, model_row(s, iz));
       qa[r] = qb[r] = 0.0f;
       const bool incell = ix < nxp && iz < nzp;
       const bool inner = xin && iz >= npml && iz < nzp - npml;
@@ -898,7 +932,8 @@ __device__ __forceinline__ int save_it(const ShotDev& s, int step)
   return step < 0 ? -step - 1 : s.it_off + s.it_dir * step;   // step < 0: absolute -step-1
 }
 // 16-bit forms: per slot two max-abs accumulators (step parity) after the
-// nt x 4 scales, cleared by the phase start and by the previous step's save
+// nt x 4 scales, cleared by the phase start and by the previous step's save;
+// word 8 is the wavefield guard (k_guard)
 __device__ __forceinline__ unsigned int* save_acc(const ShotDev& s) { return (unsigned int*)(s.sscale + (size_t)s.nt * 4); }
 
 // fp32: copy block b of the slot's step into the store
@@ -993,36 +1028,22 @@ __global__ __launch_bounds__(NTHR) void k_save_q(const ShotDev* __restrict__ sho
   }
 }
 
-// Phase start of one slot.  FWD: zero both levels of both wavefields and the
-// PML pair, and copy the shot's model window into the slot (edge replication
-// into the PML, extendvel1 Velocity_function.cpp:2011-2030, and on into the
-// margins).  REV: zero both levels of the source velocities (rtm_model.cu:
-// 366-372), the receiver set, the PML pair and the imaging accumulators.
-// the slot's model window (1/rho and rho v^2 in the field layout, margins
-// included) from its model source: the resident model, or the shot's own
-// (smoothed) padded window
-__device__ void model_window(const ShotDev& s, long long g0, long long gs, long long n)
+// 1/rho and rho v^2 of a model, rounded once (the factors the step kernels
+// read): the resident model at its creation, a shot's smoothed window
+// (pmlvelsmooth1d / velsmooth1d) whenever it changes
+__global__ __launch_bounds__(NTHR) void k_model_prep(const float* __restrict__ rho, const float* __restrict__ vv,
+                                                      float* __restrict__ ib, float* __restrict__ kk, long long n)
 {
-  for (long long e = g0; e < n; e += gs) {
-    const int row = (int)(e / s.pitch), col = (int)(e - (long long)row * s.pitch);
-    const int iz = row - RTM_MZT, ix = col - RTM_MXL;
-    const int r = imin(imax(iz - s.mz0, 0), s.mzn - 1);
-    const int c = s.mx0 + imin(imax(ix - s.mxs, 0), s.mxn - 1);
-    const size_t m = (size_t)r * s.mpitch + c;
-    const float rh = s.rho[m];
-    (s.mib - s.forg)[e] = 1.0f / rh;
-    (s.mkk - s.forg)[e] = rh * s.vv[m];
+  for (long long e = (long long)blockIdx.x * NTHR + threadIdx.x; e < n; e += (long long)gridDim.x * NTHR) {
+    const float rh = rho[e];
+    ib[e] = 1.0f / rh;
+    kk[e] = rh * vv[e];
   }
 }
 
-// model window rebuilt before the reverse pass (after velsmooth1d)
-__global__ __launch_bounds__(NTHR) void k_model_window(const ShotDev* __restrict__ shots, int slot)
-{
-  const ShotDev s = shots[slot];
-  const long long n = (long long)s.frows * s.pitch;
-  model_window(s, (long long)blockIdx.x * NTHR + threadIdx.x, (long long)gridDim.x * NTHR, n);
-}
-
+// Phase start of one slot.  FWD: zero both levels of both wavefields and the
+// PML pair.  REV: zero both levels of the source velocities (rtm_model.cu:
+// 366-372), the receiver set, the PML pair and the imaging accumulators.
 __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__ shots, int slot, int phase)
 {
   const ShotDev s = shots[slot];
@@ -1038,7 +1059,7 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
       if (k >= first) (f[k] - s.forg)[e] = 0.0f;
   if (phase == PH_FWD) {
     if (s.store_fmt != ST_F32 && g0 < 8) save_acc(s)[g0] = 0u;   // 16-bit max-abs accumulators
-    model_window(s, g0, gs, n);
+    if (g0 == 8) save_acc(s)[8] = 0u;                              // wavefield guard word
   } else {
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
     if (s.ic)
@@ -1226,6 +1247,44 @@ __global__ __launch_bounds__(NTHR) void k_zero_f64(double* __restrict__ a, long 
   if (e < n) a[e] = 0.0;
 }
 
+// Wavefield guard: the device form of the reference's max|p| print every
+// ntsnap steps (rtm_model.cu:271-273, 324, 415).  The largest |p| of each
+// listed slot's pressure as written by step `step` (it = it_off + it_dir *
+// step wrote level (it+1)&1; the source field, and in the reverse phase also
+// the receiver field) goes into the slot's guard word and into *flag as float
+// bits: non-negative floats order as their bits, and an infinity (0x7F800000)
+// or a NaN (0x7FC00000 and up) compares above every finite value, so the host
+// detects a blow-up (unstable dt, bad model) from one word.  Grid: (slots, GW).
+__global__ __launch_bounds__(NTHR) void k_guard(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                 int step, unsigned int* __restrict__ flag)
+{
+  __shared__ unsigned int red[NTHR / 64];
+  const ShotDev& s = shot_of(shots, slots[blockIdx.x]);
+  const int it = s.it_off + s.it_dir * step;
+  const bool lev = (it + 1) & 1;
+  const float* ps = lev ? s.sp[1] : s.sp[0];
+  const float* pr = lev ? s.rp[1] : s.rp[0];
+  const bool rev = s.phase == PH_REV;
+  const long long n = (long long)s.nzp * s.nxp;
+  unsigned int m = 0u;
+  for (long long e = (long long)blockIdx.y * NTHR + threadIdx.x; e < n; e += (long long)gridDim.y * NTHR) {
+    const int iz = (int)(e / s.nxp), ix = (int)(e - (long long)iz * s.nxp);
+    const long long o = (long long)iz * s.pitch + ix;
+    unsigned int a = __float_as_uint(GCF(ps)[o]) & 0x7FFFFFFFu;
+    if (rev) a = max(a, __float_as_uint(GCF(pr)[o]) & 0x7FFFFFFFu);
+    m = max(m, a);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NTHR / 64; w++) m = max(m, red[w]);
+    atomicMax(save_acc(s) + 8, m);
+    if (flag) atomicMax(flag, m);
+  }
+}
+
 // advance the time origin of the slots of a captured chunk of `k` steps so
 // that replaying the same graph continues where the last replay stopped
 __global__ void k_bump(ShotDev* __restrict__ shots, const int* __restrict__ slots, int nslots, int k)
@@ -1286,9 +1345,10 @@ hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int 
   return hipGetLastError();
 }
 
-hipError_t rtm_launch_model_window(const ShotDev* shots, int slot, int nblocks, hipStream_t st)
+hipError_t rtm_launch_model_prep(const float* rho, const float* vv, float* ib, float* kk, long long n, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_model_window, dim3(nblocks), dim3(NTHR), 0, st, shots, slot);
+  const long long nb = (n + NTHR - 1) / NTHR;
+  hipLaunchKernelGGL(k_model_prep, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(NTHR), 0, st, rho, vv, ib, kk, n);
   return hipGetLastError();
 }
 
@@ -1344,6 +1404,14 @@ hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, l
 hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st)
 {
   hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, a, n);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_guard(const ShotDev* shots, const int* slots, int nslots, int step, unsigned int* flag,
+                            hipStream_t st)
+{
+  if (nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_guard, dim3(nslots, 16), dim3(NTHR), 0, st, shots, slots, step, flag);
   return hipGetLastError();
 }
 

@@ -13,8 +13,8 @@ hipError_t rtm_launch_step(const LaunchArgs& a, int phase, int tz, hipStream_t s
 hipError_t rtm_launch_save(const ShotDev* shots, const int* slots, int nslots, int step, int store_fmt,
                            hipStream_t st);
 hipError_t rtm_launch_phase_init(const ShotDev* shots, int slot, int phase, int nblocks, hipStream_t st);
-// the slot's 1/rho and rho v^2 rebuilt from its model source (after smoothing)
-hipError_t rtm_launch_model_window(const ShotDev* shots, int slot, int nblocks, hipStream_t st);
+// 1/rho and rho v^2 of n model cells (ib = 1/rho, kk = rho v^2, each rounded once)
+hipError_t rtm_launch_model_prep(const float* rho, const float* vv, float* ib, float* kk, long long n, hipStream_t st);
 hipError_t rtm_launch_seed(const ShotDev* shots, int slot, int nrec, hipStream_t st);
 hipError_t rtm_launch_stack(const void* stack_shots, int nsh, int nx, int nz, double* i1, double* i2, double* il,
                             hipStream_t st);
@@ -28,6 +28,12 @@ hipError_t rtm_launch_laplace(const float* im, float* out, int nx, int nz, int d
 hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, long long n, hipStream_t st);
 hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st);
 hipError_t rtm_launch_bump(ShotDev* shots, const int* slots, int nslots, int k, hipStream_t st);
+// wavefield guard of the listed slots after step `step` (largest |p| as float
+// bits into each slot's guard word and into *flag; flag may be null)
+hipError_t rtm_launch_guard(const ShotDev* shots, const int* slots, int nslots, int step, unsigned int* flag,
+                            hipStream_t st);
+// a slot's guard word (device address)
+inline unsigned int* rtm_guard_word(const ShotDev& d) { return (unsigned int*)(d.sscale + (size_t)d.nt * 4) + 8; }
 
 // per-shot entry of the stacking kernel (k_stack)
 struct StackShotHost { const float* g1; const float* illum; const float* taper; int nx1, nxl; };

@@ -26,12 +26,14 @@
 #include <vector>
 
 #include "../../include/rtm_hip.h"
+#include "admission.h"
 #include "rtm_engine.h"
 #include "rtm_launch.h"
 
 namespace rtm {
 
 static inline size_t a256(size_t n) { return (n + 255) & ~(size_t)255; }
+static constexpr long long kGuardTicks = 64;   // served form: wavefield guard interval (a power of two)
 
 struct SrvRequest {
   int id = 0, model_id = 0;
@@ -54,7 +56,10 @@ struct SrvRequest {
   double t_submit = 0, t_start = -1, t_done = -1, cells = 0;
   hipEvent_t done_ev = nullptr;
   bool finalizing = false, complete = false;
-  int status = 0;
+  int status = 0;                          // 0, or the error of a failed request
+  std::string err;
+  int resident = 0;                        // shots of this request in slots
+  std::vector<unsigned int> guard;         // per shot: largest |p| bits (k_guard)
 };
 
 struct SrvSlot {
@@ -71,6 +76,7 @@ class Server {
   ~Server();
   int start();
   int submit(int model_id, const rtmh_params& P, const float* coef, int* id);
+  int request_bytes(int model_id, const rtmh_params& P, const float* coef, double* shot_max, double* request);
   int wait(int id, double timeout, rtmh_outputs* out, rtmh_request_stats* st);
   void query(rtmh_server_stats* st);
 
@@ -80,8 +86,12 @@ class Server {
   {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
   }
-  bool admit_one(SrvRequest& r);           // under mu_
+  Admit admit_one(SrvRequest& r, std::string& why);   // under mu_
   int request_device(SrvRequest& r);       // under mu_
+  void fail_request(SrvRequest& r, int status, const std::string& why);   // under mu_
+  void complete_request(SrvRequest& r);    // under mu_
+  static size_t request_bytes(const SrvRequest& r);
+  size_t shot_need(const SrvRequest& r, int k) const;
   void transition(int sid);
   void finish_shot(int sid);
   void stack_ready(SrvRequest& r);
@@ -103,6 +113,10 @@ class Server {
   ShotDev* d_ds_ = nullptr;
   Tile *d_tf_ = nullptr, *d_tr_ = nullptr;
   int *d_qs_ = nullptr;
+  int *d_as_ = nullptr;                    // every resident slot (periodic guard)
+  int *d_id_ = nullptr;                    // 0..max_slots-1 (single-slot launches)
+  int na_ = 0;
+  std::vector<int> aslots_;
   size_t tile_cap_ = 0;
   int ntf_ = 0, ntr_ = 0, nq_ = 0;
   std::vector<Tile> tf_, tr_;
@@ -134,9 +148,19 @@ int Server::start()
   slots_.assign(opt_.max_slots, SrvSlot());
   hds_.assign(opt_.max_slots, ShotDev());
   if (hipMalloc(&d_ds_, sizeof(ShotDev) * opt_.max_slots) != hipSuccess ||
-      hipMalloc(&d_qs_, sizeof(int) * opt_.max_slots) != hipSuccess) {
+      hipMalloc(&d_qs_, sizeof(int) * opt_.max_slots) != hipSuccess ||
+      hipMalloc(&d_as_, sizeof(int) * opt_.max_slots) != hipSuccess ||
+      hipMalloc(&d_id_, sizeof(int) * opt_.max_slots) != hipSuccess) {
     set_error("out of device memory (server control)");
     return -3;
+  }
+  {
+    std::vector<int> id(opt_.max_slots);
+    for (int i = 0; i < opt_.max_slots; i++) id[i] = i;
+    if (hipMemcpy(d_id_, id.data(), id.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("server control upload");
+      return -10;
+    }
   }
   th_ = std::thread([this] { loop(); });
   return 0;
@@ -157,7 +181,8 @@ Server::~Server()
   }
   for (auto& s : slots_)
     if (s.mem) pool_.put(s.mem);
-  (void)hipFree(d_ds_); (void)hipFree(d_qs_); (void)hipFree(d_tf_); (void)hipFree(d_tr_);
+  (void)hipFree(d_ds_); (void)hipFree(d_qs_); (void)hipFree(d_as_); (void)hipFree(d_id_);
+  (void)hipFree(d_tf_); (void)hipFree(d_tr_);
   pool_.trim();
   if (st_) (void)hipStreamDestroy(st_);
 }
@@ -166,6 +191,31 @@ void Server::fail(const char* what, hipError_t e)
 {
   err_ = -10;
   err_msg_ = std::string(what) + ": " + hipGetErrorString(e);
+}
+
+int Server::request_bytes(int model_id, const rtmh_params& P, const float* coef, double* shot_max, double* request)
+{
+  std::string why;
+  if (params_check(P, why) != 0) { set_error(why); return -1; }
+  const Engine::Model* M = eng_->model(model_id);
+  if (!M || M->nx != P.nx || M->nz != P.nz) { set_error("no such model or size mismatch"); return -1; }
+  SrvRequest r;
+  r.P = P;
+  r.coef.assign(coef, coef + 100);
+  for (int is = P.startshot; is <= P.nshot; is += P.dshot) r.shots.push_back(is);
+  r.setup.resize(r.shots.size());
+  int nwin = 50;
+  size_t need = 0;
+  for (size_t k = 0; k < r.shots.size(); k++) {
+    if (!shot_geometry(P, r.shots[k], r.setup[k])) { set_error("shot window outside the model"); return -2; }
+    shot_setup(P, M->vp0.data(), M->colmax.data(), r.setup[k]);
+    while (2 * nwin >= r.setup[k].g.nxl) nwin -= 3;
+    taper_row(r.setup[k].taper, r.setup[k].g.nxl, nwin);
+    need = std::max(need, shot_need(r, (int)k));
+  }
+  *shot_max = (double)need;
+  *request = (double)request_bytes(r);
+  return 0;
 }
 
 int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* id)
@@ -200,8 +250,24 @@ int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* i
   r->pend_g1.assign(nsh, nullptr);
   r->pend_il.assign(nsh, nullptr);
   r->finished.assign(nsh, 0);
+  r->guard.assign(nsh, 0u);
+  {
+    float mv = 0.0f;
+    for (float c : M->colmax) mv = c > mv ? c : mv;
+    if (stability_check(P, coef, sqrtf(mv), why) != 0) { set_error(why); return -1; }
+  }
+  // a shot that cannot fit the budget even alone is refused here, not queued
+  size_t need = 0;
+  for (int k = 0; k < nsh; k++) need = std::max(need, shot_need(*r, k));
+  const size_t extra = request_bytes(*r);
   std::lock_guard<std::mutex> lk(mu_);
   if (err_) { set_error(err_msg_); return err_; }
+  if (admit_rule(budget_, 0.0, 0, false, (double)need, (double)extra) == Admit::kReject) {
+    set_error("a shot of this request needs " + std::to_string(need + extra) + " bytes of HBM (slot " +
+              std::to_string(need) + ", request block " + std::to_string(extra) + "), above the server's budget of " +
+              std::to_string((size_t)budget_));
+    return -3;
+  }
   r->id = next_id_++;
   r->t_submit = now();
   *id = r->id;
@@ -209,6 +275,22 @@ int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* i
   reqs_[r->id] = std::move(r);
   cv_.notify_all();
   return 0;
+}
+
+// bytes of a request's own device block (the layout of request_device)
+size_t Server::request_bytes(const SrvRequest& r)
+{
+  const size_t nm = (size_t)r.P.nx * r.P.nz, nsh = r.shots.size();
+  size_t o = a256(400) + a256((size_t)r.P.nt * 4);
+  for (size_t k = 0; k < nsh; k++) o += a256(r.setup[k].taper.size() * 4);
+  return o + a256(nsh * sizeof(StackShotHost)) + 3 * a256(nm * 8) + 5 * a256(nm * 4);
+}
+
+// bytes shot k holds while resident: its slot and, once finished, its pending
+// image and illumination until it is stacked
+size_t Server::shot_need(const SrvRequest& r, int k) const
+{
+  return Engine::slot_bytes(r.setup[k], r.P.nt, opt_.store_fmt) + 2 * a256((size_t)r.P.nz * r.setup[k].g.nxl * 4);
 }
 
 // request-level device block: coefficients, wavelet, tapers, stack entries,
@@ -260,32 +342,41 @@ int Server::request_device(SrvRequest& r)
   return 0;
 }
 
-bool Server::admit_one(SrvRequest& r)
+Admit Server::admit_one(SrvRequest& r, std::string& why)
 {
   int sid = -1;
   for (int i = 0; i < (int)slots_.size(); i++)
     if (slots_[i].phase == PH_IDLE && !slots_[i].mem) { sid = i; break; }
-  if (sid < 0) return false;
+  if (sid < 0) return Admit::kWait;
   const int k = r.next_admit;
   const ShotSetup& s = r.setup[k];
   const size_t need = Engine::slot_bytes(s, r.P.nt, opt_.store_fmt);
-  size_t extra = 0;
-  if (!r.dev) {
-    const size_t nm = (size_t)r.P.nx * r.P.nz;
-    extra = 3 * nm * 8 + 5 * nm * 4 + (size_t)r.P.nt * 4;
-  }
+  const size_t extra = r.dev ? 0 : request_bytes(r);
   // pending (finished, not yet stacked) buffers of a request are bounded by
   // its resident shots; count them with the slot
   const size_t pend = 2 * a256((size_t)r.P.nz * s.g.nxl * 4);
-  if (used_ + need + extra + pend > budget_ && active_ > 0) return false;
+  bool releasing = false;
+  for (auto& kv : reqs_) releasing |= (kv.second->finalizing && !kv.second->complete);
+  const Admit a = admit_rule(budget_, used_, active_, releasing, (double)(need + pend), (double)extra);
+  if (a != Admit::kAdmit) {
+    if (a == Admit::kReject) why = "the shot does not fit the HBM budget";
+    return a;
+  }
   if (!r.dev) {
-    if (request_device(r) != 0) return false;
+    if (request_device(r) != 0) {
+      if (err_) return Admit::kWait;
+      why = "the device refused the request block (" + std::to_string(request_bytes(r)) + " bytes)";
+      return alloc_failed_rule(active_, releasing);
+    }
     used_ += r.dev_bytes;
     r.t_start = now();
   }
   SrvSlot& sl = slots_[sid];
   sl.mem = pool_.get(need);
-  if (!sl.mem) return false;
+  if (!sl.mem) {
+    why = "the device refused a slot of " + std::to_string(need) + " bytes";
+    return alloc_failed_rule(active_, releasing);
+  }
   sl.bytes = need + pend;
   used_ += sl.bytes;
   sl.req = r.id;
@@ -295,8 +386,7 @@ bool Server::admit_one(SrvRequest& r)
   sl.uploads.clear();
   const Engine::Model* M = eng_->model(r.model_id);
   ShotDev& d = hds_[sid];
-  Engine::layout_slot(r.P, s, opt_.store_fmt, (char*)sl.mem, d, sl.uploads, r.d_coef, r.d_wav, M->d_rho, M->d_vv,
-                      r.P.nx);
+  Engine::layout_slot(r.P, s, opt_.store_fmt, (char*)sl.mem, d, sl.uploads, r.d_coef, r.d_wav, M->dev());
   d.record = 1;
   d.phase = PH_FWD;
   d.it_off = (int)(-tick_);
@@ -309,10 +399,45 @@ bool Server::admit_one(SrvRequest& r)
     fail("descriptor upload", e);
   (void)rtm_launch_phase_init(d_ds_, sid, PH_FWD, 256, st_);
   r.next_admit++;
+  r.resident++;
   active_++;
   peak_ = active_ > peak_ ? active_ : peak_;
   dirty_ = true;
-  return true;
+  return Admit::kAdmit;
+}
+
+// a request that cannot run: no more of its shots are admitted; once none is
+// resident it completes with `status` (its device block released)
+void Server::fail_request(SrvRequest& r, int status, const std::string& why)
+{
+  if (r.status == 0) { r.status = status; r.err = why; }
+  r.next_admit = (int)r.shots.size();
+  if (r.resident == 0 && !r.finalizing) complete_request(r);
+}
+
+void Server::complete_request(SrvRequest& r)
+{
+  r.complete = true;
+  r.t_done = now();
+  r.stage.clear();
+  if (r.dev) { pool_.put(r.dev); used_ -= r.dev_bytes; r.dev = nullptr; }
+  for (size_t k = 0; k < r.pend_g1.size(); k++)
+    if (r.pend_g1[k]) {
+      pool_.put(r.pend_g1[k]);
+      used_ -= 2 * a256((size_t)r.P.nz * r.setup[k].g.nxl * 4);
+      r.pend_g1[k] = r.pend_il[k] = nullptr;
+    }
+  if (r.status == 0) {
+    cells_done_ += r.cells;
+    for (size_t k = 0; k < r.guard.size(); k++)
+      if (r.guard[k] >= 0x7F800000u) {
+        r.status = -7;
+        r.err = "the wavefield of shot " + std::to_string(r.shots[k]) +
+                " is not finite (blow-up: dt above the stability bound for the model, or non-finite model values)";
+        break;
+      }
+  }
+  cv_done_.notify_all();
 }
 
 void Server::rebuild_tiles()
@@ -320,7 +445,9 @@ void Server::rebuild_tiles()
   std::vector<ShotDev> fw, rv;
   std::vector<int> fid, rid;
   qslots_.clear();
+  aslots_.clear();
   for (int i = 0; i < (int)slots_.size(); i++) {
+    if (slots_[i].phase != PH_IDLE) aslots_.push_back(i);
     if (slots_[i].phase == PH_FWD) {
       fw.push_back(hds_[i]); fid.push_back(i);
       qslots_.push_back(i);                           // strips saved after each forward step
@@ -350,10 +477,13 @@ void Server::rebuild_tiles()
     fail("tile upload", e);
   if (!qslots_.empty() && (e = hipMemcpyAsync(d_qs_, qslots_.data(), qslots_.size() * 4, hipMemcpyHostToDevice, st_)) != hipSuccess)
     fail("slot list upload", e);
+  if (!aslots_.empty() && (e = hipMemcpyAsync(d_as_, aslots_.data(), aslots_.size() * 4, hipMemcpyHostToDevice, st_)) != hipSuccess)
+    fail("slot list upload", e);
   (void)hipStreamSynchronize(st_);   // host vectors are reused next rebuild
   ntf_ = (int)tf_.size();
   ntr_ = (int)tr_.size();
   nq_ = (int)qslots_.size();
+  na_ = (int)aslots_.size();
   dirty_ = false;
 }
 
@@ -385,6 +515,10 @@ void Server::finish_shot(int sid)
   SrvRequest& r = *reqs_[sl.req];
   const int k = sl.k;
   const size_t n = (size_t)r.P.nz * r.setup[k].g.nxl * 4;
+  // the guard after the shot's last step (tick_ - 1), then its word to the host
+  (void)rtm_launch_guard(d_ds_, d_id_ + sid, 1, (int)(tick_ - 1), nullptr, st_);
+  (void)hipMemcpyAsync(&r.guard[k], rtm_guard_word(hds_[sid]), 4, hipMemcpyDeviceToHost, st_);
+  r.resident--;
   float* g = (float*)pool_.get(2 * a256(n));
   if (!g) { err_ = -3; err_msg_ = "out of device memory (pending images)"; return; }
   r.pend_g1[k] = g;
@@ -401,6 +535,10 @@ void Server::finish_shot(int sid)
   hds_[sid].phase = PH_IDLE;
   active_--;
   dirty_ = true;
+  if (r.status != 0) {                         // a failed request: drop its results
+    if (r.resident == 0 && !r.finalizing) complete_request(r);
+    return;
+  }
   stack_ready(r);
 }
 
@@ -456,21 +594,18 @@ void Server::loop()
       while (!queue_.empty() && !err_) {
         SrvRequest& r = *reqs_[queue_.front()];
         if (r.next_admit >= (int)r.shots.size()) { queue_.pop_front(); continue; }
-        if (!admit_one(r)) break;
+        std::string why;
+        const Admit a = admit_one(r, why);
+        if (a == Admit::kWait) break;
+        if (a == Admit::kReject) {
+          fail_request(r, -3, why);
+          queue_.pop_front();
+        }
       }
       // close requests whose results have landed
       for (auto& kv : reqs_) {
         SrvRequest& r = *kv.second;
-        if (r.finalizing && !r.complete && hipEventQuery(r.done_ev) == hipSuccess) {
-          r.complete = true;
-          r.t_done = now();
-          r.stage.clear();
-          pool_.put(r.dev);
-          r.dev = nullptr;
-          used_ -= r.dev_bytes;
-          cells_done_ += r.cells;
-          cv_done_.notify_all();
-        }
+        if (r.finalizing && !r.complete && hipEventQuery(r.done_ev) == hipSuccess) complete_request(r);
       }
       bool pending_final = false;
       for (auto& kv : reqs_) pending_final |= (kv.second->finalizing && !kv.second->complete);
@@ -499,6 +634,10 @@ void Server::loop()
       if (rtm_launch_step(a, PH_REV, opt_.tile_z, st_) != hipSuccess) fail("reverse launch", hipGetLastError());
       launches_ += 2;
     }
+    // wavefield guard of every resident shot (the shots' own ntsnap differ;
+    // one interval for all, and once more after each shot's last step)
+    if (na_ > 0 && (tick_ & (kGuardTicks - 1)) == kGuardTicks - 1)
+      (void)rtm_launch_guard(d_ds_, d_as_, na_, (int)tick_, nullptr, st_);
     tick_++;
     {
       std::lock_guard<std::mutex> lk(mu_);
@@ -535,6 +674,11 @@ int Server::wait(int id, double timeout, rtmh_outputs* out, rtmh_request_stats* 
     if (cv_done_.wait_until(lk, until) == std::cv_status::timeout && !r.complete) return 1;
   }
   if (err_ && !r.complete) { set_error(err_msg_); return err_; }
+  if (r.status != 0) {
+    if (st) { st->submit_s = r.t_submit; st->start_s = r.t_start; st->done_s = r.t_done; st->status = r.status; }
+    set_error(r.err);
+    return r.status;
+  }
   if (out) {
     const size_t nm = (size_t)r.P.nx * r.P.nz * 4;
     if (out->image1) memcpy(out->image1, r.image1.data(), nm);
@@ -591,6 +735,13 @@ extern "C" int rtmh_server_submit(rtmh_server* srv, int model_id, const rtmh_par
 {
   if (!srv || !p || !diffcoef || !request_id) { rtm::set_error("null argument"); return -1; }
   return srv->s->submit(model_id, *p, diffcoef, request_id);
+}
+
+extern "C" int rtmh_server_request_bytes(rtmh_server* srv, int model_id, const rtmh_params* p, const float* diffcoef,
+                                         double* shot_max, double* request)
+{
+  if (!srv || !p || !diffcoef || !shot_max || !request) { rtm::set_error("null argument"); return -1; }
+  return srv->s->request_bytes(model_id, *p, diffcoef, shot_max, request);
 }
 
 extern "C" int rtmh_server_wait(rtmh_server* srv, int request_id, double timeout_s, rtmh_outputs* out,

@@ -29,9 +29,10 @@ extern "C" {
  * GPU).  Outputs are host [nz*nx] arrays, overwritten: image1 (cross-
  * correlation stack), image2s (per-shot illumination-normalised stack),
  * image2m (= image1, the reference's normalisation calls are commented out),
- * illum (illumination stack).  image3s/4s/5s/3m/4m/5m (decomposed, Poynting
- * and energy-norm conditions) are outside this implementation and are
- * returned as zeros.  File side effects as the reference: mode 0 writes
+ * illum (illumination stack), image3s/4s/5s and image3m/4m/5m (wavefield
+ * decomposition, Poynting and energy-norm conditions, normalised by the
+ * illumination or not; kernel.cu:1417-1579).  File side effects as the
+ * reference: mode 0 writes
  * ./seisReal/<ishot>seisreal.su, mode 1 reads it; ./output/Image*temp.su
  * every 50 shots; ./output/<idevice>Imagetrue.su.  On failure prints the
  * error and exits with status 1 (the reference exits 0 from check_gpu_error,
@@ -193,7 +194,15 @@ int rtmh_shot_geometry(const rtmh_params *p, int ishot, rtmh_geom *g);
 int rtmh_gather_geometry(const rtmh_params *p, int ishot, const float *field, rtmh_geom *g);
 
 /* Whole job as rtm_model() does it, with options; returns 0 or a negative
- * status (message in rtmh_last_error()). */
+ * status (message in rtmh_last_error()): -1 bad parameters, including a dt
+ * above the stability bound min(dx,dz)/(sqrt(2) sum|c| vmax) of the model
+ * (get_constant, RTM_GPU_velstr.cpp:355-361, where the driver refines dt
+ * instead); -2 a shot window outside the model; -3 out of device memory;
+ * -4 an input record missing; -5 a checkpoint of another job, or one with
+ * more shots done than the job has; -6 a checkpoint not written; -7 a
+ * wavefield that is not finite (checked on the device every
+ * ntsnap = tsnap/dt steps, or after every replayed graph chunk, the
+ * reference's max|p| print of rtm_model.cu:271-273); -10/-11 HIP errors. */
 int rtmh_run(rtmh_ctx *ctx, const rtmh_params *p, const float *vp0, const float *rho0, const float *diffcoef,
              const rtmh_options *opt, rtmh_outputs *out, rtmh_stats *stats);
 
@@ -307,10 +316,24 @@ typedef struct rtmh_request_stats {
   int nshots, status;
 } rtmh_request_stats;
 int rtmh_server_create(rtmh_ctx *ctx, const rtmh_server_options *opt, rtmh_server **out);
-/* queue one job; returns its request id */
+/* queue one job; returns its request id.  Refused (-1) like rtmh_run: bad
+ * parameters, dt above the stability bound of the model (get_constant,
+ * RTM_GPU_velstr.cpp:355-361); refused (-3) when one shot cannot fit the
+ * budget. */
 int rtmh_server_submit(rtmh_server *srv, int model_id, const rtmh_params *p, const float *diffcoef,
                        int *request_id);
-/* 0 when done (outputs copied, stats filled), 1 on timeout, < 0 on error */
+/* HBM the server holds for a request of this job (no counterpart in the
+ * reference; for admission planning): *shot_max = bytes of its largest shot
+ * while resident (slot, plus its images until they are stacked), *request =
+ * the request's own block (coefficients, wavelet, tapers, FP64 stacks,
+ * outputs).  rtmh_server_submit refuses (-3) a job with shot_max + request
+ * above the server's budget; admission waits while the budget is held by
+ * other shots and fails the request (-3 from rtmh_server_wait) when the
+ * device refuses memory that nothing resident will free. */
+int rtmh_server_request_bytes(rtmh_server *srv, int model_id, const rtmh_params *p, const float *diffcoef,
+                              double *shot_max, double *request);
+/* 0 when done (outputs copied, stats filled), 1 on timeout, < 0 on error:
+ * the request failed (-3 memory, -7 its wavefield blew up), or the server did */
 int rtmh_server_wait(rtmh_server *srv, int request_id, double timeout_s, rtmh_outputs *out,
                      rtmh_request_stats *stats);
 /* server-wide counters: ticks, launches, peak resident shots, busy seconds */

@@ -1,0 +1,203 @@
+// host_logic.cpp -- CPU checks of the host logic of librtm_hip.so that needs
+// no GPU: the served form's admission rules (driven here by a fake device
+// allocator) and the checkpoint resume decision.  Built and run by
+// tests/test_host_logic.py; prints one line per check and exits non-zero on
+// the first failure.
+#include <stdio.h>
+#include <stdlib.h>
+#include <unistd.h>
+
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "admission.h"
+#include "rtm_engine.h"
+
+using rtm::Admit;
+
+static int g_fail = 0;
+#define CHECK(c, ...)                                                  \
+  do {                                                                 \
+    if (!(c)) {                                                        \
+      printf("FAIL %s:%d: %s -- ", __FILE__, __LINE__, #c);            \
+      printf(__VA_ARGS__);                                             \
+      printf("\n");                                                    \
+      g_fail = 1;                                                      \
+    }                                                                  \
+  } while (0)
+
+// ---- a served-form admission loop over a fake device -------------------------
+// Mirrors the server's loop: shots are admitted first come first served while
+// admit_rule allows and the fake device grants the memory; resident shots
+// finish after 10 ticks and release their bytes; a request whose shot is
+// rejected fails at once.
+struct FakeDevice {
+  double capacity, held = 0;
+  bool alloc(double b) { if (held + b > capacity) return false; held += b; return true; }
+  void release(double b) { held -= b; }
+};
+struct Req {
+  std::vector<double> need;  // per shot
+  double extra;              // request block
+  int next = 0, resident = 0, done = 0;
+  bool failed = false, has_block = false;
+  std::vector<long long> admitted_at;
+};
+struct Sim {
+  double budget, used = 0;
+  FakeDevice dev;
+  std::deque<int> queue;
+  std::vector<Req> reqs;
+  struct Res { int r, k; long long end; double bytes; };
+  std::vector<Res> res;
+  int peak = 0;
+  long long tick = 0;
+  Sim(double b, double cap) : budget(b), dev{cap} {}
+  int active() const { return (int)res.size(); }
+  void admission()
+  {
+    while (!queue.empty()) {
+      Req& r = reqs[queue.front()];
+      if (r.failed || r.next >= (int)r.need.size()) { queue.pop_front(); continue; }
+      const double extra = r.has_block ? 0.0 : r.extra;
+      Admit a = rtm::admit_rule(budget, used, active(), false, r.need[r.next], extra);
+      if (a == Admit::kAdmit) {
+        if (!r.has_block) {
+          if (!dev.alloc(r.extra)) a = rtm::alloc_failed_rule(active(), false);
+          else { r.has_block = true; used += r.extra; }
+        }
+        if (a == Admit::kAdmit && !dev.alloc(r.need[r.next])) a = rtm::alloc_failed_rule(active(), false);
+      }
+      if (a == Admit::kWait) break;
+      if (a == Admit::kReject) {                 // the request fails; its block is released
+        if (r.has_block) { used -= r.extra; dev.release(r.extra); r.has_block = false; }
+        r.failed = true;
+        queue.pop_front();
+        continue;
+      }
+      used += r.need[r.next];
+      res.push_back({queue.front(), r.next, tick + 10, r.need[r.next]});
+      r.admitted_at.push_back(tick);
+      r.next++;
+      r.resident++;
+      peak = active() > peak ? active() : peak;
+    }
+  }
+  // runs until every request is done or failed; false on a stall
+  bool run(long long max_ticks = 100000)
+  {
+    for (int i = 0; i < (int)reqs.size(); i++) queue.push_back(i);
+    for (tick = 0; tick < max_ticks; tick++) {
+      admission();
+      for (size_t i = 0; i < res.size();) {
+        if (res[i].end <= tick) {
+          Req& r = reqs[res[i].r];
+          used -= res[i].bytes;
+          dev.release(res[i].bytes);
+          r.resident--;
+          r.done++;
+          if (r.done == (int)r.need.size()) { used -= r.extra; dev.release(r.extra); }
+          res.erase(res.begin() + i);
+        } else {
+          i++;
+        }
+      }
+      bool all = true;
+      for (auto& r : reqs) all = all && (r.failed || r.done == (int)r.need.size());
+      if (all && res.empty()) return true;
+    }
+    return false;
+  }
+};
+
+static void admission_checks()
+{
+  // a shot that cannot fit even alone: refused before it is queued
+  CHECK(rtm::admit_rule(100.0, 0.0, 0, false, 90.0, 20.0) == Admit::kReject, "90+20 > 100");
+  CHECK(rtm::admit_rule(100.0, 0.0, 0, false, 80.0, 20.0) == Admit::kAdmit, "80+20 <= 100");
+  // does not fit now: waits while something resident or releasing will free memory
+  CHECK(rtm::admit_rule(100.0, 50.0, 1, false, 60.0, 0.0) == Admit::kWait, "resident");
+  CHECK(rtm::admit_rule(100.0, 50.0, 0, true, 60.0, 0.0) == Admit::kWait, "releasing");
+  CHECK(rtm::admit_rule(100.0, 50.0, 0, false, 60.0, 0.0) == Admit::kReject, "nothing will free it");
+  CHECK(rtm::alloc_failed_rule(0, false) == Admit::kReject, "device full, nothing resident");
+  CHECK(rtm::alloc_failed_rule(2, false) == Admit::kWait, "device full, shots resident");
+  printf("ok admission rules\n");
+
+  {  // a budget that holds one shot (and the request block) at a time
+    Sim s(130.0, 1e9);
+    s.reqs.push_back({{100, 100, 100}, 25});
+    s.reqs.push_back({{100, 100}, 25});
+    CHECK(s.run(), "stalled");
+    CHECK(s.peak == 1, "peak %d", s.peak);
+    CHECK(!s.reqs[0].failed && !s.reqs[1].failed, "failed");
+    CHECK(s.reqs[0].admitted_at.size() == 3 && s.reqs[1].admitted_at.size() == 2, "admitted");
+    CHECK(s.reqs[0].admitted_at.back() < s.reqs[1].admitted_at.front(), "first come first served");
+    printf("ok one shot at a time, in order\n");
+  }
+  {  // the budget holds several shots: they run together
+    Sim s(1000.0, 1e9);
+    s.reqs.push_back({{100, 100, 100}, 25});
+    s.reqs.push_back({{100, 100}, 25});
+    CHECK(s.run(), "stalled");
+    CHECK(s.peak == 5, "peak %d", s.peak);
+    printf("ok shots of several requests resident together\n");
+  }
+  {  // the budget claims more than the device holds: the first shot that the
+     // device refuses with nothing resident fails its request; later requests
+     // that fit still run
+    Sim s(1000.0, 150.0);
+    s.reqs.push_back({{200}, 25});      // never fits the device
+    s.reqs.push_back({{100, 100}, 25}); // fits one shot at a time
+    CHECK(s.run(), "stalled");
+    CHECK(s.reqs[0].failed, "the oversized request must fail");
+    CHECK(!s.reqs[1].failed && s.reqs[1].done == 2, "the fitting request must complete");
+    CHECK(s.peak == 1, "peak %d", s.peak);
+    printf("ok device refusal fails the request instead of queueing it\n");
+  }
+  {  // the device refuses while shots are resident: wait, then run
+    Sim s(1000.0, 260.0);
+    s.reqs.push_back({{100, 100}, 25});
+    s.reqs.push_back({{100}, 25});
+    CHECK(s.run(), "stalled");
+    CHECK(!s.reqs[0].failed && !s.reqs[1].failed, "failed");
+    CHECK(s.reqs[1].admitted_at.front() >= 10, "second request waits for memory");
+    printf("ok device refusal with shots resident waits\n");
+  }
+}
+
+// ---- checkpoint resume of a shrunken job --------------------------------------
+static void checkpoint_checks(const char* dir)
+{
+  const std::string path = std::string(dir) + "/ckpt.bin";
+  const int nx = 7, nz = 5, nstk = 3;
+  std::vector<double> st((size_t)nx * nz * nstk);
+  for (size_t i = 0; i < st.size(); i++) st[i] = 0.5 * (double)i;
+  const uint64_t fp = 0x1234abcdULL;
+  CHECK(rtm::Engine::write_checkpoint(path.c_str(), fp, nx, nz, 5, st) == 0, "write");
+  int done = -1;
+  std::vector<double> back;
+  std::string why;
+  int rc = rtm::Engine::resume_point(path.c_str(), fp, nx, nz, nstk, 5, done, back, why);
+  CHECK(rc == 1 && done == 5 && back == st, "same job rc %d done %d", rc, done);
+  rc = rtm::Engine::resume_point(path.c_str(), fp, nx, nz, nstk, 8, done, back, why);
+  CHECK(rc == 1 && done == 5, "longer job rc %d done %d", rc, done);
+  why.clear();
+  rc = rtm::Engine::resume_point(path.c_str(), fp, nx, nz, nstk, 3, done, back, why);
+  CHECK(rc == -5 && why.find("more than this job") != std::string::npos, "shrunken job rc %d (%s)", rc, why.c_str());
+  rc = rtm::Engine::resume_point(path.c_str(), fp + 1, nx, nz, nstk, 5, done, back, why);
+  CHECK(rc == -5, "another job's checkpoint rc %d", rc);
+  rc = rtm::Engine::resume_point((path + ".none").c_str(), fp, nx, nz, nstk, 5, done, back, why);
+  CHECK(rc == 0 && done == 0, "no file rc %d", rc);
+  unlink(path.c_str());
+  printf("ok checkpoint resume (same, longer, shrunken, other job, missing)\n");
+}
+
+int main(int argc, char** argv)
+{
+  admission_checks();
+  checkpoint_checks(argc > 1 ? argv[1] : ".");
+  if (g_fail) return 1;
+  printf("all host logic checks passed\n");
+  return 0;
+}

@@ -67,13 +67,21 @@ def test_served_alone_and_mixed_equal_batch_engine(gpu_ctx, setup, fmt):
         assert q.peak_slots >= 3
 
 
-def test_server_budget_admits_in_order(gpu_ctx, setup):
-    """A budget that holds one shot at a time still completes every request."""
+def test_served_vs_oracle(gpu_ctx, setup):
+    """Served requests (FP32 store) against the kernel-order oracle's job,
+    bit for bit, and against the reference order within the image bar."""
+    from oracle import oracle as O
+    from tests.conftest import rel_l2
+
     mods, params = setup
-    P = params[0]
-    need = 1.0
-    with G.Server(gpu_ctx, max_slots=4, mem_budget=need) as srv:
-        rid = srv.submit(mods[0], P)
-        i1, *_rest, st = srv.wait(rid, P, timeout=120)
-        assert st.nshots == 3 and np.isfinite(i1).all()
-        assert srv.query().peak_slots == 1
+    with G.Server(gpu_ctx, max_slots=6) as srv:
+        ids = [srv.submit(m, P) for m, P in zip(mods, params)]
+        for k, (rid, P) in enumerate(zip(ids, params)):
+            i1, i2, il, lap, nrm, st = srv.wait(rid, P, timeout=120)
+            vp, rho = model(P.nz, P.nx, k)
+            Po = O.make_params(**{n: getattr(P, n) for n, _ in P._fields_})
+            k1, k2, kl = O.rtm(Po, vp, rho, korder=True)
+            assert np.array_equal(i1, k1) and np.array_equal(i2, k2) and np.array_equal(il, kl), k
+            f1, f2, fl = O.rtm(Po, vp, rho)
+            assert rel_l2(i1, f1) <= 1e-4 and rel_l2(i2, f2) <= 1e-4 and rel_l2(il, fl) <= 1e-4, k
+            print(f"[server] request {k} bit-exact vs the kernel-order oracle", flush=True)
