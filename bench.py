@@ -16,6 +16,15 @@ the per-step launches carry enough tiles to keep the tail short.  The velocity m
 HBM before the timed region (rtmh_model_create); every step runs through the
 C ABI (rtmh_run_model), which returns after its stream has drained.
 
+The default run also measures, as sub-objects of the same line (skip with
+--quick): "batched" (one 64-shot launch batch of the ragged 2048x768
+survey), "precision" (2048x1024 dx=7.5 m nt=6000 shots with FP32 / FP16 /
+BF16 boundary stores, and the image error of each against the FP64 oracle
+on a shot the CPU affords), "served" (a Poisson stream of --requests
+requests against the plain one-shot-per-launch form) and, in
+"cpu_baseline", the oracle on all host cores and on one core, on the
+single-shot configuration (400x200, nt 2000) and on survey shots.
+
 Other workloads (one JSON line each, same keys):
   --workload batched    2048x768, 256 shots with per-shot aperture windows
                         U[400,1600] columns, 64 shots per launch, hipGraph
@@ -62,32 +71,64 @@ def cells_of(P, geoms):
     return sum(2.0 * nt * g.nxp * g.nzp + (nt - 1.0) * P.nz * g.nxl for g in geoms)
 
 
-def cpu_baseline(P_full, vp, rho, min_seconds=10.0, max_shots=6, first=100):
-    """The oracle (FP32 restatement, OpenMP over rows) on a bounded sample of
-    the same survey: whole shots (forward + reconstruction + receiver +
-    imaging at the full nt) from the middle of the survey, until at least
-    min_seconds of CPU work (at most max_shots)."""
+def _omp_threads(n):
+    """OpenMP threads of the oracle's parallel loops (0: the runtime default)."""
+    import ctypes
+
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        return
+    gomp.omp_set_num_threads(int(n) if n > 0 else (os.cpu_count() or 1))
+
+
+def _host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+
+def cpu_shots(P_full, vp, rho, shots, min_seconds, threads):
+    """The oracle (FP32 restatement, OpenMP over rows) on whole shots of a job
+    (forward + reconstruction + receiver + imaging at the full nt) until at
+    least min_seconds of CPU work: (cells, seconds, shots run)."""
     from oracle import oracle as O
 
+    _omp_threads(threads)
     d = {n: getattr(P_full, n) for n, _ in P_full._fields_}
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    cells = 0.0
-    shots = []
+    cells, done = 0.0, []
     t0 = time.perf_counter()
-    for k in range(max_shots):
-        ishot = first + k
+    for ishot in shots:
         d.update(nshot=ishot, startshot=ishot)
         Po = O.make_params(**d)
         g = O.geometry(Po, ishot)
         O.run_shot(Po, ishot, vp, rho)
         cells += 2.0 * Po.nt * g.nxp * g.nzp + (Po.nt - 1.0) * Po.nz * g.nxl
-        shots.append(ishot)
+        done.append(ishot)
         if time.perf_counter() - t0 >= min_seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": round(cells / dt / 1e9, 4), "unit": "Gcell/s", "cores": threads, "kind": "port",
-            "sample": f"{len(shots)} whole shot(s) #{shots[0]}-#{shots[-1]} of the same survey (nt={P_full.nt}): "
-                      f"{cells / 1e9:.2f} Gcell in {dt:.1f} s; {len(shots) / dt * 3600.0:.0f} shots/hour"}
+    _omp_threads(0)
+    return cells, dt, done
+
+
+def cpu_baseline(P_full, vp, rho, synth, quick=False):
+    """The reported CPU baseline: the oracle on all host cores over whole
+    survey shots (value), and on the single-shot configuration (the
+    reference's CPU-runnable case) on all cores and on one core."""
+    nth = _host_threads()
+    cells, dt, done = cpu_shots(P_full, vp, rho, range(100, 106), 10.0, nth)
+    out = {"value": round(cells / dt / 1e9, 4), "unit": "Gcell/s", "cores": nth, "kind": "port",
+           "sample": f"{len(done)} whole shot(s) #{done[0]}-#{done[-1]} of the 200-shot survey (nt={P_full.nt}): "
+                     f"{cells / 1e9:.2f} Gcell in {dt:.1f} s; {len(done) / dt * 3600.0:.0f} shots/hour"}
+    if quick:
+        return out
+    P1, v1, r1 = synth.single_shot_anomaly()
+    c1, d1, _ = cpu_shots(P1, v1, r1, [1], 0.0, nth)
+    cs, ds, _ = cpu_shots(P1, v1, r1, [1], 0.0, 1)
+    out["single_shot_anomaly"] = {
+        "config": "400x200, npml 40, nt 2000, 400 receivers, one whole shot",
+        "all_cores": {"value": round(c1 / d1 / 1e9, 4), "unit": "Gcell/s", "cores": nth, "s_per_shot": round(d1, 2)},
+        "one_core": {"value": round(cs / ds / 1e9, 4), "unit": "Gcell/s", "cores": 1, "s_per_shot": round(ds, 2)}}
+    return out
 
 
 def traffic_ratio(kernel):
@@ -179,7 +220,6 @@ def run_survey(args, G, synth):
     elapsed = time.perf_counter() - t0
     cells = sum(s.cell_updates for s, _, _ in recs)
     nshots = sum(len(sh) for _, _, sh in recs)
-    lat = np.concatenate([np.full(len(sh), dt) for _, dt, sh in recs])
     roof = roofline_of(recs, P, copy_gbs)
     out = base_line(
         args, round(cells / elapsed / 1e9, 3), "Gcell/s", elapsed,
@@ -190,17 +230,156 @@ def run_survey(args, G, synth):
          "shots_per_step": S, "batch": recs[0][0].batch, "graph_chunk": args.chunk,
          "tile_z": recs[0][0].tile_z, "tile_z_rev": recs[0][0].tile_z_rev, "store": args.store},
         {"shots_per_hour": round(nshots / elapsed * 3600.0, 1),
-         "p50_shot_latency_s": round(float(np.percentile(lat, 50)), 3),
-         "p99_shot_latency_s": round(float(np.percentile(lat, 99)), 3),
+         "survey_latency_s": round(elapsed / max(args.steps, 1), 3),
          "hbm_pct_roofline": round(100.0 * roof["achieved"] / HBM_PEAK_GBS, 1),
          "roofline": roof, "cpu_baseline": None})
     ctx.model_destroy(mid)
+    if not args.quick:
+        # the other configurations, as sub-objects of the same line
+        out["batched"] = batched_sub(args, G, synth, ctx)
+        out["precision"] = precision_sub(args, G, synth, ctx)
+        out["served"] = served_sub(args, G, synth, ctx)
     ctx.close()
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(P, vp, rho)
+        out["cpu_baseline"] = cpu_baseline(P, vp, rho, synth, quick=args.quick)
         log(f"[bench] cpu baseline {out['cpu_baseline']}")
     return out
+
+
+# ------------------------------------------------------ configuration subs --
+def batched_sub(args, G, synth, ctx):
+    """One 64-shot launch batch of the ragged 2048x768 survey (windows drawn
+    from 400-1600 columns), graph chunk 50, autotuned tile heights: one
+    untimed batch (it tunes), then one timed."""
+    P, vp, rho, om = synth.batched_ragged()
+    if args.nt:
+        P.nt = args.nt
+    P.startshot, P.nshot, P.dshot = 1, 64, 1
+    mid = ctx.model_create(vp, rho)
+    kw = dict(model=mid, batch=64, graph_chunk=args.chunk, tile_z=0, want_post=True, shot_offsetmax=om[:64])
+    ctx.run(P, **kw)
+    t0 = time.perf_counter()
+    r = ctx.run(P, **kw)
+    dt = time.perf_counter() - t0
+    ctx.model_destroy(mid)
+    roof = roofline_of([(r.stats, dt, None)], P)
+    widths = [int(np.floor(om[i] / P.dx + 1)) for i in range(64)]
+    out = {"workload": f"batched RTM, 2048x768 dx=10 m, shots 1-64 of 256 every 80 m, windows U[400,1600] "
+                       f"columns (drawn {min(widths)}..{max(widths)}, clipped at the edges), nt={P.nt}, "
+                       f"64 shots per launch batch, graph chunk {args.chunk}, autotuned tile heights",
+           "value": round(r.stats.cell_updates / dt / 1e9, 3), "unit": "Gcell/s", "seconds": round(dt, 3),
+           "shots_per_hour": round(64 / dt * 3600.0, 1), "tile_z": r.stats.tile_z, "tile_z_rev": r.stats.tile_z_rev,
+           "k_rev_achieved_gbs": roof["achieved"], "k_rev_frac": roof["frac"]}
+    log(f"[bench] batched: {out['value']} Gcell/s, k_rev {roof['frac']:.3f} of 8 TB/s")
+    return out
+
+
+def precision_sub(args, G, synth, ctx):
+    """2048x1024 dx=7.5 m 25 Hz dt=0.6 ms nt=6000 full-aperture shots per
+    boundary store (FP32 / FP16 / BF16), one untimed shot then --prec-shots
+    timed; image error of each store against the FP64 oracle on a shot the
+    CPU affords (the same model, 401-column window, nt 2000)."""
+    P, vp, rho = synth.precision_grid()
+    if args.nt:
+        P.nt = args.nt
+    S = args.prec_shots
+    mid = ctx.model_create(vp, rho)
+    res = {}
+    for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
+        Q = G.Params.from_buffer_copy(P)
+        Q.startshot, Q.nshot, Q.dshot = 61, 61, 1
+        ctx.run(Q, model=mid, store_fmt=fmt, graph_chunk=args.chunk, want_post=False)
+        Q.nshot = 60 + S
+        t0 = time.perf_counter()
+        r = ctx.run(Q, model=mid, batch=S, store_fmt=fmt, graph_chunk=args.chunk, want_post=True)
+        dt = time.perf_counter() - t0
+        res[name] = {"gcell_s": round(r.stats.cell_updates / dt / 1e9, 2), "shots_per_hour": round(S / dt * 3600, 1),
+                     "seconds": round(dt, 2)}
+        log(f"[bench] precision {name}: {S} shots {dt:.2f} s, {res[name]['gcell_s']} Gcell/s")
+    out = {"workload": f"precision variant, 2048x1024 dx=7.5 m, PML 50, Ricker 25 Hz, dt=0.6 ms nt={P.nt}, "
+                       f"full-aperture shots (2148x1124 padded cells), {S} shots per launch batch per store",
+           "per_store": res, "error_vs_fp64_oracle": None}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        Q = G.Params.from_buffer_copy(P)
+        Q.startshot = Q.nshot = 64
+        Q.flag_layout, Q.offsetmax, Q.nt = 1, 3000.0, min(P.nt, 2000)
+        Qo = O.make_params(**{n: getattr(Q, n) for n, _ in Q._fields_})
+        _omp_threads(_host_threads())
+        t0 = time.perf_counter()
+        i1, i2, il = O.rtm(Qo, vp, rho, f64=True)
+        log(f"[bench] precision: FP64 oracle {time.perf_counter() - t0:.1f} s")
+        err = {"sample": f"shot 64 of the precision model with a 401-column window (layout 1, offsetmax 3000 m), "
+                         f"nt={Q.nt}", "rel_l2": {}}
+        for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
+            r = ctx.run(Q, model=mid, batch=1, store_fmt=fmt, want_post=False)
+            e = {k: float(np.linalg.norm((a - b).astype(np.float64)) / max(np.linalg.norm(b.astype(np.float64)), 1e-300))
+                 for k, a, b in (("image1", r.image1, i1), ("image2s", r.image2s, i2), ("illum", r.illum, il))}
+            err["rel_l2"][name] = {k: float(f"{v:.3e}") for k, v in e.items()}
+            log(f"[bench] precision {name} vs FP64 oracle: {err['rel_l2'][name]}")
+        out["error_vs_fp64_oracle"] = err
+    ctx.model_destroy(mid)
+    return out
+
+
+def served_sub(args, G, synth, ctx):
+    """A Poisson stream (4 req/s) of --requests requests over 8 resident
+    models, served with continuous batching (FP16 boundaries), against the
+    plain one-shot-per-launch GPU form (per-step launches, no graph, FP32
+    boundaries, requests one after another) on the first --plain-requests."""
+    models = synth.serve_models()
+    reqs = synth.serve_stream(args.requests, models)
+    mids = [ctx.model_create(v, r) for v, r in models]
+    total_shots = sum(P.nshot for _, _, P in reqs)
+    total_cells = 0.0
+    for _, m, P in reqs:
+        total_cells += cells_of(P, [G.geometry(P, s) for s in range(1, P.nshot + 1)])
+    log(f"[bench] serve: {len(reqs)} requests, {total_shots} shots, {total_cells / 1e12:.2f} Tcell, "
+        f"arrivals over {reqs[-1][0]:.1f} s")
+    lat = []
+    with G.Server(ctx, store_fmt=G.STORE_F16, mem_budget=args.mem_budget) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for (ta, m, P) in reqs:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            ids.append((srv.submit(mids[m], P), P))
+        for rid, P in ids:
+            *_, st = srv.wait(rid, P, timeout=3600)
+            lat.append(st.done_s - st.submit_s)
+        served_s = time.perf_counter() - t0
+        q = srv.query()
+    log(f"[bench] served: {served_s:.1f} s, peak slots {q.peak_slots}, ticks {q.ticks}")
+    nplain = min(args.plain_requests or len(reqs), len(reqs))
+    plain_s, plain_shots, plain_cells, plain_lat = 0.0, 0, 0.0, []
+    for (ta, m, P) in reqs[:nplain]:
+        t1 = time.perf_counter()
+        r = ctx.run(P, model=mids[m], batch=1, graph_chunk=0, tile_z=32, store_fmt=G.STORE_F32, want_post=True)
+        d = time.perf_counter() - t1
+        plain_s += d
+        plain_shots += P.nshot
+        plain_cells += r.stats.cell_updates
+        plain_lat.append(d)
+    log(f"[bench] plain: {nplain} requests in {plain_s:.1f} s")
+    for m in mids:
+        ctx.model_destroy(m)
+    sph = total_shots / served_s * 3600.0
+    psph = plain_shots / plain_s * 3600.0
+    return {"workload": f"serving: Poisson stream 4 req/s, {len(reqs)} requests ({total_shots} shots, "
+                        f"{total_cells / 1e12:.2f} Tcell), grids 256^2..2048x1024, nt U[1000,6000], 1-32 shots, "
+                        "8 resident models, fp16 boundaries, continuous batching",
+            "value": round(total_cells / served_s / 1e9, 3), "unit": "Gcell/s", "seconds": round(served_s, 2),
+            "shots_per_hour": round(sph, 1), "requests": len(reqs),
+            "p50_request_latency_s": round(float(np.percentile(lat, 50)), 2),
+            "p99_request_latency_s": round(float(np.percentile(lat, 99)), 2), "peak_slots": q.peak_slots,
+            "plain_form": {"shots_per_hour": round(psph, 1), "gcell_s": round(plain_cells / plain_s / 1e9, 3),
+                           "requests": nplain, "p50_request_s": round(float(np.percentile(plain_lat, 50)), 2),
+                           "note": "one shot per launch, per-step launches, fp32 boundaries, requests one after "
+                                   "another (the first requests of the same stream)"},
+            "served_vs_plain_shots_per_hour": round(sph / psph, 2)}
 
 
 # ------------------------------------------------------------------- field --
@@ -363,113 +542,27 @@ def run_batched(args, G, synth):
 
 # --------------------------------------------------------------- precision --
 def run_precision(args, G, synth):
-    P, vp, rho = synth.precision_grid()
-    if args.nt:
-        P.nt = args.nt
-    S = args.shots_per_step or 8
     ctx = G.Context(0)
-    mid = ctx.model_create(vp, rho)
-    res = {}
-    for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
-        Q = G.Params.from_buffer_copy(P)
-        Q.startshot, Q.nshot, Q.dshot = 61, 60 + S, 1
-        ctx.run(Q, model=mid, batch=S, store_fmt=fmt, graph_chunk=args.chunk, tile_z=args.tz, want_post=False) \
-            if args.warmup else None
-        t0 = time.perf_counter()
-        r = ctx.run(Q, model=mid, batch=S, store_fmt=fmt, graph_chunk=args.chunk, tile_z=args.tz, want_post=True)
-        dt = time.perf_counter() - t0
-        res[name] = {"gcell_s": round(r.stats.cell_updates / dt / 1e9, 2), "shots_per_hour": round(S / dt * 3600, 1),
-                     "seconds": round(dt, 2)}
-        log(f"[bench] {name}: {S} shots {dt:.2f} s, {r.stats.cell_updates / dt / 1e9:.1f} Gcell/s")
-    # image error of one shot against the FP64 CPU oracle (bounded: err_nt steps)
-    err = None
-    if not args.no_cpu:
-        from oracle import oracle as O
-        Q = G.Params.from_buffer_copy(P)
-        ne = args.err_nt or P.nt
-        Q.nt = ne
-        Q.startshot = Q.nshot = 64
-        Qo = O.make_params(**{n: getattr(Q, n) for n, _ in Q._fields_})
-        log(f"[bench] FP64 oracle, shot 64, nt={ne} ...")
-        t0 = time.perf_counter()
-        i1, i2, il = O.rtm(Qo, vp, rho, f64=True)
-        log(f"[bench] oracle {time.perf_counter() - t0:.1f} s")
-        err = {"sample": f"shot 64 of the survey, nt={ne}", "rel_l2": {}}
-        for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
-            r = ctx.run(Q, model=mid, batch=1, store_fmt=fmt, want_post=False)
-            e = {k: float(np.linalg.norm((a - b).astype(np.float64)) / max(np.linalg.norm(b.astype(np.float64)), 1e-300))
-                 for k, a, b in (("image1", r.image1, i1), ("image2s", r.image2s, i2), ("illum", r.illum, il))}
-            err["rel_l2"][name] = {k: float(f"{v:.3e}") for k, v in e.items()}
-            log(f"[bench] {name} vs FP64 oracle: {err['rel_l2'][name]}")
-    ctx.model_destroy(mid)
+    sub = precision_sub(args, G, synth, ctx)
     ctx.close()
-    best = res["f16"]
+    best = sub["per_store"]["f16"]
     return base_line(
         args, best["gcell_s"], "Gcell/s", best["seconds"], "f32 stencil, f16 boundaries",
         "synthetic: seeded smooth-random layered model, data modelled in the same run",
-        {"workload": f"precision variant, 2048x1024 dx=7.5 m, PML 50, Ricker 25 Hz, dt=0.6 ms nt={P.nt}, "
-                     f"full-aperture receivers, {S} shots per launch", "shots_per_step": S, "batch": S},
-        {"per_store": res, "error_vs_fp64_oracle": err, "cpu_baseline": None})
+        {"workload": sub["workload"], "shots_per_step": args.prec_shots, "batch": args.prec_shots},
+        {"per_store": sub["per_store"], "error_vs_fp64_oracle": sub["error_vs_fp64_oracle"], "cpu_baseline": None})
 
 
 # ------------------------------------------------------------------- serve --
 def run_serve(args, G, synth):
-    models = synth.serve_models()
-    reqs = synth.serve_stream(args.requests, models)
     ctx = G.Context(0)
-    mids = [ctx.model_create(v, r) for v, r in models]
-    total_shots = sum(P.nshot for _, _, P in reqs)
-    total_cells = 0.0
-    for _, m, P in reqs:
-        total_cells += cells_of(P, [G.geometry(P, s) for s in range(1, P.nshot + 1)])
-    log(f"[bench] serve: {len(reqs)} requests, {total_shots} shots, {total_cells / 1e12:.2f} Tcell, "
-        f"arrivals over {reqs[-1][0]:.1f} s")
-    # served form: continuous batching, FP16 boundaries
-    lat = []
-    with G.Server(ctx, store_fmt=G.STORE_F16, mem_budget=args.mem_budget) as srv:
-        t0 = time.perf_counter()
-        ids = []
-        for (ta, m, P) in reqs:
-            now = time.perf_counter() - t0
-            if ta > now:
-                time.sleep(ta - now)
-            ids.append((srv.submit(mids[m], P), P))
-        for rid, P in ids:
-            *_, st = srv.wait(rid, P, timeout=3600)
-            lat.append(st.done_s - st.submit_s)
-        served_s = time.perf_counter() - t0
-        q = srv.query()
-    log(f"[bench] served: {served_s:.1f} s, peak slots {q.peak_slots}, ticks {q.ticks}")
-    # plain one-shot-per-launch GPU form: requests one after another, one shot
-    # per launch, per-step launches (no graph), FP32 boundaries
-    plain_s = 0.0
-    nplain = min(args.plain_requests or len(reqs), len(reqs))
-    plain_shots = 0
-    plain_lat = []
-    for (ta, m, P) in reqs[:nplain]:
-        t1 = time.perf_counter()
-        ctx.run(P, model=mids[m], batch=1, graph_chunk=0, tile_z=32, store_fmt=G.STORE_F32, want_post=True)
-        d = time.perf_counter() - t1
-        plain_s += d
-        plain_shots += P.nshot
-        plain_lat.append(d)
-    log(f"[bench] plain: {nplain} requests in {plain_s:.1f} s")
-    for m in mids:
-        ctx.model_destroy(m)
+    sub = served_sub(args, G, synth, ctx)
     ctx.close()
-    sph = total_shots / served_s * 3600.0
-    psph = plain_shots / plain_s * 3600.0
-    return base_line(
-        args, round(total_cells / served_s / 1e9, 3), "Gcell/s", served_s, "f32 stencil, f16 boundaries",
-        "synthetic: 8 seeded layered models, Poisson arrivals, data modelled in the same run",
-        {"workload": f"serving: Poisson stream 4 req/s, {len(reqs)} requests, grids 256^2..2048x1024, "
-                     "nt U[1000,6000], 1-32 shots, 8 resident models, fp16 boundaries, continuous batching"},
-        {"shots_per_hour": round(sph, 1), "p50_request_latency_s": round(float(np.percentile(lat, 50)), 2),
-         "p99_request_latency_s": round(float(np.percentile(lat, 99)), 2), "peak_slots": q.peak_slots,
-         "plain_form": {"shots_per_hour": round(psph, 1), "requests": nplain,
-                        "p50_request_s": round(float(np.percentile(plain_lat, 50)), 2),
-                        "note": "one shot per launch, per-step launches, fp32 boundaries, requests serially"},
-         "served_vs_plain_shots_per_hour": round(sph / psph, 2), "cpu_baseline": None})
+    extra = {k: v for k, v in sub.items() if k not in ("workload", "value", "unit", "seconds")}
+    extra["cpu_baseline"] = None
+    return base_line(args, sub["value"], "Gcell/s", sub["seconds"], "f32 stencil, f16 boundaries",
+                     "synthetic: 8 seeded layered models, Poisson arrivals, data modelled in the same run",
+                     {"workload": sub["workload"]}, extra)
 
 
 def main():
@@ -485,10 +578,13 @@ def main():
     ap.add_argument("--chunk", type=int, default=50, help="time steps per replayed hipGraph (0: plain launches)")
     ap.add_argument("--tz", type=int, default=0, help="tile height 16/32/64 (0: autotuned)")
     ap.add_argument("--nt", type=int, default=0, help="override nt (shorter runs)")
-    ap.add_argument("--err-nt", type=int, default=0, help="precision: nt of the oracle error sample (0: full)")
-    ap.add_argument("--requests", type=int, default=16, help="serve: requests in the stream")
-    ap.add_argument("--plain-requests", type=int, default=0, help="serve: requests run in the plain form (0: all)")
-    ap.add_argument("--mem-budget", type=float, default=0.0, help="serve: HBM budget in bytes (0: 85%% free)")
+    ap.add_argument("--requests", type=int, default=50, help="serve: requests in the stream")
+    ap.add_argument("--plain-requests", type=int, default=8,
+                    help="serve: first requests of the stream also run in the plain form (0: all)")
+    ap.add_argument("--prec-shots", type=int, default=16, help="precision: timed shots per store")
+    ap.add_argument("--quick", action="store_true",
+                    help="survey line only (no batched / precision / served sub-objects, no one-core baseline)")
+    ap.add_argument("--mem-budget", type=float, default=0.0, help="serve: HBM budget in bytes (0: 80%% of free memory)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="survey: size of the device-to-device copy probe (achievable bandwidth; 0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle error")

@@ -27,7 +27,7 @@ EXPORTED = [
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
     "rtmh_server_create", "rtmh_server_submit", "rtmh_server_wait", "rtmh_server_query", "rtmh_server_destroy",
     "rtmh_scan_gather", "rtmh_read_gather", "rtmh_interp_seis", "rtmh_gather_geometry",
-    "rtmh_server_request_bytes",
+    "rtmh_server_request_bytes", "rtmh_smooth_gpu",
 ]
 
 
@@ -51,7 +51,7 @@ class Options(C.Structure):
                 ("write_files", C.c_int), ("verbose", C.c_int), ("mem_budget", C.c_double), ("profile", C.c_int),
                 ("shot_offsetmax", C.c_void_p), ("checkpoint", C.c_char_p), ("resume", C.c_int),
                 ("ic_extra", C.c_int), ("shot_field", C.c_void_p), ("seis_wait", C.c_void_p),
-                ("seis_user", C.c_void_p)]
+                ("seis_user", C.c_void_p), ("snap_every", C.c_int), ("snap_fmt", C.c_int)]
 
 
 # the other imaging conditions (rtmh_outputs order)
@@ -259,12 +259,15 @@ class Context:
     def run(self, P: Params, vp0=None, rho0=None, coef=None, *, model=None, batch=0, store_fmt=STORE_F32,
             graph_chunk=0, tile_z=0, write_files=False, verbose=False, mem_budget=0.0, want_shots=False,
             want_post=True, seis_in=None, profile=False, shot_offsetmax=None, checkpoint=None,
-            resume=False, ic_extra=False, shot_field=None, seis_wait=None) -> RunResult:
+            resume=False, ic_extra=False, shot_field=None, seis_wait=None, snap_every=0,
+            snap_fmt=STORE_F16) -> RunResult:
         """One rtm_model() job; pass vp0/rho0 arrays, or model=<id> of a resident model.
         checkpoint: path written after every batch (FP64 stacks, shots done); resume=True
         continues a job from it.  shot_field: per shot (sx, offset, offsetmax) of recorded
         gathers (the geometry of rtm_real).  seis_wait: a SEIS_WAIT callback (user, shot index)
-        the engine calls before it reads shot k of seis_in, after the forward pass."""
+        the engine calls before it reads shot k of seis_in, after the forward pass.
+        snap_every: decimated source-wavefield snapshots every that many steps
+        (snap_fmt precision), restored in the reverse pass."""
         coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
         nz, nx = P.nz, P.nx
         if model is None:
@@ -321,6 +324,7 @@ class Context:
         if seis_wait is not None:
             keep.append(seis_wait)
             opt.seis_wait = C.cast(seis_wait, C.c_void_p)
+        opt.snap_every, opt.snap_fmt = int(snap_every), int(snap_fmt)
         st = Stats()
         if model is None:
             rc = lib().rtmh_run(self._h, C.byref(P), _fp(vp0), _fp(rho0), _fp(coef), C.byref(opt), C.byref(out),
@@ -331,6 +335,14 @@ class Context:
         if rc != 0:
             raise RuntimeError(f"rtmh_run failed ({rc}): {last_error()}")
         return RunResult(img1, img2, ill, lap, nrm, seis, g1s, ils, st, ic)
+
+    def smooth(self, window, npml: int = 0, nsp: int = 0) -> np.ndarray:
+        """pmlvelsmooth1d (npml > 0) then velsmooth1d (nsp > 0) of a window on the GPU."""
+        w = np.array(window, np.float32, order="C", copy=True)
+        nz, nx = w.shape
+        if lib().rtmh_smooth_gpu(self._h, _fp(w), C.c_int(nx), C.c_int(nz), C.c_int(npml), C.c_int(nsp)) != 0:
+            raise RuntimeError(f"rtmh_smooth_gpu failed: {last_error()}")
+        return w
 
     def copy_probe(self, nbytes: int, reps: int = 3) -> float:
         """Device-to-device copy bandwidth, GB/s counting read + written bytes."""

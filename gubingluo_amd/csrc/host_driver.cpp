@@ -169,15 +169,9 @@ extern "C" void rtmh_gauss_smooth(float* w, int nz, int nx, int nsp)
 {
   if (nsp <= 0) return;
   const int k = 2 * nsp + 1, nxe = nx + 2 * nsp, nze = nz + 2 * nsp;
-  const double den = 2.0 * nsp / 3.0 * nsp / 3.0;
-  std::vector<double> wt((size_t)k * k);
-  for (int i = 0; i < k; i++)
-    for (int j = 0; j < k; j++) {
-      const double d1 = i - nsp, d2 = j - nsp;
-      wt[(size_t)i * k + j] = exp(-(d1 * d1 + d2 * d2) / den);
-    }
+  std::vector<double> wt;
   double norm = 0;
-  for (double x : wt) norm += x;                      // row order, as the reference sums it
+  rtm::gauss_weights(nsp, wt, norm);                  // row order, as the reference sums them
   // edge-extended copy: each extended cell takes the nearest window cell
   std::vector<float> e((size_t)nze * nxe);
   for (int iz = 0; iz < nze; iz++) {

@@ -97,6 +97,22 @@ extern "C" int rtmh_run_model(rtmh_ctx* ctx, int model_id, const rtmh_params* p,
   return ctx->eng->run_model(model_id, *p, diffcoef, o, out, stats);
 }
 
+extern "C" int rtmh_smooth_gpu(rtmh_ctx* ctx, float* w, int nx, int nz, int npml, int nsp)
+{
+  if (!ctx || !w || nx < 2 || nz < 2 || npml < 0 || nsp < 0) { rtm::set_error("bad argument"); return -1; }
+  std::vector<float*> ws{w};
+  std::vector<int> vx{nx}, vz{nz};
+  if (npml > 0) {
+    if (!rtm::pml_smooth_on_gpu(nx, nz, npml)) {
+      rtm::set_error("window outside the GPU layer smoother's shapes (npml <= 65, nx, nz in [3 npml, 4096])");
+      return -1;
+    }
+    const int rc = ctx->eng->pml_smooth_windows(ws, vx, vz, npml);
+    if (rc) return rc;
+  }
+  return nsp > 0 ? ctx->eng->gauss_smooth_windows(ws, vx, vz, nsp) : 0;
+}
+
 extern "C" int rtmh_copy_probe(rtmh_ctx* ctx, double bytes, int reps, double* gbps)
 {
   if (!ctx || !gbps || bytes <= 0 || reps <= 0) {

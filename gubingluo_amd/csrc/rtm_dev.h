@@ -98,6 +98,15 @@ struct ShotDev {
   // each, pitch nxl), then 61 bins of g31 and 61 of g32
   float* ic;
   float dx, dz;                // grid spacings (central differences of the energy norm)
+  // decimated snapshots of the source wavefield (null when off): snapshot
+  // q (step k = snap_every * (snap_k0 + q)) holds the interior cells of p
+  // as forward step k leaves them and of vx, vz as forward step k+1 leaves
+  // them ([q][3][nzi][nxl], store_fmt codes of snap_fmt), restored into the
+  // reconstructed field after reverse step k; snap_scale [q][3] fp32
+  // scales, then [q][3] max-abs words
+  void* snap;
+  float* snap_scale;
+  int snap_every, snap_k0, nsnap, snap_fmt;
 };
 
 struct Tile {

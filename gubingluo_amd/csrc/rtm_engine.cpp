@@ -172,7 +172,24 @@ static void damping(std::vector<float>& a, std::vector<float>& b, int n, int npm
   }
 }
 
-void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, ShotSetup& s)
+// the velsmooth1d weights (Velocity_function.cpp:277-303): exp of minus the
+// squared offset over 2 (nsp/3)^2, row-major over the (2 nsp + 1)^2 window,
+// and their sum in the reference's order
+void gauss_weights(int nsp, std::vector<double>& wt, double& norm)
+{
+  const int k = 2 * nsp + 1;
+  const double den = 2.0 * nsp / 3.0 * nsp / 3.0;
+  wt.resize((size_t)k * k);
+  for (int i = 0; i < k; i++)
+    for (int j = 0; j < k; j++) {
+      const double d1 = i - nsp, d2 = j - nsp;
+      wt[(size_t)i * k + j] = exp(-(d1 * d1 + d2 * d2) / den);
+    }
+  norm = 0;
+  for (double x : wt) norm += x;
+}
+
+void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, ShotSetup& s, bool smooth_here)
 {
   const rtmh_geom& g = s.g;
   const int nxp = g.nxp, nzp = g.nzp;
@@ -184,10 +201,101 @@ void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, Sho
       s.wvv[(size_t)iz * nxp + ix] = v * v;                    // v^2 (Velocity_function.cpp:2059)
       s.wrho[(size_t)iz * nxp + ix] = win_at(rho0, P.nx, P.nz, g.nx1, g.nxl, P.npml, iz, ix);
     }
-  if (P.medium_flag) {                                         // rtm_model.cu:204-206
+  if (P.medium_flag && smooth_here) {                          // rtm_model.cu:204-206
     rtmh_pml_smooth(s.wvv.data(), nxp, nzp, P.npml);
     rtmh_pml_smooth(s.wrho.data(), nxp, nzp, P.npml);
   }
+}
+
+static void parallel_for(int n, const std::function<void(int)>& fn);
+
+bool pml_smooth_on_gpu(int nx, int nz, int npml)
+{
+  return npml >= 2 && npml - 1 <= 64 && nx >= 3 * npml && nz >= 3 * npml && nx <= 4096 && nz <= 4096;
+}
+
+// pmlvelsmooth1d of host windows on the GPU (k_pml_smooth, one workgroup per
+// window), in groups of at most ~1 GiB of windows and scratch
+int Engine::pml_smooth_windows(std::vector<float*>& wins, const std::vector<int>& nx, const std::vector<int>& nz,
+                               int npml)
+{
+  HIPCHK(hipSetDevice(device_));
+  const size_t n = wins.size();
+  size_t k0 = 0;
+  while (k0 < n) {
+    size_t k1 = k0, bytes = 0;
+    while (k1 < n && (k1 == k0 || bytes + 2 * align256((size_t)nx[k1] * nz[k1] * 4) <= ((size_t)1 << 30))) {
+      bytes += 2 * align256((size_t)nx[k1] * nz[k1] * 4);
+      k1++;
+    }
+    const size_t m = k1 - k0;
+    char* buf = (char*)pool_.get(bytes + align256(m * sizeof(SmoothWinHost)));
+    if (!buf) { set_error("out of device memory (model smoothing)"); return -3; }
+    std::vector<SmoothWinHost> d(m);
+    size_t o = 0;
+    for (size_t i = 0; i < m; i++) {
+      const size_t wb = (size_t)nx[k0 + i] * nz[k0 + i] * 4;
+      d[i].w = (float*)(buf + o); o += align256(wb);
+      d[i].v = (float*)(buf + o); o += align256(wb);
+      d[i].nx = nx[k0 + i]; d[i].nz = nz[k0 + i]; d[i].npml = npml; d[i]._pad = 0;
+      HIPCHK(hipMemcpyAsync(d[i].w, wins[k0 + i], wb, hipMemcpyHostToDevice, st_));
+    }
+    HIPCHK(hipMemcpyAsync(buf + o, d.data(), m * sizeof(SmoothWinHost), hipMemcpyHostToDevice, st_));
+    HIPCHK(rtm_launch_pml_smooth(buf + o, (int)m, st_));
+    for (size_t i = 0; i < m; i++)
+      HIPCHK(hipMemcpyAsync(wins[k0 + i], d[i].w, (size_t)nx[k0 + i] * nz[k0 + i] * 4, hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    pool_.put(buf);
+    k0 = k1;
+  }
+  return 0;
+}
+
+// velsmooth1d of host windows on the GPU (k_gauss_smooth), one at a time
+int Engine::gauss_smooth_windows(std::vector<float*>& wins, const std::vector<int>& nx, const std::vector<int>& nz,
+                                 int nsp)
+{
+  if (nsp <= 0 || wins.empty()) return 0;
+  HIPCHK(hipSetDevice(device_));
+  std::vector<double> wt;
+  double norm = 0;
+  gauss_weights(nsp, wt, norm);
+  size_t big = 0;
+  for (size_t k = 0; k < wins.size(); k++) big = std::max(big, (size_t)nx[k] * nz[k] * 4);
+  char* buf = (char*)pool_.get(2 * align256(big) + align256(wt.size() * 8));
+  if (!buf) { set_error("out of device memory (model smoothing)"); return -3; }
+  float* a = (float*)buf;
+  float* b = (float*)(buf + align256(big));
+  double* dwt = (double*)(buf + 2 * align256(big));
+  HIPCHK(hipMemcpyAsync(dwt, wt.data(), wt.size() * 8, hipMemcpyHostToDevice, st_));
+  for (size_t k = 0; k < wins.size(); k++) {
+    const size_t wb = (size_t)nx[k] * nz[k] * 4;
+    HIPCHK(hipMemcpyAsync(a, wins[k], wb, hipMemcpyHostToDevice, st_));
+    HIPCHK(rtm_launch_gauss_smooth(a, b, nz[k], nx[k], nsp, dwt, norm, st_));
+    HIPCHK(hipMemcpyAsync(wins[k], b, wb, hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+  }
+  pool_.put(buf);
+  return 0;
+}
+
+// the padded windows of every shot of a job, with medium_flag their layers
+// smoothed: on the GPU where the window allows, on the host otherwise
+int Engine::job_windows(const rtmh_params* pk, const float* vp0, const float* rho0, std::vector<ShotSetup>& setup)
+{
+  const int nsh = (int)setup.size();
+  std::vector<char> gpu(nsh, 0);
+  for (int k = 0; k < nsh; k++)
+    gpu[k] = pk[k].medium_flag && pml_smooth_on_gpu(setup[k].g.nxp, setup[k].g.nzp, pk[k].npml);
+  parallel_for(nsh, [&](int k) { shot_windows(pk[k], vp0, rho0, setup[k], !gpu[k]); });
+  std::vector<float*> w;
+  std::vector<int> nx, nz;
+  for (int k = 0; k < nsh; k++)
+    if (gpu[k]) {
+      w.push_back(setup[k].wvv.data()); w.push_back(setup[k].wrho.data());
+      for (int i = 0; i < 2; i++) { nx.push_back(setup[k].g.nxp); nz.push_back(setup[k].g.nzp); }
+    }
+  return w.empty() ? 0 : pml_smooth_windows(w, nx, nz, pk[0].npml);
 }
 
 // direct-wave mute (kernel.cu:96-173) as the last muted sample per receiver,
@@ -313,6 +421,8 @@ uint64_t Engine::job_fingerprint(const rtmh_params& P, const Model& M, const flo
   h = fnv1a(h, coef, 100 * 4);
   h = fnv1a(h, &opt.store_fmt, sizeof(opt.store_fmt));
   h = fnv1a(h, &opt.ic_extra, sizeof(opt.ic_extra));
+  h = fnv1a(h, &opt.snap_every, sizeof(opt.snap_every));
+  h = fnv1a(h, &opt.snap_fmt, sizeof(opt.snap_fmt));
   if (opt.shot_offsetmax) h = fnv1a(h, opt.shot_offsetmax, (size_t)nsh * 4);
   if (opt.shot_field) h = fnv1a(h, opt.shot_field, (size_t)nsh * 12);
   return h;
@@ -427,6 +537,7 @@ bool GraphKey::operator<(const GraphKey& o) const
   if (ic != o.ic) return ic < o.ic;
   if (nslots != o.nslots) return nslots < o.nslots;
   if (guard != o.guard) return guard < o.guard;
+  if (snap != o.snap) return snap < o.snap;
   if (shots != o.shots) return shots < o.shots;
   if (tiles != o.tiles) return tiles < o.tiles;
   return slots < o.slots;
@@ -483,7 +594,29 @@ static SmallLayout small_layout(const ShotSetup& s)
   return L;
 }
 
-size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic)
+SnapPlan snap_plan(const rtmh_params& P, int every, int fmt)
+{
+  SnapPlan sp;
+  if (every <= 0) return sp;
+  // past the wavelet (the Ricker's delay 1/fdom and its width): there the
+  // reference's reconstruction carries the one-step source-removal lag
+  const int t0 = (int)std::ceil(4.0 / ((double)P.fdom * (double)P.dt));
+  sp.every = every;
+  sp.fmt = fmt;
+  sp.k0 = std::max(1, (t0 + every - 1) / every);
+  const int kmax = (P.nt - 2) / every;
+  sp.n = std::max(0, kmax - sp.k0 + 1);
+  return sp;
+}
+
+static size_t snap_bytes(const ShotSetup& s, int nz, const SnapPlan* sp)
+{
+  if (!sp || !sp->on()) return 0;
+  const size_t e = sp->fmt == ST_F32 ? 4 : 2;
+  return align256((size_t)sp->n * 3 * nz * s.g.nxl * e) + align256((size_t)sp->n * 3 * 8);
+}
+
+size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic, const SnapPlan* snap)
 {
   const size_t pitch = pitch_of(s.g.nxp), nzp = s.g.nzp;
   const size_t F = align256((size_t)frows_of((int)nzp) * pitch * 4);
@@ -496,12 +629,13 @@ size_t Engine::slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic)
   b += align256((size_t)nt * 4 * 4 + 16 * 4);        // scales, 16-bit max-abs accumulators, guard word
   if (!s.wvv.empty()) b += 4 * align256((size_t)s.g.nxp * s.g.nzp * 4);   // smoothed windows: v^2, rho, 1/rho, rho v^2
   if (ic) b += align256((size_t)(5 + 2 * 61) * nzi * s.g.nxl * 4);         // other imaging conditions
+  b += snap_bytes(s, (int)nzi, snap);
   return b;
 }
 
 void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
                          std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                         const float* wav_dev, const ModelDev& model, bool ic)
+                         const float* wav_dev, const ModelDev& model, bool ic, const SnapPlan* snap)
 {
   memset(&d, 0, sizeof(d));
   d.dx = P.dx; d.dz = P.dz;
@@ -564,6 +698,14 @@ void Engine::layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt
     d.mpitch = g.nxp; d.mz0 = 0; d.mzn = g.nzp; d.mx0 = 0; d.mxs = 0; d.mxn = g.nxp;
   }
   if (ic) { d.ic = (float*)(base + o); o += align256((size_t)(5 + 2 * 61) * d.nzi * d.nxl * 4); }
+  if (snap && snap->on()) {
+    const size_t e = snap->fmt == ST_F32 ? 4 : 2;
+    d.snap = base + o;
+    o += align256((size_t)snap->n * 3 * d.nzi * d.nxl * e);
+    d.snap_scale = (float*)(base + o);
+    o += align256((size_t)snap->n * 3 * 8);
+    d.snap_every = snap->every; d.snap_k0 = snap->k0; d.nsnap = snap->n; d.snap_fmt = snap->fmt;
+  }
 }
 
 hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const Tile* dtiles, const int* dslots,
@@ -579,6 +721,7 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
     (void)rtm_launch_step(a, k.phase, k.tz, st_);
     if (k.save) (void)rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_);
     if (k.ic) (void)rtm_launch_ic(dshots, dslots, nslots, s, k.ic, st_);
+    if (k.snap) (void)rtm_launch_snap(dshots, dslots, nslots, k.phase, s, k.snap - 1, st_);
   }
   (void)rtm_launch_bump((ShotDev*)dshots, dslots, nslots, k.k, st_);
   // the chunk's last step, relative to the advanced time origin
@@ -614,6 +757,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
       HIPCHK(hipEventRecord(ev[1], st_));
       if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
       if (ic) HIPCHK(rtm_launch_ic(dshots, dslots, nslots, s, ic, st_));
+      if (snap_.on()) HIPCHK(rtm_launch_snap(dshots, dslots, nslots, phase, s, snap_.fmt, st_));
       HIPCHK(hipEventRecord(ev[2], st_));
       HIPCHK(hipEventSynchronize(ev[2]));
       float m0 = 0, m1 = 0;
@@ -627,7 +771,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   }
   if (chunk > 0 && nsteps >= chunk) {
     GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, ic, nslots, guard_every > 0 ? 1 : 0,
-                 dshots, dtiles, dslots};
+                 snap_.on() ? 1 + snap_.fmt : 0, dshots, dtiles, dslots};
     hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
     if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
     const int nch = nsteps / chunk;
@@ -641,6 +785,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     HIPCHK(rtm_launch_step(a, phase, tz, st_));
     if (save) HIPCHK(rtm_launch_save(dshots, dslots, nslots, s, store_fmt, st_));
     if (ic) HIPCHK(rtm_launch_ic(dshots, dslots, nslots, s, ic, st_));
+    if (snap_.on()) HIPCHK(rtm_launch_snap(dshots, dslots, nslots, phase, s, snap_.fmt, st_));
     if (guard_every > 0 && ((s + 1) % guard_every == 0 || s == rest - 1))
       HIPCHK(rtm_launch_guard(dshots, dslots, nslots, s, d_flag_, st_));
   }
@@ -847,6 +992,10 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   std::string why;
   if (!params_ok(P, why)) { set_error(why); return -1; }
   if (opt.store_fmt < 0 || opt.store_fmt > 2) { set_error("store_fmt must be 0, 1 or 2"); return -1; }
+  if (opt.snap_every < 0 || opt.snap_fmt < 0 || opt.snap_fmt > 2) {
+    set_error("snap_every must be >= 0 and snap_fmt 0, 1 or 2");
+    return -1;
+  }
   if (opt.tile_z != 0 && opt.tile_z != 16 && opt.tile_z != 32 && opt.tile_z != 64) {
     set_error("tile_z must be 0 (autotuned), 16, 32 or 64");
     return -1;
@@ -885,7 +1034,12 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
   // instead of the resident model
   const bool smooth = P.medium_flag != 0 || P.ismth > 0;
   const bool icx = opt.ic_extra != 0;
-  if (smooth) parallel_for(nsh, [&](int k) { shot_windows(pk[k], vp0, M->rho0.data(), setup[k]); });
+  snap_ = snap_plan(P, opt.snap_every, opt.snap_fmt);
+  const SnapPlan* snp = snap_.on() ? &snap_ : nullptr;
+  if (smooth) {
+    const int wr = job_windows(pk.data(), vp0, M->rho0.data(), setup);
+    if (wr != 0) return wr;
+  }
   int nwin = 50;
   for (int k = 0; k < nsh; k++) {
     shot_setup(pk[k], vp0, M->colmax.data(), setup[k], smooth ? setup[k].wvv.data() : nullptr);
@@ -971,7 +1125,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     int k1 = k0;
     double used = 0;
     while (k1 < nsh && k1 - k0 < bmax) {
-      const double b = (double)slot_bytes(setup[k1], nt, opt.store_fmt, icx);
+      const double b = (double)slot_bytes(setup[k1], nt, opt.store_fmt, icx, snp);
       if (k1 > k0 && used + b > budget) break;
       used += b;
       k1++;
@@ -983,9 +1137,9 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     std::vector<ShotDev> ds(nb);
     std::vector<std::pair<void*, std::vector<char>>> uploads;
     for (int j = 0; j < nb; j++) {
-      mem[j] = pool_.get(slot_bytes(setup[k0 + j], nt, opt.store_fmt, icx));
+      mem[j] = pool_.get(slot_bytes(setup[k0 + j], nt, opt.store_fmt, icx, snp));
       if (!mem[j]) { set_error("out of device memory (shot slots)"); rc = -3; break; }
-      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, mdev, icx);
+      layout_slot(P, setup[k0 + j], opt.store_fmt, (char*)mem[j], ds[j], uploads, d_coef, d_wav, mdev, icx, snp);
       ds[j].record = (P.mode == 0) ? 1 : 0;
     }
     if (rc != 0) { for (void* p : mem) pool_.put(p); break; }
@@ -1033,6 +1187,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
           HIPCHK(rtm_launch_step(a, phase, cands[c], st_));
           if (phase == PH_FWD) HIPCHK(rtm_launch_save(d_ds_, d_slots_, nb, a.step, opt.store_fmt, st_));
           else if (ic_cells_) HIPCHK(rtm_launch_ic(d_ds_, d_slots_, nb, a.step, ic_cells_, st_));
+          if (snap_.on()) HIPCHK(rtm_launch_snap(d_ds_, d_slots_, nb, phase, a.step, snap_.fmt, st_));
         }
         HIPCHK(hipEventRecord(te[c + 1], st_));
       }
@@ -1140,27 +1295,41 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       for (int j = 0; j < nb; j++) ic_cells_ = std::max(ic_cells_, nz * setup[k0 + j].g.nxl);
 
     // velsmooth1d of each shot's window before its reverse pass (rtm_model.cu:
-    // 402-405): reconstruction and receiver propagation use the smoothed model
+    // 402-405): reconstruction and receiver propagation use the smoothed
+    // model.  On the GPU, out of place through the slot's factor arrays (they
+    // are rebuilt from the smoothed windows right after)
     if (P.ismth > 0) {
-      parallel_for(nb, [&](int j) {
-        ShotSetup& s = setup[k0 + j];
-        rtmh_gauss_smooth(s.wvv.data(), s.g.nzp, s.g.nxp, P.ismth);
-        rtmh_gauss_smooth(s.wrho.data(), s.g.nzp, s.g.nxp, P.ismth);
-      });
+      std::vector<double> wt;
+      double norm = 0;
+      gauss_weights(P.ismth, wt, norm);
+      double* d_wt = (double*)pool_.get(wt.size() * 8);
+      if (!d_wt) { set_error("out of device memory (smoothing weights)"); for (void* p : mem) pool_.put(p); rc = -3; break; }
+      HIPCHK(hipMemcpyAsync(d_wt, wt.data(), wt.size() * 8, hipMemcpyHostToDevice, st_));
       for (int j = 0; j < nb; j++) {
         ShotSetup& s = setup[k0 + j];
-        const size_t wb = s.wvv.size() * 4;
-        HIPCHK(hipMemcpyAsync((void*)ds[j].vv, s.wvv.data(), wb, hipMemcpyHostToDevice, st_));
-        HIPCHK(hipMemcpyAsync((void*)ds[j].rho, s.wrho.data(), wb, hipMemcpyHostToDevice, st_));
-        HIPCHK(rtm_launch_model_prep(ds[j].rho, ds[j].vv, (float*)ds[j].mib, (float*)ds[j].mkk,
-                                     (long long)s.g.nxp * s.g.nzp, st_));
+        const int nzw = s.g.nzp, nxw = s.g.nxp;
+        const size_t wb = (size_t)nzw * nxw * 4;
+        float* vv = (float*)ds[j].vv;
+        float* rh = (float*)ds[j].rho;
+        float* t1 = (float*)ds[j].mib;
+        float* t2 = (float*)ds[j].mkk;
+        HIPCHK(rtm_launch_gauss_smooth(vv, t1, nzw, nxw, P.ismth, d_wt, norm, st_));
+        HIPCHK(rtm_launch_gauss_smooth(rh, t2, nzw, nxw, P.ismth, d_wt, norm, st_));
+        HIPCHK(hipMemcpyAsync(vv, t1, wb, hipMemcpyDeviceToDevice, st_));
+        HIPCHK(hipMemcpyAsync(rh, t2, wb, hipMemcpyDeviceToDevice, st_));
+        HIPCHK(rtm_launch_model_prep(rh, vv, t1, t2, (long long)nzw * nxw, st_));
         if (opt.shot_field) {
           // a recorded gather is muted after the smoothing (rtm_real.cu:370-403),
           // so with the smoothed velocities
+          HIPCHK(hipMemcpyAsync(s.wvv.data(), vv, wb, hipMemcpyDeviceToHost, st_));
+          HIPCHK(hipStreamSynchronize(st_));
           mute_cuts(pk[k0 + j], vp0, s, s.wvv.data());
           HIPCHK(hipMemcpyAsync((void*)ds[j].cut, s.cut.data(), s.cut.size() * 4, hipMemcpyHostToDevice, st_));
+          HIPCHK(hipStreamSynchronize(st_));
         }
       }
+      HIPCHK(hipStreamSynchronize(st_));
+      pool_.put(d_wt);
     }
 
     // ---- reverse pass: reconstruction + receiver propagation + imaging

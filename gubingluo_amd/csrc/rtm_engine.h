@@ -43,7 +43,12 @@ void shot_setup(const rtmh_params& P, const float* vp0, const float* colmax_vv, 
                 const float* wvv = nullptr);
 // the padded windows of a shot (extractvel1 / extractrho1 + extendvel1) and,
 // with medium_flag, their absorbing layers smoothed (pmlvelsmooth1d)
-void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, ShotSetup& s);
+void shot_windows(const rtmh_params& P, const float* vp0, const float* rho0, ShotSetup& s, bool smooth_here = true);
+// windows the GPU layer smoother takes (one wave per layer side, mirrors of
+// interior cells only, edge copies in LDS)
+bool pml_smooth_on_gpu(int nx, int nz, int npml);
+// velsmooth1d weights of half-width nsp (row-major) and their sum
+void gauss_weights(int nsp, std::vector<double>& wt, double& norm);
 // Ricker of kernel.cu:13-22
 void ricker(std::vector<float>& w, int nt, float fdom, float dt);
 // taper row (kernel.cu:1591-1604)
@@ -71,6 +76,14 @@ struct ModelDev {
   int pitch;
 };
 
+// decimated snapshots of a job: every `every` steps from k0*every while
+// t >= 4/fdom (the wavelet has passed), n of them up to nt-2; fmt the store format
+struct SnapPlan {
+  int every = 0, k0 = 0, n = 0, fmt = 0;
+  bool on() const { return every > 0 && n > 0; }
+};
+SnapPlan snap_plan(const rtmh_params& P, int every, int fmt);
+
 struct SlotMem {
   void* base = nullptr;
   size_t bytes = 0;
@@ -86,6 +99,7 @@ struct GraphKey {
   int ic;                                  // cells of the other imaging conditions after every reverse step (0: off)
   int nslots;                              // slots of the batch (k_save / k_ic grids, k_bump count)
   int guard;                               // wavefield guard after the chunk
+  int snap;                                // snapshot launches after every step: 0 off, else 1 + format
   const void* shots;
   const void* tiles;
   const void* slots;
@@ -130,6 +144,12 @@ class Engine {
           rtmh_outputs* out, rtmh_stats* stats);
   int run_model(int model_id, const rtmh_params& P, const float* coef, const rtmh_options& opt, rtmh_outputs* out,
                 rtmh_stats* stats);
+  // pmlvelsmooth1d of host windows (nz[k] rows of nx[k]) on the GPU, in place
+  int pml_smooth_windows(std::vector<float*>& wins, const std::vector<int>& nx, const std::vector<int>& nz, int npml);
+  // every shot's padded windows (and, with medium_flag, their smoothed layers)
+  int job_windows(const rtmh_params* pk, const float* vp0, const float* rho0, std::vector<ShotSetup>& setup);
+  // velsmooth1d of host windows on the GPU, in place
+  int gauss_smooth_windows(std::vector<float*>& wins, const std::vector<int>& nx, const std::vector<int>& nz, int nsp);
   int fd_steps(const rtmh_params& P, int ishot, const float* vp0, const float* rho0, const float* coef, int nsteps,
                int it0, int with_source, float* pf, float* vxf, float* vzf, float* p1f, float* p2f);
 
@@ -137,13 +157,15 @@ class Engine {
   int copy_probe(size_t bytes, int reps, double* gbps);
 
   // bytes one slot of this geometry needs on the device
-  static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic = false);
+  static size_t slot_bytes(const ShotSetup& s, int nt, int store_fmt, bool ic = false,
+                           const SnapPlan* snap = nullptr);
   // build the device descriptor of one slot inside `base` (uploads: host
   // images of its small arrays to copy to the device)
   // (the slot reads `model`, or its own smoothed windows when s carries them)
   static void layout_slot(const rtmh_params& P, const ShotSetup& s, int store_fmt, char* base, ShotDev& d,
                           std::vector<std::pair<void*, std::vector<char>>>& uploads, const float* coef_dev,
-                          const float* wav_dev, const ModelDev& model, bool ic = false);
+                          const float* wav_dev, const ModelDev& model, bool ic = false,
+                          const SnapPlan* snap = nullptr);
   // offset of the taper row inside a slot block
   static size_t taper_offset(const ShotSetup& s);
   std::mutex& models_mutex() { return models_mu_; }
@@ -173,6 +195,7 @@ class Engine {
   struct StackShotHost* d_stk_ = nullptr;
   unsigned int* d_flag_ = nullptr;      // wavefield guard of the running pass (largest |p| bits)
   int ic_cells_ = 0;                    // the other imaging conditions of the running batch: interior cells (0: off)
+  SnapPlan snap_;                       // snapshots of the running job
   int ensure_cb(int nslots, size_t ntiles);
 
   // guard_every > 0: the wavefield guard after every guard_every steps (plain

@@ -1041,6 +1041,308 @@ __global__ __launch_bounds__(NTHR) void k_model_prep(const float* __restrict__ r
   }
 }
 
+// ------------------------------------------------------------ snapshots --
+// Decimated snapshots of the source wavefield.  The reverse pass runs the
+// leapfrog backwards, velocities first: after reverse step k it holds the
+// pressure of forward step k and the velocities of forward step k+1 (its
+// strips are read the same way, rtm_model.cu:424-434).  Snapshot k (k =
+// snap_every * (snap_k0 + q)) therefore keeps the interior of p as forward
+// step k leaves it and of vx, vz as forward step k+1 leaves them, and the
+// reverse pass writes them back after its step k, so that the steps below k
+// start from the stored field.  Block (q, f), f = 0 p, 1 vx, 2 vz: nzi x nxl
+// codes; 16-bit forms with one max-abs scale per block (the strips' rule:
+// max(maxabs / 32768, FLT_MIN), 1 for an all-zero block).
+__device__ __forceinline__ int snap_index(const ShotDev& s, int k)
+{
+  if (!s.snap || k < 0 || k % s.snap_every != 0) return -1;
+  const int q = k / s.snap_every - s.snap_k0;
+  return (q >= 0 && q < s.nsnap) ? q : -1;
+}
+// the snapshot a forward step `it` contributes field f to: p of step k = it,
+// velocities of step it = k + 1
+__device__ __forceinline__ int snap_of_fwd(const ShotDev& s, int it, int f)
+{
+  return snap_index(s, f == 0 ? it : it - 1);
+}
+__device__ __forceinline__ const float* snap_field(const ShotDev& s, int f, bool lev)
+{
+  return f == 0 ? (lev ? s.sp[1] : s.sp[0]) : (f == 1 ? (lev ? s.svx[1] : s.svx[0]) : (lev ? s.svz[1] : s.svz[0]));
+}
+__device__ __forceinline__ unsigned int* snap_acc(const ShotDev& s) { return (unsigned int*)(s.snap_scale + 3 * s.nsnap); }
+
+// forward, pass 1 (16-bit forms): max-abs of the workgroup's share of block
+// (q, f) after forward step `step` (its new level).  Grid (3 slots, GY).
+__global__ __launch_bounds__(NTHR) void k_snap_max(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                    int step)
+{
+  __shared__ float red[NTHR / 64];
+  const ShotDev& s = shot_of(shots, slots[blockIdx.x / 3]);
+  const int it = s.it_off + s.it_dir * step, f = blockIdx.x % 3;
+  const int q = snap_of_fwd(s, it, f);
+  if (q < 0) return;
+  const float* fld = snap_field(s, f, (it + 1) & 1);
+  const int nxl = s.nxl, n = s.nzi * nxl;
+  float m = 0.0f;
+  for (int e = blockIdx.y * NTHR + threadIdx.x; e < n; e += gridDim.y * NTHR) {
+    const int r = e / nxl, c = e - r * nxl;
+    m = fmaxf(m, fabsf(GCF(fld)[(long long)(s.npml + r) * s.pitch + s.npml + c]));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NTHR / 64; w++) m = fmaxf(m, red[w]);
+    atomicMax(snap_acc(s) + q * 3 + f, __float_as_uint(m));
+  }
+}
+
+// forward, pass 2: block (q, f) stored as codes (fp32: a copy) with its scale
+__global__ __launch_bounds__(NTHR) void k_snap_q(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                  int step)
+{
+  const ShotDev& s = shot_of(shots, slots[blockIdx.x / 3]);
+  const int it = s.it_off + s.it_dir * step, f = blockIdx.x % 3;
+  const int q = snap_of_fwd(s, it, f);
+  if (q < 0) return;
+  const float* fld = snap_field(s, f, (it + 1) & 1);
+  const int nxl = s.nxl, n = s.nzi * nxl;
+  const long long base = (long long)(q * 3 + f) * n;
+  float sc = 1.0f;
+  if (s.snap_fmt != ST_F32) {
+    const float m = __uint_as_float(snap_acc(s)[q * 3 + f]);
+    sc = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
+  }
+  if (blockIdx.y == 0 && threadIdx.x == 0) s.snap_scale[q * 3 + f] = sc;
+  for (int e = blockIdx.y * NTHR + threadIdx.x; e < n; e += gridDim.y * NTHR) {
+    const int r = e / nxl, c = e - r * nxl;
+    const float v = GCF(fld)[(long long)(s.npml + r) * s.pitch + s.npml + c];
+    if (s.snap_fmt == ST_F32) {
+      GF(s.snap)[base + e] = v;
+    } else if (s.snap_fmt == ST_F16) {
+      ((__half*)s.snap)[base + e] = __float2half_rn(v / sc);
+    } else {
+      unsigned int w = __float_as_uint(v / sc);       // bf16, round to nearest even
+      w += 0x7FFFu + ((w >> 16) & 1u);
+      ((unsigned short*)s.snap)[base + e] = (unsigned short)(w >> 16);
+    }
+  }
+}
+
+// reverse: after reverse step `step` = k (its new level (k+1)&1), snapshot
+// k replaces the reconstructed interior cells: p on the
+// interior, vx on its columns npml..nxp-npml-2, vz on its rows
+// npml..nzp-npml-2 (the cells the reconstruction computes; the strips come
+// from the boundary store as on every step)
+__global__ __launch_bounds__(NTHR) void k_snap_load(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
+                                                     int step)
+{
+  const ShotDev& s = shot_of(shots, slots[blockIdx.x / 3]);
+  const int it = s.it_off + s.it_dir * step;
+  const int q = snap_index(s, it);
+  if (q < 0) return;
+  const int f = blockIdx.x % 3;
+  float* fld = (float*)snap_field(s, f, (it + 1) & 1);
+  const int nxl = s.nxl, n = s.nzi * nxl;
+  const long long base = (long long)(q * 3 + f) * n;
+  const float sc = s.snap_scale[q * 3 + f];
+  for (int e = blockIdx.y * NTHR + threadIdx.x; e < n; e += gridDim.y * NTHR) {
+    const int r = e / nxl, c = e - r * nxl;
+    if ((f == 1 && c == nxl - 1) || (f == 2 && r == s.nzi - 1)) continue;
+    float v;
+    if (s.snap_fmt == ST_F32) v = GCF(s.snap)[base + e];
+    else if (s.snap_fmt == ST_F16) v = __half2float(((const __half*)s.snap)[base + e]) * sc;
+    else v = __uint_as_float((unsigned int)((const unsigned short*)s.snap)[base + e] << 16) * sc;
+    GF(fld)[(long long)(s.npml + r) * s.pitch + s.npml + c] = v;
+  }
+}
+
+// ------------------------------------------------------- model smoothing --
+// velsmooth1d (Velocity_function.cpp:199-313) of an nz x nx window: every
+// output cell is the sum over the (2 nsp + 1)^2 window around it of
+// weight x value, in double, rows then columns as the reference loops, on
+// the window extended by its edge values (a clamp), rounded to float and
+// then divided (in double) by the sum of the weights.  The weights are
+// tabulated on the host with the reference's expression (wt, row-major),
+// so every term is the reference's.
+__global__ __launch_bounds__(NTHR) void k_gauss_smooth(const float* __restrict__ in, float* __restrict__ out, int nz,
+                                                        int nx, int nsp, const double* __restrict__ wt, double norm)
+{
+  const long long e = (long long)blockIdx.x * NTHR + threadIdx.x;
+  if (e >= (long long)nz * nx) return;
+  const int iz = (int)(e / nx), ix = (int)(e - (long long)iz * nx);
+  const int k = 2 * nsp + 1;
+  double acc = 0.0;
+  for (int i = 0; i < k; i++) {
+    const float* row = in + (size_t)imin(imax(iz + i - nsp, 0), nz - 1) * nx;
+    const double* w = wt + (size_t)i * k;
+    for (int j = 0; j < k; j++) acc += w[j] * (double)row[imin(imax(ix + j - nsp, 0), nx - 1)];
+  }
+  out[e] = (float)((double)(float)acc / norm);
+}
+
+// pmlvelsmooth1d (Velocity_function.cpp:9-103) of one nz x nx window per
+// workgroup (wins[blockIdx.x]: the window w, in place, and a scratch window
+// v of the same size).  The reference mirrors the interior into the
+// absorbing layer, then relaxes the layer in place -- each cell becomes the
+// average of its four neighbours, taking those already updated in the sweep
+// as updated -- until a sweep changes no cell by more than 5e-4 (relative)
+// or 1001 sweeps ran.  The in-place order is kept exactly:
+//   * left and right layers (columns 1..npml-1, rows 1..nz-2 in order, the
+//     reference's ix-outer / iz-inner loop): lane t of one wave owns column
+//     t+1 and runs one row behind lane t-1, which hands it the new value of
+//     its left neighbour by a lane shuffle; the cell above is the lane's own
+//     previous result; the cells right and below are not updated yet in this
+//     sweep and come from memory.  Left and right are independent (two waves).
+//   * top and bottom layers (rows 1..npml-1, columns 1..nx-2) the same with
+//     lanes over rows: the new value above from the lane before, the one on
+//     the left the lane's own.
+//   * the edge columns and rows: four independent recurrences, one lane
+//     each, on copies staged in LDS; then the four corners.
+// Each sweep ends with the largest relative change over the layer ring (the
+// rest of the window never changes) and copies the ring back into w.  Sums
+// are formed in the reference's operand order, as floats, scaled by the
+// double literal 0.25.  Needs npml - 1 <= 64, nx, nz >= 3 npml (the
+// mirrors read only interior cells) and nx, nz <= RTM_SMOOTH_MAXN.
+#define RTM_SMOOTH_MAXN 4096
+struct SmoothWin { float* w; float* v; int nx, nz, npml, _pad; };
+
+__device__ __forceinline__ float avg4(float a, float b, float c, float d)
+{
+  return (float)(0.25 * (double)(a + b + c + d));
+}
+
+// one side of the layer, one wave: lanes over `lanes` lines starting at
+// line0, positions 1..len along each line.  LR: lines are columns and
+// positions rows (sum E + W + S + N with W from the lane before, N the
+// lane's own); otherwise lines are rows and positions columns (W the lane's
+// own, N from the lane before).
+template <bool LR>
+__device__ void smooth_side(float* __restrict__ v, int nx, int line0, int lanes, int len)
+{
+  const int t = threadIdx.x & 63;
+  float prev = 0.0f;                                 // the lane's new value at the previous position
+  for (int k = 0; k < len + lanes - 1; k++) {
+    const float before = __shfl_up(prev, 1, 64);     // lane t-1's new value at this lane's position
+    const int j = 1 + k - t;
+    if (t < lanes && j >= 1 && j <= len) {
+      const int ln = line0 + t;
+      const size_t c = LR ? (size_t)j * nx + ln : (size_t)ln * nx + j;
+      const float E = v[c + 1], S = v[c + nx];
+      float W, N;
+      if (LR) { W = t > 0 ? before : v[c - 1]; N = j > 1 ? prev : v[c - nx]; }
+      else { W = j > 1 ? prev : v[c - 1]; N = t > 0 ? before : v[c - nx]; }
+      prev = avg4(E, W, S, N);
+      v[c] = prev;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NTHR) void k_pml_smooth(const SmoothWin* __restrict__ wins)
+{
+  __shared__ float ed[8 * RTM_SMOOTH_MAXN];          // col 0, 1, nx-1, nx-2 (nz each), row 0, 1, nz-1, nz-2 (nx each)
+  __shared__ float red[NTHR / 64];
+  __shared__ float err_s;
+  const SmoothWin sw = wins[blockIdx.x];
+  float* __restrict__ w = sw.w;
+  float* __restrict__ v = sw.v;
+  const int nx = sw.nx, nz = sw.nz, npml = sw.npml, tid = threadIdx.x, wave = tid >> 6;
+  // mirror the interior into the layer: columns (rows 1..nz-2), then rows
+  // (columns 1..nx-2, which include the mirrored columns)
+  for (int e = tid; e < npml * (nz - 2); e += NTHR) {
+    const int ix = e % npml, iz = 1 + e / npml;
+    w[(size_t)iz * nx + ix] = w[(size_t)iz * nx + 2 * npml - ix - 1];
+    w[(size_t)iz * nx + nx - npml + ix] = w[(size_t)iz * nx + nx - npml - ix - 1];
+  }
+  __syncthreads();
+  for (int e = tid; e < npml * (nx - 2); e += NTHR) {
+    const int iz = e / (nx - 2), ix = 1 + e % (nx - 2);
+    w[(size_t)iz * nx + ix] = w[(size_t)(2 * npml - iz - 1) * nx + ix];
+    w[(size_t)(nz - npml + iz) * nx + ix] = w[(size_t)(nz - npml - iz - 1) * nx + ix];
+  }
+  __syncthreads();
+  for (long long e = tid; e < (long long)nx * nz; e += NTHR) v[e] = w[e];
+  __syncthreads();
+  float* c0 = ed;
+  float* c1 = ed + nz;
+  float* cN = ed + 2 * nz;
+  float* cM = ed + 3 * nz;
+  float* r0 = ed + 4 * nz;
+  float* r1 = r0 + nx;
+  float* rN = r0 + 2 * nx;
+  float* rM = r0 + 3 * nx;
+  const int ring = 2 * npml * nx + 2 * npml * (nz - 2 * npml);
+  float err = 1.0f;
+  for (int num = 0; num <= 1000 && (double)err > 0.0005; num++) {
+    // left and right layers
+    if (wave == 0) smooth_side<true>(v, nx, 1, npml - 1, nz - 2);
+    else if (wave == 1) smooth_side<true>(v, nx, nx - npml, npml - 1, nz - 2);
+    __syncthreads();
+    // top and bottom layers
+    if (wave == 0) smooth_side<false>(v, nx, 1, npml - 1, nx - 2);
+    else if (wave == 1) smooth_side<false>(v, nx, nz - npml, npml - 1, nx - 2);
+    __syncthreads();
+    // edges: stage, four recurrences, corners, write back
+    for (int i = tid; i < nz; i += NTHR) {
+      c0[i] = v[(size_t)i * nx]; c1[i] = v[(size_t)i * nx + 1];
+      cN[i] = v[(size_t)i * nx + nx - 1]; cM[i] = v[(size_t)i * nx + nx - 2];
+    }
+    for (int i = tid; i < nx; i += NTHR) {
+      r0[i] = v[i]; r1[i] = v[nx + i];
+      rN[i] = v[(size_t)(nz - 1) * nx + i]; rM[i] = v[(size_t)(nz - 2) * nx + i];
+    }
+    __syncthreads();
+    if (tid == 0) {                                  // column 0: C + E + S + N
+      float up = c0[0];
+      for (int iz = 1; iz < nz - 1; iz++) { up = avg4(c0[iz], c1[iz], c0[iz + 1], up); c0[iz] = up; }
+    } else if (tid == 64) {                          // column nx-1
+      float up = cN[0];
+      for (int iz = 1; iz < nz - 1; iz++) { up = avg4(cN[iz], cM[iz], cN[iz + 1], up); cN[iz] = up; }
+    } else if (tid == 128) {                         // row 0: C + S + W + E
+      float lf = r0[0];
+      for (int ix = 1; ix < nx - 1; ix++) { lf = avg4(r0[ix], r1[ix], lf, r0[ix + 1]); r0[ix] = lf; }
+    } else if (tid == 192) {                         // row nz-1
+      float lf = rN[0];
+      for (int ix = 1; ix < nx - 1; ix++) { lf = avg4(rN[ix], rM[ix], lf, rN[ix + 1]); rN[ix] = lf; }
+    }
+    __syncthreads();
+    if (tid == 0) {                                  // corners (none reads another)
+      const float a = avg4(c0[0], c0[1], r0[1], c1[1]);
+      const float b = avg4(c0[nz - 1], c0[nz - 2], rN[1], c1[nz - 2]);
+      const float c = avg4(cN[0], r0[nx - 2], cN[1], cM[1]);
+      const float d = avg4(cN[nz - 1], cN[nz - 2], rN[nx - 2], cM[nz - 2]);
+      c0[0] = a; r0[0] = a; c0[nz - 1] = b; rN[0] = b; cN[0] = c; r0[nx - 1] = c; cN[nz - 1] = d; rN[nx - 1] = d;
+    }
+    __syncthreads();
+    for (int i = tid; i < nz; i += NTHR) { v[(size_t)i * nx] = c0[i]; v[(size_t)i * nx + nx - 1] = cN[i]; }
+    for (int i = tid; i < nx; i += NTHR) { v[i] = r0[i]; v[(size_t)(nz - 1) * nx + i] = rN[i]; }
+    __syncthreads();
+    // largest relative change over the ring, then the ring back into w
+    float m = -1.0e+5f;
+    for (int e = tid; e < ring; e += NTHR) {
+      int iz, ix;
+      if (e < 2 * npml * nx) { const int r = e / nx; iz = r < npml ? r : nz - 2 * npml + r; ix = e - r * nx; }
+      else { const int f = e - 2 * npml * nx, r = f / (2 * npml), c = f - r * 2 * npml;
+             iz = npml + r; ix = c < npml ? c : nx - 2 * npml + c; }
+      const size_t o = (size_t)iz * nx + ix;
+      const float a = w[o], b = v[o];
+      m = fmaxf(m, fabsf(a - b) / fabsf(a));
+      w[o] = b;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((tid & 63) == 0) red[wave] = m;
+    __syncthreads();
+    if (tid == 0) {
+      float x = red[0];
+      for (int i = 1; i < NTHR / 64; i++) x = fmaxf(x, red[i]);
+      err_s = x;
+    }
+    __syncthreads();
+    err = err_s;
+  }
+}
+
 // Phase start of one slot.  FWD: zero both levels of both wavefields and the
 // PML pair.  REV: zero both levels of the source velocities (rtm_model.cu:
 // 366-372), the receiver set, the PML pair and the imaging accumulators.
@@ -1060,6 +1362,8 @@ __global__ __launch_bounds__(NTHR) void k_phase_init(const ShotDev* __restrict__
   if (phase == PH_FWD) {
     if (s.store_fmt != ST_F32 && g0 < 8) save_acc(s)[g0] = 0u;   // 16-bit max-abs accumulators
     if (g0 == 8) save_acc(s)[8] = 0u;                              // wavefield guard word
+    if (s.snap)
+      for (long long e = g0; e < 3LL * s.nsnap; e += gs) snap_acc(s)[e] = 0u;   // snapshot max-abs words
   } else {
     for (long long e = g0; e < ni; e += gs) { s.g1[e] = 0.f; s.illum[e] = 0.f; }
     if (s.ic)
@@ -1404,6 +1708,36 @@ hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, l
 hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st)
 {
   hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, a, n);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_snap(const ShotDev* shots, const int* slots, int nslots, int phase, int step, int snap_fmt,
+                           hipStream_t st)
+{
+  if (nslots <= 0) return hipSuccess;
+  const dim3 g(3 * nslots, 16), b(NTHR);
+  if (phase == PH_FWD) {
+    if (snap_fmt != ST_F32) hipLaunchKernelGGL(k_snap_max, g, b, 0, st, shots, slots, step);
+    hipLaunchKernelGGL(k_snap_q, g, b, 0, st, shots, slots, step);
+  } else {
+    hipLaunchKernelGGL(k_snap_load, g, b, 0, st, shots, slots, step);
+  }
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_gauss_smooth(const float* in, float* out, int nz, int nx, int nsp, const double* wt, double norm,
+                                   hipStream_t st)
+{
+  const long long n = (long long)nz * nx;
+  hipLaunchKernelGGL(k_gauss_smooth, dim3((unsigned)((n + NTHR - 1) / NTHR)), dim3(NTHR), 0, st, in, out, nz, nx, nsp,
+                     wt, norm);
+  return hipGetLastError();
+}
+
+hipError_t rtm_launch_pml_smooth(const void* wins, int n, hipStream_t st)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pml_smooth, dim3(n), dim3(NTHR), 0, st, (const SmoothWin*)wins);
   return hipGetLastError();
 }
 

@@ -28,6 +28,18 @@ hipError_t rtm_launch_laplace(const float* im, float* out, int nx, int nz, int d
 hipError_t rtm_launch_norm_multi(const float* sg, const float* ss, float* out, long long n, hipStream_t st);
 hipError_t rtm_launch_zero_f64(double* a, long long n, hipStream_t st);
 hipError_t rtm_launch_bump(ShotDev* shots, const int* slots, int nslots, int k, hipStream_t st);
+// pmlvelsmooth1d of n windows in place, one workgroup each (wins: device
+// array of SmoothWinHost)
+struct SmoothWinHost { float* w; float* v; int nx, nz, npml, _pad; };
+hipError_t rtm_launch_pml_smooth(const void* wins, int n, hipStream_t st);
+// velsmooth1d of an nz x nx window (out-of-place): weights wt [(2nsp+1)^2]
+// (device, double, the host's table) and their sum `norm`
+hipError_t rtm_launch_gauss_smooth(const float* in, float* out, int nz, int nx, int nsp, const double* wt, double norm,
+                                   hipStream_t st);
+// decimated snapshots after step `step` of the listed slots: FWD saves the
+// snapshot of time it+1 (if any), REV restores the one of time it (if any)
+hipError_t rtm_launch_snap(const ShotDev* shots, const int* slots, int nslots, int phase, int step, int snap_fmt,
+                           hipStream_t st);
 // wavefield guard of the listed slots after step `step` (largest |p| as float
 // bits into each slot's guard word and into *flag; flag may be null)
 hipError_t rtm_launch_guard(const ShotDev* shots, const int* slots, int nslots, int step, unsigned int* flag,

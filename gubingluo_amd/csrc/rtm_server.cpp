@@ -48,6 +48,8 @@ struct SrvRequest {
   size_t dev_bytes = 0;
   float *d_coef = nullptr, *d_wav = nullptr, *d_taper = nullptr;
   StackShotHost* d_stk = nullptr;
+  double* d_wt = nullptr;                  // velsmooth1d weights (ismth > 0)
+  double wt_norm = 0;
   double *d_i1 = nullptr, *d_i2 = nullptr, *d_il = nullptr;
   float *d_f1 = nullptr, *d_f2 = nullptr, *d_f3 = nullptr, *d_lap = nullptr, *d_nrm = nullptr;
   std::vector<size_t> taper_off;
@@ -105,6 +107,7 @@ class Server {
   DevPool pool_;
   std::thread th_;
   std::mutex mu_;
+  std::mutex smooth_mu_;                   // submitters preparing smoothed windows on the engine
   std::condition_variable cv_, cv_done_;
   std::deque<int> queue_;
   std::map<int, std::unique_ptr<SrvRequest>> reqs_;
@@ -223,10 +226,6 @@ int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* i
   std::string why;
   if (params_check(P, why) != 0) { set_error(why); return -1; }
   if (P.mode != 0) { set_error("served requests model their data (mode 0)"); return -1; }
-  if (P.medium_flag != 0 || P.ismth != 0) {
-    set_error("model smoothing (medium_flag, ismth) is not served; run the job with rtmh_run_model");
-    return -1;
-  }
   const Engine::Model* M = eng_->model(model_id);
   if (!M || M->nx != P.nx || M->nz != P.nz) { set_error("no such model or size mismatch"); return -1; }
   auto r = std::make_unique<SrvRequest>();
@@ -237,10 +236,20 @@ int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* i
   const int nsh = (int)r->shots.size();
   if (nsh == 0) { set_error("no shots"); return -1; }
   r->setup.resize(nsh);
+  for (int k = 0; k < nsh; k++)
+    if (!shot_geometry(P, r->shots[k], r->setup[k])) { set_error("shot window outside the model"); return -2; }
+  // model smoothing: each shot's padded windows (pmlvelsmooth1d on the GPU
+  // with medium_flag), read by its slot instead of the resident model
+  const bool smooth = P.medium_flag != 0 || P.ismth > 0;
+  if (smooth) {
+    std::vector<rtmh_params> pk(nsh, P);
+    std::lock_guard<std::mutex> lk(smooth_mu_);
+    const int rc = eng_->job_windows(pk.data(), M->vp0.data(), M->rho0.data(), r->setup);
+    if (rc) return rc;
+  }
   int nwin = 50;
   for (int k = 0; k < nsh; k++) {
-    if (!shot_geometry(P, r->shots[k], r->setup[k])) { set_error("shot window outside the model"); return -2; }
-    shot_setup(P, M->vp0.data(), M->colmax.data(), r->setup[k]);
+    shot_setup(P, M->vp0.data(), M->colmax.data(), r->setup[k], smooth ? r->setup[k].wvv.data() : nullptr);
     while (2 * nwin >= r->setup[k].g.nxl) nwin -= 3;
     r->setup[k].nwin = nwin;
     taper_row(r->setup[k].taper, r->setup[k].g.nxl, nwin);
@@ -283,7 +292,8 @@ size_t Server::request_bytes(const SrvRequest& r)
   const size_t nm = (size_t)r.P.nx * r.P.nz, nsh = r.shots.size();
   size_t o = a256(400) + a256((size_t)r.P.nt * 4);
   for (size_t k = 0; k < nsh; k++) o += a256(r.setup[k].taper.size() * 4);
-  return o + a256(nsh * sizeof(StackShotHost)) + 3 * a256(nm * 8) + 5 * a256(nm * 4);
+  const size_t nw = r.P.ismth > 0 ? (size_t)(2 * r.P.ismth + 1) * (2 * r.P.ismth + 1) : 0;
+  return o + a256(nsh * sizeof(StackShotHost)) + 3 * a256(nm * 8) + 5 * a256(nm * 4) + a256(nw * 8);
 }
 
 // bytes shot k holds while resident: its slot and, once finished, its pending
@@ -308,6 +318,9 @@ int Server::request_device(SrvRequest& r)
   const size_t os = o; o += a256(nsh * sizeof(StackShotHost));
   const size_t oi = o; o += 3 * a256(nm * 8);
   const size_t of = o; o += 5 * a256(nm * 4);
+  std::vector<double> wt;
+  if (P.ismth > 0) gauss_weights(P.ismth, wt, r.wt_norm);
+  const size_t owt = o; o += a256(wt.size() * 8);
   r.dev_bytes = o;
   r.dev = (char*)pool_.get(o);
   if (!r.dev) return -3;
@@ -335,6 +348,15 @@ int Server::request_device(SrvRequest& r)
       (e = hipMemcpyAsync(r.d_taper, r.stage[2].data(), (os - ot), hipMemcpyHostToDevice, st_)) != hipSuccess) {
     fail("request upload", e);
     return -10;
+  }
+  if (!wt.empty()) {                           // velsmooth1d weights (kept alive in stage)
+    r.d_wt = (double*)(r.dev + owt);
+    r.stage.push_back(std::vector<float>(wt.size() * 2));
+    memcpy(r.stage.back().data(), wt.data(), wt.size() * 8);
+    if ((e = hipMemcpyAsync(r.d_wt, r.stage.back().data(), wt.size() * 8, hipMemcpyHostToDevice, st_)) != hipSuccess) {
+      fail("smoothing weights", e);
+      return -10;
+    }
   }
   (void)rtm_launch_zero_f64(r.d_i1, (long long)nm, st_);
   (void)rtm_launch_zero_f64(r.d_i2, (long long)nm, st_);
@@ -398,6 +420,8 @@ Admit Server::admit_one(SrvRequest& r, std::string& why)
   if ((e = hipMemcpyAsync(d_ds_ + sid, &hds_[sid], sizeof(ShotDev), hipMemcpyHostToDevice, st_)) != hipSuccess)
     fail("descriptor upload", e);
   (void)rtm_launch_phase_init(d_ds_, sid, PH_FWD, 256, st_);
+  if (!s.wvv.empty())                          // the factors of the shot's (smoothed) windows
+    (void)rtm_launch_model_prep(d.rho, d.vv, (float*)d.mib, (float*)d.mkk, (long long)s.g.nxp * s.g.nzp, st_);
   r.next_admit++;
   r.resident++;
   active_++;
@@ -493,6 +517,21 @@ void Server::transition(int sid)
   SrvRequest& r = *reqs_[sl.req];
   if (sl.phase == PH_FWD) {
     ShotDev& d = hds_[sid];
+    if (r.P.ismth > 0) {
+      // velsmooth1d of the shot's windows before its reverse pass
+      // (rtm_model.cu:402-405), through the factor arrays as scratch
+      const ShotSetup& s = r.setup[sl.k];
+      const int nzw = s.g.nzp, nxw = s.g.nxp;
+      const size_t wb = (size_t)nzw * nxw * 4;
+      float *vv = (float*)d.vv, *rh = (float*)d.rho, *t1 = (float*)d.mib, *t2 = (float*)d.mkk;
+      (void)rtm_launch_gauss_smooth(vv, t1, nzw, nxw, r.P.ismth, r.d_wt, r.wt_norm, st_);
+      (void)rtm_launch_gauss_smooth(rh, t2, nzw, nxw, r.P.ismth, r.d_wt, r.wt_norm, st_);
+      hipError_t e1;
+      if ((e1 = hipMemcpyAsync(vv, t1, wb, hipMemcpyDeviceToDevice, st_)) != hipSuccess ||
+          (e1 = hipMemcpyAsync(rh, t2, wb, hipMemcpyDeviceToDevice, st_)) != hipSuccess)
+        fail("smoothing copy", e1);
+      (void)rtm_launch_model_prep(rh, vv, t1, t2, (long long)nzw * nxw, st_);
+    }
     d.phase = PH_REV;
     d.it_off = (int)(r.P.nt - 2 + tick_);      // it = nt-2 - (tick - tick_start)
     d.it_dir = -1;

@@ -137,6 +137,17 @@ typedef struct rtmh_options {
    * meanwhile; non-zero return aborts the job.  NULL: records are ready. */
   int (*seis_wait)(void *user, int shot);
   void *seis_user;
+  /* Decimated snapshots of the source wavefield (no counterpart in the
+   * reference, which reconstructs the source wavefield from the boundary
+   * strips alone): with snap_every = k > 0 the forward pass saves the
+   * interior of p, vx and vz at every k-th time step past the wavelet
+   * (t >= 4/fdom) in snap_fmt (RTMH_STORE_*; 16-bit with one max-abs scale
+   * per field and snapshot), and the reverse pass restores them into the
+   * reconstructed field at those steps, as checkpoints that bound how far
+   * the reconstruction can drift from the forward field.  Memory per shot:
+   * 3 nz nxl values per snapshot.  0: off. */
+  int snap_every;
+  int snap_fmt;
 } rtmh_options;
 
 /* Optional outputs; any pointer may be NULL. Images are [nz*nx]. */
@@ -277,6 +288,16 @@ void rtmh_interp_model(const float *vp0, float *vp, int nx0, int nz0, int nx, in
  * the reference's values bit for bit. */
 void rtmh_pml_smooth(float *w, int nx, int nz, int npml);
 void rtmh_gauss_smooth(float *w, int nz, int nx, int nsp);
+
+/* The same two smoothers on the GPU of a context, in place on a host window
+ * (x fastest, nz rows of nx): pmlvelsmooth1d when npml > 0 (the sweeps in the
+ * reference's in-place order, one workgroup per window; windows with
+ * npml <= 65 and nx, nz in [3 npml, 4096]), then velsmooth1d when nsp > 0.
+ * Bit-identical to rtmh_pml_smooth / rtmh_gauss_smooth.  The engine smooths
+ * each shot's windows this way (medium_flag before the forward pass, ismth
+ * before the reverse pass, on the slot's device windows); this entry point
+ * exposes it for checking and for callers preparing models themselves. */
+int rtmh_smooth_gpu(rtmh_ctx *ctx, float *w, int nx, int nz, int npml, int nsp);
 
 /* selfdoc banner (RTM_GPU_velstr.cpp:155-182) on stdout. */
 void rtmh_selfdoc(const rtmh_parfile *pf, int nx0, int nz0, const rtmh_grid *g);
