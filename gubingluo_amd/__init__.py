@@ -27,7 +27,7 @@ EXPORTED = [
     "rtmh_get_constant", "rtmh_interp_model", "rtmh_selfdoc", "rtmh_postprocess", "rtmh_driver_main",
     "rtmh_server_create", "rtmh_server_submit", "rtmh_server_wait", "rtmh_server_query", "rtmh_server_destroy",
     "rtmh_scan_gather", "rtmh_read_gather", "rtmh_interp_seis", "rtmh_gather_geometry",
-    "rtmh_server_request_bytes", "rtmh_smooth_gpu",
+    "rtmh_server_request_bytes", "rtmh_smooth_gpu", "rtmh_postprocess_noalloc",
 ]
 
 

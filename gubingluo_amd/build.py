@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
 ]
 LIB_SOURCES = ["rtm_kernels.hip", "rtm_engine.cpp", "rtm_server.cpp", "rtm_capi.cpp", "host_io.cpp",
-               "host_driver.cpp", "field_data.cpp"]
+               "host_driver.cpp", "field_data.cpp", "postprocess_noalloc.hip"]
 
 
 def _hipcc() -> str:

@@ -308,6 +308,20 @@ void rtmh_selfdoc(const rtmh_parfile *pf, int nx0, int nz0, const rtmh_grid *g);
 int rtmh_postprocess(int device, const float *image1, const float *illum, int nx, int nz, float dx, float dz,
                      float *image_lap, float *image_norm);
 
+/* rtmh_postprocess without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments and results as rtmh_postprocess.  Its staging (image1,
+ * illum and one result, nx*nz floats each) lives in a file-scope device
+ * array of 3 * RTMH_NOALLOC_MAX_CELLS floats (24 MiB) that every GPU the
+ * library runs on holds until the process ends; images of more than
+ * RTMH_NOALLOC_MAX_CELLS cells are refused (-3).  Calls on one GPU share
+ * that array: calls that overlap on the same GPU race and must be
+ * serialised by the caller. */
+#define RTMH_NOALLOC_MAX_CELLS (2048 * 1024)
+int rtmh_postprocess_noalloc(int device, const float *image1, const float *illum, int nx, int nz, float dx, float dz,
+                             float *image_lap, float *image_norm);
+
 /* main() of RTM_GPU_velstr.cpp:37-317: reads ./input/cmd.in, runs, writes
  * ./output/<dev>*.su (plus <dev>Image1lap.su and <dev>Imagenorm.su). */
 int rtmh_driver_main(int argc, char **argv);

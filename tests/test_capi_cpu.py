@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(ROOT, "include", "rtm_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"^\s*#.*$", "", src, flags=re.M)          # preprocessor lines (macros are not symbols)
     # function declarations; a function-pointer member `int (*f)(...)` is not one
     return sorted(set(re.findall(r"\b([A-Za-z_]\w*)\s*\((?!\s*\*)[^;{]*\)\s*;", src)))
 
