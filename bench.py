@@ -15,11 +15,14 @@ all shots in flight in one launch batch (about 110 GB of slots), so that
 the per-step launches carry enough tiles to keep the tail short.  The velocity model is resident in
 HBM before the timed region (rtmh_model_create); every step runs through the
 C ABI (rtmh_run_model), which returns after its stream has drained.
+--ic-extra adds the other imaging conditions that the rtm_model drop-in always
+computes (decomposed, Poynting and energy-norm images).
 
 The default run also measures, as sub-objects of the same line (skip with
 --quick): "batched" (one 64-shot launch batch of the ragged 2048x768
 survey), "precision" (2048x1024 dx=7.5 m nt=6000 shots with FP32 / FP16 /
-BF16 boundary stores, and the image error of each against the FP64 oracle
+BF16 boundary stores, with FP16 / BF16 source-wavefield snapshots on top of
+the 16-bit strips, and the image error of each form against the FP64 oracle
 on a shot the CPU affords), "served" (a Poisson stream of --requests
 requests against the plain one-shot-per-launch form) and, in
 "cpu_baseline", the oracle on all host cores and on one core, on the
@@ -203,7 +206,7 @@ def run_survey(args, G, synth):
         P.dshot = 1
         t0 = time.perf_counter()
         r = ctx.run(P, model=mid, batch=args.batch, store_fmt=store, graph_chunk=args.chunk, tile_z=args.tz,
-                    want_post=True)
+                    want_post=True, ic_extra=args.ic_extra)
         return r, time.perf_counter() - t0, list(range(P.startshot, P.nshot + 1))
 
     for k in range(args.warmup):
@@ -226,7 +229,8 @@ def run_survey(args, G, synth):
         "f32" if store == G.STORE_F32 else f"f32 stencil, {args.store} boundaries",
         "synthetic: seeded smooth-random layered model, data modelled in the same run",
         {"workload": "plain RTM, 200-shot survey, 1024x512 dx=10 m, PML 50, Ricker 15 Hz, dt=0.8 ms "
-                     f"nt={P.nt}, shots every 50 m, split-spread 301 receivers, fp32, 3 imaging conditions",
+                     f"nt={P.nt}, shots every 50 m, split-spread 301 receivers, fp32, 3 imaging conditions"
+                     + (" + the drop-in's decomposed, Poynting and energy-norm images" if args.ic_extra else ""),
          "shots_per_step": S, "batch": recs[0][0].batch, "graph_chunk": args.chunk,
          "tile_z": recs[0][0].tile_z, "tile_z_rev": recs[0][0].tile_z_rev, "store": args.store},
         {"shots_per_hour": round(nshots / elapsed * 3600.0, 1),
@@ -277,29 +281,47 @@ def batched_sub(args, G, synth, ctx):
 
 def precision_sub(args, G, synth, ctx):
     """2048x1024 dx=7.5 m 25 Hz dt=0.6 ms nt=6000 full-aperture shots per
-    boundary store (FP32 / FP16 / BF16), one untimed shot then --prec-shots
-    timed; image error of each store against the FP64 oracle on a shot the
-    CPU affords (the same model, 401-column window, nt 2000)."""
+    boundary store (FP32 / FP16 / BF16), and with FP16 / BF16 snapshots of the
+    source wavefield every --snap-every steps on top of the 16-bit strips; one
+    untimed shot then --prec-shots timed per form; image error of each form
+    against the FP64 oracle on a shot the CPU affords (the same model,
+    401-column window, nt 2000)."""
     P, vp, rho = synth.precision_grid()
     if args.nt:
         P.nt = args.nt
     S = args.prec_shots
+    K = args.snap_every
     mid = ctx.model_create(vp, rho)
+    # (name, boundary store, snapshot format or None)
+    forms = [("f32", G.STORE_F32, None), ("f16", G.STORE_F16, None), ("bf16", G.STORE_BF16, None),
+             ("f16+snap_f16", G.STORE_F16, G.STORE_F16), ("bf16+snap_bf16", G.STORE_BF16, G.STORE_BF16)]
     res = {}
-    for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
+    for name, fmt, snap in forms:
+        kw = dict(store_fmt=fmt, graph_chunk=args.chunk)
+        if snap is not None:
+            kw.update(snap_every=K, snap_fmt=snap)
         Q = G.Params.from_buffer_copy(P)
         Q.startshot, Q.nshot, Q.dshot = 61, 61, 1
-        ctx.run(Q, model=mid, store_fmt=fmt, graph_chunk=args.chunk, want_post=False)
+        ctx.run(Q, model=mid, want_post=False, **kw)
         Q.nshot = 60 + S
         t0 = time.perf_counter()
-        r = ctx.run(Q, model=mid, batch=S, store_fmt=fmt, graph_chunk=args.chunk, want_post=True)
+        r = ctx.run(Q, model=mid, batch=S, want_post=True, **kw)
         dt = time.perf_counter() - t0
         res[name] = {"gcell_s": round(r.stats.cell_updates / dt / 1e9, 2), "shots_per_hour": round(S / dt * 3600, 1),
                      "seconds": round(dt, 2)}
         log(f"[bench] precision {name}: {S} shots {dt:.2f} s, {res[name]['gcell_s']} Gcell/s")
+    snaps = {}
+    for name in ("f16+snap_f16", "bf16+snap_bf16"):
+        v = res.pop(name)
+        v["vs_strips_alone"] = round(v["gcell_s"] / res[name.split("+")[0]]["gcell_s"], 4)
+        snaps[name] = v
     out = {"workload": f"precision variant, 2048x1024 dx=7.5 m, PML 50, Ricker 25 Hz, dt=0.6 ms nt={P.nt}, "
                        f"full-aperture shots (2148x1124 padded cells), {S} shots per launch batch per store",
-           "per_store": res, "error_vs_fp64_oracle": None}
+           "per_store": res,
+           "snapshots": {"every": K, "note": "decimated source-wavefield snapshots (p, vx, vz of the interior, "
+                                             "per-snapshot max-abs scale) restored in the reverse pass, on top of "
+                                             "the boundary strips of the same precision", "per_form": snaps},
+           "error_vs_fp64_oracle": None}
     if not args.no_cpu:
         from oracle import oracle as O
 
@@ -313,8 +335,9 @@ def precision_sub(args, G, synth, ctx):
         log(f"[bench] precision: FP64 oracle {time.perf_counter() - t0:.1f} s")
         err = {"sample": f"shot 64 of the precision model with a 401-column window (layout 1, offsetmax 3000 m), "
                          f"nt={Q.nt}", "rel_l2": {}}
-        for name, fmt in (("f32", G.STORE_F32), ("f16", G.STORE_F16), ("bf16", G.STORE_BF16)):
-            r = ctx.run(Q, model=mid, batch=1, store_fmt=fmt, want_post=False)
+        for name, fmt, snap in forms:
+            kw = {} if snap is None else dict(snap_every=K, snap_fmt=snap)
+            r = ctx.run(Q, model=mid, batch=1, store_fmt=fmt, want_post=False, **kw)
             e = {k: float(np.linalg.norm((a - b).astype(np.float64)) / max(np.linalg.norm(b.astype(np.float64)), 1e-300))
                  for k, a, b in (("image1", r.image1, i1), ("image2s", r.image2s, i2), ("illum", r.illum, il))}
             err["rel_l2"][name] = {k: float(f"{v:.3e}") for k, v in e.items()}
@@ -582,6 +605,9 @@ def main():
     ap.add_argument("--plain-requests", type=int, default=8,
                     help="serve: first requests of the stream also run in the plain form (0: all)")
     ap.add_argument("--prec-shots", type=int, default=16, help="precision: timed shots per store")
+    ap.add_argument("--snap-every", type=int, default=100, help="precision: steps between snapshots")
+    ap.add_argument("--ic-extra", action="store_true",
+                    help="survey: also the drop-in's other imaging conditions (decomposed, Poynting, energy norm)")
     ap.add_argument("--quick", action="store_true",
                     help="survey line only (no batched / precision / served sub-objects, no one-core baseline)")
     ap.add_argument("--mem-budget", type=float, default=0.0, help="serve: HBM budget in bytes (0: 80%% of free memory)")
