@@ -263,6 +263,9 @@ __device__ __forceinline__ int col_load(int voff, int col, int nxp)
 {
   return (col >= 0 && col < nxp) ? voff : (voff | RTM_OOB);
 }
+// a lane's offset, or the out-of-range one where the lane takes no part:
+// its load returns 0 and its store is dropped, with no exec-mask branch
+__device__ __forceinline__ int pred_off(int voff, bool on) { return on ? voff : (voff | RTM_OOB); }
 __device__ __forceinline__ float bld(rsrc_t r, int voff, int soff)
 {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
@@ -587,7 +590,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #undef PZA
       if (kz >= 10 && kz < 10 + TZ) {                  // own rows (wave-uniform)
         if (!gen) bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);   // every lane's cell exists
-        else if (ix < nxp && iz <= nzp - 2) bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);
+        else bst(Rvzn, vzn_[k], pred_off(vcol, ix < nxp && iz <= nzp - 2), srow0 + (kz - 10) * pr4);
       }
     }
   }
@@ -611,14 +614,29 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       qa[r] = qb[r] = 0.0f;
       const bool incell = ix < nxp && iz < nzp;
       const bool inner = xin && iz >= npml && iz < nzp - npml;
-      if (FK != FK_RCN && gen && incell && iz > 0 && iz < nzp - 1 && !xring && !inner) {
-        qa[r] = bld(R1, vcol, so);
-        qb[r] = bld(R2, vcol, so);
-      }
-      if (FK == FK_RCV && incell && inner) {
-        const int io = (iz - npml) * s.nxl * 4;        // wave-uniform row offset
+      const int io = imax(iz - npml, 0) * s.nxl * 4;   // wave-uniform row offset (rows >= npml used)
+      if (FK == FK_RCV && !gen) {                      // every lane's cell is interior
         qa[r] = bld(image_rsrc(s.g1, s), icol, io);
         qb[r] = bld(image_rsrc(s.illum, s), icol, io);
+      } else if (FK != FK_RCN && gen) {
+        // the split pair of damped cells, the accumulators of interior cells:
+        // loads with predicated offsets (0 where a lane takes no part), each
+        // pair issued only by waves that hold such a cell
+        const bool dm = incell && iz > 0 && iz < nzp - 1 && !xring && !inner;
+        if (any_lane(dm)) {
+          const int pq = pred_off(vcol, dm);
+          qa[r] = bld(R1, pq, so);
+          qb[r] = bld(R2, pq, so);
+        }
+        if (FK == FK_RCV) {
+          const bool im = incell && inner;
+          if (any_lane(im)) {
+            const int pi = pred_off(icol, im);
+            const float a2 = bld(image_rsrc(s.g1, s), pi, io), b2 = bld(image_rsrc(s.illum, s), pi, io);
+            qa[r] = im ? a2 : qa[r];
+            qb[r] = im ? b2 : qb[r];
+          }
+        }
       }
       if (rstrips) {                                   // reconstruction: stored p of strip cells
         const int lr = strip_row_lr(s, iz);
@@ -643,7 +661,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
                axv, bxv, vxn_[r]);
 #undef PCA
       if (!gen) bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
-      else if (iz < nzp && ix <= nxp - 2) bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
+      else bst(Rvxn, vxn_[r], pred_off(vcol, iz < nzp && ix <= nxp - 2), srow0 + j * pr4);
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
@@ -759,7 +777,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       if ((tl.kind & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && wr && xsrc)   // source removal (rtm_model.cu:437)
         p = __builtin_fmaf(-GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
       if (!gen) bst(Rpn, p, vcol, so);                 // every lane's cell is interior
-      else if (wr) bst(Rpn, p, vcol, so);
+      else bst(Rpn, p, pred_off(vcol, wr), so);
     } else {
       if (!gen) {
         DivV d;
@@ -778,30 +796,35 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         else d.sz = div_z(cz, r, rad, c10, ctab);
         const bool split = upd && !(xin && iz >= npml && iz < nzp - npml);
         float pu = __builtin_fmaf(-kk[r], pres_sum(d, dtx, dtz), p);
-        if (any_lane(split) && split) {                // split pair, p = p1 + p2 (kernel.cu:780-783)
+        if (any_lane(split)) {                         // split pair, p = p1 + p2 (kernel.cu:780-783)
           const rsrc_t R1 = field_rsrc(s.q1, s), R2 = field_rsrc(s.q2, s);
           float q1 = qa[r], q2 = qb[r];
           const float dxv = (d.sx * dtx) * kk[r], dzv = (d.sz * dtz) * kk[r];
           q1 = __builtin_fmaf(ax, q1, -(dxv * bx));   // (c1 q - d)/c2 (kernel.cu:774-779)
           q2 = __builtin_fmaf(DMP[DPA + j], q2, -(dzv * DMP[DPB + j]));
-          bst(R1, q1, vcol, so);
-          bst(R2, q2, vcol, so);
-          pu = q1 + q2;
+          const int pq = pred_off(vcol, split);
+          bst(R1, q1, pq, so);
+          bst(R2, q2, pq, so);
+          if (split) pu = q1 + q2;
         }
         if (upd) p = pu;
       }
       if (FK == FK_SRC) {
         if ((tl.kind & TF_SRC) && iz >= s.sz0 && iz <= s.sz1 && xsrc)   // kernel.cu:36-47
           p = __builtin_fmaf(GCF(s.sa)[(iz - s.sz0) * (s.sx1 - s.sx0 + 1) + ix - s.sx0], GCF(s.wav)[it], p);
-        auto out = [&]() {
-          bst(Rpn, p, vcol, so);
+        auto record = [&](bool on) {
           if ((tl.kind & TF_REC) && s.record && iz == npml) {   // kernel.cu:48-71
             const int ir = ix - s.rcol0;
-            if (ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
+            if (on && ir >= 0 && ir < s.nrec) GF(s.rec)[(size_t)it * s.nrec + ir] = p;
           }
         };
-        if (!gen) out();
-        else if (incell) out();
+        if (!gen) {
+          bst(Rpn, p, vcol, so);
+          record(true);
+        } else {
+          bst(Rpn, p, pred_off(vcol, incell), so);
+          record(incell);
+        }
       } else {                                         // receiver wavefield
         if ((tl.kind & TF_REC) && iz == npml) {        // Dirichlet insertion (kernel.cu:72-95)
           const int ir = ix - s.rcol0;
@@ -818,9 +841,15 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
         if (!gen) {                                    // every lane's cell is interior
           bst(Rpn, p, vcol, so);
           image();
-        } else if (incell) {
-          bst(Rpn, p, vcol, so);
-          if (xin && iz >= npml && iz < nzp - npml) image();
+        } else {
+          bst(Rpn, p, pred_off(vcol, incell), so);
+          const bool im = incell && xin && iz >= npml && iz < nzp - npml;
+          if (any_lane(im)) {
+            const int io = imax(iz - npml, 0) * s.nxl * 4;
+            const int pi = pred_off(icol, im);
+            bst(image_rsrc(s.g1, s), __builtin_fmaf(ps[r], p, qa[r]), pi, io);
+            bst(image_rsrc(s.illum, s), __builtin_fmaf(ps[r], ps[r], qb[r]), pi, io);
+          }
         }
       }
     }
