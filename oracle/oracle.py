@@ -42,7 +42,7 @@ class Geom(C.Structure):
 class ShotOpts(C.Structure):
     _fields_ = [("literal", C.c_int), ("forward_only", C.c_int), ("nt_limit", C.c_int),
                 ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p), ("exact_reverse", C.c_int),
-                ("ic_out", C.c_void_p)]
+                ("ic_out", C.c_void_p), ("snap_every", C.c_int)]
 
 
 _lib = None
@@ -160,7 +160,8 @@ def _variant(f64, korder):
 
 
 def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal=False, forward_only=False,
-             nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False, extra_ic=False) -> ShotResult:
+             nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False, extra_ic=False,
+             snap_every=0) -> ShotResult:
     g = geometry(P, ishot)
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     rt = np.float64 if f64 else np.float32
@@ -171,7 +172,8 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal
     sr = np.zeros((g.nzp, g.nxp), np.float32) if snap_it >= 0 else None
     ic = np.zeros((4, P.nz, g.nxl), np.float32) if extra_ic else None
     o = ShotOpts(literal=int(literal), forward_only=int(forward_only), nt_limit=int(nt_limit),
-                 snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse), ic_out=_fp(ic))
+                 snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse), ic_out=_fp(ic),
+                 snap_every=int(snap_every))
     vp0 = np.ascontiguousarray(vp0, np.float32)
     rho0 = np.ascontiguousarray(rho0, np.float32)
     si = None if seis_in is None else np.ascontiguousarray(seis_in, np.float32)
@@ -228,8 +230,10 @@ def alloc_fields(g, npml, f64=False):
                 pb1=np.zeros((npml, nxl), rt), pb2=np.zeros((npml, nxl), rt))
 
 
-def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, korder=False):
-    """Whole rtm_model() job: returns (image1, image2s, illum) as float32 (nz, nx)."""
+def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, korder=False, snap_every=0):
+    """Whole rtm_model() job: returns (image1, image2s, illum) as float32 (nz, nx).
+    snap_every > 0: with the library's decimated snapshots (exact values) restored
+    into the reconstructed source field, which is not the reference's semantics."""
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     img1 = np.zeros((P.nz, P.nx), np.float32)
     img2 = np.zeros((P.nz, P.nx), np.float32)
@@ -237,7 +241,7 @@ def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, kord
     done = C.c_int(0)
     fn = getattr(lib(), "orc_rtm_" + _variant(f64, korder))
     rc = fn(C.byref(P), _fp(np.ascontiguousarray(vp0, np.float32)), _fp(np.ascontiguousarray(rho0, np.float32)),
-            _fp(coef), C.c_int(int(literal)), C.c_int(1 if (stack64 or f64) else 0),
+            _fp(coef), C.c_int(int(literal)), C.c_int(1 if (stack64 or f64) else 0), C.c_int(int(snap_every)),
             _fp(img1), _fp(img2), _fp(ill), C.byref(done))
     if rc != 0:
         raise RuntimeError(f"oracle rtm failed rc={rc}")

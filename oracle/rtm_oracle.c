@@ -95,6 +95,15 @@ typedef struct {
     float *ic_out;        /* optional: the other imaging conditions of the shot,
                              4 x nz x nxl (g2 decomposition sum, g3 Poynting
                              theta sum, g4 energy norm, g3_true), untapered */
+    int snap_every;       /* >0: decimated snapshots of the library's option
+                             snap_every (not reference semantics): the forward
+                             pass keeps the interior of p of step k and of the
+                             velocities of step k+1 for k = k0*every, ...,
+                             k <= nt-2, k0 = max(1, ceil(t0/every)),
+                             t0 = ceil(4/(fdom*dt)); after reverse step k
+                             (imaging included) they replace the reconstructed
+                             source field's interior cells, exactly (no
+                             quantisation) */
 } orc_shot_opts;
 
 /* ------------------------------------------------------------------ */

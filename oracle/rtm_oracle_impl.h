@@ -704,6 +704,14 @@ ORC_EXPORT void SFX(orc_ic_cells)(int nxp, int nzp, int npml, const RT *ps, cons
     SFX(extra_ic)(&S, &R, &m, ps_pre, p_pre, icb, dx, dz, dt);
 }
 
+/* index of the snapshot of step k, -1 if k is not a snapshot step */
+static int SFX(snap_q)(int k, int every, int k0, int n)
+{
+    if (every <= 0 || k < 0 || k % every != 0) return -1;
+    const int q = k / every - k0;
+    return (q >= 0 && q < n) ? q : -1;
+}
+
 static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const float *rho0,
                          const float *diffcoef, const orc_shot_opts *o,
                          float *seis_out, const float *seis_in,
@@ -722,6 +730,19 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     SFX(wfield) S, Rw;
     SFX(wf_alloc)(&S, nxp, nzp, npml);
     size_t slen = SFX(strip_len)(nzi, nxl);
+    /* decimated snapshots (orc_shot_opts.snap_every): snapshot q of step
+     * k = (k0 + q) * every holds p of step k and vx, vz of step k + 1 (what the
+     * reverse leapfrog holds after reverse step k), interior cells only */
+    int sn_every = o->snap_every, sn_k0 = 0, sn_n = 0;
+    if (sn_every > 0) {
+        const int t0 = (int)ceil(4.0 / ((double)P->fdom * (double)P->dt));
+        sn_k0 = (t0 + sn_every - 1) / sn_every;
+        if (sn_k0 < 1) sn_k0 = 1;
+        sn_n = (nt - 2) / sn_every - sn_k0 + 1;
+        if (sn_n < 0) sn_n = 0;
+    }
+    const size_t sn_cells = (size_t)nzi * nxl;
+    RT *snp = sn_n > 0 ? (RT *)malloc((size_t)sn_n * 3 * sn_cells * sizeof(RT)) : NULL;
     RT *store = (RT *)malloc(slen * (size_t)nt * sizeof(RT));
     RT *rec = (RT *)calloc((size_t)nt * g.nrec, sizeof(RT));
     const size_t rrow = (size_t)npml * nxp;        /* surface row of the padded grid */
@@ -736,6 +757,18 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
         SFX(strips_io)(&S, store + (size_t)it * slen, 1, 3);
         if (snap_fwd && it == o->snap_it)
             for (size_t i = 0; i < (size_t)nxp * nzp; i++) snap_fwd[i] = (float)S.p[i];
+        if (snp) {
+            const int qp = SFX(snap_q)(it, sn_every, sn_k0, sn_n), qv = SFX(snap_q)(it - 1, sn_every, sn_k0, sn_n);
+            for (int iz = 0; iz < nzi; iz++)
+                for (int ix = 0; ix < nxl; ix++) {
+                    const size_t e = (size_t)iz * nxl + ix;
+                    if (qp >= 0) snp[(size_t)(3 * qp) * sn_cells + e] = S.p[(size_t)(iz + npml) * nxp + ix + npml];
+                    if (qv >= 0) {
+                        snp[(size_t)(3 * qv + 1) * sn_cells + e] = S.vx[(size_t)(iz + npml) * (nxp - 1) + ix + npml];
+                        snp[(size_t)(3 * qv + 2) * sn_cells + e] = S.vz[(size_t)(iz + npml) * nxp + ix + npml];
+                    }
+                }
+        }
     }
     if (seis_out)
         for (size_t i = 0; i < (size_t)nt * g.nrec; i++) seis_out[i] = (float)rec[i];
@@ -819,7 +852,21 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
         if (icb) SFX(extra_ic)(&S, &Rw, &m, ps_pre, p_pre, icb, P->dx, P->dz, P->dt);
         if (snap_rec && it == o->snap_it)
             for (size_t i = 0; i < (size_t)nxp * nzp; i++) snap_rec[i] = (float)S.p[i];
+        /* snapshot restore after reverse step it: p on the interior, vx on
+         * its columns npml..nxp-npml-2, vz on its rows npml..nzp-npml-2 */
+        const int qr = snp ? SFX(snap_q)(it, sn_every, sn_k0, sn_n) : -1;
+        if (qr >= 0)
+            for (int iz = 0; iz < nzi; iz++)
+                for (int ix = 0; ix < nxl; ix++) {
+                    const size_t e = (size_t)iz * nxl + ix;
+                    S.p[(size_t)(iz + npml) * nxp + ix + npml] = snp[(size_t)(3 * qr) * sn_cells + e];
+                    if (ix < nxl - 1)
+                        S.vx[(size_t)(iz + npml) * (nxp - 1) + ix + npml] = snp[(size_t)(3 * qr + 1) * sn_cells + e];
+                    if (iz < nzi - 1)
+                        S.vz[(size_t)(iz + npml) * nxp + ix + npml] = snp[(size_t)(3 * qr + 2) * sn_cells + e];
+                }
     }
+    free(snp);
     if (icb) {
         /* cuda_stack_udlr, cuda_stack_theta (kernel.cu:1530-1555) */
         for (size_t i = 0; i < nn; i++) {
@@ -880,7 +927,7 @@ ORC_EXPORT int SFX(orc_fd_steps)(const orc_params *P, int ishot, const float *vp
  * stack_mode 0: FP32 stacks exactly as rtm_model.cu:477-488;
  * stack_mode 1: FP64 stacks (image sums kept in double, rounded at the end). */
 ORC_EXPORT int SFX(orc_rtm)(const orc_params *P, const float *vp0, const float *rho0,
-                            const float *diffcoef, int literal, int stack_mode,
+                            const float *diffcoef, int literal, int stack_mode, int snap_every,
                             float *image1, float *image2s, float *illum, int *nshots_done)
 {
     const int nx = P->nx, nz = P->nz;
@@ -894,7 +941,7 @@ ORC_EXPORT int SFX(orc_rtm)(const orc_params *P, const float *vp0, const float *
     int nwin = 50, done = 0, rc = 0;
     orc_shot_opts o;
     memset(&o, 0, sizeof(o));
-    o.literal = literal; o.snap_it = -1;
+    o.literal = literal; o.snap_it = -1; o.snap_every = snap_every;
     for (int ishot = P->startshot; ishot <= P->nshot; ishot += P->dshot) {
         orc_geom g;
         if (orc_shot_geometry(P, ishot, &g) != 0) { rc = -1; break; }

@@ -4,7 +4,10 @@ restores them into the reconstructed field): images against the FP64
 oracle within 1e-4 (FP32 snapshots) and 1e-3 (FP16 / BF16 snapshots, with
 FP16 / BF16 boundary strips too), graph replay equal to plain launches, and
 snapshots of a job whose wavefield has left the grid before nt leave the
-reconstruction unchanged to rounding."""
+reconstruction unchanged to rounding.  While the wavefield is still on the
+grid at nt the snapshots change the images (they replace the reference's
+zero-velocity reconstruction start): there FP32 snapshots equal the
+kernel-order oracle with the same snapshots bit for bit."""
 import numpy as np
 import pytest
 
@@ -63,3 +66,27 @@ def test_snapshots_graph_equals_plain(gpu_ctx):
     # and they do act: a restore replaces the reconstructed interior
     c = gpu_ctx.run(P, vp, rho, store_fmt=G.STORE_F16, batch=3, want_shots=True)
     assert not all(np.array_equal(x, y) for x, y in zip(a.shot_g1, c.shot_g1))
+
+
+@pytest.mark.parametrize("fmt", [G.STORE_F32, G.STORE_F16, G.STORE_BF16])
+def test_snapshots_while_the_wavefield_is_on_the_grid(gpu_ctx, fmt):
+    """nt = 500: the wavefield is still on the grid at nt, where the
+    reference's zero-velocity start makes its reconstruction depart from the
+    forward field and the snapshots change the images.  FP32 snapshots (with
+    FP32 strips) equal the kernel-order oracle with the same snapshots bit for
+    bit, shot by shot; FP16 / BF16 ones are within 1e-3 of the FP64 oracle
+    with the same snapshots."""
+    kw, vp, rho = job(nt=500)
+    P, Po = G.make_params(**kw), O.make_params(**kw)
+    print(f"[snapshots] on the grid, format {fmt}, every 50 steps", flush=True)
+    r = gpu_ctx.run(P, vp, rho, store_fmt=fmt, snap_every=50, snap_fmt=fmt, batch=3, want_shots=True)
+    for k, ishot in enumerate((1, 2, 3)):
+        if fmt == G.STORE_F32:
+            o = O.run_shot(Po, ishot, vp, rho, korder=True, snap_every=50)
+            assert np.array_equal(r.shot_g1[k], o.g1)
+            assert np.array_equal(r.shot_illum[k], o.illum)
+        else:
+            o = O.run_shot(Po, ishot, vp, rho, f64=True, snap_every=50)
+            e = (rel_l2(r.shot_g1[k], o.g1), rel_l2(r.shot_illum[k], o.illum))
+            print(f"[snapshots] shot {ishot} g1 {e[0]:.2e} illum {e[1]:.2e}", flush=True)
+            assert max(e) <= 1e-3, e
