@@ -111,6 +111,9 @@ class Engine {
   explicit Engine(int device);
   ~Engine();
   int device() const { return device_; }
+  // device memory the engine keeps for reuse by later jobs, released
+  // (a served form sizes its budget from the memory left free)
+  void release_cached() { pool_.trim(); }
   hipStream_t stream() const { return st_; }
 
   // device-resident velocity models (rho and v^2), shared by many jobs

@@ -145,6 +145,10 @@ int Server::start()
 {
   if (hipSetDevice(eng_->device()) != hipSuccess) { set_error("hipSetDevice failed"); return -10; }
   if (hipStreamCreateWithFlags(&st_, hipStreamNonBlocking) != hipSuccess) { set_error("stream"); return -10; }
+  // the context runs no rtmh_run* job while it serves: slot memory its
+  // earlier jobs left cached goes back to the device before the budget is
+  // taken from the free memory
+  eng_->release_cached();
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   budget_ = opt_.mem_budget > 0 ? opt_.mem_budget : 0.8 * (double)fr;

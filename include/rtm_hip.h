@@ -340,7 +340,9 @@ int rtmh_driver_main(int argc, char **argv);
  * time. */
 typedef struct rtmh_server rtmh_server;
 typedef struct rtmh_server_options {
-  double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of free */
+  double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of the
+                         memory free once the context's cached slot memory is
+                         released */
   int max_slots;      /* shots resident at once; 0 = 256                          */
   int tile_z;         /* 16 / 32 / 64; 0 = 16                                     */
   int store_fmt;      /* boundary store precision of every request                */
