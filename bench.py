@@ -19,7 +19,10 @@ C ABI (rtmh_run_model), which returns after its stream has drained.
 computes (decomposed, Poynting and energy-norm images).
 
 The default run also measures, as sub-objects of the same line (skip with
---quick): "batched" (one 64-shot launch batch of the ragged 2048x768
+--quick): "plain_form" (survey shots one per launch with per-step launches,
+the plain GPU form), "error_vs_oracle" (the GPU's images of the survey shots
+the CPU baseline runs, against those of the oracle), "batched" (one 64-shot
+launch batch of the ragged 2048x768
 survey), "precision" (2048x1024 dx=7.5 m nt=6000 shots with FP32 / FP16 /
 BF16 boundary stores, with FP16 / BF16 source-wavefield snapshots on top of
 the 16-bit strips, and the image error of each form against the FP64 oracle
@@ -89,10 +92,11 @@ def _host_threads():
     return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
 
 
-def cpu_shots(P_full, vp, rho, shots, min_seconds, threads):
+def cpu_shots(P_full, vp, rho, shots, min_seconds, threads, keep=None):
     """The oracle (FP32 restatement, OpenMP over rows) on whole shots of a job
     (forward + reconstruction + receiver + imaging at the full nt) until at
-    least min_seconds of CPU work: (cells, seconds, shots run)."""
+    least min_seconds of CPU work: (cells, seconds, shots run).  keep: a dict
+    that receives each shot's (g1, illum)."""
     from oracle import oracle as O
 
     _omp_threads(threads)
@@ -103,7 +107,9 @@ def cpu_shots(P_full, vp, rho, shots, min_seconds, threads):
         d.update(nshot=ishot, startshot=ishot)
         Po = O.make_params(**d)
         g = O.geometry(Po, ishot)
-        O.run_shot(Po, ishot, vp, rho)
+        r = O.run_shot(Po, ishot, vp, rho)
+        if keep is not None:
+            keep[ishot] = (r.g1, r.illum)
         cells += 2.0 * Po.nt * g.nxp * g.nzp + (Po.nt - 1.0) * Po.nz * g.nxl
         done.append(ishot)
         if time.perf_counter() - t0 >= min_seconds:
@@ -113,12 +119,15 @@ def cpu_shots(P_full, vp, rho, shots, min_seconds, threads):
     return cells, dt, done
 
 
-def cpu_baseline(P_full, vp, rho, synth, quick=False):
+CPU_SHOTS = range(100, 106)    # survey shots the CPU baseline runs (as many as fit its time)
+
+
+def cpu_baseline(P_full, vp, rho, synth, quick=False, keep=None):
     """The reported CPU baseline: the oracle on all host cores over whole
     survey shots (value), and on the single-shot configuration (the
     reference's CPU-runnable case) on all cores and on one core."""
     nth = _host_threads()
-    cells, dt, done = cpu_shots(P_full, vp, rho, range(100, 106), 10.0, nth)
+    cells, dt, done = cpu_shots(P_full, vp, rho, CPU_SHOTS, 10.0, nth, keep=keep)
     out = {"value": round(cells / dt / 1e9, 4), "unit": "Gcell/s", "cores": nth, "kind": "port",
            "sample": f"{len(done)} whole shot(s) #{done[0]}-#{done[-1]} of the 200-shot survey (nt={P_full.nt}): "
                      f"{cells / 1e9:.2f} Gcell in {dt:.1f} s; {len(done) / dt * 3600.0:.0f} shots/hour"}
@@ -237,6 +246,15 @@ def run_survey(args, G, synth):
          "survey_latency_s": round(elapsed / max(args.steps, 1), 3),
          "hbm_pct_roofline": round(100.0 * roof["achieved"] / HBM_PEAK_GBS, 1),
          "roofline": roof, "cpu_baseline": None})
+    gpu_shots = None
+    if not args.quick:
+        out["plain_form"] = plain_sub(args, G, ctx, P, mid, out["value"])
+        if not args.no_cpu:
+            # the survey shots the CPU baseline runs, as the GPU images them
+            Q = G.Params.from_buffer_copy(P)
+            Q.startshot, Q.nshot, Q.dshot = CPU_SHOTS[0], CPU_SHOTS[-1], 1
+            r = ctx.run(Q, model=mid, batch=len(CPU_SHOTS), graph_chunk=args.chunk, want_shots=True, want_post=False)
+            gpu_shots = {CPU_SHOTS[k]: (r.shot_g1[k], r.shot_illum[k]) for k in range(len(CPU_SHOTS))}
     ctx.model_destroy(mid)
     if not args.quick:
         # the other configurations, as sub-objects of the same line
@@ -246,7 +264,10 @@ def run_survey(args, G, synth):
     ctx.close()
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(P, vp, rho, synth, quick=args.quick)
+        keep = {}
+        out["cpu_baseline"] = cpu_baseline(P, vp, rho, synth, quick=args.quick, keep=keep)
+        if gpu_shots:
+            out["error_vs_oracle"] = survey_error(gpu_shots, keep)
         log(f"[bench] cpu baseline {out['cpu_baseline']}")
     return out
 
@@ -276,6 +297,39 @@ def batched_sub(args, G, synth, ctx):
            "shots_per_hour": round(64 / dt * 3600.0, 1), "tile_z": r.stats.tile_z, "tile_z_rev": r.stats.tile_z_rev,
            "k_rev_achieved_gbs": roof["achieved"], "k_rev_frac": roof["frac"]}
     log(f"[bench] batched: {out['value']} Gcell/s, k_rev {roof['frac']:.3f} of 8 TB/s")
+    return out
+
+
+def plain_sub(args, G, ctx, P, mid, batched_value):
+    """The plain one-shot-per-launch GPU form on survey shots 100-107: one shot
+    per launch batch, per-step launches (no graph), FP32 boundaries, 16-row
+    tiles."""
+    Q = G.Params.from_buffer_copy(P)
+    Q.startshot, Q.nshot, Q.dshot = 100, 107, 1
+    n = Q.nshot - Q.startshot + 1
+    ctx.run(Q, model=mid, batch=1, graph_chunk=0, tile_z=16, want_post=False)      # warm-up
+    t0 = time.perf_counter()
+    r = ctx.run(Q, model=mid, batch=1, graph_chunk=0, tile_z=16, want_post=True)
+    dt = time.perf_counter() - t0
+    v = r.stats.cell_updates / dt / 1e9
+    log(f"[bench] plain form: {n} survey shots {dt:.2f} s, {v:.1f} Gcell/s")
+    return {"shots": n, "gcell_s": round(v, 3), "shots_per_hour": round(n / dt * 3600.0, 1), "seconds": round(dt, 2),
+            "note": "survey shots 100-107, one shot per launch batch, per-step launches, fp32 boundaries, 16-row tiles",
+            "batched_vs_plain": round(batched_value / v, 2)}
+
+
+def survey_error(gpu_shots, cpu_shots_done):
+    """Relative L2 of the GPU's per-shot images against the FP32 oracle (the
+    reference's operation order) on the survey shots the CPU baseline ran."""
+    worst = {"g1": 0.0, "illum": 0.0}
+    for ishot, (og1, oil) in cpu_shots_done.items():
+        g1, il = gpu_shots[ishot]
+        for k, a, b in (("g1", g1, og1), ("illum", il, oil)):
+            e = float(np.linalg.norm((a - b).astype(np.float64)) / max(np.linalg.norm(b.astype(np.float64)), 1e-300))
+            worst[k] = max(worst[k], e)
+    out = {"sample": f"survey shots {sorted(cpu_shots_done)} at nt 4000, per-shot cross-correlation and illumination",
+           "reference": "FP32 oracle in the reference's operation order", "max_rel_l2": {k: float(f"{v:.3e}") for k, v in worst.items()}}
+    log(f"[bench] survey shots vs oracle: {out['max_rel_l2']}")
     return out
 
 

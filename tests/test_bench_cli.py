@@ -53,3 +53,16 @@ def test_precision_grid_and_serve_shapes():
     t = np.array([r[0] for r in reqs])
     assert np.all(np.diff(t) > 0)
     assert all(1000 <= r[2].nt <= 6000 and 1 <= r[2].nshot <= 32 for r in reqs)
+
+
+def test_survey_error_is_the_worst_shot():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    rng = np.random.default_rng(3)
+    ref = {k: (rng.standard_normal((8, 5)).astype(np.float32), rng.random((8, 5)).astype(np.float32)) for k in (1, 2)}
+    gpu = {k: (a.copy(), b.copy()) for k, (a, b) in ref.items()}
+    gpu[2][0][0, 0] += 0.5                               # one shot off in g1 only
+    e = bench.survey_error(gpu, ref)["max_rel_l2"]
+    want = np.linalg.norm(gpu[2][0] - ref[2][0]) / np.linalg.norm(ref[2][0])
+    assert abs(e["g1"] - want) <= 1e-3 * want and e["illum"] == 0.0
