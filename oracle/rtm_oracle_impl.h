@@ -773,7 +773,7 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     if (seis_out)
         for (size_t i = 0; i < (size_t)nt * g.nrec; i++) seis_out[i] = (float)rec[i];
     if (o->forward_only) {
-        SFX(wf_free)(&S); free(store); free(rec); SFX(med_free)(&m);
+        SFX(wf_free)(&S); free(store); free(rec); free(snp); SFX(med_free)(&m);
         return 0;
     }
     if (seis_in)
