@@ -742,6 +742,7 @@ void Server::query(rtmh_server_stats* st)
   std::lock_guard<std::mutex> lk(mu_);
   st->ticks = tick_; st->launches = launches_; st->peak_slots = peak_; st->active_slots = active_;
   st->queued_requests = (int)queue_.size(); st->busy_s = busy_; st->cell_updates = cells_done_;
+  st->budget_bytes = budget_;
 }
 
 }  // namespace rtm

@@ -179,6 +179,28 @@ def test_server_budget_and_blow_up(gpu_ctx):
         gpu_ctx.model_destroy(mbad)
 
 
+def test_server_budget_counts_the_context_cached_memory(gpu_ctx):
+    """Slot memory that the context's jobs keep cached for reuse goes back to
+    the device when a server starts, so the default budget (80 % of the free
+    memory) is the same right after a large job as on an idle context."""
+    from gubingluo_amd import synth
+
+    P, vp, rho = synth.survey200(nshot=200)
+    P.startshot, P.nshot, P.dshot = 1, 24, 1                  # 24 survey slots at nt 4000: ~13 GB
+    print("[server budget] 24 survey shots, then two servers", flush=True)
+    mid = gpu_ctx.model_create(vp, rho)
+    try:
+        gpu_ctx.run(P, model=mid, want_post=False)
+        with G.Server(gpu_ctx, max_slots=4) as a:
+            b1 = a.query().budget_bytes
+        with G.Server(gpu_ctx, max_slots=4) as b:
+            b2 = b.query().budget_bytes
+    finally:
+        gpu_ctx.model_destroy(mid)
+    print(f"[server budget] after the job {b1 / 1e9:.2f} GB, idle {b2 / 1e9:.2f} GB", flush=True)
+    assert b1 >= 0.995 * b2
+
+
 def test_field_gathers_in_unequal_batches(gpu_ctx):
     """rtm_real's batching (batches of the job, the record callback before
     each gather's reverse pass, graph replay): five gathers in batches of
