@@ -428,7 +428,7 @@ def served_sub(args, G, synth, ctx):
     log(f"[bench] serve: {len(reqs)} requests, {total_shots} shots, {total_cells / 1e12:.2f} Tcell, "
         f"arrivals over {reqs[-1][0]:.1f} s")
     lat = []
-    with G.Server(ctx, store_fmt=G.STORE_F16, mem_budget=args.mem_budget) as srv:
+    with G.Server(ctx, store_fmt=G.STORE_F16, mem_budget=args.mem_budget, tile_z=args.serve_tz) as srv:
         t0 = time.perf_counter()
         ids = []
         for (ta, m, P) in reqs:
@@ -677,6 +677,7 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="survey line only (no batched / precision / served sub-objects, no one-core baseline)")
     ap.add_argument("--mem-budget", type=float, default=0.0, help="serve: HBM budget in bytes (0: 80%% of free memory)")
+    ap.add_argument("--serve-tz", type=int, default=0, help="serve: tile height 16/32/64 (0: the server's default)")
     ap.add_argument("--copy-mib", type=int, default=256,
                     help="survey: size of the device-to-device copy probe (achievable bandwidth; 0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline / oracle error")

@@ -344,7 +344,7 @@ typedef struct rtmh_server_options {
                          memory free once the context's cached slot memory is
                          released */
   int max_slots;      /* shots resident at once; 0 = 256                          */
-  int tile_z;         /* 16 / 32 / 64; 0 = 16                                     */
+  int tile_z;         /* 16 / 32 / 64; 0 = 32                                     */
   int store_fmt;      /* boundary store precision of every request                */
 } rtmh_server_options;
 typedef struct rtmh_request_stats {
