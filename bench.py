@@ -639,7 +639,8 @@ def run_precision(args, G, synth):
         args, best["gcell_s"], "Gcell/s", best["seconds"], "f32 stencil, f16 boundaries",
         "synthetic: seeded smooth-random layered model, data modelled in the same run",
         {"workload": sub["workload"], "shots_per_step": args.prec_shots, "batch": args.prec_shots},
-        {"per_store": sub["per_store"], "error_vs_fp64_oracle": sub["error_vs_fp64_oracle"], "cpu_baseline": None})
+        {"per_store": sub["per_store"], "snapshots": sub["snapshots"], "error_vs_fp64_oracle": sub["error_vs_fp64_oracle"],
+         "cpu_baseline": None})
 
 
 # ------------------------------------------------------------------- serve --
