@@ -255,6 +255,7 @@ extern "C" int rtmh_postprocess(int device, const float* image1, const float* il
   float *a = nullptr, *b = nullptr, *c = nullptr;
   if (hipMalloc(&a, n * 4) != hipSuccess || hipMalloc(&b, n * 4) != hipSuccess || hipMalloc(&c, n * 4) != hipSuccess) {
     (void)hipFree(a); (void)hipFree(b);
+    (void)hipGetLastError();                         // reported here, not by a later launch
     rtm::set_error("out of device memory");
     return -3;
   }

@@ -501,8 +501,15 @@ void* DevPool::get(size_t bytes)
   }
   void* p = nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) {
+    // handled here (cached blocks go back, then a second try; the caller
+    // reports a null return): the failure must not stay the thread's last
+    // HIP error, which the next launch check would otherwise report
+    (void)hipGetLastError();
     trim();
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
   }
   size_[p] = bytes;
   held_ += bytes;
@@ -915,6 +922,7 @@ int Engine::model_create(int nx, int nz, const float* vp0, const float* rho0, in
   if (hipMalloc(&m.d_rho, nm * 4) != hipSuccess || hipMalloc(&m.d_vv, nm * 4) != hipSuccess ||
       hipMalloc(&m.d_ib, nm * 4) != hipSuccess || hipMalloc(&m.d_kk, nm * 4) != hipSuccess) {
     (void)hipFree(m.d_rho); (void)hipFree(m.d_vv); (void)hipFree(m.d_ib); (void)hipFree(m.d_kk);
+    (void)hipGetLastError();                         // reported here, not by a later launch
     set_error("out of device memory (model)");
     return -3;
   }

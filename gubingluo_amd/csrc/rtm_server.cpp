@@ -160,6 +160,7 @@ int Server::start()
       hipMalloc(&d_qs_, sizeof(int) * opt_.max_slots) != hipSuccess ||
       hipMalloc(&d_as_, sizeof(int) * opt_.max_slots) != hipSuccess ||
       hipMalloc(&d_id_, sizeof(int) * opt_.max_slots) != hipSuccess) {
+    (void)hipGetLastError();
     set_error("out of device memory (server control)");
     return -3;
   }
@@ -494,6 +495,7 @@ void Server::rebuild_tiles()
     tile_cap_ = need * 2 + 1024;
     if (hipMalloc(&d_tf_, tile_cap_ * sizeof(Tile)) != hipSuccess ||
         hipMalloc(&d_tr_, tile_cap_ * sizeof(Tile)) != hipSuccess) {
+      (void)hipGetLastError();                       // reported through err_, not by a launch
       err_ = -3; err_msg_ = "out of device memory (tiles)";
       return;
     }

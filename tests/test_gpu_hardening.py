@@ -201,6 +201,32 @@ def test_server_budget_counts_the_context_cached_memory(gpu_ctx):
     assert b1 >= 0.995 * b2
 
 
+def test_job_after_a_failed_pool_allocation():
+    """A job whose slots do not fit beside the blocks an earlier job left
+    cached: the pool's first hipMalloc fails, the cached blocks go back to the
+    device and the second try succeeds -- and that handled failure must not
+    surface as the error of a later kernel launch.  Survey shots with source
+    snapshots at every step hold ~7.4 GB (FP32) or ~4 GB (FP16) per slot, so
+    24 FP32 slots (~177 GB cached, not reusable for the smaller FP16 slots)
+    leave too little for 32 FP16 ones (~128 GB)."""
+    from gubingluo_amd import synth
+
+    P, vp, rho = synth.survey200(nshot=200)
+    ctx = G.Context(0)                                   # its own pool, released at the end
+    try:
+        mid = ctx.model_create(vp, rho)
+        P.startshot, P.nshot, P.dshot = 1, 24, 1
+        print("[pool] 24 shots, FP32 snapshots every step", flush=True)
+        ctx.run(P, model=mid, snap_every=1, snap_fmt=G.STORE_F32, graph_chunk=50, want_post=False)
+        P.nshot = 32
+        print("[pool] 32 shots, FP16 snapshots every step", flush=True)
+        r = ctx.run(P, model=mid, snap_every=1, snap_fmt=G.STORE_F16, graph_chunk=50, want_post=False)
+        assert np.isfinite(r.image1).all() and np.abs(r.image1).max() > 0
+        ctx.model_destroy(mid)
+    finally:
+        ctx.close()
+
+
 def test_field_gathers_in_unequal_batches(gpu_ctx):
     """rtm_real's batching (batches of the job, the record callback before
     each gather's reverse pass, graph replay): five gathers in batches of
