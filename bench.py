@@ -441,7 +441,8 @@ def served_sub(args, G, synth, ctx):
             lat.append(st.done_s - st.submit_s)
         served_s = time.perf_counter() - t0
         q = srv.query()
-    log(f"[bench] served: {served_s:.1f} s, peak slots {q.peak_slots}, ticks {q.ticks}")
+    log(f"[bench] served: {served_s:.1f} s, peak slots {q.peak_slots}, ticks {q.ticks}, "
+        f"{q.allocations} device allocations in {q.alloc_s:.2f} s")
     nplain = min(args.plain_requests or len(reqs), len(reqs))
     plain_s, plain_shots, plain_cells, plain_lat = 0.0, 0, 0.0, []
     for (ta, m, P) in reqs[:nplain]:
@@ -464,6 +465,7 @@ def served_sub(args, G, synth, ctx):
             "shots_per_hour": round(sph, 1), "requests": len(reqs),
             "p50_request_latency_s": round(float(np.percentile(lat, 50)), 2),
             "p99_request_latency_s": round(float(np.percentile(lat, 99)), 2), "peak_slots": q.peak_slots,
+            "device_allocations": q.allocations, "allocation_s": round(q.alloc_s, 3),
             "plain_form": {"shots_per_hour": round(psph, 1), "gcell_s": round(plain_cells / plain_s / 1e9, 3),
                            "requests": nplain, "p50_request_s": round(float(np.percentile(plain_lat, 50)), 2),
                            "note": "one shot per launch, per-step launches, fp32 boundaries, requests one after "

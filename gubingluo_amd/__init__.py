@@ -376,7 +376,8 @@ class RequestStats(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("ticks", C.c_longlong), ("launches", C.c_longlong), ("peak_slots", C.c_int),
                 ("active_slots", C.c_int), ("queued_requests", C.c_int), ("busy_s", C.c_double),
-                ("cell_updates", C.c_double), ("budget_bytes", C.c_double)]
+                ("cell_updates", C.c_double), ("budget_bytes", C.c_double), ("allocations", C.c_longlong),
+                ("alloc_s", C.c_double)]
 
 
 class Server:

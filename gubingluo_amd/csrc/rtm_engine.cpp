@@ -500,6 +500,12 @@ void* DevPool::get(size_t bytes)
     return p;
   }
   void* p = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  struct Clock {   // the host time of this call's device allocations
+    DevPool* pool; std::chrono::steady_clock::time_point t0;
+    ~Clock() { pool->alloc_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  } clock{this, t0};
+  nalloc_++;
   if (hipMalloc(&p, bytes) != hipSuccess) {
     // handled here (cached blocks go back, then a second try; the caller
     // reports a null return): the failure must not stay the thread's last
@@ -524,12 +530,14 @@ void DevPool::put(void* p)
 
 void DevPool::trim()
 {
+  const auto t0 = std::chrono::steady_clock::now();
   for (auto& kv : free_) {
     (void)hipFree(kv.second);
     held_ -= size_[kv.second];
     size_.erase(kv.second);
   }
   free_.clear();
+  alloc_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // ---------------------------------------------------------------- engine ----

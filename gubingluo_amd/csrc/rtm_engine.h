@@ -62,10 +62,15 @@ class DevPool {
   void put(void* p);
   void trim();
   size_t held() const { return held_; }
+  // host seconds spent in hipMalloc / hipFree, and the number of hipMalloc calls
+  double alloc_seconds() const { return alloc_s_; }
+  long long allocations() const { return nalloc_; }
  private:
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> size_;
   size_t held_ = 0;
+  double alloc_s_ = 0;
+  long long nalloc_ = 0;
 };
 
 // ---- engine -----------------------------------------------------------------

@@ -747,6 +747,8 @@ void Server::query(rtmh_server_stats* st)
   st->ticks = tick_; st->launches = launches_; st->peak_slots = peak_; st->active_slots = active_;
   st->queued_requests = (int)queue_.size(); st->busy_s = busy_; st->cell_updates = cells_done_;
   st->budget_bytes = budget_;
+  st->allocations = pool_.allocations();
+  st->alloc_s = pool_.alloc_seconds();
 }
 
 }  // namespace rtm

@@ -374,12 +374,15 @@ int rtmh_server_request_bytes(rtmh_server *srv, int model_id, const rtmh_params 
 int rtmh_server_wait(rtmh_server *srv, int request_id, double timeout_s, rtmh_outputs *out,
                      rtmh_request_stats *stats);
 /* server-wide counters: ticks, launches, peak resident shots, busy seconds,
- * and the HBM budget the server admits against (bytes) */
+ * the HBM budget the server admits against (bytes), and the device
+ * allocations its slot pool made (count, host seconds in hipMalloc/hipFree) */
 typedef struct rtmh_server_stats {
   long long ticks, launches;
   int peak_slots, active_slots, queued_requests;
   double busy_s, cell_updates;
   double budget_bytes;
+  long long allocations;
+  double alloc_s;
 } rtmh_server_stats;
 int rtmh_server_query(rtmh_server *srv, rtmh_server_stats *st);
 int rtmh_server_destroy(rtmh_server *srv);
