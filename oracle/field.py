@@ -13,6 +13,12 @@ rtmh_interp_seis and of rtm_real's geometry; never part of the product).
 FFTW's transforms are restated with numpy.fft (pocketfft): fftw FORWARD is
 np.fft.fft* and BACKWARD is n * np.fft.ifft* (FFTW does not normalise).  The
 f-k results therefore agree with any other FFT to rounding, not bit for bit.
+
+Parity unpinned: fk_interp's steps for lx >= 2 that the lx = 1 closed form
+of the tests leaves out (the block average of the padded spectrum, the
+|H| > lx^2 clip, Interp_function.cpp:170-199) rest on reading the source;
+Interp_function.cpp needs FFTW, which is absent, so no output of the
+reference's own code pins them.
 """
 from __future__ import annotations
 

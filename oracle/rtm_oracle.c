@@ -42,8 +42,18 @@
  *                   per-cell formulas, for any npml >= 10.  Identical to the
  *                   literal form at npml == 32 (checked by the tests).
  *
- * Pinning of each function is listed in DESIGN.md (oracle section); the
- * checks live in tests/test_oracle_*.py.
+ * Also restated, not reference semantics: the library's decimated snapshots
+ * (orc_shot_opts.snap_every), restored into the reconstruction.
+ *
+ * Checks: tests/test_oracle_*.py and tests/test_golden.py pin each function
+ * by golden vectors of the reference's own host code, closed forms, physical
+ * properties (Green's function, mirror symmetry, plane-wave attenuation in
+ * the absorbing layer, image and traveltime locations) or brute force.
+ * Parity unpinned: the index quirks of the other imaging conditions
+ * (kernel.cu:1431,1468-1469: vx read at the p index, the source Poynting
+ * vector with the receiver's vz) rest on reading the source, as do the last
+ * bits of CUDA's acosf/cosf/sinf there and of its expf/logf in the set-up
+ * values (wavelet, source, PML d0); the CUDA path cannot be built here.
  */
 #include <math.h>
 #include <stdint.h>
