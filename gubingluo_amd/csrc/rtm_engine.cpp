@@ -866,6 +866,23 @@ void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tile
   }
 }
 
+// Longest tiles first.  Each XCD takes a contiguous run of the table (the
+// kernels' xcd_tile: run c is [c*per, (c+1)*per), per = ceil(n/8)) in order;
+// within each run the long tiles go first, so that a launch ends on short
+// ones: in the reverse step the tiles holding reconstructed cells (two field
+// steps, the others one), in the forward step those of the absorbing layer
+// (the split pair and the per-cell rules).  Measured on the 200-shot survey,
+// four alternations on one box: reverse pass -1.0 to -1.3 %.
+void order_longest_first(std::vector<Tile>& tiles, int phase)
+{
+  const int flag = phase == PH_REV ? TF_INT : TF_GEN;
+  const size_t n = tiles.size(), per = (n + 7) / 8;
+  for (size_t c = 0; c < 8 && c * per < n; c++) {
+    auto b = tiles.begin() + c * per, e = tiles.begin() + std::min(n, (c + 1) * per);
+    std::stable_partition(b, e, [flag](const Tile& t) { return (t.kind & flag) != 0; });
+  }
+}
+
 int params_check(const rtmh_params& P, std::string& why);
 static bool params_ok(const rtmh_params& P, std::string& why) { return params_check(P, why) == 0; }
 
@@ -1364,10 +1381,9 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       tuned_[rkey] = tz_r;
       rev_done = 3 * kTuneSteps;
     }
-    if (tune_r || tz_r != tz) {                        // (tuning left the last candidate's tiles)
-      build_tiles(ds, tz_r, tiles);
-      HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
-    }
+    build_tiles(ds, tz_r, tiles);
+    order_longest_first(tiles, PH_REV);
+    HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     rc = run_pass(PH_REV, nt - 1 - rev_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz_r, opt.store_fmt,
                   opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every);
     last_tz = tz;
