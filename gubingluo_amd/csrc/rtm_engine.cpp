@@ -872,7 +872,8 @@ void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tile
 // ones: in the reverse step the tiles holding reconstructed cells (two field
 // steps, the others one), in the forward step those of the absorbing layer
 // (the split pair and the per-cell rules).  Measured on the 200-shot survey,
-// four alternations on one box: reverse pass -1.0 to -1.3 %.
+// alternated on one box: reverse pass -1.0 to -1.3 % (four pairs), forward
+// pass -4 % (three pairs).
 void order_longest_first(std::vector<Tile>& tiles, int phase)
 {
   const int flag = phase == PH_REV ? TF_INT : TF_GEN;
@@ -1204,6 +1205,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       tcap = std::max(tcap, tcand[c].size());
     }
     build_tiles(ds, tz, tiles);
+    order_longest_first(tiles, PH_FWD);
     tcap = std::max(tcap, tiles.size());
     // the first 3 x kTuneSteps steps of a pass, kTuneSteps with each candidate
     // height, timed with events; every candidate computes the same cells, so
@@ -1266,6 +1268,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       tuned_[tkey] = tz;
       fwd_done = 3 * kTuneSteps;
       build_tiles(ds, tz, tiles);
+      order_longest_first(tiles, PH_FWD);
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     }
     rc = run_pass(PH_FWD, nt - fwd_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt,
