@@ -866,22 +866,51 @@ void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tile
   }
 }
 
-// Longest tiles first.  Each XCD takes a contiguous run of the table (the
-// kernels' xcd_tile: run c is [c*per, (c+1)*per), per = ceil(n/8)) in order;
-// within each run the long tiles go first, so that a launch ends on short
-// ones: in the reverse step the tiles holding reconstructed cells (two field
-// steps, the others one), in the forward step those of the absorbing layer
-// (the split pair and the per-cell rules).  Measured on the 200-shot survey,
-// alternated on one box: reverse pass -1.0 to -1.3 % (four pairs), forward
-// pass -4 % (three pairs).
-void order_longest_first(std::vector<Tile>& tiles, int phase)
+// Runs of the tile table per XCD.  The step kernels deal workgroups to the 8
+// XCDs round-robin and give XCD c the entries [c*per, (c+1)*per) of the
+// table in order (xcd_tile; per = launched workgroups / 8).  The table is cut
+// into 8 consecutive runs of equal estimated cost -- the slowest XCD sets the
+// length of a launch -- and each run is padded to the longest with empty
+// entries (slot -1), whose workgroups return at once.  Within a run the long
+// tiles go first, so that a launch ends on short ones: in the reverse step the
+// tiles holding reconstructed cells (two field steps against one), in the
+// forward step those of the absorbing layer (the split pair and the per-cell
+// rules; about 1.55 times an interior tile).  Measured on the 200-shot
+// survey, alternated on one box: long tiles first, reverse pass -1.0 to
+// -1.3 % (four pairs) and forward pass -4 % (three pairs); runs of equal cost
+// instead of equal count, a further -0.3 to -0.5 % (three pairs).
+void layout_runs(std::vector<Tile>& tiles, int phase)
 {
+  const size_t n = tiles.size();
+  if (n == 0) return;
   const int flag = phase == PH_REV ? TF_INT : TF_GEN;
-  const size_t n = tiles.size(), per = (n + 7) / 8;
-  for (size_t c = 0; c < 8 && c * per < n; c++) {
-    auto b = tiles.begin() + c * per, e = tiles.begin() + std::min(n, (c + 1) * per);
-    std::stable_partition(b, e, [flag](const Tile& t) { return (t.kind & flag) != 0; });
+  const double heavy = phase == PH_REV ? 2.0 : 1.55;
+  auto cost = [&](const Tile& t) { return (t.kind & flag) ? heavy : 1.0; };
+  double total = 0;
+  for (const Tile& t : tiles) total += cost(t);
+  size_t cut[9];
+  cut[0] = 0;
+  int c = 1;
+  double acc = 0;
+  for (size_t i = 0; i < n && c < 8; i++) {
+    acc += cost(tiles[i]);
+    while (c < 8 && acc >= total * c / 8.0) cut[c++] = i + 1;
   }
+  while (c <= 8) cut[c++] = n;
+  size_t lmax = 0;
+  for (int k = 0; k < 8; k++) lmax = std::max(lmax, cut[k + 1] - cut[k]);
+  const size_t per = (n + 7) / 8;
+  if (lmax > per + per / 4) {                          // runs far from equal in count: equal counts
+    for (int k = 0; k <= 8; k++) cut[k] = std::min(n, (size_t)k * per);
+    lmax = per;
+  }
+  std::vector<Tile> out(8 * lmax, Tile{-1, 0, 0, 0});
+  for (int k = 0; k < 8; k++) {
+    auto b = out.begin() + (size_t)k * lmax;
+    std::copy(tiles.begin() + cut[k], tiles.begin() + cut[k + 1], b);
+    std::stable_partition(b, b + (cut[k + 1] - cut[k]), [flag](const Tile& t) { return (t.kind & flag) != 0; });
+  }
+  tiles.swap(out);
 }
 
 int params_check(const rtmh_params& P, std::string& why);
@@ -1205,8 +1234,9 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       tcap = std::max(tcap, tcand[c].size());
     }
     build_tiles(ds, tz, tiles);
-    order_longest_first(tiles, PH_FWD);
-    tcap = std::max(tcap, tiles.size());
+    layout_runs(tiles, PH_FWD);
+    // room for any height's padded table (runs at most 1.25 x the even share)
+    tcap = std::max(tcap, tiles.size()) * 5 / 4 + 16;
     // the first 3 x kTuneSteps steps of a pass, kTuneSteps with each candidate
     // height, timed with events; every candidate computes the same cells, so
     // these are steps of the pass.  Returns the fastest height.
@@ -1268,7 +1298,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       tuned_[tkey] = tz;
       fwd_done = 3 * kTuneSteps;
       build_tiles(ds, tz, tiles);
-      order_longest_first(tiles, PH_FWD);
+      layout_runs(tiles, PH_FWD);
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     }
     rc = run_pass(PH_FWD, nt - fwd_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt,
@@ -1385,7 +1415,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       rev_done = 3 * kTuneSteps;
     }
     build_tiles(ds, tz_r, tiles);
-    order_longest_first(tiles, PH_REV);
+    layout_runs(tiles, PH_REV);
     HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     rc = run_pass(PH_REV, nt - 1 - rev_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz_r, opt.store_fmt,
                   opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every);

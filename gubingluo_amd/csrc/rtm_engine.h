@@ -223,8 +223,9 @@ class Engine {
 // tile table of a list of slot descriptors with the per-tile flags
 // (tiles carry the index of the descriptor in `ds` plus `slot_base`)
 void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tiles, const std::vector<int>* slot_ids = nullptr);
-// longest tiles first within each XCD's run of the table (phase: PH_FWD / PH_REV)
-void order_longest_first(std::vector<Tile>& tiles, int phase);
+// the table as 8 per-XCD runs of equal estimated cost, long tiles first in
+// each, padded with empty entries (slot -1) (phase: PH_FWD / PH_REV)
+void layout_runs(std::vector<Tile>& tiles, int phase);
 int params_check(const rtmh_params& P, std::string& why);
 // the stability bound of get_constant (RTM_GPU_velstr.cpp:355-361):
 // min(dx, dz) / (sqrt(2) sum|c| vmax), sum over the order-20 coefficient row

@@ -870,6 +870,7 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a)
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
+  if (tl.slot < 0) return;                             // padding of a shorter XCD run
   const ShotDev& s = shot_of(a.shots, tl.slot);
   if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;
@@ -893,6 +894,7 @@ __global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(G == 4 &
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
   const Tile tl = a.tiles[t];
+  if (tl.slot < 0) return;                             // padding of a shorter XCD run
   const ShotDev& s = shot_of(a.shots, tl.slot);
   if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;

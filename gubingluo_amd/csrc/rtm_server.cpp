@@ -488,8 +488,8 @@ void Server::rebuild_tiles()
   }
   build_tiles(fw, opt_.tile_z, tf_, &fid);
   build_tiles(rv, opt_.tile_z, tr_, &rid);
-  order_longest_first(tf_, PH_FWD);
-  order_longest_first(tr_, PH_REV);
+  layout_runs(tf_, PH_FWD);
+  layout_runs(tr_, PH_REV);
   const size_t need = tf_.size() > tr_.size() ? tf_.size() : tr_.size();
   if (need > tile_cap_) {
     (void)hipStreamSynchronize(st_);
