@@ -442,7 +442,7 @@ def served_sub(args, G, synth, ctx):
         served_s = time.perf_counter() - t0
         q = srv.query()
     log(f"[bench] served: {served_s:.1f} s, peak slots {q.peak_slots}, ticks {q.ticks}, "
-        f"{q.allocations} device allocations in {q.alloc_s:.2f} s")
+        f"{q.allocations} device allocation(s) in {q.alloc_s:.2f} s (the budget reserved at server start)")
     nplain = min(args.plain_requests or len(reqs), len(reqs))
     plain_s, plain_shots, plain_cells, plain_lat = 0.0, 0, 0.0, []
     for (ta, m, P) in reqs[:nplain]:

@@ -489,10 +489,36 @@ static void parallel_for(int n, const std::function<void(int)>& fn)
 }
 
 // ------------------------------------------------------------------ pool ----
-DevPool::~DevPool() { trim(); }
+DevPool::~DevPool()
+{
+  trim();
+  if (arena_) (void)hipFree(arena_);
+}
+
+bool DevPool::reserve(size_t bytes)
+{
+  if (arena_ || bytes < 256) return false;
+  bytes &= ~(size_t)255;
+  void* p = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  nalloc_++;
+  const bool ok = hipMalloc(&p, bytes) == hipSuccess;
+  if (!ok) (void)hipGetLastError();                  // handled: the pool stays lazy
+  alloc_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!ok) return false;
+  trim();
+  arena_ = (char*)p;
+  arena_bytes_ = bytes;
+  ext_.reset(bytes);
+  return true;
+}
 
 void* DevPool::get(size_t bytes)
 {
+  if (arena_) {
+    const size_t off = ext_.get(bytes);
+    return off == Extents::kNone ? nullptr : arena_ + off;   // none: the caller waits or refuses
+  }
   auto it = free_.lower_bound(bytes);
   if (it != free_.end() && it->first <= bytes + bytes / 4) {
     void* p = it->second;
@@ -525,6 +551,10 @@ void* DevPool::get(size_t bytes)
 void DevPool::put(void* p)
 {
   if (!p) return;
+  if (arena_) {
+    (void)ext_.put((size_t)((char*)p - arena_));
+    return;
+  }
   free_.insert({size_[p], p});
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <string>
@@ -55,13 +56,74 @@ void ricker(std::vector<float>& w, int nt, float fdom, float dt);
 void taper_row(std::vector<float>& t, int nxl, int nwin);
 
 // ---- device memory pool ------------------------------------------------------
+// First-fit carving of one range [0, bytes) into blocks of whole 256-byte
+// units; put() merges a freed block with the free ranges either side of it.
+// Offsets only: the device pool lays it over one device allocation.
+class Extents {
+ public:
+  static constexpr size_t kNone = ~(size_t)0;
+  void reset(size_t bytes)
+  {
+    holes_.clear();
+    blocks_.clear();
+    if (bytes) holes_[0] = bytes;
+  }
+  // offset of a block of at least `bytes`, kNone when no free range holds it
+  size_t get(size_t bytes)
+  {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    for (auto it = holes_.begin(); it != holes_.end(); ++it) {
+      if (it->second < need) continue;
+      const size_t off = it->first, len = it->second;
+      holes_.erase(it);
+      if (len > need) holes_[off + need] = len - need;
+      blocks_[off] = need;
+      return off;
+    }
+    return kNone;
+  }
+  // false for an offset that is not the start of a block in use
+  bool put(size_t off)
+  {
+    auto bl = blocks_.find(off);
+    if (bl == blocks_.end()) return false;
+    size_t start = off, len = bl->second;
+    blocks_.erase(bl);
+    auto next = holes_.lower_bound(off);
+    if (next != holes_.end() && next->first == off + len) {
+      len += next->second;
+      next = holes_.erase(next);
+    }
+    if (next != holes_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == start) {
+        start = prev->first;
+        len += prev->second;
+        holes_.erase(prev);
+      }
+    }
+    holes_[start] = len;
+    return true;
+  }
+  const std::map<size_t, size_t>& holes() const { return holes_; }
+  const std::map<size_t, size_t>& blocks() const { return blocks_; }
+ private:
+  std::map<size_t, size_t> holes_;    // offset -> length of the free ranges
+  std::map<size_t, size_t> blocks_;   // offset -> length of the blocks in use
+};
+
 class DevPool {
  public:
   ~DevPool();
   void* get(size_t bytes);
   void put(void* p);
   void trim();
-  size_t held() const { return held_; }
+  // Take `bytes` of device memory in one allocation now and carve every later
+  // get() from it (first fit, 256-byte blocks, free neighbours merged on
+  // put()): no device allocation call after this one.  False (and the pool
+  // stays as it was) if the device refuses.
+  bool reserve(size_t bytes);
+  size_t held() const { return arena_ ? arena_bytes_ : held_; }
   // host seconds spent in hipMalloc / hipFree, and the number of hipMalloc calls
   double alloc_seconds() const { return alloc_s_; }
   long long allocations() const { return nalloc_; }
@@ -71,6 +133,9 @@ class DevPool {
   size_t held_ = 0;
   double alloc_s_ = 0;
   long long nalloc_ = 0;
+  char* arena_ = nullptr;
+  size_t arena_bytes_ = 0;
+  Extents ext_;
 };
 
 // ---- engine -----------------------------------------------------------------

@@ -68,6 +68,7 @@ struct SrvSlot {
   int req = -1, k = -1, phase = PH_IDLE;
   int steps_left = 0;
   void* mem = nullptr;
+  void* pend = nullptr;   // the shot's image and illumination once it finishes, until stacked
   size_t bytes = 0;
   std::vector<std::pair<void*, std::vector<char>>> uploads;
 };
@@ -154,6 +155,12 @@ int Server::start()
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   budget_ = opt_.mem_budget > 0 ? opt_.mem_budget : 0.8 * (double)fr;
+  // the whole budget in one device allocation, carved into request blocks and
+  // slots as shots come and go: a hipMalloc in the serving loop stalls the
+  // stream it feeds (tens of ms each in a process whose device memory is
+  // fragmented).  A budget the device cannot hold in one piece is served by
+  // allocations on demand, as before.
+  if (pool_.reserve((size_t)budget_)) budget_ = (double)pool_.held();
   slots_.assign(opt_.max_slots, SrvSlot());
   hds_.assign(opt_.max_slots, ShotDev());
   if (hipMalloc(&d_ds_, sizeof(ShotDev) * opt_.max_slots) != hipSuccess ||
@@ -189,8 +196,10 @@ Server::~Server()
     if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
     if (kv.second->dev) pool_.put(kv.second->dev);
   }
-  for (auto& s : slots_)
+  for (auto& s : slots_) {
     if (s.mem) pool_.put(s.mem);
+    if (s.pend) pool_.put(s.pend);
+  }
   (void)hipFree(d_ds_); (void)hipFree(d_qs_); (void)hipFree(d_as_); (void)hipFree(d_id_);
   (void)hipFree(d_tf_); (void)hipFree(d_tr_);
   pool_.trim();
@@ -402,8 +411,10 @@ Admit Server::admit_one(SrvRequest& r, std::string& why)
   }
   SrvSlot& sl = slots_[sid];
   sl.mem = pool_.get(need);
-  if (!sl.mem) {
-    why = "the device refused a slot of " + std::to_string(need) + " bytes";
+  sl.pend = sl.mem ? pool_.get(pend) : nullptr;
+  if (!sl.pend) {
+    if (sl.mem) { pool_.put(sl.mem); sl.mem = nullptr; }
+    why = "the device refused a slot of " + std::to_string(need + pend) + " bytes";
     return alloc_failed_rule(active_, releasing);
   }
   sl.bytes = need + pend;
@@ -568,8 +579,7 @@ void Server::finish_shot(int sid)
   (void)rtm_launch_guard(d_ds_, d_id_ + sid, 1, (int)(tick_ - 1), nullptr, st_);
   (void)hipMemcpyAsync(&r.guard[k], rtm_guard_word(hds_[sid]), 4, hipMemcpyDeviceToHost, st_);
   r.resident--;
-  float* g = (float*)pool_.get(2 * a256(n));
-  if (!g) { err_ = -3; err_msg_ = "out of device memory (pending images)"; return; }
+  float* g = (float*)sl.pend;                 // taken with the slot
   r.pend_g1[k] = g;
   r.pend_il[k] = (float*)((char*)g + a256(n));
   hipError_t e;

@@ -342,7 +342,10 @@ typedef struct rtmh_server rtmh_server;
 typedef struct rtmh_server_options {
   double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of the
                          memory free once the context's cached slot memory is
-                         released */
+                         released.  rtmh_server_create takes the whole budget
+                         in one device allocation (about 21 ms per GB) and
+                         carves slots from it; a budget the device cannot
+                         give in one piece is allocated as shots need it */
   int max_slots;      /* shots resident at once; 0 = 256                          */
   int tile_z;         /* 16 / 32 / 64; 0 = 32                                     */
   int store_fmt;      /* boundary store precision of every request                */
@@ -375,7 +378,8 @@ int rtmh_server_wait(rtmh_server *srv, int request_id, double timeout_s, rtmh_ou
                      rtmh_request_stats *stats);
 /* server-wide counters: ticks, launches, peak resident shots, busy seconds,
  * the HBM budget the server admits against (bytes), and the device
- * allocations its slot pool made (count, host seconds in hipMalloc/hipFree) */
+ * allocations it made (count, host seconds in hipMalloc/hipFree; 1 and the
+ * time of the reservation when the budget was taken at start) */
 typedef struct rtmh_server_stats {
   long long ticks, launches;
   int peak_slots, active_slots, queued_requests;

@@ -1,6 +1,7 @@
 // host_logic.cpp -- CPU checks of the host logic of librtm_hip.so that needs
 // no GPU: the served form's admission rules (driven here by a fake device
-// allocator) and the checkpoint resume decision.  Built and run by
+// allocator), the first-fit arena its slots are carved from, and the
+// checkpoint resume decision.  Built and run by
 // tests/test_host_logic.py; prints one line per check and exits non-zero on
 // the first failure.
 #include <stdio.h>
@@ -193,8 +194,80 @@ static void checkpoint_checks(const char* dir)
   printf("ok checkpoint resume (same, longer, shrunken, other job, missing)\n");
 }
 
+// ---- the server's arena: first fit over one range, neighbours merged ---------
+// Random gets and puts against a byte map of the range: blocks never overlap,
+// stay inside the range and are 256-byte aligned; the free ranges and the
+// blocks in use tile the range exactly with no two free ranges adjacent; the
+// lowest-offset hole that fits is the one taken; once everything is returned
+// the range is one free range again.
+static void extents_checks()
+{
+  rtm::Extents e;
+  const size_t R = 256 * 1000;
+  e.reset(R);
+  std::vector<int> owner(R / 256, -1);
+  std::vector<std::pair<size_t, size_t>> live;   // offset, units
+  unsigned rng = 12345;
+  auto rnd = [&](unsigned m) { rng = rng * 1103515245u + 12345u; return (rng >> 8) % m; };
+  int bad = 0, got = 0, refused = 0;
+  for (int it = 0; it < 20000 && !bad; it++) {
+    if (live.empty() || rnd(100) < 55) {
+      const size_t bytes = 1 + rnd(256 * 40);
+      const size_t units = (bytes + 255) / 256;
+      // the first hole (lowest offset) of at least `units` in the byte map
+      size_t expect = rtm::Extents::kNone, run = 0;
+      for (size_t u = 0; u < owner.size(); u++) {
+        run = owner[u] < 0 ? run + 1 : 0;
+        if (run == units) { expect = (u + 1 - units) * 256; break; }
+      }
+      const size_t off = e.get(bytes);
+      if (off != expect) { bad = 1; CHECK(off == expect, "get(%zu) gave %zu, first fit is %zu", bytes, off, expect); }
+      if (off == rtm::Extents::kNone) { refused++; continue; }
+      got++;
+      for (size_t u = off / 256; u < off / 256 + units; u++) owner[u] = it;
+      live.push_back({off, units});
+    } else {
+      const size_t i = rnd((unsigned)live.size());
+      CHECK(e.put(live[i].first), "put of a live block at %zu refused", live[i].first);
+      for (size_t u = live[i].first / 256; u < live[i].first / 256 + live[i].second; u++) owner[u] = -1;
+      live[i] = live.back();
+      live.pop_back();
+    }
+    // the maps tile [0, R): sorted, gap-free, holes never adjacent
+    size_t pos = 0;
+    auto h = e.holes().begin();
+    auto b = e.blocks().begin();
+    bool prev_hole = false;
+    while (pos < R && !bad) {
+      if (h != e.holes().end() && h->first == pos) {
+        if (prev_hole) { bad = 1; CHECK(!prev_hole, "two adjacent free ranges at %zu", pos); }
+        for (size_t u = pos / 256; u < (pos + h->second) / 256; u++)
+          if (owner[u] >= 0) { bad = 1; CHECK(owner[u] < 0, "free range over a live block at unit %zu", u); break; }
+        pos += h->second; ++h; prev_hole = true;
+      } else if (b != e.blocks().end() && b->first == pos) {
+        pos += b->second; ++b; prev_hole = false;
+      } else {
+        bad = 1;
+        CHECK(false, "offset %zu is neither free nor in use", pos);
+      }
+    }
+    if (!bad && (pos != R || h != e.holes().end() || b != e.blocks().end())) {
+      bad = 1;
+      CHECK(false, "the maps end at %zu of %zu", pos, R);
+    }
+  }
+  CHECK(got > 1000 && refused > 10, "the sequence exercised too little (%d blocks, %d refusals)", got, refused);
+  CHECK(!e.put(1), "put of an offset that starts no block was accepted");
+  for (auto& l : live) CHECK(e.put(l.first), "put at %zu refused", l.first);
+  CHECK(e.holes().size() == 1 && e.holes().begin()->first == 0 && e.holes().begin()->second == R && e.blocks().empty(),
+        "after every block came back: %zu free ranges, %zu blocks", e.holes().size(), e.blocks().size());
+  CHECK(e.get(R) == 0 && e.get(1) == rtm::Extents::kNone, "the whole range as one block");
+  printf("extents: %d blocks, %d refusals, first fit and merging hold\n", got, refused);
+}
+
 int main(int argc, char** argv)
 {
+  extents_checks();
   admission_checks();
   checkpoint_checks(argc > 1 ? argv[1] : ".");
   if (g_fail) return 1;

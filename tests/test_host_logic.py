@@ -3,7 +3,8 @@ small C++ program (tests/host_logic.cpp) linked against the library: the
 served form's admission rules driven by a fake device allocator (a shot
 larger than the budget is refused, never queued forever; a device that
 refuses memory with nothing resident fails the request; one-shot budgets
-admit in arrival order) and the checkpoint resume decision (a checkpoint
+admit in arrival order), the server's first-fit device arena (blocks never
+overlap, the lowest fitting hole is taken, freed neighbours merge) and the checkpoint resume decision (a checkpoint
 with more shots done than the job has is refused)."""
 import os
 import subprocess
