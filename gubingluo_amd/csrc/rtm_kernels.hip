@@ -1487,10 +1487,16 @@ __global__ __launch_bounds__(NTHR) void k_ic(const ShotDev* __restrict__ shots, 
   const float vtxs = -ps * vxs, vtzs = -ps * vzs, vtxr = -p * vxr, vtzr = -p * vzr;
   const float psu = vtzs < 0 ? ps : 0.0f, psd = vtzs > 0 ? ps : 0.0f, psl = vtxs < 0 ? ps : 0.0f, psr = vtxs > 0 ? ps : 0.0f;
   const float pu = vtzr < 0 ? p : 0.0f, pd = vtzr > 0 ? p : 0.0f, pl = vtxr < 0 ? p : 0.0f, pr = vtxr > 0 ? p : 0.0f;
-  ic[e] = __builtin_fmaf(psu, pd, ic[e]);
-  ic[nn + e] = __builtin_fmaf(psd, pu, ic[nn + e]);
-  ic[2 * nn + e] = __builtin_fmaf(psl, pr, ic[2 * nn + e]);
-  ic[3 * nn + e] = __builtin_fmaf(psr, pl, ic[3 * nn + e]);
+  // An accumulator plane is read and written only where its product is not
+  // zero: the planes start at +0 and fma(a, b, c) with a*b = +-0 returns c
+  // bitwise for any c other than -0, which a sum that starts at +0 never
+  // reaches (exact cancellation rounds to +0).  The up/down (left/right)
+  // conditions exclude each other, and a wave travelling one way keeps them
+  // constant over a region, so whole cache lines are skipped.
+  if (psu != 0.0f && pd != 0.0f) ic[e] = __builtin_fmaf(psu, pd, ic[e]);
+  if (psd != 0.0f && pu != 0.0f) ic[nn + e] = __builtin_fmaf(psd, pu, ic[nn + e]);
+  if (psl != 0.0f && pr != 0.0f) ic[2 * nn + e] = __builtin_fmaf(psl, pr, ic[2 * nn + e]);
+  if (psr != 0.0f && pl != 0.0f) ic[3 * nn + e] = __builtin_fmaf(psr, pl, ic[3 * nn + e]);
   // cuda_poynting (kernel.cu:1452-1488)
   const float ksx = ps * vxs, ksz = ps * vzr, krx = p * vxr, krz = p * vzr;
   const float qd = sqrtf(__builtin_fmaf(ksx, ksx, ksz * ksz) * __builtin_fmaf(krx, krx, krz * krz));
@@ -1514,7 +1520,8 @@ __global__ __launch_bounds__(NTHR) void k_ic(const ShotDev* __restrict__ shots, 
   const float px = (float)(0.5 * (double)(prn[id + 1] - prn[id - 1]) / (double)s.dx);
   const float pz = (float)(0.5 * (double)(prn[id + pitch] - prn[id - pitch]) / (double)s.dz);
   const float t = -(ps - pso[id]) * (p - pro[id]) / (s.dt * s.dt * vv);
-  ic[4 * nn + e] = ic[4 * nn + e] + __builtin_fmaf(psz, pz, __builtin_fmaf(psx, px, t));
+  const float en = __builtin_fmaf(psz, pz, __builtin_fmaf(psx, px, t));
+  if (en != 0.0f) ic[4 * nn + e] = ic[4 * nn + e] + en;     // +-0 leaves the sum as it is (above)
 }
 
 // one shot's other images into the FP64 stacks (kernel.cu:1530-1579): g2 as
