@@ -10,7 +10,9 @@
                   max|p| print every ntsnap steps, rtm_model.cu:271-273)
   budget          a request whose shot cannot fit the server's budget is
                   refused at submit; one that fits one shot at a time runs
-                  one shot at a time
+                  one shot at a time; the budget is reserved in one device
+                  allocation at start (on demand when the device cannot
+                  give it in one piece)
   field batches   recorded gathers migrated in unequal batches with graph
                   replay and the record callback equal one batch of plain
                   launches and the kernel-order oracle, gather by gather
@@ -177,6 +179,41 @@ def test_server_budget_and_blow_up(gpu_ctx):
     finally:
         gpu_ctx.model_destroy(mid)
         gpu_ctx.model_destroy(mbad)
+
+
+def test_server_arena_and_on_demand_fallback(gpu_ctx):
+    """A server takes its budget in one device allocation at start and serves
+    every request block and slot from it (one allocation in all); a budget
+    the device cannot give in one piece (here 4 TB) leaves the server
+    allocating as shots are admitted, with the same images."""
+    vp, rho = model(60, 160, 11)
+    kw = kw_base(nshot=3, flag_layout=1, nx=160, spx0=300.0, dspx=300.0, offsetmax=800.0)
+    P = G.make_params(**kw)
+    mid = gpu_ctx.model_create(vp, rho)
+    try:
+        ref = gpu_ctx.run(P, model=mid, batch=3)
+        print("[server arena] default budget", flush=True)
+        with G.Server(gpu_ctx, max_slots=4) as srv:
+            ids = [srv.submit(mid, P) for _ in range(2)]
+            for rid in ids:
+                i1, i2, il, *_rest = srv.wait(rid, P, timeout=120)
+                assert np.array_equal(i1, ref.image1) and np.array_equal(il, ref.illum)
+            q = srv.query()
+            assert q.allocations == 1, q.allocations
+            assert q.budget_bytes > 0 and int(q.budget_bytes) % 256 == 0
+        print("[server arena] 4 TB budget: on demand", flush=True)
+        with G.Server(gpu_ctx, max_slots=4, mem_budget=4e12) as srv:
+            q0 = srv.query()
+            assert q0.budget_bytes == 4e12 and q0.allocations == 1     # the reservation that failed
+            rid = srv.submit(mid, P)
+            i1, i2, il, *_rest = srv.wait(rid, P, timeout=120)
+            assert np.array_equal(i1, ref.image1) and np.array_equal(il, ref.illum)
+            assert srv.query().allocations > 1
+        # the refused reservation left no error behind for the next job
+        r = gpu_ctx.run(P, model=mid, batch=3)
+        assert np.array_equal(r.image1, ref.image1)
+    finally:
+        gpu_ctx.model_destroy(mid)
 
 
 def test_server_budget_counts_the_context_cached_memory(gpu_ctx):
