@@ -1607,14 +1607,26 @@ __global__ __launch_bounds__(NTHR) void k_guard(const ShotDev* __restrict__ shot
   const float* ps = lev ? s.sp[1] : s.sp[0];
   const float* pr = lev ? s.rp[1] : s.rp[0];
   const bool rev = s.phase == PH_REV;
-  const long long n = (long long)s.nzp * s.nxp;
+  // one wave per row, rows dealt over the waves of gridDim.y workgroups; a
+  // lane takes GU columns 64 apart per iteration (independent loads in
+  // flight, no index division, lanes busy on windows of any width)
+  constexpr int GU = 4, WPB = NTHR / 64;
+  const int lane = threadIdx.x & 63, nxp = s.nxp;
   unsigned int m = 0u;
-  for (long long e = (long long)blockIdx.y * NTHR + threadIdx.x; e < n; e += (long long)gridDim.y * NTHR) {
-    const int iz = (int)(e / s.nxp), ix = (int)(e - (long long)iz * s.nxp);
-    const long long o = (long long)iz * s.pitch + ix;
-    unsigned int a = __float_as_uint(GCF(ps)[o]) & 0x7FFFFFFFu;
-    if (rev) a = max(a, __float_as_uint(GCF(pr)[o]) & 0x7FFFFFFFu);
-    m = max(m, a);
+  for (int iz = blockIdx.y * WPB + (threadIdx.x >> 6); iz < s.nzp; iz += gridDim.y * WPB) {
+    const gfloat* rs = GCF(ps) + (long long)iz * s.pitch;
+    const gfloat* rr = GCF(pr) + (long long)iz * s.pitch;
+    for (int x0 = lane; x0 < nxp; x0 += GU * 64) {
+      unsigned int a[GU];
+#pragma unroll
+      for (int u = 0; u < GU; u++) {
+        const int ix = x0 + u * 64;
+        a[u] = ix < nxp ? __float_as_uint(rs[ix]) & 0x7FFFFFFFu : 0u;
+        if (rev && ix < nxp) a[u] = max(a[u], __float_as_uint(rr[ix]) & 0x7FFFFFFFu);
+      }
+#pragma unroll
+      for (int u = 0; u < GU; u++) m = max(m, a[u]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, o, 64));
@@ -1783,7 +1795,10 @@ hipError_t rtm_launch_guard(const ShotDev* shots, const int* slots, int nslots, 
                             hipStream_t st)
 {
   if (nslots <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_guard, dim3(nslots, 16), dim3(NTHR), 0, st, shots, slots, step, flag);
+  // about 2048 workgroups in all: 10 per slot for 200 survey slots, 64 for
+  // a few wide ones
+  const int gy = nslots >= 256 ? 8 : std::min(64, std::max(8, 2048 / nslots));
+  hipLaunchKernelGGL(k_guard, dim3(nslots, gy), dim3(NTHR), 0, st, shots, slots, step, flag);
   return hipGetLastError();
 }
 
