@@ -91,7 +91,7 @@ struct ShotDev {
   const int* cut;              // mute: samples it <= cut[ir] are zero
   // boundary store
   void* store;
-  float* sscale;               // nt x 4 scales (16-bit formats), then 8 max-abs accumulators
+  float* sscale;               // nt x 4 scales (16-bit formats), then the wavefield guard word
   long long strip_len;         // values per step
   int store_fmt, _pad4;
   // the other imaging conditions (null when off): ud du lr rl g4 (nzi x nxl

@@ -45,7 +45,7 @@ hipError_t rtm_launch_snap(const ShotDev* shots, const int* slots, int nslots, i
 hipError_t rtm_launch_guard(const ShotDev* shots, const int* slots, int nslots, int step, unsigned int* flag,
                             hipStream_t st);
 // a slot's guard word (device address)
-inline unsigned int* rtm_guard_word(const ShotDev& d) { return (unsigned int*)(d.sscale + (size_t)d.nt * 4) + 8; }
+inline unsigned int* rtm_guard_word(const ShotDev& d) { return (unsigned int*)(d.sscale + (size_t)d.nt * 4); }
 
 // per-shot entry of the stacking kernel (k_stack)
 struct StackShotHost { const float* g1; const float* illum; const float* taper; int nx1, nxl; };

@@ -42,7 +42,7 @@ class Geom(C.Structure):
 class ShotOpts(C.Structure):
     _fields_ = [("literal", C.c_int), ("forward_only", C.c_int), ("nt_limit", C.c_int),
                 ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p), ("exact_reverse", C.c_int),
-                ("ic_out", C.c_void_p), ("snap_every", C.c_int)]
+                ("ic_out", C.c_void_p), ("snap_every", C.c_int), ("store_fmt", C.c_int)]
 
 
 _lib = None
@@ -161,7 +161,7 @@ def _variant(f64, korder):
 
 def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal=False, forward_only=False,
              nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False, extra_ic=False,
-             snap_every=0) -> ShotResult:
+             snap_every=0, store_fmt=0) -> ShotResult:
     g = geometry(P, ishot)
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     rt = np.float64 if f64 else np.float32
@@ -173,7 +173,7 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal
     ic = np.zeros((4, P.nz, g.nxl), np.float32) if extra_ic else None
     o = ShotOpts(literal=int(literal), forward_only=int(forward_only), nt_limit=int(nt_limit),
                  snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse), ic_out=_fp(ic),
-                 snap_every=int(snap_every))
+                 snap_every=int(snap_every), store_fmt=int(store_fmt))
     vp0 = np.ascontiguousarray(vp0, np.float32)
     rho0 = np.ascontiguousarray(rho0, np.float32)
     si = None if seis_in is None else np.ascontiguousarray(seis_in, np.float32)
@@ -186,6 +186,17 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal
     if ic is not None:
         res.ic = {"g2": ic[0], "g3": ic[1], "g4": ic[2], "g3_true": ic[3]}
     return res
+
+
+def store16_block(values, fmt):
+    """The library's 16-bit boundary store on one strip block: returns (read
+    back values, 16-bit codes, scale); fmt 1 binary16, 2 bfloat16."""
+    v = np.array(values, np.float32, copy=True)
+    codes = np.zeros(v.size, np.uint16)
+    f = lib().orc_store16_block
+    f.restype = C.c_float
+    sc = f(_fp(v), C.c_longlong(v.size), C.c_int(int(fmt)), _fp(codes))
+    return v, codes, float(sc)
 
 
 def ic_cells(ps, vxs, vzs, p, vxr, vzr, ps_pre, p_pre, vv, npml, dx, dz, dt, korder=True):
@@ -230,10 +241,12 @@ def alloc_fields(g, npml, f64=False):
                 pb1=np.zeros((npml, nxl), rt), pb2=np.zeros((npml, nxl), rt))
 
 
-def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, korder=False, snap_every=0):
+def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, korder=False, snap_every=0,
+        store_fmt=0):
     """Whole rtm_model() job: returns (image1, image2s, illum) as float32 (nz, nx).
     snap_every > 0: with the library's decimated snapshots (exact values) restored
-    into the reconstructed source field, which is not the reference's semantics."""
+    into the reconstructed source field, which is not the reference's semantics.
+    store_fmt 1 / 2: the library's binary16 / bfloat16 boundary store (FP32 forms)."""
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     img1 = np.zeros((P.nz, P.nx), np.float32)
     img2 = np.zeros((P.nz, P.nx), np.float32)
@@ -242,7 +255,7 @@ def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, kord
     fn = getattr(lib(), "orc_rtm_" + _variant(f64, korder))
     rc = fn(C.byref(P), _fp(np.ascontiguousarray(vp0, np.float32)), _fp(np.ascontiguousarray(rho0, np.float32)),
             _fp(coef), C.c_int(int(literal)), C.c_int(1 if (stack64 or f64) else 0), C.c_int(int(snap_every)),
-            _fp(img1), _fp(img2), _fp(ill), C.byref(done))
+            C.c_int(int(store_fmt)), _fp(img1), _fp(img2), _fp(ill), C.byref(done))
     if rc != 0:
         raise RuntimeError(f"oracle rtm failed rc={rc}")
     return img1, img2, ill

@@ -42,7 +42,9 @@
  *                   per-cell formulas, for any npml >= 10.  Identical to the
  *                   literal form at npml == 32 (checked by the tests).
  *
- * Also restated, not reference semantics: the library's decimated snapshots
+ * Also restated, not reference semantics: the library's 16-bit boundary
+ * store (orc_store16_block: per-step block scales, binary16 / bfloat16 codes
+ * rounded to nearest even) and its decimated snapshots
  * (orc_shot_opts.snap_every), restored into the reconstruction.
  *
  * Checks: tests/test_oracle_*.py and tests/test_golden.py pin each function
@@ -114,6 +116,10 @@ typedef struct {
                              (imaging included) they replace the reconstructed
                              source field's interior cells, exactly (no
                              quantisation) */
+    int store_fmt;        /* boundary store of the library's option store_fmt
+                             (not reference semantics; FP32 forms only):
+                             0 exact, 1 binary16, 2 bfloat16 with one scale
+                             per step and strip block (orc_store16_block) */
 } orc_shot_opts;
 
 /* ------------------------------------------------------------------ */
@@ -450,6 +456,81 @@ ORC_EXPORT void orc_coef_table(float *out)
     };
     for (int r = 0; r < 10; r++)
         for (int i = 0; i < 10; i++) out[r * 10 + i] = t[r][i];
+}
+
+/* ------------------------------------------------------------------ */
+/* The library's 16-bit boundary store (not reference semantics).  Each step's
+ * strips are four blocks (vx lr, vz tb, p lr, p tb); a block is stored as
+ * codes of value / scale with scale = max(maxabs / 32768, FLT_MIN) (1 for an
+ * all-zero block; NaN values do not raise the max), rounded to nearest even
+ * into IEEE binary16 or bfloat16, and read back as code * scale.  The
+ * conversions below follow the IEEE 754 definitions of the two formats; the
+ * bfloat16 one rounds the top 16 bits of the binary32 pattern (ties to even)
+ * as the device code does, NaN patterns included. */
+static uint16_t orc_f2h(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, a = x & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) return (uint16_t)(sign | (a > 0x7F800000u ? 0x7E00u : 0x7C00u));
+    if (a >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);     /* 65520 and up: to infinity */
+    if (a < 0x38800000u) {                                        /* below 2^-14: subnormal */
+        float af;
+        memcpy(&af, &a, 4);
+        return (uint16_t)(sign | (uint32_t)rintf(af * 16777216.0f));   /* units of 2^-24, exact scaling */
+    }
+    uint32_t r = a - 0x38000000u;                                 /* exponent bias 127 -> 15 */
+    r += 0x0FFFu + ((r >> 13) & 1u);
+    return (uint16_t)(sign | (r >> 13));
+}
+static float orc_h2f(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    uint32_t x;
+    if (e == 0) {
+        float f = (float)m * 5.9604644775390625e-8f;            /* m * 2^-24, exact */
+        memcpy(&x, &f, 4);
+    } else if (e == 31) {
+        x = 0x7F800000u | (m << 13);
+    } else {
+        x = ((e + 112u) << 23) | (m << 13);
+    }
+    x |= sign;
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+static uint16_t orc_f2bf(float f)
+{
+    uint32_t w;
+    memcpy(&w, &f, 4);
+    w += 0x7FFFu + ((w >> 16) & 1u);
+    return (uint16_t)(w >> 16);
+}
+static float orc_bf2f(uint16_t h)
+{
+    const uint32_t w = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &w, 4);
+    return f;
+}
+/* one block in place: values replaced by what the store reads back; codes
+ * (optional) receive the 16-bit patterns; returns the scale */
+ORC_EXPORT float orc_store16_block(float *v, long long n, int fmt, uint16_t *codes)
+{
+    float mx = 0.0f;
+    for (long long i = 0; i < n; i++) {
+        const float a = fabsf(v[i]);
+        if (a > mx) mx = a;
+    }
+    const float sc = mx > 0.0f ? fmaxf(mx * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
+    for (long long i = 0; i < n; i++) {
+        const float q = v[i] / sc;
+        const uint16_t c = fmt == 1 ? orc_f2h(q) : orc_f2bf(q);
+        if (codes) codes[i] = c;
+        v[i] = (fmt == 1 ? orc_h2f(c) : orc_bf2f(c)) * sc;
+    }
+    return sc;
 }
 
 /* ------------------------------------------------------------------ */

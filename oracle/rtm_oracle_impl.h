@@ -724,6 +724,7 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     const int nxl = g.nxl, nzi = P->nz;
     if (o->literal && (npml != 32 || ORACLE_KORDER)) return -2;
     if (npml < 10) return -3;
+    if (o->store_fmt && sizeof(RT) != sizeof(float)) return -4;     /* 16-bit store: FP32 forms */
     SFX(medium) m;
     SFX(med_setup)(&m, &g, P, vp0, rho0, diffcoef, o->vvpad, o->rhopad);
 
@@ -755,6 +756,14 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
         SFX(add_source)(S.p, &m, it, +1);
         for (int ir = 0; ir < g.nrec; ir++) rec[(size_t)it * g.nrec + ir] = S.p[rrow + g.rcol0 + ir];
         SFX(strips_io)(&S, store + (size_t)it * slen, 1, 3);
+        if (o->store_fmt) {                      /* the library's 16-bit store, block by block */
+            RT *st = store + (size_t)it * slen;
+            const long long nb[4] = {20LL * nzi, 20LL * nxl, 20LL * nzi, 20LL * nxl};
+            for (int b = 0; b < 4; b++) {
+                orc_store16_block((float *)st, nb[b], o->store_fmt, NULL);
+                st += nb[b];
+            }
+        }
         if (snap_fwd && it == o->snap_it)
             for (size_t i = 0; i < (size_t)nxp * nzp; i++) snap_fwd[i] = (float)S.p[i];
         if (snp) {
@@ -927,7 +936,7 @@ ORC_EXPORT int SFX(orc_fd_steps)(const orc_params *P, int ishot, const float *vp
  * stack_mode 0: FP32 stacks exactly as rtm_model.cu:477-488;
  * stack_mode 1: FP64 stacks (image sums kept in double, rounded at the end). */
 ORC_EXPORT int SFX(orc_rtm)(const orc_params *P, const float *vp0, const float *rho0,
-                            const float *diffcoef, int literal, int stack_mode, int snap_every,
+                            const float *diffcoef, int literal, int stack_mode, int snap_every, int store_fmt,
                             float *image1, float *image2s, float *illum, int *nshots_done)
 {
     const int nx = P->nx, nz = P->nz;
@@ -941,7 +950,7 @@ ORC_EXPORT int SFX(orc_rtm)(const orc_params *P, const float *vp0, const float *
     int nwin = 50, done = 0, rc = 0;
     orc_shot_opts o;
     memset(&o, 0, sizeof(o));
-    o.literal = literal; o.snap_it = -1; o.snap_every = snap_every;
+    o.literal = literal; o.snap_it = -1; o.snap_every = snap_every; o.store_fmt = store_fmt;
     for (int ishot = P->startshot; ishot <= P->nshot; ishot += P->dshot) {
         orc_geom g;
         if (orc_shot_geometry(P, ishot, &g) != 0) { rc = -1; break; }

@@ -1,6 +1,7 @@
 """Reduced-precision boundary store (FP16 / BF16 with per-step max-abs
 scaling, FP32 stencil, FP64 stacks) against the FP64 oracle: the images stay
-within 1e-3 relative L2; the FP32 form within 1e-4."""
+within 1e-3 relative L2; the FP32 form within 1e-4.  Against the kernel-order
+oracle with the same 16-bit store (orc_store16_block): bit for bit."""
 import numpy as np
 import pytest
 
@@ -46,3 +47,13 @@ def test_store_16bit_graph_matches_plain(gpu_ctx):
     a = gpu_ctx.run(P, vp, rho, store_fmt=G.STORE_F16, batch=3, graph_chunk=0)
     b = gpu_ctx.run(P, vp, rho, store_fmt=G.STORE_F16, batch=1, graph_chunk=32)
     assert np.array_equal(a.image1, b.image1) and np.array_equal(a.image2s, b.image2s)
+
+
+@pytest.mark.parametrize("fmt", [G.STORE_F16, G.STORE_BF16])
+def test_store_16bit_bit_exact_vs_oracle(gpu_ctx, fmt):
+    kw, vp, rho = job()
+    print(f"[precision] store_fmt={fmt} against the kernel-order oracle", flush=True)
+    r = gpu_ctx.run(G.make_params(**kw), vp, rho, store_fmt=fmt, batch=3)
+    i1, i2, il = O.rtm(O.make_params(**kw), vp, rho, stack64=True, korder=True, store_fmt=fmt)
+    for name, a, b in (("image1", r.image1, i1), ("image2s", r.image2s, i2), ("illum", r.illum, il)):
+        assert np.array_equal(a, b), (name, rel_l2(a, b))
