@@ -42,7 +42,8 @@ class Geom(C.Structure):
 class ShotOpts(C.Structure):
     _fields_ = [("literal", C.c_int), ("forward_only", C.c_int), ("nt_limit", C.c_int),
                 ("snap_it", C.c_int), ("vvpad", C.c_void_p), ("rhopad", C.c_void_p), ("exact_reverse", C.c_int),
-                ("ic_out", C.c_void_p), ("snap_every", C.c_int), ("store_fmt", C.c_int)]
+                ("ic_out", C.c_void_p), ("snap_every", C.c_int), ("store_fmt", C.c_int),
+                ("snap_fmt", C.c_int)]
 
 
 _lib = None
@@ -161,7 +162,7 @@ def _variant(f64, korder):
 
 def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal=False, forward_only=False,
              nt_limit=0, snap_it=-1, seis_in=None, exact_reverse=False, extra_ic=False,
-             snap_every=0, store_fmt=0) -> ShotResult:
+             snap_every=0, store_fmt=0, snap_fmt=0) -> ShotResult:
     g = geometry(P, ishot)
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     rt = np.float64 if f64 else np.float32
@@ -173,7 +174,7 @@ def run_shot(P, ishot, vp0, rho0, coef=None, *, f64=False, korder=False, literal
     ic = np.zeros((4, P.nz, g.nxl), np.float32) if extra_ic else None
     o = ShotOpts(literal=int(literal), forward_only=int(forward_only), nt_limit=int(nt_limit),
                  snap_it=int(snap_it), vvpad=None, rhopad=None, exact_reverse=int(exact_reverse), ic_out=_fp(ic),
-                 snap_every=int(snap_every), store_fmt=int(store_fmt))
+                 snap_every=int(snap_every), store_fmt=int(store_fmt), snap_fmt=int(snap_fmt))
     vp0 = np.ascontiguousarray(vp0, np.float32)
     rho0 = np.ascontiguousarray(rho0, np.float32)
     si = None if seis_in is None else np.ascontiguousarray(seis_in, np.float32)
@@ -242,11 +243,12 @@ def alloc_fields(g, npml, f64=False):
 
 
 def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, korder=False, snap_every=0,
-        store_fmt=0):
+        store_fmt=0, snap_fmt=0):
     """Whole rtm_model() job: returns (image1, image2s, illum) as float32 (nz, nx).
     snap_every > 0: with the library's decimated snapshots (exact values) restored
     into the reconstructed source field, which is not the reference's semantics.
-    store_fmt 1 / 2: the library's binary16 / bfloat16 boundary store (FP32 forms)."""
+    store_fmt / snap_fmt 1 / 2: the library's binary16 / bfloat16 boundary store and
+    snapshots (FP32 forms)."""
     coef = coef_table() if coef is None else np.ascontiguousarray(coef, np.float32)
     img1 = np.zeros((P.nz, P.nx), np.float32)
     img2 = np.zeros((P.nz, P.nx), np.float32)
@@ -255,7 +257,7 @@ def rtm(P, vp0, rho0, coef=None, *, literal=False, stack64=True, f64=False, kord
     fn = getattr(lib(), "orc_rtm_" + _variant(f64, korder))
     rc = fn(C.byref(P), _fp(np.ascontiguousarray(vp0, np.float32)), _fp(np.ascontiguousarray(rho0, np.float32)),
             _fp(coef), C.c_int(int(literal)), C.c_int(1 if (stack64 or f64) else 0), C.c_int(int(snap_every)),
-            C.c_int(int(store_fmt)), _fp(img1), _fp(img2), _fp(ill), C.byref(done))
+            C.c_int(int(store_fmt)), C.c_int(int(snap_fmt)), _fp(img1), _fp(img2), _fp(ill), C.byref(done))
     if rc != 0:
         raise RuntimeError(f"oracle rtm failed rc={rc}")
     return img1, img2, ill

@@ -44,7 +44,7 @@
  *
  * Also restated, not reference semantics: the library's 16-bit boundary
  * store (orc_store16_block: per-step block scales, binary16 / bfloat16 codes
- * rounded to nearest even) and its decimated snapshots
+ * rounded to nearest even) and its decimated snapshots (exact or 16-bit)
  * (orc_shot_opts.snap_every), restored into the reconstruction.
  *
  * Checks: tests/test_oracle_*.py and tests/test_golden.py pin each function
@@ -120,6 +120,10 @@ typedef struct {
                              (not reference semantics; FP32 forms only):
                              0 exact, 1 binary16, 2 bfloat16 with one scale
                              per step and strip block (orc_store16_block) */
+    int snap_fmt;         /* snapshots of snap_every stored the same way
+                             (FP32 forms): 0 exact, 1 binary16, 2 bfloat16,
+                             one scale per snapshot and field over its
+                             nz x nxl interior */
 } orc_shot_opts;
 
 /* ------------------------------------------------------------------ */

@@ -724,7 +724,7 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
     const int nxl = g.nxl, nzi = P->nz;
     if (o->literal && (npml != 32 || ORACLE_KORDER)) return -2;
     if (npml < 10) return -3;
-    if (o->store_fmt && sizeof(RT) != sizeof(float)) return -4;     /* 16-bit store: FP32 forms */
+    if ((o->store_fmt || o->snap_fmt) && sizeof(RT) != sizeof(float)) return -4;   /* 16-bit: FP32 forms */
     SFX(medium) m;
     SFX(med_setup)(&m, &g, P, vp0, rho0, diffcoef, o->vvpad, o->rhopad);
 
@@ -777,6 +777,12 @@ static int SFX(run_shot)(const orc_params *P, int ishot, const float *vp0, const
                         snp[(size_t)(3 * qv + 2) * sn_cells + e] = S.vz[(size_t)(iz + npml) * nxp + ix + npml];
                     }
                 }
+            if (o->snap_fmt) {                   /* the library's 16-bit snapshots: one scale per field */
+                if (qp >= 0) orc_store16_block((float *)(snp + (size_t)(3 * qp) * sn_cells), (long long)sn_cells, o->snap_fmt, NULL);
+                if (qv >= 0)
+                    for (int f = 1; f < 3; f++)
+                        orc_store16_block((float *)(snp + (size_t)(3 * qv + f) * sn_cells), (long long)sn_cells, o->snap_fmt, NULL);
+            }
         }
     }
     if (seis_out)
@@ -937,7 +943,7 @@ ORC_EXPORT int SFX(orc_fd_steps)(const orc_params *P, int ishot, const float *vp
  * stack_mode 1: FP64 stacks (image sums kept in double, rounded at the end). */
 ORC_EXPORT int SFX(orc_rtm)(const orc_params *P, const float *vp0, const float *rho0,
                             const float *diffcoef, int literal, int stack_mode, int snap_every, int store_fmt,
-                            float *image1, float *image2s, float *illum, int *nshots_done)
+                            int snap_fmt, float *image1, float *image2s, float *illum, int *nshots_done)
 {
     const int nx = P->nx, nz = P->nz;
     size_t nimg = (size_t)nx * nz;
@@ -950,7 +956,7 @@ ORC_EXPORT int SFX(orc_rtm)(const orc_params *P, const float *vp0, const float *
     int nwin = 50, done = 0, rc = 0;
     orc_shot_opts o;
     memset(&o, 0, sizeof(o));
-    o.literal = literal; o.snap_it = -1; o.snap_every = snap_every; o.store_fmt = store_fmt;
+    o.literal = literal; o.snap_it = -1; o.snap_every = snap_every; o.store_fmt = store_fmt; o.snap_fmt = snap_fmt;
     for (int ishot = P->startshot; ishot <= P->nshot; ishot += P->dshot) {
         orc_geom g;
         if (orc_shot_geometry(P, ishot, &g) != 0) { rc = -1; break; }

@@ -6,8 +6,8 @@ FP16 / BF16 boundary strips too), graph replay equal to plain launches, and
 snapshots of a job whose wavefield has left the grid before nt leave the
 reconstruction unchanged to rounding.  While the wavefield is still on the
 grid at nt the snapshots change the images (they replace the reference's
-zero-velocity reconstruction start): there FP32 snapshots equal the
-kernel-order oracle with the same snapshots bit for bit."""
+zero-velocity reconstruction start): there every form equals the
+kernel-order oracle with the same store and snapshots bit for bit."""
 import numpy as np
 import pytest
 
@@ -72,10 +72,10 @@ def test_snapshots_graph_equals_plain(gpu_ctx):
 def test_snapshots_while_the_wavefield_is_on_the_grid(gpu_ctx, fmt):
     """nt = 500: the wavefield is still on the grid at nt, where the
     reference's zero-velocity start makes its reconstruction depart from the
-    forward field and the snapshots change the images.  FP32 snapshots (with
-    FP32 strips) equal the kernel-order oracle with the same snapshots bit for
-    bit, shot by shot; FP16 / BF16 ones are within 1e-3 of the FP64 oracle
-    with the same snapshots."""
+    forward field and the snapshots change the images.  Each form (FP32, or
+    FP16 / BF16 strips and snapshots) equals the kernel-order oracle with the
+    same store and snapshots bit for bit, shot by shot; the 16-bit forms are
+    also within 1e-3 of the FP64 oracle with exact snapshots."""
     kw, vp, rho = job(nt=500)
     P, Po = G.make_params(**kw), O.make_params(**kw)
     print(f"[snapshots] on the grid, format {fmt}, every 50 steps", flush=True)
@@ -86,6 +86,9 @@ def test_snapshots_while_the_wavefield_is_on_the_grid(gpu_ctx, fmt):
             assert np.array_equal(r.shot_g1[k], o.g1)
             assert np.array_equal(r.shot_illum[k], o.illum)
         else:
+            o = O.run_shot(Po, ishot, vp, rho, korder=True, snap_every=50, store_fmt=fmt, snap_fmt=fmt)
+            assert np.array_equal(r.shot_g1[k], o.g1)
+            assert np.array_equal(r.shot_illum[k], o.illum)
             o = O.run_shot(Po, ishot, vp, rho, f64=True, snap_every=50)
             e = (rel_l2(r.shot_g1[k], o.g1), rel_l2(r.shot_illum[k], o.illum))
             print(f"[snapshots] shot {ishot} g1 {e[0]:.2e} illum {e[1]:.2e}", flush=True)

@@ -7,7 +7,8 @@ the scale rule fixes exactly (the block's largest magnitude becomes +-32768,
 an all-zero block keeps scale 1, a denormal maximum takes FLT_MIN, NaN does
 not raise the maximum).  A wrong rounding mode, a truncation, a scale off by
 a factor or a lost subnormal fails one of these.  Then whole shots with the
-16-bit store stay within the store's rounding of the exact-store images."""
+16-bit store, and with 16-bit snapshots, stay within the format's rounding of
+the exact images."""
 import numpy as np
 import pytest
 
@@ -96,3 +97,11 @@ def test_shot_with_16bit_store(fmt, tol):
         assert 0 < e <= tol, e
     with pytest.raises(RuntimeError):
         O.run_shot(P, 1, vp, rho, f64=True, store_fmt=fmt)
+    # 16-bit snapshots (one scale per snapshot and field): the restores move
+    # the images by the format's rounding only
+    c = O.run_shot(P, 1, vp, rho, korder=True, snap_every=50)
+    d = O.run_shot(P, 1, vp, rho, korder=True, snap_every=50, snap_fmt=fmt)
+    e = np.linalg.norm((c.g1 - d.g1).astype(np.float64)) / np.linalg.norm(c.g1.astype(np.float64))
+    assert 0 < e <= tol, e
+    with pytest.raises(RuntimeError):
+        O.run_shot(P, 1, vp, rho, f64=True, snap_every=50, snap_fmt=fmt)
