@@ -1485,21 +1485,26 @@ __global__ __launch_bounds__(NTHR) void k_ic(const ShotDev* __restrict__ shots, 
   if (psd != 0.0f && pu != 0.0f) ic[nn + e] = __builtin_fmaf(psd, pu, ic[nn + e]);
   if (psl != 0.0f && pr != 0.0f) ic[2 * nn + e] = __builtin_fmaf(psl, pr, ic[2 * nn + e]);
   if (psr != 0.0f && pl != 0.0f) ic[3 * nn + e] = __builtin_fmaf(psr, pl, ic[3 * nn + e]);
-  // cuda_poynting (kernel.cu:1452-1488)
+  // cuda_poynting (kernel.cu:1452-1488).  Both bins add a multiple of ps*p:
+  // where that product is zero (either wavefield zero, or the product
+  // underflows) the bins keep their value bit for bit (the argument above),
+  // and the angle, its cosine and sine and the FP64 quotient are not needed.
   const float ksx = ps * vxs, ksz = ps * vzr, krx = p * vxr, krz = p * vzr;
-  const float qd = sqrtf(__builtin_fmaf(ksx, ksx, ksz * ksz) * __builtin_fmaf(krx, krx, krz * krz));
-  float w = (float)((double)__builtin_fmaf(ksx, krx, ksz * krz) / (qd > 0 ? (double)qd : 1.0e-6));
-  if (w >= -1.0 && w <= 1.0) {
-    const double pi = 3.1415926535897932;
-    w = (float)(0.5 * (double)acosf(w));
-    w = (float)(pi / 2.0 - (double)w);
-    if (w > 0 && w <= pi / 3.0) {
-      const float dw = (float)(pi / 180);
-      const size_t ib = (size_t)(int)(w / dw) * nn + e;
-      float* g31 = ic + 5 * (size_t)nn;
-      float* g32 = ic + 66 * (size_t)nn;
-      g31[ib] = __builtin_fmaf(ps * p, cosf(w), g31[ib]);
-      g32[ib] = (float)((double)g32[ib] + (double)(ps * p) * 16.0 * pi * (double)sqrtf(vv) / (double)sinf(w));
+  if (ps * p != 0.0f) {
+    const float qd = sqrtf(__builtin_fmaf(ksx, ksx, ksz * ksz) * __builtin_fmaf(krx, krx, krz * krz));
+    float w = (float)((double)__builtin_fmaf(ksx, krx, ksz * krz) / (qd > 0 ? (double)qd : 1.0e-6));
+    if (w >= -1.0 && w <= 1.0) {
+      const double pi = 3.1415926535897932;
+      w = (float)(0.5 * (double)acosf(w));
+      w = (float)(pi / 2.0 - (double)w);
+      if (w > 0 && w <= pi / 3.0) {
+        const float dw = (float)(pi / 180);
+        const size_t ib = (size_t)(int)(w / dw) * nn + e;
+        float* g31 = ic + 5 * (size_t)nn;
+        float* g32 = ic + 66 * (size_t)nn;
+        g31[ib] = __builtin_fmaf(ps * p, cosf(w), g31[ib]);
+        g32[ib] = (float)((double)g32[ib] + (double)(ps * p) * 16.0 * pi * (double)sqrtf(vv) / (double)sinf(w));
+      }
     }
   }
   // cuda_energynorm (kernel.cu:1489-1529)
