@@ -988,8 +988,9 @@ __global__ __launch_bounds__(NTHR) void k_save(const ShotDev* __restrict__ shots
 // all-zero block; a denormal max-abs, e.g. source tails before the wave
 // arrives, must not underflow the scale to zero: |x| <= 32768 * FLT_MIN then
 // anyway) and stores x / scale rounded to nearest even.  Blocks of up to
-// SQV x SQT values (2 x 10 x 4915 cells) stay in registers between the two
-// steps; a larger one is read twice.  (Against a max-abs launch with an
+// SQV x SQT values stay in registers between the two steps (a block holds
+// 20 x nxl or 20 x nz values: windows up to 2457 cells wide and deep); a
+// larger one is read twice.  (Against a max-abs launch with an
 // atomic per workgroup followed by a quantising launch: 20.2 against 22.5 us
 // per step on 16 precision slots, one launch fewer.)
 constexpr int SQT = 1024, SQV = 48;

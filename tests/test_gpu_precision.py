@@ -57,3 +57,22 @@ def test_store_16bit_bit_exact_vs_oracle(gpu_ctx, fmt):
     i1, i2, il = O.rtm(O.make_params(**kw), vp, rho, stack64=True, korder=True, store_fmt=fmt)
     for name, a, b in (("image1", r.image1, i1), ("image2s", r.image2s, i2), ("illum", r.illum, il)):
         assert np.array_equal(a, b), (name, rel_l2(a, b))
+
+
+@pytest.mark.parametrize("fmt", [G.STORE_F16, G.STORE_BF16])
+def test_store_16bit_wide_window_bit_exact_vs_oracle(gpu_ctx, fmt):
+    """a 2600-column window: its top/bottom strip blocks hold 2 x 10 x 2600 =
+    52000 values, more than the save kernel keeps in registers (48 x 1024), so
+    the block is read twice -- still bit for bit the oracle's store"""
+    nz, nx = 40, 2600
+    rng = np.random.default_rng(12)
+    vp = (2200.0 + rng.uniform(-80, 80, (nz, nx))).astype(np.float32)
+    rho = np.full((nz, nx), 2000.0, np.float32)
+    kw = dict(nshot=1, startshot=1, dshot=1, flag_layout=2, nx=nx, nz=nz, nt=150, dx=10.0, dz=10.0, dt=1e-3,
+              npml=20, fdom=20.0, spx0=13000.0, spz0=20.0, offsetmax=13000.0)
+    P, Po = G.make_params(**kw), O.make_params(**kw)
+    assert 20 * G.geometry(P, 1).nxl > 48 * 1024
+    print(f"[precision] wide window, store_fmt={fmt}", flush=True)
+    r = gpu_ctx.run(P, vp, rho, store_fmt=fmt, want_shots=True)
+    o = O.run_shot(Po, 1, vp, rho, korder=True, store_fmt=fmt)
+    assert np.array_equal(r.shot_g1[0], o.g1) and np.array_equal(r.shot_illum[0], o.illum)
