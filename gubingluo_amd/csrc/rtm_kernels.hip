@@ -1091,6 +1091,14 @@ __device__ __forceinline__ const float* snap_field(const ShotDev& s, int f, bool
 }
 __device__ __forceinline__ unsigned int* snap_acc(const ShotDev& s) { return (unsigned int*)(s.snap_scale + 3 * s.nsnap); }
 
+// The snapshot kernels run after every step of a pass (a captured graph
+// holds the same launches for every step) and return at once unless the step
+// takes or restores a snapshot.  When it does, a block is the nz x nxl
+// interior of one field: one wave per row, rows dealt over the waves of
+// gridDim.y workgroups, a lane taking SU columns 64 apart per iteration
+// (independent loads in flight, no index division).
+constexpr int SU = 4, SWB = NTHR / 64;
+
 // forward, pass 1 (16-bit forms): max-abs of the workgroup's share of block
 // (q, f) after forward step `step` (its new level).  Grid (3 slots, GY).
 __global__ __launch_bounds__(NTHR) void k_snap_max(const ShotDev* __restrict__ shots, const int* __restrict__ slots,
@@ -1102,11 +1110,17 @@ __global__ __launch_bounds__(NTHR) void k_snap_max(const ShotDev* __restrict__ s
   const int q = snap_of_fwd(s, it, f);
   if (q < 0) return;
   const float* fld = snap_field(s, f, (it + 1) & 1);
-  const int nxl = s.nxl, n = s.nzi * nxl;
+  const int nxl = s.nxl, lane = threadIdx.x & 63;
   float m = 0.0f;
-  for (int e = blockIdx.y * NTHR + threadIdx.x; e < n; e += gridDim.y * NTHR) {
-    const int r = e / nxl, c = e - r * nxl;
-    m = fmaxf(m, fabsf(GCF(fld)[(long long)(s.npml + r) * s.pitch + s.npml + c]));
+  for (int r = blockIdx.y * SWB + (threadIdx.x >> 6); r < s.nzi; r += gridDim.y * SWB) {
+    const gfloat* row = GCF(fld) + (long long)(s.npml + r) * s.pitch + s.npml;
+    for (int c0 = lane; c0 < nxl; c0 += SU * 64) {
+      float a[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) a[u] = c0 + u * 64 < nxl ? fabsf(row[c0 + u * 64]) : 0.0f;
+#pragma unroll
+      for (int u = 0; u < SU; u++) m = fmaxf(m, a[u]);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
@@ -1127,7 +1141,7 @@ __global__ __launch_bounds__(NTHR) void k_snap_q(const ShotDev* __restrict__ sho
   const int q = snap_of_fwd(s, it, f);
   if (q < 0) return;
   const float* fld = snap_field(s, f, (it + 1) & 1);
-  const int nxl = s.nxl, n = s.nzi * nxl;
+  const int nxl = s.nxl, n = s.nzi * nxl, lane = threadIdx.x & 63;
   const long long base = (long long)(q * 3 + f) * n;
   float sc = 1.0f;
   if (s.snap_fmt != ST_F32) {
@@ -1135,17 +1149,27 @@ __global__ __launch_bounds__(NTHR) void k_snap_q(const ShotDev* __restrict__ sho
     sc = m > 0.0f ? fmaxf(m * (1.0f / 32768.0f), 1.17549435e-38f) : 1.0f;
   }
   if (blockIdx.y == 0 && threadIdx.x == 0) s.snap_scale[q * 3 + f] = sc;
-  for (int e = blockIdx.y * NTHR + threadIdx.x; e < n; e += gridDim.y * NTHR) {
-    const int r = e / nxl, c = e - r * nxl;
-    const float v = GCF(fld)[(long long)(s.npml + r) * s.pitch + s.npml + c];
-    if (s.snap_fmt == ST_F32) {
-      GF(s.snap)[base + e] = v;
-    } else if (s.snap_fmt == ST_F16) {
-      ((__half*)s.snap)[base + e] = __float2half_rn(v / sc);
-    } else {
-      unsigned int w = __float_as_uint(v / sc);       // bf16, round to nearest even
-      w += 0x7FFFu + ((w >> 16) & 1u);
-      ((unsigned short*)s.snap)[base + e] = (unsigned short)(w >> 16);
+  for (int r = blockIdx.y * SWB + (threadIdx.x >> 6); r < s.nzi; r += gridDim.y * SWB) {
+    const gfloat* row = GCF(fld) + (long long)(s.npml + r) * s.pitch + s.npml;
+    const long long ob = base + (long long)r * nxl;
+    for (int c0 = lane; c0 < nxl; c0 += SU * 64) {
+      float v[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) v[u] = c0 + u * 64 < nxl ? row[c0 + u * 64] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int c = c0 + u * 64;
+        if (c >= nxl) continue;
+        if (s.snap_fmt == ST_F32) {
+          GF(s.snap)[ob + c] = v[u];
+        } else if (s.snap_fmt == ST_F16) {
+          ((__half*)s.snap)[ob + c] = __float2half_rn(v[u] / sc);
+        } else {
+          unsigned int w = __float_as_uint(v[u] / sc);   // bf16, round to nearest even
+          w += 0x7FFFu + ((w >> 16) & 1u);
+          ((unsigned short*)s.snap)[ob + c] = (unsigned short)(w >> 16);
+        }
+      }
     }
   }
 }
@@ -1164,17 +1188,28 @@ __global__ __launch_bounds__(NTHR) void k_snap_load(const ShotDev* __restrict__ 
   if (q < 0) return;
   const int f = blockIdx.x % 3;
   float* fld = (float*)snap_field(s, f, (it + 1) & 1);
-  const int nxl = s.nxl, n = s.nzi * nxl;
+  const int nxl = s.nxl, n = s.nzi * nxl, lane = threadIdx.x & 63;
+  const int ncol = f == 1 ? nxl - 1 : nxl, nrow = f == 2 ? s.nzi - 1 : s.nzi;
   const long long base = (long long)(q * 3 + f) * n;
   const float sc = s.snap_scale[q * 3 + f];
-  for (int e = blockIdx.y * NTHR + threadIdx.x; e < n; e += gridDim.y * NTHR) {
-    const int r = e / nxl, c = e - r * nxl;
-    if ((f == 1 && c == nxl - 1) || (f == 2 && r == s.nzi - 1)) continue;
-    float v;
-    if (s.snap_fmt == ST_F32) v = GCF(s.snap)[base + e];
-    else if (s.snap_fmt == ST_F16) v = __half2float(((const __half*)s.snap)[base + e]) * sc;
-    else v = __uint_as_float((unsigned int)((const unsigned short*)s.snap)[base + e] << 16) * sc;
-    GF(fld)[(long long)(s.npml + r) * s.pitch + s.npml + c] = v;
+  for (int r = blockIdx.y * SWB + (threadIdx.x >> 6); r < nrow; r += gridDim.y * SWB) {
+    gfloat* row = GF(fld) + (long long)(s.npml + r) * s.pitch + s.npml;
+    const long long ib = base + (long long)r * nxl;
+    for (int c0 = lane; c0 < ncol; c0 += SU * 64) {
+      float v[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int c = c0 + u * 64;
+        v[u] = 0.0f;
+        if (c >= ncol) continue;
+        if (s.snap_fmt == ST_F32) v[u] = GCF(s.snap)[ib + c];
+        else if (s.snap_fmt == ST_F16) v[u] = __half2float(((const __half*)s.snap)[ib + c]) * sc;
+        else v[u] = __uint_as_float((unsigned int)((const unsigned short*)s.snap)[ib + c] << 16) * sc;
+      }
+#pragma unroll
+      for (int u = 0; u < SU; u++)
+        if (c0 + u * 64 < ncol) row[c0 + u * 64] = v[u];
+    }
   }
 }
 
@@ -1758,7 +1793,10 @@ hipError_t rtm_launch_snap(const ShotDev* shots, const int* slots, int nslots, i
                            hipStream_t st)
 {
   if (nslots <= 0) return hipSuccess;
-  const dim3 g(3 * nslots, 16), b(NTHR);
+  // about 2048 workgroups when a snapshot is taken (each returns at once on
+  // the other steps): 42 per block for 16 slots, 8 for 200
+  const int gy = std::min(64, std::max(8, 2048 / (3 * nslots)));
+  const dim3 g(3 * nslots, gy), b(NTHR);
   if (phase == PH_FWD) {
     if (snap_fmt != ST_F32) hipLaunchKernelGGL(k_snap_max, g, b, 0, st, shots, slots, step);
     hipLaunchKernelGGL(k_snap_q, g, b, 0, st, shots, slots, step);
