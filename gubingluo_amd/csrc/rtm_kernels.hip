@@ -78,8 +78,8 @@ __device__ __forceinline__ int xcd_tile(int ntiles)
 }
 
 // ---------------------------------------------------------------- strips --
-// store offsets of the boundary strips (kernel.cu:1186-1293 packing); -1 if
-// the cell is not part of the block.
+// store offset of a vx strip cell (kernel.cu:1186-1293 packing), -1 off the
+// strip (the halo columns of a reconstruction tile)
 __device__ __forceinline__ int strip_vx(const ShotDev& s, int iz, int ix)
 {
   if (iz < s.npml || iz >= s.nzp - s.npml) return -1;
@@ -89,35 +89,6 @@ __device__ __forceinline__ int strip_vx(const ShotDev& s, int iz, int ix)
   if (ix >= c0 && ix < c0 + RTM_R) return RTM_R * s.nzi + r * RTM_R + ix - c0;
   return -1;
 }
-__device__ __forceinline__ int strip_vz(const ShotDev& s, int iz, int ix)
-{
-  if (ix < s.npml || ix >= s.nxp - s.npml) return -1;
-  const int base = 2 * RTM_R * s.nzi;
-  const int c = ix - s.npml;
-  if (iz >= s.npml - RTM_R && iz < s.npml) return base + (iz - (s.npml - RTM_R)) * s.nxl + c;
-  const int r0 = s.nzp - s.npml - 1;
-  if (iz >= r0 && iz < r0 + RTM_R) return base + RTM_R * s.nxl + (iz - r0) * s.nxl + c;
-  return -1;
-}
-__device__ __forceinline__ int strip_p(const ShotDev& s, int iz, int ix)
-{
-  const int base_lr = 2 * RTM_R * s.nzi + 2 * RTM_R * s.nxl;
-  const int base_tb = base_lr + 2 * RTM_R * s.nzi;
-  if (iz >= s.npml && iz < s.nzp - s.npml) {
-    const int r = iz - s.npml;
-    if (ix >= s.npml - RTM_R && ix < s.npml) return base_lr + r * RTM_R + ix - (s.npml - RTM_R);
-    const int c0 = s.nxp - s.npml;
-    if (ix >= c0 && ix < c0 + RTM_R) return base_lr + RTM_R * s.nzi + r * RTM_R + ix - c0;
-    return -1;
-  }
-  if (ix < s.npml || ix >= s.nxp - s.npml) return -1;
-  const int c = ix - s.npml;
-  if (iz >= s.npml - RTM_R && iz < s.npml) return base_tb + (iz - (s.npml - RTM_R)) * s.nxl + c;
-  const int r0 = s.nzp - s.npml;
-  if (iz >= r0 && iz < r0 + RTM_R) return base_tb + RTM_R * s.nxl + (iz - r0) * s.nxl + c;
-  return -1;
-}
-
 // The same offsets split into a row part (wave-uniform: the lanes of a wave
 // share their rows) and a column part (per lane, computed once per tile):
 // own-column cells then cost one add and a compare.
@@ -166,13 +137,6 @@ __device__ __forceinline__ int soff_vx(int lr, const StripCols& c) { return (lr 
 __device__ __forceinline__ int soff_p(int lr, int tb, const StripCols& c)
 {
   return (lr >= 0 && c.plr >= 0) ? lr + c.plr : ((tb >= 0 && c.tb >= 0) ? tb + c.tb : -1);
-}
-
-// 16-bit store: block of a strip offset (0 vx, 1 vz, 2 p-lr, 3 p-tb)
-__device__ __forceinline__ int strip_block(const ShotDev& s, int off)
-{
-  const int a = 2 * RTM_R * s.nzi, b = a + 2 * RTM_R * s.nxl, c = b + 2 * RTM_R * s.nzi;
-  return off < a ? 0 : (off < b ? 1 : (off < c ? 2 : 3));
 }
 
 // A stored strip value is fetched with the first loads of a pass (raw: the
