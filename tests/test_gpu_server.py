@@ -67,21 +67,24 @@ def test_served_alone_and_mixed_equal_batch_engine(gpu_ctx, setup, fmt):
         assert q.peak_slots >= 3
 
 
-def test_served_vs_oracle(gpu_ctx, setup):
-    """Served requests (FP32 store) against the kernel-order oracle's job,
-    bit for bit, and against the reference order within the image bar."""
+@pytest.mark.parametrize("fmt", [G.STORE_F32, G.STORE_F16])
+def test_served_vs_oracle(gpu_ctx, setup, fmt):
+    """Served requests against the kernel-order oracle's job with the same
+    boundary store (FP32, and FP16 as the served benchmark runs), bit for bit;
+    the FP32 store also against the reference order within the image bar."""
     from oracle import oracle as O
     from tests.conftest import rel_l2
 
     mods, params = setup
-    with G.Server(gpu_ctx, max_slots=6) as srv:
+    with G.Server(gpu_ctx, max_slots=6, store_fmt=fmt) as srv:
         ids = [srv.submit(m, P) for m, P in zip(mods, params)]
         for k, (rid, P) in enumerate(zip(ids, params)):
             i1, i2, il, lap, nrm, st = srv.wait(rid, P, timeout=120)
             vp, rho = model(P.nz, P.nx, k)
             Po = O.make_params(**{n: getattr(P, n) for n, _ in P._fields_})
-            k1, k2, kl = O.rtm(Po, vp, rho, korder=True)
+            k1, k2, kl = O.rtm(Po, vp, rho, korder=True, store_fmt=fmt)
             assert np.array_equal(i1, k1) and np.array_equal(i2, k2) and np.array_equal(il, kl), k
-            f1, f2, fl = O.rtm(Po, vp, rho)
-            assert rel_l2(i1, f1) <= 1e-4 and rel_l2(i2, f2) <= 1e-4 and rel_l2(il, fl) <= 1e-4, k
-            print(f"[server] request {k} bit-exact vs the kernel-order oracle", flush=True)
+            if fmt == G.STORE_F32:
+                f1, f2, fl = O.rtm(Po, vp, rho)
+                assert rel_l2(i1, f1) <= 1e-4 and rel_l2(i2, f2) <= 1e-4 and rel_l2(il, fl) <= 1e-4, k
+            print(f"[server] store {fmt} request {k} bit-exact vs the kernel-order oracle", flush=True)
