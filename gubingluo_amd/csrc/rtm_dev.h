@@ -111,6 +111,9 @@ struct ShotDev {
 
 struct Tile {
   int slot, x0, z0, kind;
+  // table positions of the 8 tiles around this one in the same slot (-1:
+  // none), for the zero-tile skip of the step kernels
+  int nbr[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 struct LaunchArgs {
@@ -118,4 +121,8 @@ struct LaunchArgs {
   const Tile* tiles;
   int ntiles;
   int step;
+  // zero-tile skip of k_fwd (nullptr: every tile steps): per tile and field
+  // level one word whose byte g says that wave g stored a non-zero value of
+  // the forward field at that level; two levels of ntiles words
+  unsigned long long* act = nullptr;
 };

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <functional>
 #include <thread>
+#include <tuple>
 
 #include "host_io.h"
 #include "rtm_launch.h"
@@ -583,6 +584,7 @@ bool GraphKey::operator<(const GraphKey& o) const
   if (nslots != o.nslots) return nslots < o.nslots;
   if (guard != o.guard) return guard < o.guard;
   if (snap != o.snap) return snap < o.snap;
+  if (skip != o.skip) return skip < o.skip;
   if (shots != o.shots) return shots < o.shots;
   if (tiles != o.tiles) return tiles < o.tiles;
   return slots < o.slots;
@@ -761,6 +763,7 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
   LaunchArgs a{dshots, dtiles, k.ntiles, 0};
+  if (k.skip) a.act = d_act_;
   for (int s = 0; s < k.k; s++) {
     a.step = s;
     (void)rtm_launch_step(a, k.phase, k.tz, st_);
@@ -782,8 +785,12 @@ hipGraphExec_t Engine::graph_for(const GraphKey& k, const ShotDev* dshots, const
 
 int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
                      const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
-                     rtmh_stats* prof, int guard_every)
+                     rtmh_stats* prof, int guard_every, bool skip)
 {
+  // zero-tile skip words: every tile active until its first step has stepped
+  // (the words then follow the fields), whatever the fields hold now
+  unsigned long long* act = skip ? d_act_ : nullptr;
+  if (act) HIPCHK(hipMemsetAsync(act, 0xFF, 2 * (size_t)ntiles * sizeof(unsigned long long), st_));
   const int save = phase == PH_FWD ? 1 : 0;
   const int ic = phase == PH_REV ? ic_cells_ : 0;          // the other imaging conditions after each reverse step
   const int nslots = (int)slots.size();
@@ -794,6 +801,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     hipEvent_t ev[3];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     LaunchArgs a{dshots, dtiles, ntiles, 0};
+    a.act = act;
     const int kb = phase == PH_FWD ? 0 : 2;
     for (int s = 0; s < nsteps; s++) {
       a.step = s;
@@ -816,7 +824,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
   }
   if (chunk > 0 && nsteps >= chunk) {
     GraphKey key{ntiles, tz, phase, save, store_fmt, chunk, ic, nslots, guard_every > 0 ? 1 : 0,
-                 snap_.on() ? 1 + snap_.fmt : 0, dshots, dtiles, dslots};
+                 snap_.on() ? 1 + snap_.fmt : 0, act ? 1 : 0, dshots, dtiles, dslots};
     hipGraphExec_t ex = graph_for(key, dshots, dtiles, dslots, nslots, store_fmt);
     if (!ex) { set_error("hipGraph capture/instantiate failed"); return -11; }
     const int nch = nsteps / chunk;
@@ -824,6 +832,7 @@ int Engine::run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* d
     done = nch * chunk;
   }
   LaunchArgs a{dshots, dtiles, ntiles, 0};
+  a.act = act;
   const int rest = nsteps - done;
   for (int s = 0; s < rest; s++) {
     a.step = s;
@@ -948,6 +957,28 @@ void layout_runs(std::vector<Tile>& tiles, int phase)
     std::stable_partition(b, b + (cut[k + 1] - cut[k]), [flag](const Tile& t) { return (t.kind & flag) != 0; });
   }
   tiles.swap(out);
+}
+
+// Tile::nbr from the final table: tiles are keyed by (slot, z0, x0); the
+// neighbours of (z0, x0) are at z0 +- tz and x0 +- 64 in the same slot
+void link_neighbours(std::vector<Tile>& tiles, int tz)
+{
+  std::map<std::tuple<int, int, int>, int> at;
+  for (int i = 0; i < (int)tiles.size(); i++)
+    if (tiles[i].slot >= 0) at[std::make_tuple(tiles[i].slot, tiles[i].z0, tiles[i].x0)] = i;
+  for (Tile& t : tiles) {
+    int n = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dx = -1; dx <= 1; dx++) {
+        if (dz == 0 && dx == 0) continue;
+        int k = -1;
+        if (t.slot >= 0) {
+          auto it = at.find(std::make_tuple(t.slot, t.z0 + dz * tz, t.x0 + dx * RTM_TX));
+          if (it != at.end()) k = it->second;
+        }
+        t.nbr[n++] = k;
+      }
+  }
 }
 
 int params_check(const rtmh_params& P, std::string& why);
@@ -1075,13 +1106,15 @@ int Engine::ensure_cb(int nslots, size_t ntiles)
   if (cb_slots_ < 64) cb_slots_ = 64;
   cb_tiles_ = ntiles > cb_tiles_ ? ntiles * 2 : cb_tiles_;
   const size_t bytes = align256(cb_slots_ * sizeof(ShotDev)) + align256(cb_tiles_ * sizeof(Tile)) +
-                       align256(cb_slots_ * 4) + align256(cb_slots_ * sizeof(StackShotHost)) + 256;
+                       align256(cb_slots_ * 4) + align256(cb_slots_ * sizeof(StackShotHost)) + 256 +
+                       align256(2 * cb_tiles_ * sizeof(unsigned long long));
   HIPCHK(hipMalloc(&cb_, bytes));
   d_ds_ = (ShotDev*)cb_;
   d_tiles_ = (Tile*)((char*)cb_ + align256(cb_slots_ * sizeof(ShotDev)));
   d_slots_ = (int*)((char*)d_tiles_ + align256(cb_tiles_ * sizeof(Tile)));
   d_stk_ = (StackShotHost*)((char*)d_slots_ + align256(cb_slots_ * 4));
   d_flag_ = (unsigned int*)((char*)d_stk_ + align256(cb_slots_ * sizeof(StackShotHost)));
+  d_act_ = (unsigned long long*)((char*)d_flag_ + 256);
   return 0;
 }
 
@@ -1272,6 +1305,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
     }
     build_tiles(ds, tz, tiles);
     layout_runs(tiles, PH_FWD);
+    link_neighbours(tiles, tz);
     // room for any height's padded table (runs at most 1.25 x the even share)
     tcap = std::max(tcap, tiles.size()) * 5 / 4 + 16;
     // the first 3 x kTuneSteps steps of a pass, kTuneSteps with each candidate
@@ -1336,10 +1370,11 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       fwd_done = 3 * kTuneSteps;
       build_tiles(ds, tz, tiles);
       layout_runs(tiles, PH_FWD);
+      link_neighbours(tiles, tz);
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     }
     rc = run_pass(PH_FWD, nt - fwd_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt,
-                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every);
+                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every, true);
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e1, st_));
     HIPCHK(hipStreamSynchronize(st_));

@@ -170,6 +170,7 @@ struct GraphKey {
   int nslots;                              // slots of the batch (k_save / k_ic grids, k_bump count)
   int guard;                               // wavefield guard after the chunk
   int snap;                                // snapshot launches after every step: 0 off, else 1 + format
+  int skip;                                // zero-tile skip of the step kernels
   const void* shots;
   const void* tiles;
   const void* slots;
@@ -267,6 +268,7 @@ class Engine {
   int* d_slots_ = nullptr;
   struct StackShotHost* d_stk_ = nullptr;
   unsigned int* d_flag_ = nullptr;      // wavefield guard of the running pass (largest |p| bits)
+  unsigned long long* d_act_ = nullptr; // zero-tile skip words of the running pass (2 x tiles)
   int ic_cells_ = 0;                    // the other imaging conditions of the running batch: interior cells (0: off)
   SnapPlan snap_;                       // snapshots of the running job
   int ensure_cb(int nslots, size_t ntiles);
@@ -275,7 +277,7 @@ class Engine {
   // launches) or after every replayed chunk, and after the last step
   int run_pass(int phase, int nsteps, const ShotDev* dshots, const Tile* dtiles, int ntiles,
                const std::vector<int>& slots, const int* dslots, int tz, int store_fmt, int chunk,
-               rtmh_stats* prof, int guard_every = 0);
+               rtmh_stats* prof, int guard_every = 0, bool skip = false);
   // clears the guard flag before a pass; after it, checks the flag and names
   // the shots whose wavefield is not finite (-7)
   int guard_clear();
@@ -291,6 +293,9 @@ void build_tiles(const std::vector<ShotDev>& ds, int tz, std::vector<Tile>& tile
 // the table as 8 per-XCD runs of equal estimated cost, long tiles first in
 // each, padded with empty entries (slot -1) (phase: PH_FWD / PH_REV)
 void layout_runs(std::vector<Tile>& tiles, int phase);
+// Tile::nbr of every tile: the table positions of the tiles of height tz
+// above, below and beside it in its slot (after the final layout)
+void link_neighbours(std::vector<Tile>& tiles, int tz);
 int params_check(const rtmh_params& P, std::string& why);
 // the stability bound of get_constant (RTM_GPU_velstr.cpp:355-361):
 // min(dx, dz) / (sqrt(2) sum|c| vmax), sum over the order-20 coefficient row

@@ -385,9 +385,12 @@ template <class Q, int FK>
 __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDev& s, const Tile& tl, int it,
                                            const float* pc, const float* vxc, const float* vzc, float* pn, float* vxn,
                                            float* vzn, const float* __restrict__ ctab, const float (&ps)[Q::RPT],
-                                           float (&pnew)[Q::RPT])
+                                           float (&pnew)[Q::RPT], unsigned char* actw = nullptr)
 {
   constexpr int G = Q::G, RPT = Q::RPT, TZ = Q::TZ, NT = Q::NT, KZ = Q::KZ, KP = Q::KP, KH = Q::KH;
+  // the OR of the bit patterns of the lane's stored field values (-0 counts
+  // as non-zero): written to actw[wave] at the end for the zero-tile skip
+  unsigned int nzb = 0u;
   constexpr int PW = Q::PW, VXW = Q::VXW, ZR = Q::ZR;
   float* PS = lds;                                     // p rows z0-19.., columns x0-20..x0+83
   float* DMP = lds + Q::LDS;                           // damping table (damped passes of gen tiles)
@@ -555,6 +558,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
       if (kz >= 10 && kz < 10 + TZ) {                  // own rows (wave-uniform)
         if (!gen) bst(Rvzn, vzn_[k], vcol, srow0 + (kz - 10) * pr4);   // every lane's cell exists
         else bst(Rvzn, vzn_[k], pred_off(vcol, ix < nxp && iz <= nzp - 2), srow0 + (kz - 10) * pr4);
+        nzb |= __float_as_uint(vzn_[k]);
       }
     }
   }
@@ -626,6 +630,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 #undef PCA
       if (!gen) bst(Rvxn, vxn_[r], vcol, srow0 + j * pr4);
       else bst(Rvxn, vxn_[r], pred_off(vcol, iz < nzp && ix <= nxp - 2), srow0 + j * pr4);
+      nzb |= __float_as_uint(vxn_[r]);
     }
 #pragma unroll
     for (int k = 0; k < KH; k++) {
@@ -769,6 +774,7 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           const int pq = pred_off(vcol, split);
           bst(R1, q1, pq, so);
           bst(R2, q2, pq, so);
+          if (split) nzb |= __float_as_uint(q1) | __float_as_uint(q2);
           if (split) pu = q1 + q2;
         }
         if (upd) p = pu;
@@ -816,9 +822,43 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
           }
         }
       }
+      nzb |= __float_as_uint(p);
     }
     pnew[r] = p;
   }
+  if (FK == FK_SRC && actw) {
+    const bool any = any_lane(nzb != 0u);
+    if (lx == 0) actw[g] = any ? 1 : 0;
+  }
+}
+
+// Zero-tile skip of the forward step: a tile skips the step when the forward
+// field is zero at the level the step reads, on the tile and its 8
+// neighbours (all the step's inputs are then +0 and so would be its
+// outputs), and at the level it writes, on the tile itself (the buffer
+// already holds those +0).  The wavefield starts at +0 and spreads from the
+// source, so far tiles skip until it comes near.  Words of both levels are
+// fetched beside the tile's other first loads; `it` picks the level.  (The
+// same skip of the reverse step's receiver field cost more than it saved:
+// that field rarely has a zero neighbourhood.)
+struct SkipWords { unsigned long long self0, self1, nb0, nb1; };   // (no arrays: a runtime index would spill)
+__device__ __forceinline__ SkipWords skip_words(const unsigned long long* act, int ntiles, int t, const Tile& tl)
+{
+  SkipWords w{act[t], act[ntiles + t], 0ull, 0ull};
+#pragma unroll
+  for (int j = 0; j < 8; j++)
+    if (tl.nbr[j] >= 0) {
+      w.nb0 |= act[tl.nbr[j]];
+      w.nb1 |= act[ntiles + tl.nbr[j]];
+    }
+  return w;
+}
+__device__ __forceinline__ bool skip_of(const SkipWords& w, int it)
+{
+  const bool odd = it & 1;
+  const unsigned long long in = odd ? (w.self1 | w.nb1) : (w.self0 | w.nb0);
+  const unsigned long long out = odd ? w.self0 : w.self1;
+  return (in | out) == 0ull;
 }
 
 }  // namespace
@@ -835,16 +875,21 @@ __global__ __launch_bounds__(64 * G) void k_fwd(LaunchArgs a)
   if (t < 0) return;
   const Tile tl = a.tiles[t];
   if (tl.slot < 0) return;                             // padding of a shorter XCD run
+  const bool track = a.act && !(tl.kind & (TF_SRC | TF_REC));
+  SkipWords sw{};
+  if (track) sw = skip_words(a.act, a.ntiles, t, tl);
   const ShotDev& s = shot_of(a.shots, tl.slot);
-  if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const int it = s.it_off + s.it_dir * a.step;
+  if (track && skip_of(sw, it)) return;                // the forward field is +0 around the tile
+  if (threadIdx.x < 100) ctab[threadIdx.x] = ctab_entry(s, threadIdx.x);
   const bool odd = it & 1;                             // (a runtime index into s would spill it to scratch)
   float none[Q::RPT], pnew[Q::RPT];
 #pragma unroll
   for (int r = 0; r < Q::RPT; r++) none[r] = 0.0f;
+  unsigned char* actw = a.act ? (unsigned char*)(a.act + (size_t)((it + 1) & 1) * a.ntiles + t) : nullptr;
   field_step<Q, FK_SRC>(lds, s, tl, it, odd ? s.sp[1] : s.sp[0], odd ? s.svx[1] : s.svx[0],
                         odd ? s.svz[1] : s.svz[0], odd ? s.sp[0] : s.sp[1], odd ? s.svx[0] : s.svx[1],
-                        odd ? s.svz[0] : s.svz[1], ctab, none, pnew);
+                        odd ? s.svz[0] : s.svz[1], ctab, none, pnew, actw);
 }
 
 // reverse time step: source reconstruction, then the receiver wavefield and

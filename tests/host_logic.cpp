@@ -6,6 +6,7 @@
 // the first failure.
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <unistd.h>
 
 #include <deque>
@@ -265,8 +266,45 @@ static void extents_checks()
   printf("extents: %d blocks, %d refusals, first fit and merging hold\n", got, refused);
 }
 
+// ---- neighbour links of the zero-tile skip ----------------------------------
+// Two slots of different windows through build_tiles, the per-XCD layout and
+// link_neighbours: every link of a tile points at the tile one step away in
+// the same slot, and every such tile is linked; padding entries link nothing.
+static void neighbour_checks()
+{
+  std::vector<ShotDev> ds(2);
+  memset(ds.data(), 0, ds.size() * sizeof(ShotDev));
+  ds[0].npml = 40; ds[0].nxp = 300; ds[0].nzp = 170;
+  ds[1].npml = 40; ds[1].nxp = 701; ds[1].nzp = 150;
+  for (int tz : {16, 32}) {
+    std::vector<Tile> tiles;
+    rtm::build_tiles(ds, tz, tiles);
+    rtm::layout_runs(tiles, PH_REV);
+    rtm::link_neighbours(tiles, tz);
+    int links = 0, bad = 0;
+    for (size_t i = 0; i < tiles.size(); i++) {
+      const Tile& t = tiles[i];
+      int n = 0;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dx = -1; dx <= 1; dx++) {
+          if (dz == 0 && dx == 0) continue;
+          const int k = t.nbr[n++];
+          int want = -1;
+          if (t.slot >= 0)
+            for (size_t j = 0; j < tiles.size(); j++)
+              if (tiles[j].slot == t.slot && tiles[j].z0 == t.z0 + dz * tz && tiles[j].x0 == t.x0 + dx * 64) want = (int)j;
+          if (k != want) bad++;
+          links += k >= 0;
+        }
+    }
+    CHECK(bad == 0 && links > 0, "tz %d: %d wrong links of %d", tz, bad, links);
+  }
+  printf("ok neighbour links of the zero-tile skip\n");
+}
+
 int main(int argc, char** argv)
 {
+  neighbour_checks();
   extents_checks();
   admission_checks();
   checkpoint_checks(argc > 1 ? argv[1] : ".");
