@@ -181,7 +181,10 @@ def roofline_of(recs, P, copy_gbs=None):
             "avg_launch_us": round(avg_step_s * 1e6, 2), "alg_bytes_per_launch": step_bytes,
             "forward_step": {"kernel": "k_fwd", "achieved": round(fwd_bytes / (fwd_ms * 1e-3 / fwd_steps) / 1e9, 1),
                              "avg_launch_us": round(fwd_ms * 1e3 / fwd_steps, 2),
-                             "alg_bytes_per_launch": fwd_bytes}}
+                             "alg_bytes_per_launch": fwd_bytes,
+                             "note": "bytes of every tile of the step; tiles whose forward field is still +0 "
+                                     "around them skip the step and move none of theirs, so this rate is the "
+                                     "step's work per time, not an HBM rate"}}
 
 
 def base_line(args, value, unit, elapsed, dtype, data, config, extra):
