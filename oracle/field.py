@@ -14,11 +14,13 @@ FFTW's transforms are restated with numpy.fft (pocketfft): fftw FORWARD is
 np.fft.fft* and BACKWARD is n * np.fft.ifft* (FFTW does not normalise).  The
 f-k results therefore agree with any other FFT to rounding, not bit for bit.
 
-Parity unpinned: fk_interp's steps for lx >= 2 that the lx = 1 closed form
-of the tests leaves out (the block average of the padded spectrum, the
-|H| > lx^2 clip, Interp_function.cpp:170-199) rest on reading the source;
-Interp_function.cpp needs FFTW, which is absent, so no output of the
-reference's own code pins them.
+fk_interp is pinned by closed forms of events constant over traces: for
+lx = 1 the spectrum's path, for lx >= 2 the replication, the zero padding and
+the block average of the padded spectrum (the box sums of the lx aliases give
+every fine trace lx N(x mod lx) / ceil(nx0 / lx) times the lx = 1 output).
+Parity unpinned: the |H| > lx^2 clip (Interp_function.cpp:190-199), which
+those cases never reach, rests on reading the source; Interp_function.cpp
+needs FFTW, which is absent, so no output of the reference's own code pins it.
 """
 from __future__ import annotations
 

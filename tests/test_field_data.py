@@ -120,6 +120,31 @@ def test_fk_single_factor_closed_form():
     assert np.allclose(out, exp[:, None], rtol=0, atol=1e-6 * A)
 
 
+@pytest.mark.parametrize("lx,nx0", [(2, 8), (3, 9), (4, 8), (2, 7), (3, 8), (3, 10)])
+def test_fk_refinement_closed_form(lx, nx0):
+    """lx >= 2, an event constant over traces (a cosine at bin k): its f-k
+    spectrum lives at kx = 0 only, C repeats it at kx = m nx0, and the spectrum
+    of the zero-padded data there is the box sum D_m = sum_{x < nx0}
+    exp(-2 pi i m x / lx) times one time factor, which cancels in H = S / Z.
+    Z averages the lx aliases: sum_m D_m = lx ceil(nx0 / lx).  So fine trace x
+    gets sum_m H_m exp(2 pi i m x / lx) = lx N(x mod lx) / ceil(nx0 / lx) times
+    the lx = 1 output (pinned above), N(r) = #{x' < nx0: x' = r mod lx}: lx on
+    every trace when lx divides nx0, and alternating levels when it does not
+    (H stays below lx^2, so the clip is not reached; the 1e-5 regularisation
+    is the tolerance)"""
+    nt0, k = 64, 5
+    t = np.arange(nt0)
+    sig = np.repeat(np.cos(2 * np.pi * k * t / nt0)[:, None], nx0, 1).astype(np.float32)
+    base = F.fk_interp(sig, nx0, 1)[:, :1].astype(np.float64)
+    out = F.fk_interp(sig, lx * nx0, lx).astype(np.float64)
+    assert out.shape == (nt0, lx * nx0)
+    ceil = -(-nx0 // lx)
+    for x in range(lx * nx0):
+        n = len(range(x % lx, nx0, lx))
+        exp = lx * n / ceil * base[:, 0]
+        assert np.max(np.abs(out[:, x] - exp)) <= 1e-3 * np.max(np.abs(exp)), (x, n)
+
+
 def test_interp_dispatch_and_copy():
     rng = np.random.default_rng(4)
     a = rng.standard_normal((50, 8)).astype(np.float32)
