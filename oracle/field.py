@@ -18,7 +18,7 @@ fk_interp is pinned by closed forms of events constant over traces: for
 lx = 1 the spectrum's path, for lx >= 2 the replication, the zero padding and
 the block average of the padded spectrum (the box sums of the lx aliases give
 every fine trace lx N(x mod lx) / ceil(nx0 / lx) times the lx = 1 output).
-Parity unpinned: the |H| > lx^2 clip (Interp_function.cpp:190-199), which
+Parity unpinned: the |H| > lx^2 clip (Interp_function.cpp:205-207), which
 those cases never reach, rests on reading the source; Interp_function.cpp
 needs FFTW, which is absent, so no output of the reference's own code pins it.
 """
