@@ -1,6 +1,7 @@
 // host_logic.cpp -- CPU checks of the host logic of librtm_hip.so that needs
 // no GPU: the served form's admission rules (driven here by a fake device
-// allocator), the first-fit arena its slots are carved from, and the
+// allocator), the first-fit arena its slots are carved from, the per-XCD runs
+// of the tile table and the neighbour links of the zero-tile skip, and the
 // checkpoint resume decision.  Built and run by
 // tests/test_host_logic.py; prints one line per check and exits non-zero on
 // the first failure.
@@ -9,6 +10,8 @@
 #include <string.h>
 #include <unistd.h>
 
+#include <algorithm>
+#include <cmath>
 #include <deque>
 #include <string>
 #include <vector>
@@ -302,8 +305,74 @@ static void neighbour_checks()
   printf("ok neighbour links of the zero-tile skip\n");
 }
 
+// ---- per-XCD runs of the tile table -----------------------------------------
+// layout_runs on a ragged batch: the table becomes 8 runs of one length (the
+// kernels deal run c to XCD c), every tile of the input exactly once, empty
+// entries only at the ends of runs, each run's long tiles (reverse: holding
+// reconstructed cells; forward: in the absorbing layer) before its short
+// ones, each run a contiguous piece of the input (neighbours stay on one
+// XCD), and -- when cut by cost -- run costs within one long tile of the
+// even share.
+static void layout_checks()
+{
+  std::vector<ShotDev> ds(5);
+  memset(ds.data(), 0, ds.size() * sizeof(ShotDev));
+  const int w[5] = {300, 701, 450, 1300, 520}, h[5] = {170, 150, 260, 210, 180};
+  for (int k = 0; k < 5; k++) { ds[k].npml = 40; ds[k].nxp = w[k]; ds[k].nzp = h[k]; }
+  for (int phase : {PH_FWD, PH_REV}) {
+    std::vector<Tile> in;
+    rtm::build_tiles(ds, 32, in);
+    std::vector<Tile> t = in;
+    rtm::layout_runs(t, phase);
+    const int flag = phase == PH_REV ? TF_INT : TF_GEN;
+    const double heavy = phase == PH_REV ? 2.0 : 1.55;
+    CHECK(t.size() % 8 == 0, "phase %d: table of %zu entries is not 8 runs", phase, t.size());
+    const size_t len = t.size() / 8;
+    // every input tile exactly once
+    auto key = [](const Tile& x) { return ((long long)x.slot << 40) | ((long long)x.z0 << 20) | x.x0; };
+    std::vector<long long> a, b;
+    for (const Tile& x : in) a.push_back(key(x));
+    for (const Tile& x : t) if (x.slot >= 0) b.push_back(key(x));
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    CHECK(a == b, "phase %d: tiles lost or repeated (%zu in, %zu out)", phase, a.size(), b.size());
+    double total = 0;
+    for (const Tile& x : in) total += (x.kind & flag) ? heavy : 1.0;
+    double worst = 0;
+    size_t next = 0;                                 // the runs take the input in order, piece by piece
+    for (int c = 0; c < 8; c++) {
+      bool pad = false, shortseen = false, ok = true;
+      double cost = 0;
+      std::vector<long long> run;
+      for (size_t i = c * len; i < (c + 1) * len; i++) {
+        const Tile& x = t[i];
+        if (x.slot < 0) { pad = true; continue; }
+        if (pad) ok = false;                         // a tile after padding
+        const bool lng = (x.kind & flag) != 0;
+        if (lng && shortseen) ok = false;            // a long tile after a short one
+        shortseen |= !lng;
+        cost += lng ? heavy : 1.0;
+        run.push_back(key(x));
+      }
+      CHECK(ok, "phase %d run %d: order (long, then short, then padding) broken", phase, c);
+      // the run is the next contiguous piece of the input, reordered
+      std::vector<long long> piece;
+      for (size_t i = next; i < next + run.size() && i < in.size(); i++) piece.push_back(key(in[i]));
+      next += run.size();
+      std::sort(run.begin(), run.end());
+      std::sort(piece.begin(), piece.end());
+      CHECK(run == piece, "phase %d run %d is not a contiguous piece of the input", phase, c);
+      worst = std::max(worst, std::abs(cost - total / 8));
+    }
+    CHECK(worst <= heavy + 1e-9 || len == (in.size() + 7) / 8,
+          "phase %d: a run's cost is %.2f away from the even share %.2f", phase, worst, total / 8);
+  }
+  printf("ok per-XCD runs of the tile table\n");
+}
+
 int main(int argc, char** argv)
 {
+  layout_checks();
   neighbour_checks();
   extents_checks();
   admission_checks();

@@ -4,7 +4,10 @@ served form's admission rules driven by a fake device allocator (a shot
 larger than the budget is refused, never queued forever; a device that
 refuses memory with nothing resident fails the request; one-shot budgets
 admit in arrival order), the server's first-fit device arena (blocks never
-overlap, the lowest fitting hole is taken, freed neighbours merge) and the checkpoint resume decision (a checkpoint
+overlap, the lowest fitting hole is taken, freed neighbours merge), the
+per-XCD runs of the tile table (every tile once, long tiles first, padding at
+run ends, runs of equal cost, each a contiguous piece), the neighbour links of
+the forward step's zero-tile skip, and the checkpoint resume decision (a checkpoint
 with more shots done than the job has is refused)."""
 import os
 import subprocess
