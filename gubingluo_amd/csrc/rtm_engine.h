@@ -68,10 +68,11 @@ class Extents {
     blocks_.clear();
     if (bytes) holes_[0] = bytes;
   }
-  // offset of a block of at least `bytes`, kNone when no free range holds it
+  // offset of a block of at least `bytes` (at least one unit, so that no two
+  // blocks share an offset), kNone when no free range holds it
   size_t get(size_t bytes)
   {
-    const size_t need = (bytes + 255) & ~(size_t)255;
+    const size_t need = bytes ? (bytes + 255) & ~(size_t)255 : 256;
     for (auto it = holes_.begin(); it != holes_.end(); ++it) {
       if (it->second < need) continue;
       const size_t off = it->first, len = it->second;

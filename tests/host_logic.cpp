@@ -266,6 +266,9 @@ static void extents_checks()
   CHECK(e.holes().size() == 1 && e.holes().begin()->first == 0 && e.holes().begin()->second == R && e.blocks().empty(),
         "after every block came back: %zu free ranges, %zu blocks", e.holes().size(), e.blocks().size());
   CHECK(e.get(R) == 0 && e.get(1) == rtm::Extents::kNone, "the whole range as one block");
+  e.reset(1024);
+  const size_t z0 = e.get(0), z1 = e.get(0);
+  CHECK(z0 != z1 && z0 != rtm::Extents::kNone && z1 != rtm::Extents::kNone, "zero-byte blocks share offset %zu", z0);
   printf("extents: %d blocks, %d refusals, first fit and merging hold\n", got, refused);
 }
 
