@@ -20,7 +20,9 @@ computes (decomposed, Poynting and energy-norm images).
 
 The default run also measures, as sub-objects of the same line (skip with
 --quick): "plain_form" (survey shots one per launch with per-step launches,
-the plain GPU form), "error_vs_oracle" (the GPU's images of the survey shots
+the plain GPU form), "single_shot" (the single-shot configuration, 400x200
+nt 2000, on the GPU in the plain form and the default engine form; the CPU
+baseline runs it too), "error_vs_oracle" (the GPU's images of the survey shots
 the CPU baseline runs, against those of the oracle), "batched" (one 64-shot
 launch batch of the ragged 2048x768
 survey), "precision" (2048x1024 dx=7.5 m nt=6000 shots with FP32 / FP16 /
@@ -42,6 +44,8 @@ Other workloads (one JSON line each, same keys):
                         grids 256^2..2048x1024, nt 1000-6000, 1-32 shots,
                         FP16 boundaries, continuous batching; against the
                         plain one-shot-per-launch GPU form on the same stream
+  --workload single     the single-shot configuration (400x200, npml 40,
+                        nt 2000) on the GPU: plain form and engine form
   --workload field      field-data path on the survey model: 128 recorded
                         one-sided gathers (151 traces at 20 m, 2000 samples
                         at 1.6 ms, seeded random samples) scanned, read and
@@ -252,6 +256,7 @@ def run_survey(args, G, synth):
     gpu_shots = None
     if not args.quick:
         out["plain_form"] = plain_sub(args, G, ctx, P, mid, out["value"])
+        out["single_shot"] = single_sub(args, G, synth, ctx)
         if not args.no_cpu:
             # the survey shots the CPU baseline runs, as the GPU images them
             Q = G.Params.from_buffer_copy(P)
@@ -301,6 +306,41 @@ def batched_sub(args, G, synth, ctx):
            "k_rev_achieved_gbs": roof["achieved"], "k_rev_frac": roof["frac"]}
     log(f"[bench] batched: {out['value']} Gcell/s, k_rev {roof['frac']:.3f} of 8 TB/s")
     return out
+
+
+def single_sub(args, G, synth, ctx):
+    """The reference's CPU-runnable case on the GPU: one shot of the
+    single-shot anomaly model (400x200, npml 40, nt 2000), as the plain form
+    (one shot per launch, per-step launches, FP32 boundaries, 16-row tiles)
+    and as the default engine (graph chunk, autotuned tile heights); one
+    untimed run of each, then one timed."""
+    P, vp, rho = synth.single_shot_anomaly()
+    mid = ctx.model_create(vp, rho)
+    out = {"workload": "single-shot anomaly model 400x200, dx=10 m, npml 40, nt=2000, 400 receivers, one shot, "
+                       "images and post-processing included"}
+    forms = (("plain_form", dict(batch=1, graph_chunk=0, tile_z=16, store_fmt=G.STORE_F32)),
+             ("engine", dict(graph_chunk=args.chunk, tile_z=0)))
+    for name, kw in forms:
+        ctx.run(P, model=mid, want_post=True, **kw)
+        t0 = time.perf_counter()
+        r = ctx.run(P, model=mid, want_post=True, **kw)
+        dt = time.perf_counter() - t0
+        out[name] = {"s_per_shot": round(dt, 4), "gcell_s": round(r.stats.cell_updates / dt / 1e9, 3)}
+        log(f"[bench] single shot, {name}: {dt * 1e3:.1f} ms, {out[name]['gcell_s']} Gcell/s")
+    ctx.model_destroy(mid)
+    return out
+
+
+def run_single(args, G, synth):
+    args.steps, args.warmup = 1, 1                      # one timed shot after one untimed
+    ctx = G.Context(0)
+    sub = single_sub(args, G, synth, ctx)
+    ctx.close()
+    e = sub["engine"]
+    return base_line(args, e["gcell_s"], "Gcell/s", e["s_per_shot"], "f32",
+                     "synthetic: the two-layer anomaly model, data modelled in the same run",
+                     {"workload": sub["workload"], "shots_per_step": 1},
+                     {"plain_form": sub["plain_form"], "cpu_baseline": None})
 
 
 def plain_sub(args, G, ctx, P, mid, batched_value):
@@ -667,7 +707,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (survey default 2, others 10)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="survey", choices=["survey", "batched", "precision", "serve", "field"])
+    ap.add_argument("--workload", default="survey", choices=["survey", "batched", "precision", "serve", "field", "single"])
     ap.add_argument("--shots-per-step", type=int, default=0,
                     help="shots per step (survey default 200: the whole survey in one launch batch)")
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0: all shots of the step)")
@@ -712,6 +752,8 @@ def main():
         out = run_precision(args, G, synth)
     elif args.workload == "field":
         out = run_field(args, G, synth)
+    elif args.workload == "single":
+        out = run_single(args, G, synth)
     else:
         out = run_serve(args, G, synth)
     print(json.dumps(out), flush=True)
