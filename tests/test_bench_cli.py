@@ -66,3 +66,12 @@ def test_survey_error_is_the_worst_shot():
     e = bench.survey_error(gpu, ref)["max_rel_l2"]
     want = np.linalg.norm(gpu[2][0] - ref[2][0]) / np.linalg.norm(ref[2][0])
     assert abs(e["g1"] - want) <= 1e-3 * want and e["illum"] == 0.0
+
+
+def test_every_workload_is_a_choice():
+    """each documented workload parses (the one-GPU check runs before any GPU
+    work, so --gpus 2 ends every one of them with the same message)"""
+    for w in ("survey", "batched", "precision", "serve", "field", "single"):
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--workload", w, "--gpus", "2"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 2 and "one GPU" in r.stderr, (w, r.stderr[-300:])
