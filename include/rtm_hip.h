@@ -343,7 +343,7 @@ typedef struct rtmh_server_options {
   double mem_budget;  /* bytes of HBM for slots and request stacks; 0 = 80% of the
                          memory free once the context's cached slot memory is
                          released.  rtmh_server_create takes the whole budget
-                         in one device allocation (about 21 ms per GB) and
+                         in one device allocation (up to ~21 ms per GB) and
                          carves slots from it; a budget the device cannot
                          give in one piece is allocated as shots need it */
   int max_slots;      /* shots resident at once; 0 = 256                          */
