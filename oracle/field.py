@@ -17,10 +17,12 @@ f-k results therefore agree with any other FFT to rounding, not bit for bit.
 fk_interp is pinned by closed forms of events constant over traces: for
 lx = 1 the spectrum's path, for lx >= 2 the replication, the zero padding and
 the block average of the padded spectrum (the box sums of the lx aliases give
-every fine trace lx N(x mod lx) / ceil(nx0 / lx) times the lx = 1 output).
-Parity unpinned: the |H| > lx^2 clip (Interp_function.cpp:205-207), which
-those cases never reach, rests on reading the source; Interp_function.cpp
-needs FFTW, which is absent, so no output of the reference's own code pins it.
+every fine trace lx N(x mod lx) / ceil(nx0 / lx) times the lx = 1 output),
+and the |H| > lx^2 clip (Interp_function.cpp:205-207) by two spikes whose
+aliases cancel in Z: every |H| above lx^2 gives lx^2 times the lx = 1 output at
+the spread traces, none above it gives the data plus a shifted copy, and two
+amplitudes bracket the threshold.  Interp_function.cpp needs FFTW, which is
+absent, so no output of the reference's own code is compared.
 """
 from __future__ import annotations
 

@@ -145,6 +145,54 @@ def test_fk_refinement_closed_form(lx, nx0):
         assert np.max(np.abs(out[:, x] - exp)) <= 1e-3 * np.max(np.abs(exp)), (x, n)
 
 
+@pytest.mark.parametrize("lx,nx0,x1,x2,eps_clip,eps_free", [(2, 8, 5, 2, 0.18, 0.36), (3, 8, 4, 3, 0.095, 0.13)])
+def test_fk_clip_closed_form(lx, nx0, x1, x2, eps_clip, eps_free):
+    """The |H| > lx^2 clip (Interp_function.cpp:205-207), on two spikes at the
+    same time t0: trace x1 (x1 != 0 mod lx, amplitude 1) and trace x2
+    (x2 = 0 mod lx, amplitude eps).  The lx aliases of trace x1 cancel in Z, so
+    Z = eps exp(-2 pi i kk x2 / Nx) at every wavenumber kk of Nx = lx nx0, the
+    time factor cancels in S / Z, and H(kk) = 1 + exp(-2 pi i kk (x1 - x2) / Nx)
+    / eps, with |H| between (1 - eps) / eps and (1 + eps) / eps.
+    - eps below 1 / (lx^2 + 1): every |H| > lx^2, H = lx everywhere, and the
+      output is lx^2 times the lx = 1 output q of trace x1 at fine trace lx x1,
+      eps lx^2 q at lx x2, zero elsewhere;
+    - eps above 1 / (lx^2 - 1): no |H| reaches lx^2, H is the inverse
+      transform of delta(0) + delta(x1 - x2) / eps, so the replicated spectrum
+      (the data spread to fine traces lx m) comes back plus a copy shifted by
+      x1 - x2 and scaled by 1 / eps: lx q at lx x1, eps lx q at lx x2, lx q / eps
+      at lx x1 + x1 - x2, lx q at lx x2 + x1 - x2, zero elsewhere.
+    The two eps bracket the threshold: (1 + 0.13) / 0.13 = 8.7 < 9 < 9.5 =
+    (1 - 0.095) / 0.095 for lx = 3, so a threshold of lx or 2 lx, a clip to lx^2
+    or to a complex value, or no clip fails; the 1e-5 regularisation of Z is the
+    tolerance.  Both the oracle and the library."""
+    nt0, t0 = 64, 20
+    nx = lx * nx0
+
+    def spikes(eps):
+        a = np.zeros((nt0, nx0), np.float32)
+        a[t0, x1], a[t0, x2] = 1.0, eps
+        return a
+
+    for name, fk in (("oracle", lambda a, n, l: F.fk_interp(a, n, l)),
+                     ("library", lambda a, n, l: G.interp_seis(a, n, nt0, 0.002, 0.002))):
+        for eps, clipped in ((eps_clip, True), (eps_free, False)):
+            a = spikes(eps)
+            q = np.asarray(F.fk_interp(a, nx0, 1), np.float64)[:, x1]
+            out = np.asarray(fk(a, nx, lx), np.float64)
+            assert out.shape == (nt0, nx)
+            exp = np.zeros((nt0, nx))
+            if clipped:
+                exp[:, lx * x1] = lx * lx * q
+                exp[:, lx * x2] = lx * lx * eps * q
+            else:
+                exp[:, lx * x1] = lx * q
+                exp[:, lx * x2] = lx * eps * q
+                exp[:, lx * x1 + x1 - x2] = lx * q / eps
+                exp[:, lx * x2 + x1 - x2] = lx * q
+            err = np.max(np.abs(out - exp)) / np.max(np.abs(exp))
+            assert err <= 1e-4, (name, eps, err)
+
+
 def test_interp_dispatch_and_copy():
     rng = np.random.default_rng(4)
     a = rng.standard_normal((50, 8)).astype(np.float32)
