@@ -53,7 +53,8 @@
  * the absorbing layer, image and traveltime locations) or brute force.
  * Parity unpinned: the index quirks of the other imaging conditions
  * (kernel.cu:1431,1468-1469: vx read at the p index, the source Poynting
- * vector with the receiver's vz) rest on reading the source, as do the last
+ * vector with the receiver's vz) rest on reading the source (position-coded
+ * fields show that the oracle reads what that source names), as do the last
  * bits of CUDA's acosf/cosf/sinf there and of its expf/logf in the set-up
  * values (wavelet, source, PML d0); the CUDA path cannot be built here.
  */

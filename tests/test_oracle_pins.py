@@ -315,6 +315,28 @@ def test_imaging_conditions_decomposition_signs():
     assert np.all(r["du"] == 6.0) and np.all(r["rl"] == 6.0) and not r["ud"].any() and not r["lr"].any()
 
 
+@pytest.mark.parametrize("nxp", [26, 27])
+def test_imaging_conditions_vx_read_at_the_p_index(nxp):
+    """kernel.cu:1431,1468 index the vx arrays (rows of nxp - 1) with the
+    pressure's flat index iz*nxp + ix.  With vxs alternating in sign along its
+    flat storage and vxr = -vxs, the left-right product collects ps*p where the
+    element read is positive and the right-left product where it is negative,
+    so every cell shows which element it read: the parity of iz*nxp + ix, not
+    that of the element [iz][ix] (iz*(nxp - 1) + ix), which differs on every
+    other row"""
+    nzp, npml = 24, 6
+    f = _ic_fields(nzp, nxp, ps=2.0, p=3.0)
+    alt = np.where(np.arange(nzp * (nxp - 1)) % 2 == 0, 1.0, -1.0).astype(np.float32).reshape(nzp, nxp - 1)
+    f["vxs"], f["vxr"] = alt, -alt
+    r = O.ic_cells(**f, npml=npml, dx=10.0, dz=10.0, dt=1e-3)
+    iz, ix = np.meshgrid(np.arange(npml, nzp - npml), np.arange(npml, nxp - npml), indexing="ij")
+    even = (iz * nxp + ix) % 2 == 0
+    assert np.array_equal(r["lr"], np.where(even, 6.0, 0.0).astype(np.float32))
+    assert np.array_equal(r["rl"], np.where(even, 0.0, 6.0).astype(np.float32))
+    assert not r["ud"].any() and not r["du"].any()
+    assert not np.array_equal(even, (iz * (nxp - 1) + ix) % 2 == 0)
+
+
 def test_imaging_conditions_poynting_angle_bin():
     """kernel.cu:1452-1488 with ps = p = 1: source vector (vxs, vzr), receiver
     vector (vxr, vzr) at cos = 0.3, so theta = pi/2 - acos(0.3)/2 = 0.9377 rad
