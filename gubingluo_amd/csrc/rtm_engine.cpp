@@ -1374,7 +1374,7 @@ int Engine::run_model(int model_id, const rtmh_params& P, const float* coef, con
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st_));
     }
     rc = run_pass(PH_FWD, nt - fwd_done, d_ds, d_tiles, (int)tiles.size(), slots, d_slots, tz, opt.store_fmt,
-                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every, true);
+                  opt.graph_chunk, (opt.profile && stats) ? stats : nullptr, guard_every, fwd_skip_ok(tz));
     if (rc) { for (void* p : mem) pool_.put(p); break; }
     HIPCHK(hipEventRecord(e1, st_));
     HIPCHK(hipStreamSynchronize(st_));

@@ -297,6 +297,14 @@ void layout_runs(std::vector<Tile>& tiles, int phase);
 // Tile::nbr of every tile: the table positions of the tiles of height tz
 // above, below and beside it in its slot (after the final layout)
 void link_neighbours(std::vector<Tile>& tiles, int tz);
+// Whether the forward step's zero-tile skip may run on tiles of height tz.
+// One step reads the pressure 2*RTM_R - 1 rows above and below a cell (the
+// velocity halo recomputed from the pressure RTM_R - 1 / RTM_R rows further
+// out), and the skip looks at the 8 adjacent tiles only: that covers the
+// reach when tz >= 2*RTM_R - 1 (32 and 64 rows), not for 16-row tiles, where
+// a wavefield front in the last rows of the tile two above can reach the
+// first rows of a tile whose neighbours are still +0.
+inline bool fwd_skip_ok(int tz) { return tz >= 2 * RTM_R - 1; }
 int params_check(const rtmh_params& P, std::string& why);
 // the stability bound of get_constant (RTM_GPU_velstr.cpp:355-361):
 // min(dx, dz) / (sqrt(2) sum|c| vmax), sum over the order-20 coefficient row

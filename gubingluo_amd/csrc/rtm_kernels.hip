@@ -835,7 +835,9 @@ __device__ __forceinline__ void field_step(float* __restrict__ lds, const ShotDe
 // Zero-tile skip of the forward step: a tile skips the step when the forward
 // field is zero at the level the step reads, on the tile and its 8
 // neighbours (all the step's inputs are then +0 and so would be its
-// outputs), and at the level it writes, on the tile itself (the buffer
+// outputs: the step reads 19 rows and columns out, within the adjacent
+// tiles for tiles of 32 or 64 rows; the host passes no words for 16-row
+// tiles), and at the level it writes, on the tile itself (the buffer
 // already holds those +0).  The wavefield starts at +0 and spreads from the
 // source, so far tiles skip until it comes near.  Words of both levels are
 // fetched beside the tile's other first loads; `it` picks the level.  (The

@@ -305,6 +305,8 @@ static void neighbour_checks()
     }
     CHECK(bad == 0 && links > 0, "tz %d: %d wrong links of %d", tz, bad, links);
   }
+  // the skip only where the adjacent tiles hold the step's 19-row reach
+  CHECK(!rtm::fwd_skip_ok(16) && rtm::fwd_skip_ok(32) && rtm::fwd_skip_ok(64), "skip gate by tile height");
   printf("ok neighbour links of the zero-tile skip\n");
 }
 
