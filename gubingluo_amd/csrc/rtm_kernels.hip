@@ -1522,20 +1522,24 @@ __global__ __launch_bounds__(NTHR) void k_ic(const ShotDev* __restrict__ shots, 
   const float vtxs = -ps * vxs, vtzs = -ps * vzs, vtxr = -p * vxr, vtzr = -p * vzr;
   const float psu = vtzs < 0 ? ps : 0.0f, psd = vtzs > 0 ? ps : 0.0f, psl = vtxs < 0 ? ps : 0.0f, psr = vtxs > 0 ? ps : 0.0f;
   const float pu = vtzr < 0 ? p : 0.0f, pd = vtzr > 0 ? p : 0.0f, pl = vtxr < 0 ? p : 0.0f, pr = vtxr > 0 ? p : 0.0f;
-  // An accumulator plane is read and written only where its product is not
-  // zero: the planes start at +0 and fma(a, b, c) with a*b = +-0 returns c
-  // bitwise for any c other than -0, which a sum that starts at +0 never
-  // reaches (exact cancellation rounds to +0).  The up/down (left/right)
-  // conditions exclude each other, and a wave travelling one way keeps them
-  // constant over a region, so whole cache lines are skipped.
+  // An accumulator plane is read and written only where neither factor of its
+  // product is zero.  There the reference's fma(a, b, c) adds +-0 and returns
+  // c unchanged, except that it can turn a -0 into +0.  A plane holds -0 only
+  // after an fma whose exact product was a negative value too small for a
+  // float, so the skipped form can differ from the reference's only in the
+  // sign of a zero.  The stacks sum the planes in FP64 from +0
+  // (+0 + -0 = +0), so the images are the same bit for bit.  The up/down
+  // (left/right) conditions exclude each other, and a wave travelling one way
+  // keeps them constant over a region, so whole cache lines are skipped.
   if (psu != 0.0f && pd != 0.0f) ic[e] = __builtin_fmaf(psu, pd, ic[e]);
   if (psd != 0.0f && pu != 0.0f) ic[nn + e] = __builtin_fmaf(psd, pu, ic[nn + e]);
   if (psl != 0.0f && pr != 0.0f) ic[2 * nn + e] = __builtin_fmaf(psl, pr, ic[2 * nn + e]);
   if (psr != 0.0f && pl != 0.0f) ic[3 * nn + e] = __builtin_fmaf(psr, pl, ic[3 * nn + e]);
   // cuda_poynting (kernel.cu:1452-1488).  Both bins add a multiple of ps*p:
   // where that product is zero (either wavefield zero, or the product
-  // underflows) the bins keep their value bit for bit (the argument above),
-  // and the angle, its cosine and sine and the FP64 quotient are not needed.
+  // underflows) they add +-0, so the bins keep their value up to the sign of
+  // a zero, as above.  The angle, its cosine and sine and the FP64 quotient
+  // are then not needed.
   const float ksx = ps * vxs, ksz = ps * vzr, krx = p * vxr, krz = p * vzr;
   if (ps * p != 0.0f) {
     const float qd = sqrtf(__builtin_fmaf(ksx, ksx, ksz * ksz) * __builtin_fmaf(krx, krx, krz * krz));
@@ -1561,7 +1565,9 @@ __global__ __launch_bounds__(NTHR) void k_ic(const ShotDev* __restrict__ shots, 
   const float pz = (float)(0.5 * (double)(prn[id + pitch] - prn[id - pitch]) / (double)s.dz);
   const float t = -(ps - pso[id]) * (p - pro[id]) / (s.dt * s.dt * vv);
   const float en = __builtin_fmaf(psz, pz, __builtin_fmaf(psx, px, t));
-  if (en != 0.0f) ic[4 * nn + e] = ic[4 * nn + e] + en;     // +-0 leaves the sum as it is (above)
+  // a sum of floats is +0 on exact cancellation and never underflows to 0,
+  // so this plane never holds -0 and adding +-0 leaves it bit for bit
+  if (en != 0.0f) ic[4 * nn + e] = ic[4 * nn + e] + en;
 }
 
 // one shot's other images into the FP64 stacks (kernel.cu:1530-1579): g2 as
