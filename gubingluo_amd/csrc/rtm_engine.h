@@ -72,6 +72,7 @@ class Extents {
   // blocks share an offset), kNone when no free range holds it
   size_t get(size_t bytes)
   {
+    if (bytes > kNone - 255) return kNone;             // would wrap when rounded up
     const size_t need = bytes ? (bytes + 255) & ~(size_t)255 : 256;
     for (auto it = holes_.begin(); it != holes_.end(); ++it) {
       if (it->second < need) continue;

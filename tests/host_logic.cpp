@@ -269,6 +269,11 @@ static void extents_checks()
   e.reset(1024);
   const size_t z0 = e.get(0), z1 = e.get(0);
   CHECK(z0 != z1 && z0 != rtm::Extents::kNone && z1 != rtm::Extents::kNone, "zero-byte blocks share offset %zu", z0);
+  // sizes that would wrap when rounded up to a unit are refused, not carved as empty blocks
+  const size_t big = rtm::Extents::kNone - 100;
+  CHECK(e.get(rtm::Extents::kNone) == rtm::Extents::kNone && e.get(big) == rtm::Extents::kNone,
+        "a size near the top of size_t was carved");
+  CHECK(e.blocks().size() == 2, "%zu blocks after the refusals", e.blocks().size());
   printf("extents: %d blocks, %d refusals, first fit and merging hold\n", got, refused);
 }
 
