@@ -325,6 +325,25 @@ static void neighbour_checks()
 // even share.
 static void layout_checks()
 {
+  // the engine sizes its tile table for n * 5 / 4 + 16 entries: the padded
+  // table stays within that however the long tiles are spread (all first,
+  // all last, alternating, none), for every table length up to 300
+  for (int phase : {PH_FWD, PH_REV}) {
+    const int flag = phase == PH_REV ? TF_INT : TF_GEN;
+    for (int n = 1; n <= 300; n++)
+      for (int pat = 0; pat < 4; pat++) {
+        std::vector<Tile> t;
+        for (int i = 0; i < n; i++) {
+          const bool heavy = pat == 0 ? i < n / 2 : pat == 1 ? i >= n / 2 : pat == 2 ? (i & 1) : false;
+          t.push_back(Tile{i % 3, 64 * i, 0, heavy ? flag : 0});
+        }
+        rtm::layout_runs(t, phase);
+        size_t real = 0;
+        for (const Tile& x : t) real += x.slot >= 0;
+        CHECK(t.size() % 8 == 0 && t.size() <= (size_t)n * 5 / 4 + 16 && real == (size_t)n,
+              "phase %d n %d pattern %d: %zu entries", phase, n, pat, t.size());
+      }
+  }
   std::vector<ShotDev> ds(5);
   memset(ds.data(), 0, ds.size() * sizeof(ShotDev));
   const int w[5] = {300, 701, 450, 1300, 520}, h[5] = {170, 150, 260, 210, 180};
@@ -337,6 +356,7 @@ static void layout_checks()
     const int flag = phase == PH_REV ? TF_INT : TF_GEN;
     const double heavy = phase == PH_REV ? 2.0 : 1.55;
     CHECK(t.size() % 8 == 0, "phase %d: table of %zu entries is not 8 runs", phase, t.size());
+    CHECK(t.size() <= in.size() * 5 / 4 + 16, "phase %d: padded table of %zu for %zu tiles", phase, t.size(), in.size());
     const size_t len = t.size() / 8;
     // every input tile exactly once
     auto key = [](const Tile& x) { return ((long long)x.slot << 40) | ((long long)x.z0 << 20) | x.x0; };
