@@ -6,8 +6,10 @@ refuses memory with nothing resident fails the request; one-shot budgets
 admit in arrival order), the server's first-fit device arena (blocks never
 overlap, the lowest fitting hole is taken, freed neighbours merge), the
 per-XCD runs of the tile table (every tile once, long tiles first, padding at
-run ends, runs of equal cost, each a contiguous piece), the neighbour links of
-the forward step's zero-tile skip, and the checkpoint resume decision (a checkpoint
+run ends, runs of equal cost, each a contiguous piece, the padded table within
+the engine's capacity), the neighbour links of
+the forward step's zero-tile skip and the tile heights it runs on, and the
+checkpoint resume decision (a checkpoint
 with more shots done than the job has is refused)."""
 import os
 import subprocess
