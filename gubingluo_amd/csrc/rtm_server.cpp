@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -130,8 +131,15 @@ class Server {
   long long tick_ = 0, launches_ = 0;
   int next_id_ = 1, peak_ = 0, active_ = 0;
   double busy_ = 0, cells_done_ = 0;
-  int err_ = 0;
+  // The first device error ends the server: its code (atomic: the loop reads
+  // it without a lock, submit and wait under mu_) and its message (under
+  // err_mu_, which is taken last -- nothing else is locked while it is held --
+  // so fail() may run with or without mu_ held).
+  std::atomic<int> err_{0};
+  std::mutex err_mu_;
   std::string err_msg_;
+  void fail_with(int code, std::string msg);
+  std::string err_message();
   std::chrono::steady_clock::time_point t0_;
 };
 
@@ -208,8 +216,21 @@ Server::~Server()
 
 void Server::fail(const char* what, hipError_t e)
 {
-  err_ = -10;
-  err_msg_ = std::string(what) + ": " + hipGetErrorString(e);
+  fail_with(-10, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void Server::fail_with(int code, std::string msg)
+{
+  std::lock_guard<std::mutex> lk(err_mu_);
+  if (err_.load() != 0) return;                        // keep the first error
+  err_msg_ = std::move(msg);
+  err_.store(code);
+}
+
+std::string Server::err_message()
+{
+  std::lock_guard<std::mutex> lk(err_mu_);
+  return err_msg_;
 }
 
 int Server::request_bytes(int model_id, const rtmh_params& P, const float* coef, double* shot_max, double* request)
@@ -286,7 +307,7 @@ int Server::submit(int model_id, const rtmh_params& P, const float* coef, int* i
   for (int k = 0; k < nsh; k++) need = std::max(need, shot_need(*r, k));
   const size_t extra = request_bytes(*r);
   std::lock_guard<std::mutex> lk(mu_);
-  if (err_) { set_error(err_msg_); return err_; }
+  if (const int e = err_.load()) { set_error(err_message()); return e; }
   if (admit_rule(budget_, 0.0, 0, false, (double)need, (double)extra) == Admit::kReject) {
     set_error("a shot of this request needs " + std::to_string(need + extra) + " bytes of HBM (slot " +
               std::to_string(need) + ", request block " + std::to_string(extra) + "), above the server's budget of " +
@@ -509,7 +530,7 @@ void Server::rebuild_tiles()
     if (hipMalloc(&d_tf_, tile_cap_ * sizeof(Tile)) != hipSuccess ||
         hipMalloc(&d_tr_, tile_cap_ * sizeof(Tile)) != hipSuccess) {
       (void)hipGetLastError();                       // reported through err_, not by a launch
-      err_ = -3; err_msg_ = "out of device memory (tiles)";
+      fail_with(-3, "out of device memory (tiles)");
       return;
     }
   }
@@ -732,7 +753,7 @@ int Server::wait(int id, double timeout, rtmh_outputs* out, rtmh_request_stats* 
   while (!r.complete && !err_) {
     if (cv_done_.wait_until(lk, until) == std::cv_status::timeout && !r.complete) return 1;
   }
-  if (err_ && !r.complete) { set_error(err_msg_); return err_; }
+  if (const int e = err_.load(); e && !r.complete) { set_error(err_message()); return e; }
   if (r.status != 0) {
     if (st) { st->submit_s = r.t_submit; st->start_s = r.t_start; st->done_s = r.t_done; st->status = r.status; }
     set_error(r.err);
