@@ -128,9 +128,11 @@ class Server {
   std::vector<int> qslots_;
   bool dirty_ = true, stop_ = false;
   double budget_ = 0, used_ = 0;
-  long long tick_ = 0, launches_ = 0;
+  // written by the loop only, read by query() from other threads
+  std::atomic<long long> tick_{0}, launches_{0};
+  std::atomic<double> busy_{0};
   int next_id_ = 1, peak_ = 0, active_ = 0;
-  double busy_ = 0, cells_done_ = 0;
+  double cells_done_ = 0;
   // The first device error ends the server: its code (atomic: the loop reads
   // it without a lock, submit and wait under mu_) and its message (under
   // err_mu_, which is taken last -- nothing else is locked while it is held --
@@ -731,14 +733,14 @@ void Server::loop()
       hipEvent_t ev;
       (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
       (void)hipEventRecord(ev, st_);
-      inflight.push_back({tick_, ev});
+      inflight.push_back({tick_.load(), ev});
       while (inflight.size() > 4) {
         (void)hipEventSynchronize(inflight.front().second);
         (void)hipEventDestroy(inflight.front().second);
         inflight.pop_front();
       }
     }
-    busy_ += now() - tb;
+    busy_.store(busy_.load() + (now() - tb));
   }
   for (auto& p : inflight) { (void)hipEventSynchronize(p.second); (void)hipEventDestroy(p.second); }
 }
@@ -778,7 +780,7 @@ void Server::query(rtmh_server_stats* st)
 {
   std::lock_guard<std::mutex> lk(mu_);
   st->ticks = tick_; st->launches = launches_; st->peak_slots = peak_; st->active_slots = active_;
-  st->queued_requests = (int)queue_.size(); st->busy_s = busy_; st->cell_updates = cells_done_;
+  st->queued_requests = (int)queue_.size(); st->busy_s = busy_.load(); st->cell_updates = cells_done_;
   st->budget_bytes = budget_;
   st->allocations = pool_.allocations();
   st->alloc_s = pool_.alloc_seconds();
