@@ -499,6 +499,10 @@ DevPool::~DevPool()
 bool DevPool::reserve(size_t bytes)
 {
   if (arena_ || bytes < 256) return false;
+  // only a pool with no block in use (every block it holds is cached) takes
+  // an arena: put() maps each pointer to the arena, so no block from before
+  // it may come back there
+  if (size_.size() != free_.size()) return false;
   bytes &= ~(size_t)255;
   void* p = nullptr;
   const auto t0 = std::chrono::steady_clock::now();

@@ -123,7 +123,7 @@ class DevPool {
   // Take `bytes` of device memory in one allocation now and carve every later
   // get() from it (first fit, 256-byte blocks, free neighbours merged on
   // put()): no device allocation call after this one.  False (and the pool
-  // stays as it was) if the device refuses.
+  // stays as it was) if the device refuses or a block of the pool is in use.
   bool reserve(size_t bytes);
   size_t held() const { return arena_ ? arena_bytes_ : held_; }
   // host seconds spent in hipMalloc / hipFree, and the number of hipMalloc calls
